@@ -1,0 +1,381 @@
+"""MI355X column-compression engine: Python mirror of the reference's host API.
+
+The compute path is ``libddj_hip.so`` (hand-written HIP kernels for gfx950
+behind the C ABI declared in ``include/ddj.h``); this module only binds it
+with ctypes.  The class and method names follow the reference
+(dzitkowskik/GpuTimeSeriesCompressionOptimizer, src/):
+
+* :class:`Encoding`        -- compression/encoding.hpp (Encode / Decode /
+  GetNumberOfResults / GetReturnTypes / GetMetadataSize / GetCompressedSize)
+* :class:`CompressionTree` -- tree/compression_tree.hpp (Compress /
+  Decompress / GetPredictedSizeAfterCompression)
+* :class:`CompressionOptimizer` -- optimizer/compression_optimizer.hpp
+  (OptimizeTree / CompressData) plus the exhaustive phase-1 planner
+* :class:`ParallelTSCompressor` -- parallel_ts_compressor.hpp
+
+There is no fallback: if the library is missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Iterable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libddj_hip.so")
+
+# EncodingType (compression/encoding_type.hpp:17-30)
+DELTA, AFL, SCALE, RLE, DICT, UNIQUE, PATCH, NONE, FLOAT_TO_INT, CONST, GFC = range(11)
+OUTSIDE_PATCH = 100
+ENCODING_NAMES = {DELTA: "delta", AFL: "afl", SCALE: "scale", RLE: "rle", DICT: "dict", UNIQUE: "unique",
+                  PATCH: "patch", NONE: "none", FLOAT_TO_INT: "floatToInt", CONST: "constData", GFC: "gfc",
+                  OUTSIDE_PATCH: "patch(outside)"}
+
+# DataType codes carried in tree headers (see include/ddj.h)
+D_INT, D_TIME, D_FLOAT, D_DOUBLE, D_SHORT, D_CHAR = range(6)
+NP_DTYPE = {D_INT: np.int32, D_TIME: np.int64, D_FLOAT: np.float32, D_DOUBLE: np.float64, D_SHORT: np.int16,
+            D_CHAR: np.int8}
+DTYPE_OF_NP = {np.dtype(v): k for k, v in NP_DTYPE.items()}
+
+PLAN_EXHAUSTIVE, PLAN_OPTIMIZE30, PLAN_LEARNED = 0, 1, 2
+
+STATUS = {0: "OK", 1: "EINVAL", 2: "EUNSUPPORTED", 3: "ECORRUPT", 4: "ENOMEM", 5: "EHIP", 6: "ETIMEOUT"}
+
+
+class DdjError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"{STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class _Buf(ctypes.Structure):
+    _fields_ = [("dev", ctypes.c_void_p), ("bytes", ctypes.c_size_t)]
+
+
+class Stats(ctypes.Structure):
+    """ddj_stats_t: DataStatistics (util/statistics/cuda_array_statistics.hpp:24-34) + planner features."""
+    _fields_ = [("min", ctypes.c_double), ("max", ctypes.c_double), ("minBitCnt", ctypes.c_int32),
+                ("precision", ctypes.c_int32), ("sorted", ctypes.c_int32), ("rlMetric", ctypes.c_float),
+                ("mean", ctypes.c_double), ("size", ctypes.c_uint64), ("runs", ctypes.c_uint64),
+                ("distinct_est", ctypes.c_uint64), ("bit_hist", ctypes.c_uint32 * 65),
+                ("delta_bits_max", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libddj_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with `python -m gputimeseriescompressionoptimizer_amd.build`")
+    L = ctypes.CDLL(path)
+    vp, sz, i32, u32, u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64
+    P = ctypes.POINTER
+    sig = {
+        "ddj_ctx_create": (i32, [P(vp), i32]),
+        "ddj_ctx_destroy": (None, [vp]),
+        "ddj_ctx_sync": (i32, [vp]),
+        "ddj_last_error": (ctypes.c_char_p, [vp]),
+        "ddj_ctx_trim": (None, [vp]),
+        "ddj_ctx_stream": (vp, [vp]),
+        "ddj_buf_alloc": (i32, [vp, sz, P(_Buf)]),
+        "ddj_buf_free": (None, [vp, P(_Buf)]),
+        "ddj_memcpy_h2d": (i32, [vp, vp, vp, sz]),
+        "ddj_memcpy_d2h": (i32, [vp, vp, vp, sz]),
+        "ddj_memcpy_d2d": (i32, [vp, vp, vp, sz]),
+        "ddj_encoding_info": (i32, [i32, i32, P(u32), P(i32)]),
+        "ddj_encode": (i32, [vp, i32, i32, vp, sz, P(_Buf)]),
+        "ddj_decode": (i32, [vp, i32, i32, P(_Buf), u32, P(_Buf)]),
+        "ddj_predict_sizes": (i32, [vp, i32, i32, vp, sz, P(sz), P(sz)]),
+        "ddj_stats": (i32, [vp, i32, vp, sz, P(Stats)]),
+        "ddj_fl_pack": (i32, [vp, i32, i32, vp, sz, vp]),
+        "ddj_fl_unpack": (i32, [vp, i32, i32, vp, sz, sz, vp]),
+        "ddj_tree_compress": (i32, [vp, P(i32), sz, i32, vp, sz, P(_Buf), P(ctypes.c_double)]),
+        "ddj_tree_decompress": (i32, [vp, vp, sz, P(_Buf), P(i32)]),
+        "ddj_tree_predict_size": (i32, [vp, P(i32), sz, i32, vp, sz, P(sz)]),
+        "ddj_plan": (i32, [vp, i32, i32, vp, sz, P(i32), sz, P(sz), P(ctypes.c_double), P(u64)]),
+        "ddj_optimizer_create": (i32, [vp, P(vp)]),
+        "ddj_optimizer_destroy": (None, [vp]),
+        "ddj_optimizer_compress": (i32, [vp, i32, vp, sz, P(_Buf)]),
+        "ddj_optimizer_tree": (i32, [vp, P(i32), sz, P(sz)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def encoding_info(enc: int, dtype: int) -> Tuple[int, List[int]]:
+    """Encoding::GetNumberOfResults and GetReturnTypes."""
+    L = load_library()
+    n = ctypes.c_uint32()
+    rt = (ctypes.c_int * 2)()
+    st = L.ddj_encoding_info(enc, dtype, ctypes.byref(n), rt)
+    if st:
+        raise DdjError(st, f"encoding_info({enc}, {dtype})")
+    return n.value, [rt[0], rt[1]][: n.value]
+
+
+class DeviceBuffer:
+    """Device memory owned by a :class:`Context` (freed with the object)."""
+
+    def __init__(self, ctx: "Context", dev: int, nbytes: int, owned: bool = True):
+        self.ctx = ctx
+        self.dev = dev or 0
+        self.nbytes = nbytes
+        self._owned = owned
+
+    def __len__(self):
+        return self.nbytes
+
+    def to_bytes(self) -> bytes:
+        return self.ctx.download(self)
+
+    def to_numpy(self, dtype) -> np.ndarray:
+        return np.frombuffer(self.to_bytes(), dtype=dtype)
+
+    def view(self, offset: int, nbytes: int) -> "DeviceBuffer":
+        return DeviceBuffer(self.ctx, self.dev + offset if self.dev else 0, nbytes, owned=False)
+
+    def free(self):
+        if self._owned and self.dev and self.ctx._h:
+            b = _Buf(self.dev, self.nbytes)
+            self.ctx._L.ddj_buf_free(self.ctx._h, ctypes.byref(b))
+        self.dev = 0
+        self._owned = False
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def _as_c(self) -> _Buf:
+        return _Buf(self.dev or None, self.nbytes)
+
+
+class Context:
+    """A GPU, a HIP stream and a caching device allocator (ddj_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._L = load_library()
+        h = ctypes.c_void_p()
+        st = self._L.ddj_ctx_create(ctypes.byref(h), device)
+        if st:
+            raise DdjError(st, f"ddj_ctx_create(device={device})")
+        self._h = h
+
+    def check(self, st: int, what: str):
+        if st:
+            msg = self._L.ddj_last_error(self._h)
+            raise DdjError(st, f"{what}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ddj_ctx_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def sync(self):
+        self.check(self._L.ddj_ctx_sync(self._h), "sync")
+
+    def stream(self) -> int:
+        return self._L.ddj_ctx_stream(self._h) or 0
+
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        b = _Buf()
+        self.check(self._L.ddj_buf_alloc(self._h, nbytes, ctypes.byref(b)), "alloc")
+        return DeviceBuffer(self, b.dev, nbytes)
+
+    def upload(self, data) -> DeviceBuffer:
+        if isinstance(data, np.ndarray):
+            data = np.ascontiguousarray(data)
+            raw, n = data.ctypes.data, data.nbytes
+            keep = data
+        else:
+            keep = ctypes.create_string_buffer(bytes(data), max(len(data), 1))
+            raw, n = ctypes.addressof(keep), len(data)
+        buf = self.alloc(n)
+        if n:
+            self.check(self._L.ddj_memcpy_h2d(self._h, buf.dev, raw, n), "upload")
+        del keep
+        return buf
+
+    def download(self, buf: DeviceBuffer) -> bytes:
+        if buf.nbytes == 0:
+            return b""
+        out = ctypes.create_string_buffer(buf.nbytes)
+        self.check(self._L.ddj_memcpy_d2h(self._h, out, buf.dev, buf.nbytes), "download")
+        return out.raw
+
+    def _wrap(self, b: _Buf) -> DeviceBuffer:
+        return DeviceBuffer(self, b.dev or 0, b.bytes)
+
+    def stats(self, buf: DeviceBuffer, dtype: int) -> Stats:
+        """CudaArrayStatistics::GenerateStatistics."""
+        s = Stats()
+        self.check(self._L.ddj_stats(self._h, dtype, buf.dev or None, buf.nbytes, ctypes.byref(s)), "stats")
+        return s
+
+    def fl_pack(self, buf: DeviceBuffer, dtype: int, bits: int, n: int) -> DeviceBuffer:
+        W = 8 * np.dtype(NP_DTYPE[dtype]).itemsize
+        nbytes = (bits * n + W - 1) // W * (W // 8)
+        out = self.alloc(nbytes)
+        self.check(self._L.ddj_fl_pack(self._h, dtype, bits, buf.dev or None, n, out.dev or None), "fl_pack")
+        return out
+
+    def fl_unpack(self, buf: DeviceBuffer, dtype: int, bits: int, n: int) -> DeviceBuffer:
+        out = self.alloc(n * np.dtype(NP_DTYPE[dtype]).itemsize)
+        self.check(self._L.ddj_fl_unpack(self._h, dtype, bits, buf.dev or None, buf.nbytes, n, out.dev or None),
+                   "fl_unpack")
+        return out
+
+
+class Encoding:
+    """One reference encoding (compression/encoding.hpp:18-113)."""
+
+    def __init__(self, ctx: Context, encoding_type: int):
+        self.ctx = ctx
+        self.encoding_type = encoding_type
+
+    def GetNumberOfResults(self, dtype: int = D_INT) -> int:
+        return encoding_info(self.encoding_type, dtype)[0]
+
+    def GetReturnTypes(self, dtype: int) -> List[int]:
+        return encoding_info(self.encoding_type, dtype)[1]
+
+    def Encode(self, data: DeviceBuffer, dtype: int) -> List[DeviceBuffer]:
+        n, _ = encoding_info(self.encoding_type, dtype)
+        outs = (_Buf * (n + 1))()
+        self.ctx.check(self.ctx._L.ddj_encode(self.ctx._h, self.encoding_type, dtype, data.dev or None, data.nbytes,
+                                              outs), f"encode {ENCODING_NAMES.get(self.encoding_type)}")
+        return [self.ctx._wrap(outs[i]) for i in range(n + 1)]
+
+    def Decode(self, parts: Sequence[DeviceBuffer], dtype: int) -> DeviceBuffer:
+        ins = (_Buf * len(parts))(*[p._as_c() for p in parts])
+        out = _Buf()
+        self.ctx.check(self.ctx._L.ddj_decode(self.ctx._h, self.encoding_type, dtype, ins, len(parts),
+                                              ctypes.byref(out)), f"decode {ENCODING_NAMES.get(self.encoding_type)}")
+        return self.ctx._wrap(out)
+
+    def _sizes(self, data: DeviceBuffer, dtype: int) -> Tuple[int, int]:
+        m, d = ctypes.c_size_t(), ctypes.c_size_t()
+        self.ctx.check(self.ctx._L.ddj_predict_sizes(self.ctx._h, self.encoding_type, dtype, data.dev or None,
+                                                     data.nbytes, ctypes.byref(m), ctypes.byref(d)), "predict")
+        return m.value, d.value
+
+    def GetMetadataSize(self, data: DeviceBuffer, dtype: int) -> int:
+        return self._sizes(data, dtype)[0]
+
+    def GetCompressedSize(self, data: DeviceBuffer, dtype: int) -> int:
+        return self._sizes(data, dtype)[1]
+
+    @staticmethod
+    def GetCompressionRatio(input_size: int, output_size: int) -> float:
+        """Encoding::GetCompressionRatio (compression/encoding.hpp:78-84)."""
+        if output_size <= 0 or output_size > input_size:
+            return 1.0
+        return input_size / output_size
+
+
+def _path_arg(path: Sequence[int]):
+    arr = (ctypes.c_int * len(path))(*path)
+    return arr, len(path)
+
+
+class CompressionTree:
+    """A cascade given as a pre-order path of encodings
+    (PathGenerator::GenerateTree, optimizer/path_generator.cpp:70-91)."""
+
+    def __init__(self, ctx: Context, path: Optional[Sequence[int]] = None, dtype: int = D_INT):
+        self.ctx = ctx
+        self.path = list(path) if path is not None else None
+        self.dtype = dtype
+        self.node_ratios: List[float] = []
+
+    def Compress(self, data: DeviceBuffer) -> DeviceBuffer:
+        arr, n = _path_arg(self.path)
+        ratios = (ctypes.c_double * n)()
+        out = _Buf()
+        self.ctx.check(self.ctx._L.ddj_tree_compress(self.ctx._h, arr, n, self.dtype, data.dev or None, data.nbytes,
+                                                     ctypes.byref(out), ratios), "tree compress")
+        self.node_ratios = list(ratios)
+        return self.ctx._wrap(out)
+
+    def GetCompressionRatio(self) -> float:
+        return self.node_ratios[0] if self.node_ratios else 1.0
+
+    def Decompress(self, blob: DeviceBuffer) -> DeviceBuffer:
+        out = _Buf()
+        dt = ctypes.c_int()
+        self.ctx.check(self.ctx._L.ddj_tree_decompress(self.ctx._h, blob.dev or None, blob.nbytes, ctypes.byref(out),
+                                                       ctypes.byref(dt)), "tree decompress")
+        self.dtype = dt.value
+        return self.ctx._wrap(out)
+
+    def GetPredictedSizeAfterCompression(self, data: DeviceBuffer, dtype: Optional[int] = None) -> int:
+        arr, n = _path_arg(self.path)
+        s = ctypes.c_size_t()
+        self.ctx.check(self.ctx._L.ddj_tree_predict_size(self.ctx._h, arr, n, self.dtype if dtype is None else dtype,
+                                                         data.dev or None, data.nbytes, ctypes.byref(s)), "predict")
+        return s.value
+
+
+class CompressionOptimizer:
+    """CompressionOptimizer (optimizer/compression_optimizer.hpp:28-96)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(ctx._L.ddj_optimizer_create(ctx._h, ctypes.byref(h)), "optimizer create")
+        self._h = h
+
+    def __del__(self):
+        try:
+            if self._h:
+                self.ctx._L.ddj_optimizer_destroy(self._h)
+        except Exception:
+            pass
+
+    def _plan(self, mode: int, data: DeviceBuffer, dtype: int) -> Tuple[List[int], float, int]:
+        path = (ctypes.c_int * 4096)()
+        n = ctypes.c_size_t()
+        r = ctypes.c_double()
+        cand = ctypes.c_uint64()
+        self.ctx.check(self.ctx._L.ddj_plan(self.ctx._h, mode, dtype, data.dev or None, data.nbytes, path, 4096,
+                                            ctypes.byref(n), ctypes.byref(r), ctypes.byref(cand)), "plan")
+        return list(path[: n.value]), r.value, cand.value
+
+    def OptimizeTree(self, data: DeviceBuffer, dtype: int) -> CompressionTree:
+        """The 30-path brute force (compression_optimizer.cpp:16-36)."""
+        path, _, _ = self._plan(PLAN_OPTIMIZE30, data, dtype)
+        return CompressionTree(self.ctx, path, dtype)
+
+    def PlanExhaustive(self, data: DeviceBuffer, dtype: int) -> Tuple[CompressionTree, float, int]:
+        """Phase 1 of CompressData: FullStatisticsUpdate on the first 10,000 rows."""
+        path, ratio, cand = self._plan(PLAN_EXHAUSTIVE, data, dtype)
+        return CompressionTree(self.ctx, path, dtype), ratio, cand
+
+    def CompressData(self, part: DeviceBuffer, dtype: int) -> DeviceBuffer:
+        out = _Buf()
+        self.ctx.check(self.ctx._L.ddj_optimizer_compress(self._h, dtype, part.dev or None, part.nbytes,
+                                                          ctypes.byref(out)), "CompressData")
+        return self.ctx._wrap(out)
+
+    def GetOptimalTree(self) -> List[int]:
+        path = (ctypes.c_int * 4096)()
+        n = ctypes.c_size_t()
+        self.ctx.check(self.ctx._L.ddj_optimizer_tree(self._h, path, 4096, ctypes.byref(n)), "optimizer tree")
+        return list(path[: n.value])

@@ -1,0 +1,292 @@
+// The C ABI (include/ddj.h): status codes at the boundary, C++ inside.
+#include <cstdio>
+#include <cstring>
+
+#include "codecs.h"
+#include "planner.h"
+#include "tree.h"
+
+using namespace ddj;
+
+struct ddj_ctx {
+    Ctx c;
+};
+
+struct ddj_optimizer {
+    ddj_ctx* ctx;
+    std::unique_ptr<Optimizer> opt;
+};
+
+namespace {
+
+template <typename F> ddj_status guarded(ddj_ctx* ctx, F&& f) {
+    if (!ctx) return DDJ_EINVAL;
+    try {
+        HIPC(hipSetDevice(ctx->c.device));
+        f();
+        return DDJ_OK;
+    } catch (const Error& e) {
+        ctx->c.last_error = e.msg;
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        ctx->c.last_error = "host allocation failed";
+        return DDJ_ENOMEM;
+    } catch (const std::exception& e) {
+        ctx->c.last_error = e.what();
+        return DDJ_EINVAL;
+    }
+}
+
+ddj_buf to_buf(Ctx* c, const Part& p) {
+    // hand out a buffer the caller owns: reuse the block when the part owns
+    // all of it, otherwise copy
+    if (p.n == 0) return ddj_buf{nullptr, 0};
+    if (p.owner && p.owner.use_count() == 1 && p.owner->bytes_ptr() == p.p) {
+        DevBuf& b = *p.owner;
+        b.set_size(p.n);
+        return b.release();
+    }
+    DevBuf b(c, p.n);
+    d2d(c, b.get(), p.p, p.n);
+    return b.release();
+}
+
+std::vector<DView> views(const ddj_buf* ins, uint32_t n) {
+    std::vector<DView> v;
+    for (uint32_t i = 0; i < n; i++) v.push_back(DView{static_cast<const uint8_t*>(ins[i].dev), ins[i].bytes});
+    return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+ddj_status ddj_ctx_create(ddj_ctx** out, int device) {
+    if (!out) return DDJ_EINVAL;
+    *out = nullptr;
+    auto* ctx = new (std::nothrow) ddj_ctx;
+    if (!ctx) return DDJ_ENOMEM;
+    ctx->c.device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            ctx->c.cu_count = prop.multiProcessorCount;
+        e = hipMalloc(reinterpret_cast<void**>(&ctx->c.control), sizeof(Control));
+    }
+    if (e == hipSuccess) e = hipMemset(ctx->c.control, 0, sizeof(Control));
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        fprintf(stderr, "ddj_ctx_create: %s\n", hipGetErrorString(e));
+        if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+        delete ctx;
+        return e == hipErrorOutOfMemory ? DDJ_ENOMEM : DDJ_EHIP;
+    }
+    ctx->c.alloc.reset(new Allocator(device));
+    *out = ctx;
+    return DDJ_OK;
+}
+
+void ddj_ctx_destroy(ddj_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    ctx->c.alloc.reset();
+    (void)hipFree(ctx->c.control);
+    (void)hipStreamDestroy(ctx->c.stream);
+    delete ctx;
+}
+
+ddj_status ddj_ctx_sync(ddj_ctx* ctx) {
+    return guarded(ctx, [&] { HIPC(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+const char* ddj_last_error(ddj_ctx* ctx) { return ctx ? ctx->c.last_error.c_str() : "no context"; }
+
+void ddj_ctx_trim(ddj_ctx* ctx) {
+    if (ctx && ctx->c.alloc) ctx->c.alloc->trim();
+}
+
+void* ddj_ctx_stream(ddj_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
+
+ddj_status ddj_buf_alloc(ddj_ctx* ctx, size_t bytes, ddj_buf* out) {
+    return guarded(ctx, [&] {
+        if (!out) fail(DDJ_EINVAL, "null output");
+        DevBuf b(&ctx->c, bytes);
+        *out = b.release();
+    });
+}
+
+void ddj_buf_free(ddj_ctx* ctx, ddj_buf* buf) {
+    if (!ctx || !buf) return;
+    if (buf->dev) ctx->c.alloc->free(buf->dev);
+    buf->dev = nullptr;
+    buf->bytes = 0;
+}
+
+ddj_status ddj_memcpy_h2d(ddj_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    return guarded(ctx, [&] { h2d(&ctx->c, dst, src, bytes); });
+}
+
+ddj_status ddj_memcpy_d2h(ddj_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    return guarded(ctx, [&] { d2h(&ctx->c, dst, src, bytes); });
+}
+
+ddj_status ddj_memcpy_d2d(ddj_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    return guarded(ctx, [&] { d2d(&ctx->c, dst, src, bytes); });
+}
+
+ddj_status ddj_encoding_info(int enc, int dtype, uint32_t* n, int* rt) {
+    try {
+        if (!enc_ok(enc) || !dtype_ok(dtype) || !n || !rt) return DDJ_EINVAL;
+        *n = (uint32_t)n_results(enc);
+        return_types(enc, dtype, rt);
+        return DDJ_OK;
+    } catch (const Error& e) {
+        return e.status;
+    }
+}
+
+ddj_status ddj_encode(ddj_ctx* ctx, int enc, int dtype, const void* d_in, size_t in_bytes, ddj_buf* outs) {
+    return guarded(ctx, [&] {
+        if (!outs || (in_bytes && !d_in)) fail(DDJ_EINVAL, "null argument");
+        auto parts = encode(&ctx->c, enc, dtype, DView{static_cast<const uint8_t*>(d_in), in_bytes});
+        for (size_t i = 0; i < parts.size(); i++) outs[i] = to_buf(&ctx->c, parts[i]);
+    });
+}
+
+ddj_status ddj_decode(ddj_ctx* ctx, int enc, int dtype, const ddj_buf* ins, uint32_t n_ins, ddj_buf* out) {
+    return guarded(ctx, [&] {
+        if (!ins || !out) fail(DDJ_EINVAL, "null argument");
+        Part p = decode(&ctx->c, enc, dtype, views(ins, n_ins));
+        *out = to_buf(&ctx->c, p);
+    });
+}
+
+ddj_status ddj_predict_sizes(ddj_ctx* ctx, int enc, int dtype, const void* d_in, size_t in_bytes, size_t* meta,
+                             size_t* data) {
+    return guarded(ctx, [&] {
+        if (!meta || !data) fail(DDJ_EINVAL, "null argument");
+        predict_sizes(&ctx->c, enc, dtype, DView{static_cast<const uint8_t*>(d_in), in_bytes}, meta, data);
+    });
+}
+
+ddj_status ddj_stats(ddj_ctx* ctx, int dtype, const void* d_in, size_t in_bytes, ddj_stats_t* out) {
+    return guarded(ctx, [&] {
+        if (!out || !dtype_ok(dtype)) fail(DDJ_EINVAL, "bad argument");
+        *out = stats(&ctx->c, dtype, DView{static_cast<const uint8_t*>(d_in), in_bytes});
+    });
+}
+
+ddj_status ddj_fl_pack(ddj_ctx* ctx, int dtype, int bits, const void* d_in, size_t n, void* d_out) {
+    return guarded(ctx, [&] {
+        if (!dtype_ok(dtype) || dtype == DDJ_D_FLOAT || dtype == DDJ_D_DOUBLE) fail(DDJ_EINVAL, "integer types only");
+        if (bits <= 0 || bits > 8 * (int)dtype_size(dtype)) fail(DDJ_EINVAL, "bit width out of range");
+        launch_fl_pack(&ctx->c, dtype, d_in, n, nullptr, bits, d_out);
+    });
+}
+
+ddj_status ddj_fl_unpack(ddj_ctx* ctx, int dtype, int bits, const void* d_in, size_t in_bytes, size_t n, void* d_out) {
+    return guarded(ctx, [&] {
+        if (!dtype_ok(dtype) || dtype == DDJ_D_FLOAT || dtype == DDJ_D_DOUBLE) fail(DDJ_EINVAL, "integer types only");
+        launch_fl_unpack(&ctx->c, dtype, static_cast<const uint8_t*>(d_in), in_bytes, n, bits, d_out);
+    });
+}
+
+ddj_status ddj_tree_compress(ddj_ctx* ctx, const int* preorder_path, size_t path_len, int dtype, const void* d_in,
+                             size_t in_bytes, ddj_buf* blob, double* node_ratios) {
+    return guarded(ctx, [&] {
+        if (!preorder_path || !blob || !dtype_ok(dtype)) fail(DDJ_EINVAL, "bad argument");
+        NodeP root = tree_from_path(preorder_path, path_len, dtype);
+        fix_tree(root);
+        Part p = compress_tree(&ctx->c, root, DView{static_cast<const uint8_t*>(d_in), in_bytes});
+        *blob = to_buf(&ctx->c, p);
+        if (node_ratios) {
+            size_t i = 0;
+            std::function<void(const NodeP&)> rec = [&](const NodeP& n) {
+                if (i < path_len) node_ratios[i] = n->ratio;
+                i++;
+                for (auto& ch : n->ch) rec(ch);
+            };
+            rec(root);
+        }
+    });
+}
+
+ddj_status ddj_tree_decompress(ddj_ctx* ctx, const void* d_blob, size_t blob_bytes, ddj_buf* out, int* dtype) {
+    return guarded(ctx, [&] {
+        if (!out) fail(DDJ_EINVAL, "null output");
+        int dt = 0;
+        Part p = decompress_tree(&ctx->c, DView{static_cast<const uint8_t*>(d_blob), blob_bytes}, &dt);
+        *out = to_buf(&ctx->c, p);
+        if (dtype) *dtype = dt;
+    });
+}
+
+ddj_status ddj_tree_predict_size(ddj_ctx* ctx, const int* preorder_path, size_t path_len, int dtype, const void* d_in,
+                                 size_t in_bytes, size_t* size) {
+    return guarded(ctx, [&] {
+        if (!preorder_path || !size || !dtype_ok(dtype)) fail(DDJ_EINVAL, "bad argument");
+        NodeP root = tree_from_path(preorder_path, path_len, dtype);
+        *size = predict_tree(&ctx->c, root, DView{static_cast<const uint8_t*>(d_in), in_bytes});
+    });
+}
+
+static void write_path(const NodeP& t, int* path_out, size_t cap, size_t* len) {
+    auto pre = preorder(t);
+    if (pre.size() > cap) fail(DDJ_EINVAL, "path buffer too small");
+    memcpy(path_out, pre.data(), pre.size() * sizeof(int));
+    *len = pre.size();
+}
+
+ddj_status ddj_plan(ddj_ctx* ctx, int mode, int dtype, const void* d_in, size_t in_bytes, int* path_out,
+                    size_t path_cap, size_t* path_len, double* ratio, uint64_t* n_candidates) {
+    return guarded(ctx, [&] {
+        if (!path_out || !path_len || !dtype_ok(dtype)) fail(DDJ_EINVAL, "bad argument");
+        DView in{static_cast<const uint8_t*>(d_in), in_bytes};
+        NodeP t;
+        if (mode == DDJ_PLAN_EXHAUSTIVE) {
+            size_t sz = 0;
+            t = plan_exhaustive(&ctx->c, in, dtype, &sz, n_candidates);
+        } else if (mode == DDJ_PLAN_OPTIMIZE30) {
+            t = optimize_tree(&ctx->c, in, dtype);
+            if (n_candidates) *n_candidates = 30;
+        } else {
+            fail(DDJ_EUNSUPPORTED, "planner mode not available");
+        }
+        if (ratio) *ratio = t->ratio;
+        write_path(t, path_out, path_cap, path_len);
+    });
+}
+
+ddj_status ddj_optimizer_create(ddj_ctx* ctx, ddj_optimizer** out) {
+    return guarded(ctx, [&] {
+        if (!out) fail(DDJ_EINVAL, "null output");
+        auto* o = new ddj_optimizer;
+        o->ctx = ctx;
+        o->opt.reset(new Optimizer(&ctx->c));
+        *out = o;
+    });
+}
+
+void ddj_optimizer_destroy(ddj_optimizer* opt) { delete opt; }
+
+ddj_status ddj_optimizer_compress(ddj_optimizer* opt, int dtype, const void* d_in, size_t in_bytes, ddj_buf* blob) {
+    if (!opt) return DDJ_EINVAL;
+    return guarded(opt->ctx, [&] {
+        if (!blob || !dtype_ok(dtype)) fail(DDJ_EINVAL, "bad argument");
+        Part p = opt->opt->compress_data(DView{static_cast<const uint8_t*>(d_in), in_bytes}, dtype);
+        *blob = to_buf(&opt->ctx->c, p);
+    });
+}
+
+ddj_status ddj_optimizer_tree(ddj_optimizer* opt, int* path_out, size_t path_cap, size_t* path_len) {
+    if (!opt) return DDJ_EINVAL;
+    return guarded(opt->ctx, [&] {
+        if (!opt->opt->tree()) fail(DDJ_EINVAL, "no tree yet");
+        write_path(opt->opt->tree(), path_out, path_cap, path_len);
+    });
+}
+
+}  // extern "C"

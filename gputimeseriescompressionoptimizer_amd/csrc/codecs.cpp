@@ -1,0 +1,736 @@
+// Encode / Decode of every reference encoding.  Each function follows the
+// reference encoding's host code (file:line cited) for output arity, sizes,
+// empty-input behaviour and metadata layout; the per-row work runs in the
+// kernels of k_*.hip.
+#include "codecs.h"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <type_traits>
+
+#include "dispatch.h"
+#include "k_common.h"
+
+namespace ddj {
+
+Part own(DevBuf&& b) {
+    auto sp = std::make_shared<DevBuf>(std::move(b));
+    Part p;
+    p.owner = sp;
+    p.p = sp->bytes_ptr();
+    p.n = sp->size();
+    return p;
+}
+
+Part borrow(DView v, std::shared_ptr<DevBuf> keep) {
+    Part p;
+    p.owner = std::move(keep);
+    p.p = v.p;
+    p.n = v.n;
+    return p;
+}
+
+namespace {
+
+Part empty() { return Part{}; }
+
+Part upload(Ctx* c, const void* h, size_t n) { return own(make_buf(c, h, n)); }
+
+template <typename T> T read_dev(Ctx* c, const uint8_t* p) {
+    T v;
+    d2h(c, &v, p, sizeof(T));
+    return v;
+}
+
+// A copy at an aligned address when `v` is not aligned to `align`.
+Part aligned_view(Ctx* c, DView v, size_t align) {
+    if ((reinterpret_cast<uintptr_t>(v.p) % align) == 0) return borrow(v);
+    DevBuf b(c, v.n);
+    d2d(c, b.get(), v.p, v.n);
+    return own(std::move(b));
+}
+
+// Integer view used by delta / scale for floating types
+// (reference delta_encoding.hpp:57-64, scale_encoding.hpp:52-59).
+int int_view(int dtype) {
+    if (dtype == DDJ_D_FLOAT) return DDJ_D_INT;
+    if (dtype == DDJ_D_DOUBLE) return DDJ_D_TIME;
+    return dtype;
+}
+
+// n = floor(bytes / sizeof(T)): a CudaPtr<T> over a char buffer drops a
+// trailing partial element (reference core/cuda_ptr.hpp:60).
+size_t rows(int dtype, size_t bytes) { return bytes / dtype_size(dtype); }
+
+template <typename T> uint64_t tbits(T v) {
+    using U = std::make_unsigned_t<std::conditional_t<std::is_floating_point<T>::value,
+                                                      std::conditional_t<sizeof(T) == 8, int64_t, int32_t>, T>>;
+    U u;
+    memcpy(&u, &v, sizeof(T));
+    return (uint64_t)u;
+}
+
+struct MinMax {
+    DevBuf buf;
+    MinMaxResult host;
+};
+MinMax minmax(Ctx* c, int dtype, const void* x, size_t n) {
+    MinMax m{DevBuf(c, sizeof(MinMaxResult)), {}};
+    launch_minmax(c, dtype, x, n, m.buf.get<MinMaxResult>());
+    d2h(c, &m.host, m.buf.get(), sizeof(MinMaxResult));
+    return m;
+}
+
+// ------------------------------------------------------------------- none
+// NoneEncoding (reference compression/none/none_encoding.cu:13-49)
+std::vector<Part> none_encode(Ctx* c, DView in) {
+    size_t len = in.n;
+    DevBuf d(c, in.n);
+    d2d(c, d.get(), in.p, in.n);
+    return {upload(c, &len, sizeof len), own(std::move(d))};
+}
+
+// ------------------------------------------------------------------ delta
+// DeltaEncoding (compression/delta/delta_encoding.cu:29-102)
+std::vector<Part> delta_encode(Ctx* c, int dtype, DView in) {
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    if (n == 0) return {empty(), empty()};
+    Part x = aligned_view(c, in, es);
+    DevBuf meta(c, es), data(c, (n - 1) * es);
+    d2d(c, meta.get(), x.p, es);
+    launch_delta_encode(c, (int)es, x.p, n, data.get());
+    return {own(std::move(meta)), own(std::move(data))};
+}
+Part delta_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
+    const size_t es = dtype_size(dtype);
+    if (in[0].n == 0) return empty();
+    if (in[0].n < es) fail(DDJ_ECORRUPT, "delta metadata shorter than one value");
+    const size_t m = in[1].n / es;
+    DevBuf out(c, (m + 1) * es);
+    launch_delta_decode(c, (int)es, in[0].p, in[1].p, m, out.get());
+    return own(std::move(out));
+}
+
+// ------------------------------------------------------------------ scale
+// ScaleEncoding (compression/scale/scale_encoding.cu:30-108): m = min, but 0
+// when the overflow guard (:49-50) fires; its std::abs<T> leaves T's minimum
+// unchanged and T_MAX - max wraps for int/long.
+template <typename T> T scale_min(T mn, T mx) {
+    T m = mn;
+    if (mn < 0) {
+        long long tmax = (long long)std::numeric_limits<T>::max();
+        long long diff;
+        if (sizeof(T) >= 4)
+            diff = (long long)(T)((uint64_t)(std::make_unsigned_t<T>)tmax - (uint64_t)(std::make_unsigned_t<T>)mx);
+        else
+            diff = tmax - (long long)mx;
+        long long absmin = (mn == std::numeric_limits<T>::min()) ? (long long)mn : -(long long)mn;
+        if (diff < absmin) m = 0;
+    }
+    return m;
+}
+std::vector<Part> scale_encode(Ctx* c, int dtype, DView in) {
+    const int iv = int_view(dtype);
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    if (n == 0) return {empty(), empty()};
+    Part x = aligned_view(c, in, es);
+    MinMax mm = minmax(c, iv, x.p, n);
+    return dispatch_int((int)es, [&](auto tag) -> std::vector<Part> {
+        using T = decltype(tag);
+        T mn, mx;
+        decode_minmax(iv, mm.host, &mn, &mx);
+        T m = scale_min(mn, mx);
+        DevBuf data(c, n * es);
+        launch_sub_scalar(c, (int)es, x.p, n, tbits(m), data.get());
+        return {upload(c, &m, sizeof m), own(std::move(data))};
+    });
+}
+Part scale_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
+    const size_t es = dtype_size(dtype);
+    if (in[1].n == 0) return empty();
+    if (in[0].n < es) fail(DDJ_ECORRUPT, "scale metadata shorter than one value");
+    const size_t n = in[1].n / es;
+    uint64_t m = 0;
+    d2h(c, &m, in[0].p, es);
+    Part x = aligned_view(c, in[1], es);
+    DevBuf out(c, n * es);
+    launch_add_scalar(c, (int)es, x.p, n, m, out.get());
+    return own(std::move(out));
+}
+
+// -------------------------------------------------------------------- afl
+// AflEncoding::Encode<T> (compression/afl/afl_encoding.cu:15-56): metadata
+// {minBit, rest, 0, 0}; data = ceil(n*b/W) words of T.
+std::vector<Part> afl_encode_int(Ctx* c, int dtype, DView in) {
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    if (n == 0) return {empty(), empty()};
+    Part x = aligned_view(c, in, es);
+    DevBuf mmb(c, sizeof(MinMaxResult));
+    launch_minmax(c, dtype, x.p, n, mmb.get<MinMaxResult>());
+    const uint64_t W = 8 * es;
+    DevBuf data(c, (size_t)((W * n + W - 1) / W) * es);  // worst case b = W
+    launch_fl_pack(c, dtype, x.p, n, mmb.get<MinMaxResult>(), 0, data.get());
+    MinMaxResult h;
+    d2h(c, &h, mmb.get(), sizeof h);
+    int b = dispatch_int((int)es, [&](auto tag) {
+        using T = decltype(tag);
+        T mn, mx;
+        decode_minmax(dtype, h, &mn, &mx);
+        return bitlen_of<T>(mn, mx);
+    });
+    const uint64_t bytes = (b * (uint64_t)n + W - 1) / W * es;
+    data.set_size((size_t)bytes);
+    uint8_t meta[4] = {(uint8_t)b, (uint8_t)((bytes * 8 - (uint64_t)n * b) & 0xff), 0, 0};
+    return {upload(c, meta, 4), own(std::move(data))};
+}
+
+Part fl_decode_raw(Ctx* c, int dtype, const uint8_t* data, size_t bytes, int b, int rest) {
+    const int W = 8 * (int)dtype_size(dtype);
+    if (b <= 0 || b > W) fail(DDJ_ECORRUPT, "AFL bit width out of range");
+    long long bits = (long long)bytes * 8 - rest;
+    if (bits < 0) fail(DDJ_ECORRUPT, "AFL rest larger than the data");
+    const size_t n = (size_t)(bits / b);
+    DevBuf out(c, n * dtype_size(dtype));
+    launch_fl_unpack(c, dtype, data, bytes, n, b, out.get());
+    return own(std::move(out));
+}
+
+// DecodeAfl / AflEncoding::Decode<T> (afl_encoding.cu:74-113)
+Part afl_decode_int(Ctx* c, int dtype, const std::vector<DView>& in) {
+    if (in[1].n == 0) return empty();
+    if (in[0].n < 2) fail(DDJ_ECORRUPT, "AFL metadata shorter than 2 bytes");
+    int8_t m[2];
+    d2h(c, m, in[0].p, 2);
+    return fl_decode_raw(c, dtype, in[1].p, in[1].n, m[0], m[1]);
+}
+
+// AflEncoding::Encode<float> (afl_encoding.cu:115-167)
+std::vector<Part> afl_encode_float(Ctx* c, DView in) {
+    const size_t n = rows(DDJ_D_FLOAT, in.n);
+    if (n == 0) return {empty(), empty()};
+    Part x = aligned_view(c, in, 4);
+    MinMax mm = minmax(c, DDJ_D_FLOAT, x.p, n);
+    float mn, mx;
+    decode_minmax(DDJ_D_FLOAT, mm.host, &mn, &mx);
+    const char sign = (char)((mn >= 0 ? 1 : 0) + (mx < 0 ? 2 : 0));
+    DevBuf mant(c, 4 * n), expo(c, 4 * n), sgn(c, 4 * n);
+    launch_float_split(c, reinterpret_cast<const float*>(x.p), n, mant.get<int32_t>(), expo.get<int32_t>(),
+                       sgn.get<int32_t>());
+    auto a = afl_encode_int(c, DDJ_D_INT, DView{mant.bytes_ptr(), 4 * n});
+    auto b = afl_encode_int(c, DDJ_D_INT, DView{expo.bytes_ptr(), 4 * n});
+    size_t msize = a[1].n;
+    const size_t stencil_bytes = sign == 0 ? 1 + (n + 7) / 8 : 0;
+    DevBuf meta(c, sizeof(size_t) + 1 + stencil_bytes);
+    uint8_t head[sizeof(size_t) + 1];
+    memcpy(head, &msize, sizeof(size_t));
+    head[sizeof(size_t)] = (uint8_t)sign;
+    h2d(c, meta.get(), head, sizeof head);
+    if (sign == 0) {
+        PredArgs p{};
+        p.kind = PRED_NOT_EQUAL;
+        p.a_bits = 0;
+        DevBuf cnt(c, 8);
+        launch_split(c, DDJ_D_INT, sgn.get(), n, p, meta.bytes_ptr() + sizeof(size_t) + 1, nullptr, nullptr, nullptr,
+                     cnt.get<uint64_t>());
+    }
+    DevBuf data(c, a[0].n + a[1].n + b[0].n + b[1].n);
+    size_t off = 0;
+    for (const Part* q : {&a[0], &a[1], &b[0], &b[1]}) {
+        d2d(c, data.bytes_ptr() + off, q->p, q->n);
+        off += q->n;
+    }
+    return {own(std::move(meta)), own(std::move(data))};
+}
+
+// AflEncoding::Decode<float> (afl_encoding.cu:185-254)
+Part afl_decode_float(Ctx* c, const std::vector<DView>& in) {
+    if (in[1].n == 0) return empty();
+    if (in[0].n < sizeof(size_t) + 1 || in[1].n < 8) fail(DDJ_ECORRUPT, "AFL float metadata too short");
+    uint8_t head[sizeof(size_t) + 1];
+    d2h(c, head, in[0].p, sizeof head);
+    size_t msize;
+    memcpy(&msize, head, sizeof(size_t));
+    const int sign = (int8_t)head[sizeof(size_t)];
+    if (msize + 8 > in[1].n) fail(DDJ_ECORRUPT, "AFL float mantissa size out of range");
+    int8_t ma[2], ea[2];
+    d2h(c, ma, in[1].p, 2);
+    const uint8_t* eb = in[1].p + 4 + msize;
+    d2h(c, ea, eb, 2);
+    Part mant = fl_decode_raw(c, DDJ_D_INT, in[1].p + 4, msize, ma[0], ma[1]);
+    Part expo = fl_decode_raw(c, DDJ_D_INT, eb + 4, in[1].n - msize - 8, ea[0], ea[1]);
+    const size_t n = mant.n / 4;
+    if (expo.n / 4 < n) fail(DDJ_ECORRUPT, "AFL float exponent shorter than mantissa");
+    const uint8_t* bits = nullptr;
+    if (sign == 0) {
+        if (in[0].n < sizeof(size_t) + 2 + (n + 7) / 8) fail(DDJ_ECORRUPT, "AFL float sign stencil too short");
+        bits = in[0].p + sizeof(size_t) + 2;
+    }
+    DevBuf out(c, 4 * n);
+    launch_float_compose(c, reinterpret_cast<const int32_t*>(mant.p), reinterpret_cast<const int32_t*>(expo.p), bits,
+                         sign - 1, n, out.get<float>());
+    return own(std::move(out));
+}
+
+// -------------------------------------------------------------------- rle
+// RleEncoding (compression/rle/rle_encode.cu:18-62): metadata = int32 run
+// count (also for empty input), lengths int32[R], values T[R].
+std::vector<Part> rle_encode(Ctx* c, int dtype, DView in) {
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    int32_t R = 0;
+    if (n == 0) return {upload(c, &R, 4), empty(), empty()};
+    Part x = aligned_view(c, in, es);
+    DevBuf lens(c, 4 * n), vals(c, es * n), runs(c, 8);
+    launch_rle_encode(c, dtype, x.p, n, lens.get<int32_t>(), vals.get(), runs.get<uint64_t>());
+    uint64_t r = 0;
+    d2h(c, &r, runs.get(), 8);
+    R = (int32_t)r;
+    lens.set_size(4 * r);
+    vals.set_size(es * r);
+    return {upload(c, &R, 4), own(std::move(lens)), own(std::move(vals))};
+}
+// rle_decode.cu:19-73
+Part rle_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
+    const size_t es = dtype_size(dtype);
+    if (in[1].n == 0) return empty();
+    if (in[0].n < 4) fail(DDJ_ECORRUPT, "RLE metadata shorter than 4 bytes");
+    const int32_t R = read_dev<int32_t>(c, in[0].p);
+    if (R < 0 || (size_t)R * 4 > in[1].n || (size_t)R * es > in[2].n) fail(DDJ_ECORRUPT, "RLE run count out of range");
+    if (R == 0) return empty();
+    return own(rle_decode_dev(c, dtype, in[1].p, in[2].p, (size_t)R));
+}
+
+// ------------------------------------------------------- stencil helpers
+size_t stencil_rows(Ctx* c, DView st) {
+    // Stencil::unpack (util/stencil/stencil.cu:69-96)
+    if (st.n < 1) fail(DDJ_ECORRUPT, "empty stencil");
+    const int8_t rest = read_dev<int8_t>(c, st.p);
+    long long n = (long long)(st.n - 1) * 8;
+    if (rest) n -= 8 - rest;
+    if (n < 0) fail(DDJ_ECORRUPT, "stencil length out of range");
+    return (size_t)n;
+}
+
+// ------------------------------------------------------------------ patch
+// PatchEncodingFactory::Get(data) + LowerPatchEncoding / OutsidePatchEncoding
+// (compression/patch/patch_encoding_factory.hpp:40-57, lower_patch_encoding.cu:22-66,
+// outside_patch_encoding.cu:22-60)
+std::vector<Part> patch_encode(Ctx* c, int dtype, DView in, bool outside) {
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    if (n == 0) return {empty(), empty(), empty()};
+    Part x = aligned_view(c, in, es);
+    MinMax mm = minmax(c, dtype, x.p, n);
+    PredArgs p{};
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        T mnT, mxT;
+        decode_minmax(dtype, mm.host, &mnT, &mxT);
+        volatile double mn = (double)mnT, mx = (double)mxT, factor = 0.1;
+        const T dist = (T)(mx - mn);
+        volatile double prod = factor * (double)dist;
+        if (!outside) {
+            p.kind = PRED_LOWER;
+            p.a_bits = tbits((T)(mx - prod));
+        } else {
+            p.kind = PRED_INSIDE;
+            p.a_bits = tbits((T)(mn + prod));
+            p.b_bits = tbits((T)(mx - prod));
+        }
+    });
+    DevBuf st(c, 1 + (n + 7) / 8), yes(c, n * es), no(c, n * es), cnt(c, 8);
+    launch_split(c, dtype, x.p, n, p, st.bytes_ptr(), yes.get(), no.get(), nullptr, cnt.get<uint64_t>());
+    uint64_t m = 0;
+    d2h(c, &m, cnt.get(), 8);
+    yes.set_size(m * es);
+    no.set_size((n - m) * es);
+    return {own(std::move(st)), own(std::move(yes)), own(std::move(no))};
+}
+Part patch_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
+    const size_t es = dtype_size(dtype);
+    if (in[1].n == 0 && in[2].n == 0) return empty();
+    const size_t n = stencil_rows(c, in[0]);
+    Part y = aligned_view(c, in[1], es), z = aligned_view(c, in[2], es);
+    if (y.n / es + z.n / es < n) fail(DDJ_ECORRUPT, "patch outputs shorter than the stencil");
+    DevBuf out(c, n * es);
+    launch_merge(c, dtype, in[0].p + 1, n, y.p, z.p ? z.p : y.p, 0, out.get());
+    return own(std::move(out));
+}
+
+// ------------------------------------------------------------------ const
+// ConstEncoding (compression/const/const_encoding.cu:43-131)
+std::vector<Part> const_encode(Ctx* c, int dtype, DView in) {
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    if (n == 0) return {empty(), empty()};
+    Part x = aligned_view(c, in, es);
+    uint64_t key[4];
+    topk_most_frequent(c, dtype, x.p, n, 1, key, nullptr);
+    DevBuf meta(c, es + 1 + (n + 7) / 8), data(c, n * es), cnt(c, 8);
+    h2d(c, meta.get(), key, es);
+    PredArgs p{};
+    p.kind = PRED_NOT_EQUAL;
+    p.a_bits = key[0];
+    launch_split(c, dtype, x.p, n, p, meta.bytes_ptr() + es, data.get(), nullptr, nullptr, cnt.get<uint64_t>());
+    uint64_t m = 0;
+    d2h(c, &m, cnt.get(), 8);
+    data.set_size(m * es);
+    return {own(std::move(meta)), own(std::move(data))};
+}
+Part const_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
+    const size_t es = dtype_size(dtype);
+    if (in[0].n == 0) return empty();
+    if (in[0].n < es + 1) fail(DDJ_ECORRUPT, "const metadata too short");
+    uint64_t cv = 0;
+    d2h(c, &cv, in[0].p, es);
+    const size_t n = stencil_rows(c, DView{in[0].p + es, in[0].n - es});
+    Part y = aligned_view(c, in[1], es);
+    DevBuf out(c, n * es);
+    launch_merge(c, dtype, in[0].p + es + 1, n, y.p, nullptr, cv, out.get());
+    return own(std::move(out));
+}
+
+// ------------------------------------------------------------------- dict
+// DictEncoding (compression/dict/dict_encoding.cu:66-130) with K = 4
+// (dict_encoding.hpp:30) and UniqueEncoding::CompressUnique
+// (unique_encoding.cu:88-131): out1 = [size_t K][size_t m][K x T][codes].
+constexpr int kDictK = 4;
+std::vector<Part> dict_encode(Ctx* c, int dtype, DView in) {
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    if (n == 0) return {empty(), empty(), empty()};
+    Part x = aligned_view(c, in, es);
+    uint64_t keys[4];
+    topk_most_frequent(c, dtype, x.p, n, kDictK, keys, nullptr);
+    PredArgs p{};
+    p.kind = PRED_IN_SET;
+    p.set_n = kDictK;
+    for (int i = 0; i < kDictK; i++) p.set_bits[i] = keys[i];
+    DevBuf st(c, 1 + (n + 7) / 8), codes(c, n), others(c, n * es), cnt(c, 8);
+    launch_split(c, dtype, x.p, n, p, st.bytes_ptr(), nullptr, others.get(), codes.get<uint8_t>(),
+                 cnt.get<uint64_t>());
+    uint64_t m = 0;
+    d2h(c, &m, cnt.get(), 8);
+    others.set_size((n - m) * es);
+    Part freq = empty();
+    if (m > 0) {
+        const int bits = 2;  // ALT_BITLEN(K - 1)
+        const size_t words = (m + 15) / 16;
+        const size_t head = 16 + kDictK * es;
+        DevBuf f(c, head + 4 * words);
+        std::vector<uint8_t> h(head);
+        size_t sizes[2] = {(size_t)kDictK, (size_t)m};
+        memcpy(h.data(), sizes, 16);
+        for (int i = 0; i < kDictK; i++) memcpy(h.data() + 16 + i * es, &keys[i], es);
+        h2d(c, f.get(), h.data(), head);
+        launch_pack_codes(c, codes.get<uint8_t>(), m, bits, reinterpret_cast<uint32_t*>(f.bytes_ptr() + head));
+        freq = own(std::move(f));
+    }
+    return {own(std::move(st)), freq, own(std::move(others))};
+}
+// DecompressUnique (unique_encoding.cu:215-259) then Splitter::Merge
+Part dict_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
+    const size_t es = dtype_size(dtype);
+    if (in[1].n == 0 && in[2].n == 0) return empty();
+    const size_t n = stencil_rows(c, in[0]);
+    Part freq = empty();
+    if (in[1].n > 0) {
+        if (in[1].n < 16) fail(DDJ_ECORRUPT, "dict header too short");
+        size_t sizes[2];
+        d2h(c, sizes, in[1].p, 16);
+        const size_t K = sizes[0], m = sizes[1];
+        if (K == 0 || K > (1u << 30) || 16 + K * es > in[1].n) fail(DDJ_ECORRUPT, "dict key count out of range");
+        int bits = 1;
+        while ((1ull << bits) < K) bits++;  // ALT_BITLEN(K - 1) for K >= 2; 1 for K == 1
+        if (K == 1) bits = 1;
+        const int P = 32 / bits;
+        const size_t off = 16 + K * es;
+        if ((m + P - 1) / P * 4 > in[1].n - off) fail(DDJ_ECORRUPT, "dict codes shorter than their count");
+        DevBuf f(c, m * es);
+        launch_unpack_codes(c, dtype, in[1].p + off, m, bits, in[1].p + 16, f.get());
+        freq = own(std::move(f));
+    }
+    Part z = aligned_view(c, in[2], es);
+    if (freq.n / es + z.n / es < n) fail(DDJ_ECORRUPT, "dict outputs shorter than the stencil");
+    DevBuf out(c, n * es);
+    launch_merge(c, dtype, in[0].p + 1, n, freq.p ? freq.p : z.p, z.p ? z.p : freq.p, 0, out.get());
+    return own(std::move(out));
+}
+
+// ------------------------------------------------------------- floatToInt
+// FloatEncoding (compression/float/float_encoding.cu:17-88)
+std::vector<Part> float_encode(Ctx* c, int dtype, DView in) {
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    if (n == 0) return {empty(), empty()};
+    Part x = aligned_view(c, in, es);
+    int precision = 0;
+    MinMaxResult mmr;
+    if (dtype == DDJ_D_FLOAT || dtype == DDJ_D_DOUBLE) {
+        DevBuf s(c, sizeof(DevStats));
+        launch_stats(c, dtype, x.p, n, s.get<DevStats>());
+        DevStats h;
+        d2h(c, &h, s.get(), sizeof h);
+        precision = (int)h.precision;
+        mmr.min_key = h.min_key;
+        mmr.max_key = h.max_key;
+    } else {
+        mmr = minmax(c, dtype, x.p, n).host;  // float(integer) always has precision 0
+    }
+    bool transform = false;
+    if (precision < 10) {
+        static const double pw[11] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8, 1e9, 1e10};
+        const int scale = (int)pw[precision];
+        dispatch_native(dtype, [&](auto tag) {
+            using T = decltype(tag);
+            T mn, mx;
+            decode_minmax(dtype, mmr, &mn, &mx);
+            bool ok;
+            if constexpr (std::is_floating_point<T>::value) {
+                volatile T lo = mn * (T)scale, hi = mx * (T)scale;
+                ok = lo > (T)std::numeric_limits<int32_t>::min() && hi < (T)std::numeric_limits<int32_t>::max();
+            } else {
+                long long lo = (long long)mn * scale, hi = (long long)mx * scale;
+                if (sizeof(T) == 4) {  // int * int wraps in the reference
+                    lo = (int32_t)(uint32_t)(uint64_t)lo;
+                    hi = (int32_t)(uint32_t)(uint64_t)hi;
+                }
+                ok = lo > std::numeric_limits<int32_t>::min() && hi < std::numeric_limits<int32_t>::max();
+            }
+            transform = ok;
+        });
+        if (!transform) precision = 10;
+    }
+    Part data;
+    if (transform) {
+        DevBuf y(c, 4 * n);
+        launch_float_to_int(c, dtype, x.p, n, precision, y.get<int32_t>());
+        data = own(std::move(y));
+    } else {
+        DevBuf y(c, n * es);
+        d2d(c, y.get(), x.p, n * es);
+        data = own(std::move(y));
+    }
+    int32_t p32 = precision;
+    return {upload(c, &p32, 4), data};
+}
+Part float_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
+    const size_t es = dtype_size(dtype);
+    if (in[1].n == 0) return empty();
+    if (in[0].n < 4) fail(DDJ_ECORRUPT, "floatToInt metadata shorter than 4 bytes");
+    const int32_t p = read_dev<int32_t>(c, in[0].p);
+    if (p < 10) {
+        if (p < 0) fail(DDJ_ECORRUPT, "negative precision");
+        const size_t m = in[1].n / 4;
+        Part y = aligned_view(c, in[1], 4);
+        DevBuf out(c, m * es);
+        launch_int_to_float(c, dtype, y.p, m, p, out.get());
+        return own(std::move(out));
+    }
+    DevBuf out(c, in[1].n);
+    d2d(c, out.get(), in[1].p, in[1].n);
+    return own(std::move(out));
+}
+
+// -------------------------------------------------------------------- gfc
+// GfcEncoding (compression/gfc/gfc_encoding.cpp:16-166): float/double; the
+// integer types go through AFL plus an empty third output.
+std::vector<Part> gfc_encode_any(Ctx* c, int dtype, DView in) {
+    if (dtype != DDJ_D_FLOAT && dtype != DDJ_D_DOUBLE) {
+        auto r = afl_encode_int(c, dtype, in);
+        r.push_back(empty());
+        return r;
+    }
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    if (n == 0) return {empty(), empty(), empty()};
+    if (n > (size_t)std::numeric_limits<int32_t>::max() / 9) fail(DDJ_EUNSUPPORTED, "GFC offsets are int32");
+    Part x = aligned_view(c, in, es);
+    DevBuf meta, offs, pay;
+    gfc_encode(c, dtype == DDJ_D_DOUBLE, x.p, n, meta, offs, pay);
+    return {own(std::move(meta)), own(std::move(offs)), own(std::move(pay))};
+}
+Part gfc_decode_any(Ctx* c, int dtype, const std::vector<DView>& in) {
+    if (dtype != DDJ_D_FLOAT && dtype != DDJ_D_DOUBLE) return afl_decode_int(c, dtype, in);
+    if (in[2].n == 0) return empty();
+    DevBuf out;
+    gfc_decode(c, dtype == DDJ_D_DOUBLE, in[0].p, in[0].n, in[1].p, in[1].n, in[2].p, in[2].n, out);
+    return own(std::move(out));
+}
+
+}  // namespace
+
+bool enc_ok(int enc) {
+    return (enc >= DDJ_ENC_DELTA && enc <= DDJ_ENC_GFC) || enc == DDJ_ENC_OUTSIDE_PATCH;
+}
+
+int n_results(int enc) {
+    switch (enc) {
+    case DDJ_ENC_NONE: return 0;
+    case DDJ_ENC_DELTA:
+    case DDJ_ENC_SCALE:
+    case DDJ_ENC_AFL:
+    case DDJ_ENC_UNIQUE:
+    case DDJ_ENC_CONST:
+    case DDJ_ENC_FLOAT_TO_INT: return 1;
+    case DDJ_ENC_RLE:
+    case DDJ_ENC_DICT:
+    case DDJ_ENC_PATCH:
+    case DDJ_ENC_OUTSIDE_PATCH:
+    case DDJ_ENC_GFC: return 2;
+    default: fail(DDJ_EINVAL, "unknown encoding " + std::to_string(enc));
+    }
+}
+
+void return_types(int enc, int dtype, int out[2]) {
+    out[0] = dtype;
+    out[1] = dtype;
+    if (enc == DDJ_ENC_RLE || enc == DDJ_ENC_FLOAT_TO_INT) out[0] = DDJ_D_INT;  // rle_encoding.hpp:35, float_encoding.hpp:31
+}
+
+std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in) {
+    if (!dtype_ok(dtype)) fail(DDJ_EINVAL, "unknown data type");
+    switch (enc) {
+    case DDJ_ENC_NONE: return none_encode(c, in);
+    case DDJ_ENC_DELTA: return delta_encode(c, int_view(dtype), in);
+    case DDJ_ENC_SCALE: return scale_encode(c, dtype, in);
+    case DDJ_ENC_AFL:
+        if (dtype == DDJ_D_DOUBLE) fail(DDJ_EUNSUPPORTED, "AFL on double returns no outputs (afl_encoding.cu:268-271)");
+        if (dtype == DDJ_D_FLOAT) return afl_encode_float(c, in);
+        return afl_encode_int(c, dtype, in);
+    case DDJ_ENC_RLE: return rle_encode(c, dtype, in);
+    case DDJ_ENC_DICT: return dict_encode(c, dtype, in);
+    case DDJ_ENC_CONST: return const_encode(c, dtype, in);
+    case DDJ_ENC_PATCH: return patch_encode(c, dtype, in, false);
+    case DDJ_ENC_OUTSIDE_PATCH: return patch_encode(c, dtype, in, true);
+    case DDJ_ENC_FLOAT_TO_INT: return float_encode(c, dtype, in);
+    case DDJ_ENC_GFC: return gfc_encode_any(c, dtype, in);
+    case DDJ_ENC_UNIQUE: fail(DDJ_EUNSUPPORTED, "standalone unique encoding is not on the planner's path");
+    default: fail(DDJ_EINVAL, "unknown encoding " + std::to_string(enc));
+    }
+}
+
+Part decode(Ctx* c, int enc, int dtype, const std::vector<DView>& in) {
+    if (!dtype_ok(dtype)) fail(DDJ_EINVAL, "unknown data type");
+    const size_t need = 1 + (size_t)n_results(enc);
+    if (in.size() < std::max<size_t>(need, 2)) fail(DDJ_EINVAL, "too few inputs for decode");
+    switch (enc) {
+    case DDJ_ENC_NONE: {
+        DevBuf out(c, in[1].n);
+        d2d(c, out.get(), in[1].p, in[1].n);
+        return own(std::move(out));
+    }
+    case DDJ_ENC_DELTA: return delta_decode(c, int_view(dtype), in);
+    case DDJ_ENC_SCALE: return scale_decode(c, int_view(dtype), in);
+    case DDJ_ENC_AFL:
+        if (dtype == DDJ_D_DOUBLE) fail(DDJ_EUNSUPPORTED, "AFL on double");
+        if (dtype == DDJ_D_FLOAT) return afl_decode_float(c, in);
+        return afl_decode_int(c, dtype, in);
+    case DDJ_ENC_RLE: return rle_decode(c, dtype, in);
+    case DDJ_ENC_DICT: return dict_decode(c, dtype, in);
+    case DDJ_ENC_CONST: return const_decode(c, dtype, in);
+    case DDJ_ENC_PATCH:
+    case DDJ_ENC_OUTSIDE_PATCH: return patch_decode(c, dtype, in);
+    case DDJ_ENC_FLOAT_TO_INT: return float_decode(c, dtype, in);
+    case DDJ_ENC_GFC: return gfc_decode_any(c, dtype, in);
+    case DDJ_ENC_UNIQUE: fail(DDJ_EUNSUPPORTED, "standalone unique encoding is not on the planner's path");
+    default: fail(DDJ_EINVAL, "unknown encoding " + std::to_string(enc));
+    }
+}
+
+// GetMetadataSize / GetCompressedSize of each encoding (its .hpp / .cu)
+void predict_sizes(Ctx* c, int enc, int dtype, DView in, size_t* meta, size_t* data) {
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    auto sum_outs = [&](const std::vector<Part>& o, size_t from) {
+        size_t s = 0;
+        for (size_t i = from; i < o.size(); i++) s += o[i].n;
+        return s;
+    };
+    switch (enc) {
+    case DDJ_ENC_NONE: *meta = sizeof(size_t); *data = in.n; return;
+    case DDJ_ENC_DELTA: *meta = n ? es : 0; *data = n ? in.n - es : 0; return;
+    case DDJ_ENC_SCALE: *meta = n ? es : 0; *data = n ? in.n : 0; return;
+    case DDJ_ENC_FLOAT_TO_INT: *meta = n ? 4 : 0; *data = in.n; return;
+    case DDJ_ENC_GFC: *meta = 12; *data = 0; return;
+    case DDJ_ENC_PATCH:
+    case DDJ_ENC_OUTSIDE_PATCH: *meta = n ? (n + 7) / 8 + 1 : 0; *data = in.n; return;
+    case DDJ_ENC_RLE:
+        *meta = n ? 4 : 0;
+        if (!n) { *data = 0; return; }
+        if (dtype != DDJ_D_INT && dtype != DDJ_D_FLOAT) fail(DDJ_EUNSUPPORTED, "RLE size predictor: int/float only");
+        *data = sum_outs(encode(c, enc, dtype, in), 1);
+        return;
+    case DDJ_ENC_CONST:
+        *meta = n ? (n + 7) / 8 + es + 1 : 0;
+        if (!n) { *data = 0; return; }
+        if (dtype != DDJ_D_INT && dtype != DDJ_D_FLOAT) fail(DDJ_EUNSUPPORTED, "const size predictor: int/float only");
+        *data = sum_outs(encode(c, enc, dtype, in), 1);
+        return;
+    case DDJ_ENC_DICT: {
+        *meta = (n + 7) / 8 + 1;
+        if (!n) { *data = 0; return; }
+        if (dtype != DDJ_D_INT && dtype != DDJ_D_FLOAT) fail(DDJ_EUNSUPPORTED, "dict size predictor: int/float only");
+        auto o = encode(c, enc, dtype, in);
+        size_t m = 0;
+        if (o[1].n >= 16) d2h(c, &m, o[1].p + 8, 8);
+        *data = 16 + 4 * es + (m + 15) / 16 * 4 + o[2].n;  // dict_encoding.cu:156-172
+        return;
+    }
+    case DDJ_ENC_AFL: {
+        if (!n) { *meta = 0; *data = 0; return; }
+        if (dtype != DDJ_D_INT && dtype != DDJ_D_FLOAT) fail(DDJ_EUNSUPPORTED, "AFL size predictor: int/float only");
+        auto o = encode(c, enc, dtype, in);
+        *meta = dtype == DDJ_D_INT ? 4 : sizeof(size_t) + 1;
+        *data = o[1].n;
+        return;
+    }
+    case DDJ_ENC_UNIQUE: fail(DDJ_EUNSUPPORTED, "standalone unique encoding is not on the planner's path");
+    default: fail(DDJ_EINVAL, "unknown encoding");
+    }
+}
+
+ddj_stats_t stats(Ctx* c, int dtype, DView in) {
+    ddj_stats_t s;
+    memset(&s, 0, sizeof s);
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    s.size = n;
+    if (n == 0) return s;
+    Part x = aligned_view(c, in, es);
+    DevBuf d(c, sizeof(DevStats));
+    launch_stats(c, dtype, x.p, n, d.get<DevStats>());
+    DevStats h;
+    d2h(c, &h, d.get(), sizeof h);
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        MinMaxResult r{h.min_key, h.max_key};
+        T mn, mx;
+        decode_minmax(dtype, r, &mn, &mx);
+        s.min = (double)mn;
+        s.max = (double)mx;
+        if constexpr (std::is_floating_point<T>::value) {
+            s.mean = (double)(T)(h.sum_f / (double)n);
+        } else {
+            using U = std::make_unsigned_t<T>;
+            T sum = (T)(U)h.sum_bits;
+            s.mean = (double)(T)(sum / (double)n);
+        }
+    });
+    s.minBitCnt = 64;  // BITLEN(double, double) (core/cuda_macros.cuh:187-189)
+    s.precision = (int32_t)h.precision;
+    s.sorted = h.unsorted ? 0 : 1;
+    if (n <= 2)
+        s.rlMetric = 1.0f;
+    else
+        s.rlMetric = (float)((double)((n - 2) + h.rl_eq) / (double)(n - 2));
+    s.runs = h.runs;
+    for (int i = 0; i < 65; i++) s.bit_hist[i] = h.bit_hist[i];
+    s.delta_bits_max = h.delta_bits_max;
+    // 64-register HyperLogLog estimate with the small-range correction
+    double sum = 0;
+    int zeros = 0;
+    for (int i = 0; i < 64; i++) {
+        sum += std::ldexp(1.0, -(int)h.hll[i]);
+        zeros += h.hll[i] == 0;
+    }
+    double est = 0.709 * 64.0 * 64.0 / sum;
+    if (est <= 2.5 * 64 && zeros > 0) est = 64.0 * std::log(64.0 / zeros);
+    s.distinct_est = (uint64_t)std::llround(std::min(est, (double)n));
+    return s;
+}
+
+}  // namespace ddj

@@ -1,0 +1,37 @@
+// Encodings: Encode / Decode / size predictors of every reference encoding,
+// orchestrated on the host, computed by the HIP kernels.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "ddj_internal.h"
+#include "kernels.h"
+
+namespace ddj {
+
+// A device byte range that keeps its storage alive.
+struct Part {
+    std::shared_ptr<DevBuf> owner;  // may be null for caller-owned memory
+    const uint8_t* p = nullptr;
+    size_t n = 0;
+    DView view() const { return DView{p, n}; }
+};
+Part own(DevBuf&& b);
+Part borrow(DView v, std::shared_ptr<DevBuf> keep = nullptr);
+
+int n_results(int enc);
+// GetReturnTypes (each encoding's .hpp)
+void return_types(int enc, int dtype, int out[2]);
+bool enc_ok(int enc);
+
+// Encoding::Encode: out[0] = metadata, out[1..n_results(enc)] = outputs.
+std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in);
+// Encoding::Decode: in[0] = metadata, in[1..] = children outputs.
+Part decode(Ctx* c, int enc, int dtype, const std::vector<DView>& in);
+// GetMetadataSize / GetCompressedSize.
+void predict_sizes(Ctx* c, int enc, int dtype, DView in, size_t* meta, size_t* data);
+// CudaArrayStatistics::GenerateStatistics plus planner features.
+ddj_stats_t stats(Ctx* c, int dtype, DView in);
+
+}  // namespace ddj

@@ -1,0 +1,129 @@
+// Internal declarations of libddj_hip.so: context, allocator, error
+// plumbing and the host-side launchers of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ddj.h"
+
+namespace ddj {
+
+// Thrown inside the library, turned into a ddj_status at the C boundary.
+struct Error {
+    ddj_status status;
+    std::string msg;
+};
+
+[[noreturn]] void fail(ddj_status st, const std::string& msg);
+void hip_check(hipError_t e, const char* what, const char* file, int line);
+#define HIPC(x) ::ddj::hip_check((x), #x, __FILE__, __LINE__)
+
+size_t dtype_size(int dtype);
+inline bool dtype_ok(int dtype) { return dtype >= DDJ_D_INT && dtype <= DDJ_D_CHAR; }
+
+// Stream-ordered caching allocator: every kernel of a context runs on one
+// stream, so a block freed by the host can be handed to the next request
+// without synchronising.  Requests round up to a size class; blocks go back
+// to a per-class free list.
+class Allocator {
+public:
+    explicit Allocator(int device) : device_(device) {}
+    ~Allocator();
+    void* alloc(size_t bytes);
+    void free(void* p);
+    void trim();
+    size_t in_use() const { return in_use_; }
+
+private:
+    static size_t round_size(size_t bytes);
+    int device_;
+    std::map<size_t, std::vector<void*>> free_;
+    std::map<void*, size_t> live_;
+    size_t in_use_ = 0;
+};
+
+// Device-side control block: counters that must start at zero for the
+// "last workgroup finishes the reduction" pattern and for look-back scans.
+struct Control {
+    uint32_t counters[64];
+    uint32_t error_flag;
+    uint32_t pad[63];
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<Allocator> alloc;
+    Control* control = nullptr;  // device
+    int cu_count = 256;
+    std::string last_error;
+    std::mutex mu;
+};
+
+// RAII device buffer from the context allocator.
+class DevBuf {
+public:
+    DevBuf() = default;
+    DevBuf(Ctx* c, size_t bytes) : ctx_(c), bytes_(bytes) { p_ = bytes ? c->alloc->alloc(bytes) : nullptr; }
+    ~DevBuf() { reset(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept { *this = std::move(o); }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            ctx_ = o.ctx_;
+            p_ = o.p_;
+            bytes_ = o.bytes_;
+            o.p_ = nullptr;
+            o.bytes_ = 0;
+        }
+        return *this;
+    }
+    void reset() {
+        if (p_ && ctx_) ctx_->alloc->free(p_);
+        p_ = nullptr;
+        bytes_ = 0;
+    }
+    template <typename T = void> T* get() const { return static_cast<T*>(p_); }
+    uint8_t* bytes_ptr() const { return static_cast<uint8_t*>(p_); }
+    size_t size() const { return bytes_; }
+    void set_size(size_t b) { bytes_ = b; }  // logical size <= allocated size
+    // Hand the block to a ddj_buf (the caller becomes the owner).
+    ddj_buf release() {
+        ddj_buf b{p_, bytes_};
+        p_ = nullptr;
+        bytes_ = 0;
+        return b;
+    }
+
+private:
+    Ctx* ctx_ = nullptr;
+    void* p_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// A read-only view of device bytes (input of an encoder or decoder).
+struct DView {
+    const uint8_t* p = nullptr;
+    size_t n = 0;
+    template <typename T> const T* as() const { return reinterpret_cast<const T*>(p); }
+};
+
+inline DView view(const DevBuf& b) { return DView{b.bytes_ptr(), b.size()}; }
+
+void d2h(Ctx* c, void* dst, const void* src, size_t bytes);  // synchronises the stream
+void h2d(Ctx* c, void* dst, const void* src, size_t bytes);  // stream-ordered
+void d2d(Ctx* c, void* dst, const void* src, size_t bytes);  // stream-ordered
+DevBuf make_buf(Ctx* c, const void* host, size_t bytes);     // stream-ordered upload
+
+}  // namespace ddj

@@ -1,0 +1,214 @@
+// Device helpers shared by the HIP kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+namespace ddj {
+
+constexpr int kWave = 64;
+
+// Native 16-byte vector (works with the nontemporal load/store builtins).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kBlock = 256;  // 4 waves per workgroup
+
+// ------------------------------------------------------------ type helpers
+template <typename T> struct UnsignedOf;
+template <> struct UnsignedOf<int8_t> { using type = uint8_t; };
+template <> struct UnsignedOf<int16_t> { using type = uint16_t; };
+template <> struct UnsignedOf<int32_t> { using type = uint32_t; };
+template <> struct UnsignedOf<int64_t> { using type = uint64_t; };
+template <> struct UnsignedOf<uint32_t> { using type = uint32_t; };
+template <> struct UnsignedOf<uint64_t> { using type = uint64_t; };
+template <> struct UnsignedOf<float> { using type = uint32_t; };
+template <> struct UnsignedOf<double> { using type = uint64_t; };
+
+template <typename T> __device__ __forceinline__ uint64_t ubits(T v) {
+    using U = typename UnsignedOf<T>::type;
+    U u;
+    __builtin_memcpy(&u, &v, sizeof(T));
+    return (uint64_t)u;
+}
+
+// Order-preserving map of a value to an unsigned key (min/max by atomics or
+// integer compares).  Floats: flip the sign bit for positives, all bits for
+// negatives.
+template <typename T> __device__ __forceinline__ uint64_t order_key(T v) {
+    if constexpr (std::is_floating_point<T>::value) {
+        uint64_t b = ubits(v);
+        constexpr int W = 8 * sizeof(T);
+        const uint64_t sign = 1ull << (W - 1);
+        const uint64_t mask = (W == 64) ? ~0ull : ((1ull << W) - 1);
+        return (b & sign) ? (~b & mask) : (b | sign);
+    } else {
+        constexpr int W = 8 * sizeof(T);
+        if constexpr (W == 64)
+            return (uint64_t)(int64_t)v ^ (1ull << 63);
+        else
+            return (uint64_t)((int64_t)v + (1ll << (W - 1)));
+    }
+}
+template <typename T> __device__ __host__ __forceinline__ T from_order_key(uint64_t k) {
+    if constexpr (std::is_floating_point<T>::value) {
+        constexpr int W = 8 * sizeof(T);
+        const uint64_t sign = 1ull << (W - 1);
+        const uint64_t mask = (W == 64) ? ~0ull : ((1ull << W) - 1);
+        uint64_t b = (k & sign) ? (k & ~sign) : (~k & mask);
+        T v;
+        using U = typename UnsignedOf<T>::type;
+        U u = (U)b;
+        __builtin_memcpy(&v, &u, sizeof(T));
+        return v;
+    } else {
+        constexpr int W = 8 * sizeof(T);
+        if constexpr (W == 64)
+            return (T)(int64_t)(k ^ (1ull << 63));
+        else
+            return (T)((int64_t)k - (1ll << (W - 1)));
+    }
+}
+
+// ---------------------------------------------------------- wave reductions
+template <typename T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+template <typename T> __device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, kWave);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+template <typename T> __device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, kWave);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_or(uint64_t v) {
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+template <typename T> __device__ __forceinline__ T wave_inclusive_scan(T v) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        T w = __shfl_up(v, o, kWave);
+        if (lane >= o) v += w;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread (kBlock threads).
+// Returns the exclusive prefix; *total receives the block sum.
+template <typename T> __device__ __forceinline__ T block_exclusive_scan(T v, T* total, T* smem /* kBlock/kWave */) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    T inc = wave_inclusive_scan(v);
+    if (lane == kWave - 1) smem[wid] = inc;
+    __syncthreads();
+    T wave_off = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; w++) {
+        T s = smem[w];
+        if (w < wid) wave_off += s;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return wave_off + inc - v;
+}
+
+// ------------------------------------------------ decoupled look-back scan
+// Tiles take tickets in start order, publish their aggregate, and sum the
+// predecessors' aggregates until one with an inclusive prefix is found.
+// Every wait is bounded: on expiry the error flag is raised and the tile
+// treats the missing prefix as zero, so a kernel always finishes.
+constexpr uint32_t kFlagEmpty = 0, kFlagAggregate = 1, kFlagInclusive = 2;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+template <typename V> struct LookbackState {
+    uint32_t* flags;    // [tiles], zeroed before launch
+    V* aggregate;       // [tiles]
+    V* inclusive;       // [tiles]
+    uint32_t* ticket;   // zeroed before launch
+    uint32_t* error;    // set to 1 on an expired wait
+};
+
+__device__ __forceinline__ uint32_t atomic_load_acq(uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_store_rel(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Called by thread 0 only.  Returns the exclusive prefix of `tile`.
+template <typename V> __device__ V lookback_prefix(LookbackState<V> st, uint32_t tile, V aggregate) {
+    if (tile == 0) {
+        st.inclusive[0] = aggregate;
+        atomic_store_rel(&st.flags[0], kFlagInclusive);
+        return V(0);
+    }
+    st.aggregate[tile] = aggregate;
+    atomic_store_rel(&st.flags[tile], kFlagAggregate);
+    V excl = V(0);
+    int64_t t = (int64_t)tile - 1;
+    while (t >= 0) {
+        uint32_t f;
+        uint32_t spins = 0;
+        while ((f = atomic_load_acq(&st.flags[t])) == kFlagEmpty) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > kSpinLimit) break;
+        }
+        if (f == kFlagEmpty) {  // expired: report and stop waiting
+            __hip_atomic_store(st.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        if (f == kFlagInclusive) {
+            excl += __hip_atomic_load(&st.inclusive[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+        excl += __hip_atomic_load(&st.aggregate[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        t--;
+    }
+    st.inclusive[tile] = excl + aggregate;
+    atomic_store_rel(&st.flags[tile], kFlagInclusive);
+    return excl;
+}
+
+// Ticket of the calling workgroup (all threads get it).
+template <typename V> __device__ __forceinline__ uint32_t take_ticket(LookbackState<V> st, uint32_t* smem) {
+    if (threadIdx.x == 0) *smem = atomicAdd(st.ticket, 1u);
+    __syncthreads();
+    uint32_t t = *smem;
+    __syncthreads();
+    return t;
+}
+
+// -------------------------------------------------------------- bit helpers
+__device__ __host__ __forceinline__ uint64_t lowmask64(unsigned n) { return n >= 64 ? ~0ull : ((1ull << n) - 1ull); }
+
+// BITLEN (reference core/cuda_macros.cuh:168-175) from min and max.
+template <typename T> __device__ __host__ __forceinline__ int bitlen_of(T mn, T mx) {
+    if constexpr (std::is_floating_point<T>::value) {
+        return 8 * sizeof(T);
+    } else {
+        if (mn < 0) return 8 * sizeof(T);
+        uint64_t m = (uint64_t)mx;
+        int r = 0;
+        while (m >>= 1) r++;
+        return r + 1;
+    }
+}
+
+}  // namespace ddj

@@ -1,0 +1,292 @@
+// Elementwise kernels: delta, scale, float split/compose (AFL on floats),
+// floatToInt and its inverse, zero-fill, gathered copies; and the delta
+// decode scan (single pass, decoupled look-back).
+#include "dispatch.h"
+#include "k_common.h"
+#include "kernels.h"
+
+namespace ddj {
+
+namespace {
+
+inline unsigned grid_for(Ctx* c, size_t work_items, int per_thread = 1) {
+    size_t threads = (work_items + per_thread - 1) / per_thread;
+    size_t g = (threads + kBlock - 1) / kBlock;
+    size_t cap = (size_t)c->cu_count * 32;
+    if (g > cap) g = cap;
+    if (g == 0) g = 1;
+    return (unsigned)g;
+}
+
+// DeltaEncoding deltaEncodeKernel (reference compression/delta/delta_encoding.cu:19-27):
+// out[i] = x[i+1] - x[i], wrapping in T.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_delta_encode(const T* __restrict__ x, size_t n, T* __restrict__ out) {
+    constexpr int V = 16 / sizeof(T);
+    const size_t m = n - 1;
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    const bool vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    const size_t nv = vec ? m / V : 0;
+    for (size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x; g < nv; g += stride) {
+        const size_t i0 = g * V;
+        u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(x) + g);
+        T v[V + 1];
+        __builtin_memcpy(v, &a, 16);
+        v[V] = x[i0 + V];  // i0 + V <= m < n
+        T d[V];
+#pragma unroll
+        for (int k = 0; k < V; k++) d[k] = (T)(ubits(v[k + 1]) - ubits(v[k]));
+        u32x4 o;
+        __builtin_memcpy(&o, d, 16);
+        __builtin_nontemporal_store(o, reinterpret_cast<u32x4*>(out) + g);
+    }
+    for (size_t i = nv * V + (size_t)blockIdx.x * kBlock + threadIdx.x; i < m; i += stride)
+        out[i] = (T)(ubits(x[i + 1]) - ubits(x[i]));
+}
+
+// x +/- s wrapping in T (scaleEncodeKernel / scaleDecodeKernel,
+// reference compression/scale/scale_encoding.cu:21-28, :70-76).
+template <typename T, bool ADD>
+__global__ __launch_bounds__(kBlock) void k_add_scalar(const T* __restrict__ x, size_t n, uint64_t s,
+                                                       T* __restrict__ out) {
+    constexpr int V = 16 / sizeof(T);
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    const bool vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    const size_t nv = vec ? n / V : 0;
+    for (size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x; g < nv; g += stride) {
+        u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(x) + g);
+        T v[V];
+        __builtin_memcpy(v, &a, 16);
+#pragma unroll
+        for (int k = 0; k < V; k++) v[k] = ADD ? (T)(ubits(v[k]) + s) : (T)(ubits(v[k]) - s);
+        __builtin_memcpy(&a, v, 16);
+        __builtin_nontemporal_store(a, reinterpret_cast<u32x4*>(out) + g);
+    }
+    for (size_t i = nv * V + (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+        out[i] = ADD ? (T)(ubits(x[i]) + s) : (T)(ubits(x[i]) - s);
+}
+
+// _splitFloatKernel (reference compression/afl/afl_encoding.cu:58-72)
+__global__ __launch_bounds__(kBlock) void k_float_split(const float* __restrict__ x, size_t n, int32_t* mant,
+                                                        int32_t* expo, int32_t* sign) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        uint32_t u = __float_as_uint(x[i]);
+        mant[i] = (int32_t)(u & 0x7fffffu);
+        expo[i] = (int32_t)((u >> 23) & 0xffu);
+        sign[i] = (int32_t)(u >> 31);
+    }
+}
+
+// _composeFloatKernel (afl_encoding.cu:169-183).  The sign comes from a
+// packed stencil, or is constant (sign_const 0/1) when the stencil is null.
+__global__ __launch_bounds__(kBlock) void k_float_compose(const int32_t* __restrict__ mant,
+                                                          const int32_t* __restrict__ expo, const uint8_t* bits,
+                                                          int sign_const, size_t n, float* __restrict__ out) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        uint32_t s = bits ? ((bits[i >> 3] >> (i & 7)) & 1u) : (uint32_t)sign_const;
+        uint32_t u = (s << 31) | (((uint32_t)expo[i] & 0xffu) << 23) | ((uint32_t)mant[i] & 0x7fffffu);
+        out[i] = __uint_as_float(u);
+    }
+}
+
+__constant__ double kPow10[11] = {1e0, 1e1, 1e2, 1e3, 1e4, 1e5, 1e6, 1e7, 1e8, 1e9, 1e10};
+
+// FloatingPointToIntegerOperator (reference util/transform/transform_operators.hpp:109-119):
+// int(lrint(x * 10^p)) with the product in double.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_float_to_int(const T* __restrict__ x, size_t n, int p,
+                                                         int32_t* __restrict__ out) {
+    const double f = kPow10[p];
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        double prod = __dmul_rn((double)x[i], f);
+        out[i] = (int32_t)(long)__builtin_rint(prod);
+    }
+}
+
+// IntegerToFloatingPointOperator (transform_operators.hpp:121-131): (T)x / 10^p in double.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_int_to_float(const int32_t* __restrict__ x, size_t n, int p,
+                                                         T* __restrict__ out) {
+    const double f = kPow10[p];
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const int32_t v = x[i];
+        out[i] = (T)__ddiv_rn((double)(T)v, f);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_zero(uint32_t* p, size_t words) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < words; i += stride) p[i] = 0u;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gather_copy(const CopyRange* __restrict__ r, uint8_t* __restrict__ dst) {
+    const CopyRange cr = r[blockIdx.y];
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < cr.bytes; i += stride)
+        dst[cr.dst_off + i] = cr.src[i];
+}
+
+// ------------------------------------------------------------- delta decode
+// out[0] = x0; out[i+1] = x0 + d[0] + ... + d[i] (wrapping).  Tiles of 4096
+// deltas; each thread sums 16 consecutive deltas staged through LDS, the
+// workgroup scans, and the tile prefix comes from the decoupled look-back.
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_delta_decode(const uint8_t* first_bytes, const uint8_t* __restrict__ din,
+                                                         size_t m, T* __restrict__ out, LookbackState<uint64_t> st) {
+    T first;
+    __builtin_memcpy(&first, first_bytes, sizeof(T));
+    __shared__ uint64_t buf[kScanTile + kScanTile / 32];
+    __shared__ uint64_t red[kBlock / kWave];
+    __shared__ uint32_t tk;
+    __shared__ uint64_t tile_prefix;
+    const uint32_t tile = take_ticket(st, &tk);
+    const size_t base = (size_t)tile * kScanTile;
+    const int cnt = (int)((m - base) < (size_t)kScanTile ? (m - base) : (size_t)kScanTile);
+    auto pad = [](int i) { return i + (i >> 5); };
+    // striped, coalesced load (input may sit at any byte offset)
+    const bool aligned = (reinterpret_cast<uintptr_t>(din) % sizeof(T)) == 0;
+    const T* dt = reinterpret_cast<const T*>(din);
+    for (int i = threadIdx.x; i < kScanTile; i += kBlock) {
+        uint64_t v = 0;
+        if (i < cnt) {
+            T t;
+            if (aligned)
+                t = dt[base + i];
+            else
+                __builtin_memcpy(&t, din + (base + i) * sizeof(T), sizeof(T));
+            v = ubits(t);
+        }
+        buf[pad(i)] = v;
+    }
+    __syncthreads();
+    uint64_t local[kScanItems];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        local[k] = buf[pad(threadIdx.x * kScanItems + k)];
+        sum += local[k];
+    }
+    uint64_t total;
+    uint64_t excl = block_exclusive_scan<uint64_t>(sum, &total, red);
+    if (threadIdx.x == 0) tile_prefix = lookback_prefix<uint64_t>(st, tile, total);
+    __syncthreads();
+    uint64_t run = ubits(first) + tile_prefix + excl;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        run += local[k];
+        buf[pad(threadIdx.x * kScanItems + k)] = run;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += kBlock) out[base + i + 1] = (T)buf[pad(i)];
+    if (tile == 0 && threadIdx.x == 0) out[0] = first;
+}
+
+}  // namespace
+
+void launch_delta_encode(Ctx* c, int elem_size, const void* x, size_t n, void* out) {
+    if (n < 2) return;
+    dispatch_int(elem_size, [&](auto tag) {
+        using T = decltype(tag);
+        hipLaunchKernelGGL(k_delta_encode<T>, dim3(grid_for(c, n, 16 / sizeof(T))), dim3(kBlock), 0, c->stream,
+                           static_cast<const T*>(x), n, static_cast<T*>(out));
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_add_scalar(Ctx* c, int elem_size, const void* x, size_t n, uint64_t s, void* out) {
+    if (!n) return;
+    dispatch_int(elem_size, [&](auto tag) {
+        using T = decltype(tag);
+        hipLaunchKernelGGL((k_add_scalar<T, true>), dim3(grid_for(c, n, 16 / sizeof(T))), dim3(kBlock), 0, c->stream,
+                           static_cast<const T*>(x), n, s, static_cast<T*>(out));
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_sub_scalar(Ctx* c, int elem_size, const void* x, size_t n, uint64_t s, void* out) {
+    if (!n) return;
+    dispatch_int(elem_size, [&](auto tag) {
+        using T = decltype(tag);
+        hipLaunchKernelGGL((k_add_scalar<T, false>), dim3(grid_for(c, n, 16 / sizeof(T))), dim3(kBlock), 0,
+                           c->stream, static_cast<const T*>(x), n, s, static_cast<T*>(out));
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_float_split(Ctx* c, const float* x, size_t n, int32_t* mant, int32_t* expo, int32_t* sign) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_float_split, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream, x, n, mant, expo, sign);
+    HIPC(hipGetLastError());
+}
+
+void launch_float_compose(Ctx* c, const int32_t* mant, const int32_t* expo, const uint8_t* bits, int sign_const,
+                          size_t n, float* out) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_float_compose, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream, mant, expo, bits,
+                       sign_const, n, out);
+    HIPC(hipGetLastError());
+}
+
+void launch_float_to_int(Ctx* c, int dtype, const void* x, size_t n, int precision, int32_t* out) {
+    if (!n) return;
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        hipLaunchKernelGGL(k_float_to_int<T>, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream,
+                           static_cast<const T*>(x), n, precision, out);
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_int_to_float(Ctx* c, int dtype, const uint8_t* x, size_t n, int precision, void* out) {
+    if (!n) return;
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        hipLaunchKernelGGL(k_int_to_float<T>, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream,
+                           reinterpret_cast<const int32_t*>(x), n, precision, static_cast<T*>(out));
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_fill_zero(Ctx* c, void* p, size_t bytes) {
+    if (!bytes) return;
+    size_t words = (bytes + 3) / 4;  // allocations are padded, whole words are safe
+    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(c, words)), dim3(kBlock), 0, c->stream, static_cast<uint32_t*>(p),
+                       words);
+    HIPC(hipGetLastError());
+}
+
+void launch_gather_copy(Ctx* c, const CopyRange* d_ranges, int n_ranges, uint8_t* dst, uint64_t max_bytes) {
+    if (n_ranges <= 0 || max_bytes == 0) return;
+    unsigned gx = grid_for(c, max_bytes);
+    gx = gx > 1024 ? 1024 : gx;
+    hipLaunchKernelGGL(k_gather_copy, dim3(gx, n_ranges), dim3(kBlock), 0, c->stream, d_ranges, dst);
+    HIPC(hipGetLastError());
+}
+
+void launch_delta_decode(Ctx* c, int elem_size, const void* first, const uint8_t* d, size_t m, void* out) {
+    dispatch_int(elem_size, [&](auto tag) {
+        using T = decltype(tag);
+        if (m == 0) {
+            d2d(c, out, first, sizeof(T));
+            return;
+        }
+        const size_t tiles = (m + kScanTile - 1) / kScanTile;
+        DevBuf flags(c, 4 * tiles + 4);
+        DevBuf agg(c, 8 * tiles), inc(c, 8 * tiles);
+        launch_fill_zero(c, flags.get(), 4 * tiles + 4);
+        LookbackState<uint64_t> st{flags.get<uint32_t>(), agg.get<uint64_t>(), inc.get<uint64_t>(),
+                                   flags.get<uint32_t>() + tiles, &c->control->error_flag};
+        hipLaunchKernelGGL(k_delta_decode<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                           static_cast<const uint8_t*>(first), d, m, static_cast<T*>(out), st);
+        HIPC(hipGetLastError());
+    });
+}
+
+}  // namespace ddj

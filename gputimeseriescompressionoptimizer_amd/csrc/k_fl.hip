@@ -1,0 +1,141 @@
+// FL bit-packing (the reference's AFL encoding as it is actually called,
+// run_afl_compress_gpu<T,1> / run_afl_decompress_gpu<T,1>,
+// compression/afl/afl_gpu.cu:67-195).
+//
+// The reference stream is a contiguous LSB-first bit stream in words of T:
+// value i occupies stream bits [i*b, (i+1)*b), stream bit p being bit p%W of
+// word p/W.  On a little-endian machine that stream is the same byte sequence
+// for every word width, so both kernels work on 32-bit words whatever T is;
+// only the total length is rounded to whole T words.
+//
+// Pack: a workgroup owns a tile of 4096 values (a whole number of output
+// words for any b), stages the values in LDS with 16-byte loads and emits
+// the tile's 128*b output words, one word per thread-iteration, each word
+// OR-ing the <= ceil(32/b)+1 values that overlap it.  Loads and stores are
+// both fully coalesced.
+//
+// Unpack: a workgroup stages the tile's aligned input words in LDS (an
+// aligned 4-byte word containing a valid byte never crosses a page, so the
+// stream may start at any byte address) and every thread extracts values
+// with a 64/96-bit funnel shift.
+#include "dispatch.h"
+#include "k_common.h"
+#include "kernels.h"
+
+namespace ddj {
+
+namespace {
+
+constexpr int kFlTile = 4096;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fl_pack(const T* __restrict__ x, size_t n, const MinMaxResult* mm,
+                                                    int fixed_bits, uint32_t* __restrict__ out, size_t out_words) {
+    __shared__ __attribute__((aligned(16))) T vals[kFlTile];
+    int b = fixed_bits;
+    if (mm) b = bitlen_of<T>(from_order_key<T>(mm->min_key), from_order_key<T>(mm->max_key));
+    const size_t tile = blockIdx.x;
+    const size_t v0 = tile * kFlTile;
+    const int nv = (int)((n - v0) < (size_t)kFlTile ? (n - v0) : (size_t)kFlTile);
+    const T* src = x + v0;
+    if (nv == kFlTile && (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+        const u32x4* s4 = reinterpret_cast<const u32x4*>(src);
+        u32x4* d4 = reinterpret_cast<u32x4*>(vals);
+        constexpr int N4 = kFlTile * (int)sizeof(T) / 16;
+#pragma unroll
+        for (int i = threadIdx.x; i < N4; i += kBlock) d4[i] = __builtin_nontemporal_load(s4 + i);
+    } else {
+        for (int i = threadIdx.x; i < kFlTile; i += kBlock) vals[i] = i < nv ? src[i] : T(0);
+    }
+    __syncthreads();
+    const int tile_words = kFlTile / 32 * b;  // 128 * b
+    const size_t w0 = tile * (size_t)tile_words;
+    const uint64_t mask = lowmask64(b);
+    for (int j = threadIdx.x; j < tile_words; j += kBlock) {
+        if (w0 + j >= out_words) break;
+        const int bit0 = j * 32;
+        const int va = bit0 / b;
+        int vb = (bit0 + 31) / b;
+        vb = vb < kFlTile - 1 ? vb : kFlTile - 1;
+        uint64_t word = 0;
+        for (int v = va; v <= vb; v++) {
+            const uint64_t val = ubits(vals[v]) & mask;
+            const int off = v * b - bit0;
+            word |= off >= 0 ? (val << off) : (val >> (-off));
+        }
+        out[w0 + j] = (uint32_t)word;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fl_unpack(const uint8_t* __restrict__ in, size_t in_bytes, size_t n,
+                                                      int b, T* __restrict__ out) {
+    constexpr int kMaxWords = kFlTile * 64 / 32 + 8;
+    __shared__ uint32_t w[kMaxWords];
+    const size_t tile = blockIdx.x;
+    const size_t v0 = tile * kFlTile;
+    const int nv = (int)((n - v0) < (size_t)kFlTile ? (n - v0) : (size_t)kFlTile);
+    const size_t B0 = v0 * (size_t)b / 8;  // exact: v0 is a multiple of 8
+    const size_t need = ((size_t)nv * b + 7) / 8;
+    const size_t avail_total = in_bytes > B0 ? in_bytes - B0 : 0;
+    const size_t avail = need < avail_total ? need : avail_total;
+    const uint8_t* base = in + B0;
+    const int a = (int)(reinterpret_cast<uintptr_t>(base) & 3);
+    const uint32_t* wp = reinterpret_cast<const uint32_t*>(base - a);
+    const int nw_valid = (int)((a + avail + 3) / 4);
+    int nw_fill = nw_valid + 3;
+    nw_fill = nw_fill < kMaxWords ? nw_fill : kMaxWords;
+    for (int i = threadIdx.x; i < nw_fill; i += kBlock) {
+        uint32_t v = i < nw_valid ? wp[i] : 0u;
+        // bytes before the stream start or after its end are masked out
+        if (i == 0 && a) v &= ~0u << (8 * a);
+        const long long last_byte = (long long)a + (long long)avail;  // exclusive, relative to wp
+        if (i < nw_valid && (long long)(i + 1) * 4 > last_byte) {
+            const int keep = (int)(last_byte - (long long)i * 4);
+            v &= keep >= 4 ? ~0u : ((1u << (8 * keep)) - 1u);
+        }
+        w[i] = v;
+    }
+    __syncthreads();
+    const uint64_t mask = lowmask64(b);
+    for (int k = threadIdx.x; k < nv; k += kBlock) {
+        const uint64_t p = (uint64_t)k * b + 8 * a;
+        const uint32_t q = (uint32_t)(p >> 5), s = (uint32_t)(p & 31);
+        uint64_t val = ((uint64_t)w[q] | ((uint64_t)w[q + 1] << 32)) >> s;
+        if (s + b > 64) val |= (uint64_t)w[q + 2] << (64 - s);
+        out[v0 + k] = (T)(val & mask);
+    }
+}
+
+}  // namespace
+
+void launch_fl_pack(Ctx* c, int dtype, const void* x, size_t n, const MinMaxResult* mm, int bits, void* out) {
+    if (n == 0) return;
+    const int esize = (int)dtype_size(dtype);
+    dispatch_int(esize, [&](auto tag) {
+        using T = decltype(tag);
+        const int maxb = mm ? 8 * (int)sizeof(T) : bits;
+        const uint64_t W = 8 * sizeof(T);
+        const size_t out_bytes = (size_t)((maxb * (uint64_t)n + W - 1) / W) * sizeof(T);
+        const size_t out_words = (out_bytes + 3) / 4;
+        const size_t tiles = (n + kFlTile - 1) / kFlTile;
+        hipLaunchKernelGGL(k_fl_pack<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, static_cast<const T*>(x),
+                           n, mm, bits, static_cast<uint32_t*>(out), out_words);
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_fl_unpack(Ctx* c, int dtype, const uint8_t* in, size_t in_bytes, size_t n, int bits, void* out) {
+    if (n == 0) return;
+    const int esize = (int)dtype_size(dtype);
+    dispatch_int(esize, [&](auto tag) {
+        using T = decltype(tag);
+        if (bits <= 0 || bits > 8 * (int)sizeof(T)) fail(DDJ_ECORRUPT, "FL bit width out of range");
+        const size_t tiles = (n + kFlTile - 1) / kFlTile;
+        hipLaunchKernelGGL(k_fl_unpack<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, in, in_bytes, n, bits,
+                           static_cast<T*>(out));
+        HIPC(hipGetLastError());
+    });
+}
+
+}  // namespace ddj

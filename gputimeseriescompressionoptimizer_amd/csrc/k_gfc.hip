@@ -1,0 +1,301 @@
+// GFC float/double codec (reference compression/gfc/gfc_encoding.cpp:16-108,
+// gfc_encoding_impl_double.cu:31-303, gfc_encoding_impl_float.cu:25-289).
+//
+// Stream semantics kept from the reference: rows are zero-padded to a
+// multiple of 32 and cut into Wc = blocks * warpsPerBlock chunks of `per`
+// rows; inside a chunk every 32-row subchunk codes diff = v - last, with
+// last = 0 for the chunk's first subchunk and the chunk's row 31 for the
+// others; a subchunk is 16 bytes of sign|code nibbles then each row's low
+// nb bytes of |diff|.  out[1] holds, per chunk, the end offset in the
+// reference's scratch layout; out[2] is the concatenated chunk payloads.
+//
+// MI355X mapping: one workgroup per chunk (<= 2048 rows).  Encode computes
+// all byte counts, scans them in the workgroup, assembles the chunk payload
+// in LDS and gets its output position from a decoupled look-back over chunk
+// sizes.  Decode stages the chunk payload in LDS, walks the subchunk headers
+// with one wave (the only sequential step of the format) and then decodes
+// all rows in parallel.
+#include "dispatch.h"
+#include "k_common.h"
+#include "kernels.h"
+
+namespace ddj {
+
+namespace {
+
+constexpr int kGfcMaxPer = 2048;  // rows per chunk is at most 2048 (double) / 1024 (float)
+
+struct GfcGeom {
+    long long n, N, blocks, wpb, Wc, per, S;
+};
+
+GfcGeom gfc_geom(bool dbl, long long n) {
+    GfcGeom g;
+    g.n = n;
+    g.wpb = dbl ? 16 : 32;
+    g.blocks = (n + 32768 - 1) / 32768;
+    g.Wc = g.blocks * g.wpb;
+    g.N = (n + 31) & ~31LL;
+    long long per = (g.N + g.Wc - 1) / g.Wc;
+    if (per < 32) per = 32;
+    g.per = (per + 31) & ~31LL;
+    g.S = dbl ? 16 : 8;
+    return g;
+}
+
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_gfc_encode(const U* __restrict__ x, long long n, long long N, long long per,
+                                                       int S, uint8_t* __restrict__ payload,
+                                                       int32_t* __restrict__ offsets, LookbackState<uint64_t> st,
+                                                       uint64_t* d_total, uint32_t n_chunks) {
+    constexpr int RPT = kGfcMaxPer / kBlock;  // rows per thread (8)
+    constexpr bool dbl = sizeof(U) == 8;
+    __shared__ __attribute__((aligned(16))) uint8_t out[kGfcMaxPer / 32 * (16 + 32 * (int)sizeof(U)) + 16];
+    __shared__ U s_last;
+    __shared__ uint64_t red[kBlock / kWave];
+    __shared__ uint32_t tk;
+    __shared__ uint64_t s_prefix;
+    const uint32_t chunk = take_ticket(st, &tk);
+    const long long c0 = (long long)chunk * per;
+    long long c1 = c0 + per;
+    c1 = c1 < N ? c1 : N;
+    const int rows = c0 < N ? (int)(c1 - c0) : 0;
+    if (threadIdx.x == 0) s_last = (rows > 0 && c0 + 31 < n) ? x[c0 + 31] : U(0);
+    __syncthreads();
+    const U last_c = s_last;
+    U d[RPT];
+    uint32_t nibs = 0;  // 4 bits per row
+    uint32_t nbsum = 0;
+    int nbs[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; k++) {
+        const int r = threadIdx.x * RPT + k;
+        nbs[k] = 0;
+        d[k] = 0;
+        if (r < rows) {
+            const long long g = c0 + r;
+            const U v = g < n ? x[g] : U(0);
+            const U last = (r < 32) ? U(0) : last_c;
+            U diff = (U)(v - last);
+            const uint32_t sign = (uint32_t)(diff >> (8 * sizeof(U) - 1)) & 1u;
+            if (sign) diff = (U)(U(0) - diff);
+            int nb, code;
+            if (dbl) {
+                const int z = (diff == 0) ? 8 : (__clzll((unsigned long long)diff) >> 3);
+                nb = (z == 6) ? 3 : 8 - z;
+                code = nb >= 3 ? nb - 1 : nb;
+            } else {
+                const int z = (diff == 0) ? 4 : (__clz((unsigned)diff) >> 3);
+                nb = 4 - z;
+                code = nb;
+            }
+            nbs[k] = nb;
+            d[k] = diff;
+            nibs |= (uint32_t)((sign << 3) | code) << (4 * k);
+            nbsum += nb;
+        }
+    }
+    uint64_t total_nb;
+    const uint64_t excl = block_exclusive_scan<uint64_t>(nbsum, &total_nb, red);
+    const int nsub = rows / 32;
+    const uint64_t chunk_bytes = 16ull * nsub + total_nb;
+    // thread t covers rows [8t, 8t+8) of subchunk s = t/4, i.e. header bytes
+    // 4*(t%4) .. +3; the subchunk starts after 16 header bytes per earlier
+    // subchunk plus the byte counts of all earlier rows (the exclusive scan
+    // value of the subchunk's first thread).
+    const int sub = threadIdx.x / 4;
+    __shared__ uint32_t sub_excl[kGfcMaxPer / 32];
+    if ((threadIdx.x & 3) == 0 && threadIdx.x * RPT < rows) sub_excl[sub] = (uint32_t)excl;
+    __syncthreads();
+    if (threadIdx.x * RPT < rows) {
+        const uint32_t sub_off = 16u * sub + sub_excl[sub];
+        const uint32_t h = sub_off + 4u * (threadIdx.x & 3);
+#pragma unroll
+        for (int j = 0; j < 4; j++) out[h + j] = (uint8_t)((nibs >> (8 * j)) & 0xff);
+        uint32_t pos = 16u * (sub + 1) + (uint32_t)excl;
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            uint64_t dv = (uint64_t)d[k];
+            for (int b = 0; b < nbs[k]; b++) out[pos + b] = (uint8_t)(dv >> (8 * b));
+            pos += nbs[k];
+        }
+    }
+    if (threadIdx.x == 0) s_prefix = lookback_prefix<uint64_t>(st, chunk, chunk_bytes);
+    __syncthreads();
+    const uint64_t O = s_prefix;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)chunk_bytes; i += kBlock) payload[O + i] = out[i];
+    if (threadIdx.x == 0) {
+        offsets[chunk] = (int32_t)((c0 < N ? (c0 + 1) / 2 : (N + 1) / 2) * (S + 1) + (long long)chunk_bytes);
+        if (chunk == n_chunks - 1) *d_total = O + chunk_bytes;
+    }
+}
+
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict__ payload, size_t payload_bytes,
+                                                       const uint8_t* __restrict__ offsets, long long n, long long N,
+                                                       long long per, int S, U* __restrict__ out,
+                                                       LookbackState<uint64_t> st) {
+    constexpr int RPT = kGfcMaxPer / kBlock;
+    constexpr bool dbl = sizeof(U) == 8;
+    constexpr int kMaxBytes = kGfcMaxPer / 32 * (16 + 32 * (int)sizeof(U));
+    __shared__ __attribute__((aligned(16))) uint8_t buf[kMaxBytes + 16];
+    __shared__ uint32_t sub_off[kGfcMaxPer / 32 + 1];
+    __shared__ U vals[kGfcMaxPer];
+    __shared__ U s_last;
+    __shared__ uint32_t tk;
+    __shared__ uint64_t s_prefix;
+    __shared__ int s_bad;
+    const uint32_t chunk = take_ticket(st, &tk);
+    const long long c0 = (long long)chunk * per;
+    long long c1 = c0 + per;
+    c1 = c1 < N ? c1 : N;
+    const int rows = c0 < N ? (int)(c1 - c0) : 0;
+    const int nsub = rows / 32;
+    int32_t end;
+    __builtin_memcpy(&end, offsets + 4 * (size_t)chunk, 4);
+    const long long start_scr = (c0 < N ? (c0 + 1) / 2 : (N + 1) / 2) * (S + 1);
+    long long cb = (long long)end - start_scr;
+    if (cb < 0 || cb > kMaxBytes) cb = 0;  // corrupt: decode as empty (the host validates totals)
+    const uint64_t chunk_bytes = (uint64_t)cb;
+    if (threadIdx.x == 0) {
+        s_prefix = lookback_prefix<uint64_t>(st, chunk, chunk_bytes);
+        s_bad = 0;
+    }
+    __syncthreads();
+    const uint64_t P = s_prefix;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)chunk_bytes; i += kBlock)
+        buf[i] = (P + i < payload_bytes) ? payload[P + i] : 0;
+    __syncthreads();
+    // header walk: one wave, lanes 0..31 read one nibble each
+    if (threadIdx.x < kWave) {
+        const int lane = threadIdx.x;
+        uint32_t off = 0;
+        for (int s = 0; s < nsub; s++) {
+            int nb = 0;
+            if (lane < 32 && off + 16 <= chunk_bytes) {
+                const uint8_t byte = buf[off + (lane >> 1)];
+                const int nib = (lane & 1) ? (byte >> 4) : (byte & 15);
+                const int code = nib & 7;
+                nb = dbl ? (code >= 2 ? code + 1 : code) : code;
+            }
+            const int tot = wave_sum(nb);
+            if (lane == 0) sub_off[s] = off;
+            off += 16 + tot;
+        }
+        if (lane == 0) {
+            sub_off[nsub] = off;
+            if (off != chunk_bytes) s_bad = 1;
+        }
+    }
+    __syncthreads();
+    const int sub = threadIdx.x / 4;
+    U dec[RPT];
+    if (threadIdx.x * RPT < rows) {
+        const uint32_t so = sub_off[sub];
+        // nibbles of rows 0..31 of the subchunk; this thread's rows start at lane 8*(t%4)
+        int pre = 0;
+        const int first_lane = 8 * (threadIdx.x & 3);
+        for (int l = 0; l < first_lane; l++) {
+            const uint8_t byte = buf[so + (l >> 1)];
+            const int code = ((l & 1) ? (byte >> 4) : byte) & 7;
+            pre += dbl ? (code >= 2 ? code + 1 : code) : code;
+        }
+        uint32_t pos = so + 16 + pre;
+#pragma unroll
+        for (int k = 0; k < RPT; k++) {
+            const int l = first_lane + k;
+            const uint8_t byte = buf[so + (l >> 1)];
+            const int nib = (l & 1) ? (byte >> 4) : (byte & 15);
+            const int code = nib & 7;
+            const int nb = dbl ? (code >= 2 ? code + 1 : code) : code;
+            uint64_t dv = 0;
+            for (int b = nb - 1; b >= 0; b--) dv = (dv << 8) | (pos + b < sizeof(buf) ? buf[pos + b] : 0);
+            pos += nb;
+            U diff = (U)dv;
+            if (nib & 8) diff = (U)(U(0) - diff);
+            dec[k] = diff;
+        }
+    }
+    // row 31 of the chunk is the predictor of every later subchunk
+    if (threadIdx.x == 3 && rows > 0) s_last = dec[RPT - 1];
+    __syncthreads();
+    if (threadIdx.x * RPT < rows) {
+        const U last = sub == 0 ? U(0) : s_last;
+#pragma unroll
+        for (int k = 0; k < RPT; k++) vals[threadIdx.x * RPT + k] = (U)(last + dec[k]);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < rows; i += kBlock)
+        if (c0 + i < n) out[c0 + i] = vals[i];
+    if (threadIdx.x == 0 && s_bad) __hip_atomic_store(st.error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace
+
+void gfc_encode(Ctx* c, bool dbl, const void* x, size_t n, DevBuf& meta, DevBuf& offsets, DevBuf& payload) {
+    const GfcGeom g = gfc_geom(dbl, (long long)n);
+    const size_t tiles = (size_t)g.Wc;
+    DevBuf flags(c, 4 * tiles + 4), agg(c, 8 * tiles), inc(c, 8 * tiles), total(c, 8);
+    launch_fill_zero(c, flags.get(), 4 * tiles + 4);
+    LookbackState<uint64_t> st{flags.get<uint32_t>(), agg.get<uint64_t>(), inc.get<uint64_t>(),
+                               flags.get<uint32_t>() + tiles, &c->control->error_flag};
+    offsets = DevBuf(c, 4 * tiles);
+    const size_t worst = (size_t)(g.N / 32) * (16 + 32 * (dbl ? 8 : 4));
+    payload = DevBuf(c, worst);
+    if (dbl)
+        hipLaunchKernelGGL(k_gfc_encode<uint64_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                           static_cast<const uint64_t*>(x), g.n, g.N, g.per, (int)g.S, payload.get<uint8_t>(),
+                           offsets.get<int32_t>(), st, total.get<uint64_t>(), (uint32_t)tiles);
+    else
+        hipLaunchKernelGGL(k_gfc_encode<uint32_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                           static_cast<const uint32_t*>(x), g.n, g.N, g.per, (int)g.S, payload.get<uint8_t>(),
+                           offsets.get<int32_t>(), st, total.get<uint64_t>(), (uint32_t)tiles);
+    HIPC(hipGetLastError());
+    int32_t h_meta[3] = {(int32_t)g.blocks, (int32_t)g.wpb, (int32_t)n};
+    meta = make_buf(c, h_meta, 12);
+    uint64_t tot = 0;
+    d2h(c, &tot, total.get(), 8);
+    payload.set_size(tot);
+}
+
+void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const uint8_t* offsets, size_t off_bytes,
+                const uint8_t* payload, size_t payload_bytes, DevBuf& out) {
+    if (meta_bytes < 12) fail(DDJ_ECORRUPT, "GFC metadata shorter than 12 bytes");
+    int32_t h_meta[3];
+    d2h(c, h_meta, meta, 12);
+    if (h_meta[0] <= 0 || h_meta[1] <= 0 || h_meta[2] <= 0) fail(DDJ_ECORRUPT, "GFC metadata out of range");
+    const long long n = h_meta[2];
+    GfcGeom g = gfc_geom(dbl, n);
+    if (h_meta[0] != g.blocks || h_meta[1] != g.wpb) {
+        // other launch shapes are legal in the format: recompute the chunking from them
+        g.blocks = h_meta[0];
+        g.wpb = h_meta[1];
+        g.Wc = g.blocks * g.wpb;
+        long long per = (g.N + g.Wc - 1) / g.Wc;
+        if (per < 32) per = 32;
+        g.per = (per + 31) & ~31LL;
+        if (g.per > kGfcMaxPer) fail(DDJ_EUNSUPPORTED, "GFC chunks longer than 2048 rows");
+    }
+    const size_t tiles = (size_t)g.Wc;
+    if (off_bytes < 4 * tiles) fail(DDJ_ECORRUPT, "GFC offsets shorter than the chunk count");
+    DevBuf flags(c, 4 * tiles + 4), agg(c, 8 * tiles), inc(c, 8 * tiles);
+    launch_fill_zero(c, flags.get(), 4 * tiles + 4);
+    uint32_t zero = 0;
+    h2d(c, &c->control->error_flag, &zero, 4);
+    LookbackState<uint64_t> st{flags.get<uint32_t>(), agg.get<uint64_t>(), inc.get<uint64_t>(),
+                               flags.get<uint32_t>() + tiles, &c->control->error_flag};
+    out = DevBuf(c, (size_t)n * (dbl ? 8 : 4));
+    if (dbl)
+        hipLaunchKernelGGL(k_gfc_decode<uint64_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
+                           payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<uint64_t>(), st);
+    else
+        hipLaunchKernelGGL(k_gfc_decode<uint32_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
+                           payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<uint32_t>(), st);
+    HIPC(hipGetLastError());
+    uint32_t err = 0;
+    d2h(c, &err, &c->control->error_flag, 4);
+    if (err == 2) fail(DDJ_ECORRUPT, "GFC chunk sizes disagree with their headers");
+    if (err) fail(DDJ_ETIMEOUT, "look-back wait expired");
+}
+
+}  // namespace ddj

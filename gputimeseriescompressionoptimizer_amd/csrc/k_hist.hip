@@ -1,0 +1,249 @@
+// Most-frequent values (Histogram::GetMostFrequent, reference
+// util/histogram/histogram.cpp:29-41 -> thrust_sparse_histogram.cu:20-69 and
+// most_frequent.cu:10-26): exact counts of every distinct value, then the K
+// largest counts, equal counts ordered by the smaller value (what the
+// reference's stable sort of key-sorted bins yields).
+//
+// Counting: each workgroup aggregates its rows in an LDS hash table and
+// flushes it into a global open-addressing table (64-bit keys, ~0 = empty);
+// rows that find no LDS slot go straight to the global table.  A 64-bit
+// value equal to the empty marker is counted on the side.  Floats count
+// -0.0 with +0.0 (operator==), reporting -0.0 when one was seen (it sorts
+// first in the reference).  If the global table fills up the host retries
+// with a table sized for every row distinct.
+#include "dispatch.h"
+#include "k_common.h"
+#include "kernels.h"
+
+namespace ddj {
+
+namespace {
+
+constexpr uint64_t kEmpty = ~0ull;
+constexpr int kLdsSlots = 2048;
+constexpr int kMaxK = 4;
+
+struct HistCtl {
+    unsigned long long distinct;
+    unsigned long long special;  // rows whose 64-bit pattern equals kEmpty
+    uint32_t overflow;
+    uint32_t negzero;
+};
+
+__device__ __forceinline__ uint64_t hmix(uint64_t z) {
+    z ^= z >> 33;
+    z *= 0xff51afd7ed558ccdull;
+    z ^= z >> 33;
+    z *= 0xc4ceb9fe1a85ec53ull;
+    z ^= z >> 33;
+    return z;
+}
+
+template <typename T> __device__ __forceinline__ uint64_t hist_key(T v, bool& negzero) {
+    if constexpr (std::is_floating_point<T>::value) {
+        if (v == T(0)) {
+            if (__builtin_signbit(v)) negzero = true;
+            return 0;
+        }
+    }
+    return ubits(v);
+}
+
+__device__ bool global_insert(unsigned long long* keys, unsigned long long* counts, uint64_t cap_mask, uint64_t key,
+                              unsigned long long cnt, HistCtl* ctl) {
+    uint64_t h = hmix(key) & cap_mask;
+    for (uint64_t p = 0; p <= cap_mask && p < 4096; p++) {
+        const uint64_t s = (h + p) & cap_mask;
+        unsigned long long k = __hip_atomic_load(&keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == key) {
+            atomicAdd(&counts[s], cnt);
+            return true;
+        }
+        if (k == kEmpty) {
+            unsigned long long prev = atomicCAS(&keys[s], kEmpty, (unsigned long long)key);
+            if (prev == kEmpty || prev == key) {
+                atomicAdd(&counts[s], cnt);
+                if (prev == kEmpty) atomicAdd(&ctl->distinct, 1ull);
+                return true;
+            }
+        }
+    }
+    atomicOr(&ctl->overflow, 1u);
+    return false;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, size_t n, unsigned long long* keys,
+                                                       unsigned long long* counts, uint64_t cap_mask, HistCtl* ctl) {
+    __shared__ unsigned long long skey[kLdsSlots];
+    __shared__ uint32_t scnt[kLdsSlots];
+    for (int i = threadIdx.x; i < kLdsSlots; i += kBlock) {
+        skey[i] = kEmpty;
+        scnt[i] = 0;
+    }
+    __syncthreads();
+    bool negzero = false;
+    unsigned long long special = 0;
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const uint64_t key = hist_key(x[i], negzero);
+        if (key == kEmpty) {
+            special++;
+            continue;
+        }
+        const uint32_t h = (uint32_t)hmix(key) & (kLdsSlots - 1);
+        bool done = false;
+        for (int p = 0; p < 16 && !done; p++) {
+            const uint32_t s = (h + p) & (kLdsSlots - 1);
+            unsigned long long k = skey[s];
+            if (k == kEmpty) k = atomicCAS(&skey[s], kEmpty, (unsigned long long)key);
+            if (k == kEmpty || k == key) {
+                atomicAdd(&scnt[s], 1u);
+                done = true;
+            }
+        }
+        if (!done) global_insert(keys, counts, cap_mask, key, 1ull, ctl);
+    }
+    __syncthreads();
+    for (int s = threadIdx.x; s < kLdsSlots; s += kBlock)
+        if (skey[s] != kEmpty) global_insert(keys, counts, cap_mask, skey[s], scnt[s], ctl);
+    const uint64_t nz = wave_or((uint64_t)negzero);
+    const unsigned long long sp = wave_sum(special);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (nz) atomicOr(&ctl->negzero, 1u);
+        if (sp) atomicAdd(&ctl->special, sp);
+    }
+}
+
+struct Cand {
+    unsigned long long count;
+    uint64_t key;
+};
+
+template <typename T> __device__ __forceinline__ bool better(const Cand& a, const Cand& b) {
+    if (a.count != b.count) return a.count > b.count;
+    T va, vb;
+    using U = typename UnsignedOf<T>::type;
+    U ua = (U)a.key, ub = (U)b.key;
+    __builtin_memcpy(&va, &ua, sizeof(T));
+    __builtin_memcpy(&vb, &ub, sizeof(T));
+    return order_key(va) < order_key(vb);
+}
+
+template <typename T> __device__ __forceinline__ void push(Cand* top, int K, Cand c) {
+    if (c.count == 0) return;
+    for (int i = 0; i < K; i++) {
+        if (top[i].count == 0 || better<T>(c, top[i])) {
+            for (int j = K - 1; j > i; j--) top[j] = top[j - 1];
+            top[i] = c;
+            return;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_hist_topk(const unsigned long long* keys, const unsigned long long* counts,
+                                                      uint64_t cap, int K, Cand* partials, uint32_t* counter,
+                                                      const HistCtl* ctl, Cand* out) {
+    __shared__ Cand sc[kBlock * kMaxK];
+    __shared__ bool last;
+    Cand top[kMaxK];
+    for (int i = 0; i < kMaxK; i++) top[i] = Cand{0, 0};
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t s = (size_t)blockIdx.x * kBlock + threadIdx.x; s < cap; s += stride) {
+        const uint64_t k = keys[s];
+        if (k != kEmpty) push<T>(top, K, Cand{counts[s], k});
+    }
+    for (int i = 0; i < K; i++) sc[threadIdx.x * K + i] = top[i];
+    __syncthreads();
+    if (threadIdx.x < kWave) {  // 256 lists -> 64 lists
+        for (int r = 1; r < kBlock / kWave; r++)
+            for (int i = 0; i < K; i++) push<T>(top, K, sc[(threadIdx.x + r * kWave) * K + i]);
+    }
+    __syncthreads();
+    if (threadIdx.x < kWave)
+        for (int i = 0; i < K; i++) sc[threadIdx.x * K + i] = top[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Cand b[kMaxK];
+        for (int i = 0; i < kMaxK; i++) b[i] = Cand{0, 0};
+        for (int i = 0; i < kWave * K; i++) push<T>(b, K, sc[i]);
+        for (int i = 0; i < K; i++) partials[blockIdx.x * K + i] = b[i];
+        uint32_t prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last || threadIdx.x != 0) return;
+    Cand b[kMaxK];
+    for (int i = 0; i < kMaxK; i++) b[i] = Cand{0, 0};
+    for (uint32_t i = 0; i < gridDim.x * (uint32_t)K; i++) {
+        Cand c;
+        c.count = __hip_atomic_load(&partials[i].count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        c.key = __hip_atomic_load(&partials[i].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        push<T>(b, K, c);
+    }
+    if (ctl->special) push<T>(b, K, Cand{ctl->special, kEmpty});
+    for (int i = 0; i < K; i++) out[i] = b[i];
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+}  // namespace
+
+void topk_most_frequent(Ctx* c, int dtype, const void* x, size_t n, int K, uint64_t* keys_bits_host,
+                        int* n_distinct_host) {
+    if (K < 1 || K > kMaxK) fail(DDJ_EINVAL, "top-K supports K in 1..4");
+    if (n == 0) fail(DDJ_EINVAL, "top-K of an empty array");
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        uint64_t cap = 1u << 16;
+        for (int attempt = 0; attempt < 2; attempt++) {
+            DevBuf keys(c, 8 * cap), counts(c, 8 * cap), ctl(c, sizeof(HistCtl));
+            const unsigned fg = (unsigned)std::min<uint64_t>((cap + kBlock - 1) / kBlock, (uint64_t)c->cu_count * 16);
+            hipLaunchKernelGGL(k_fill_u64, dim3(fg), dim3(kBlock), 0, c->stream, keys.get<unsigned long long>(), cap,
+                               (unsigned long long)kEmpty);
+            launch_fill_zero(c, counts.get(), 8 * cap);
+            launch_fill_zero(c, ctl.get(), sizeof(HistCtl));
+            size_t g = (n + kBlock * 16 - 1) / (kBlock * 16);
+            g = std::max<size_t>(1, std::min<size_t>(g, (size_t)c->cu_count * 4));
+            hipLaunchKernelGGL(k_hist_count<T>, dim3((unsigned)g), dim3(kBlock), 0, c->stream,
+                               static_cast<const T*>(x), n, keys.get<unsigned long long>(),
+                               counts.get<unsigned long long>(), cap - 1, ctl.get<HistCtl>());
+            HIPC(hipGetLastError());
+            HistCtl h;
+            d2h(c, &h, ctl.get(), sizeof h);
+            if (h.overflow || h.distinct * 10 > cap * 7) {
+                if (attempt == 1) fail(DDJ_ENOMEM, "histogram table overflow");
+                uint64_t want = 1;
+                while (want < 2 * n + 2) want <<= 1;
+                cap = std::max<uint64_t>(want, cap << 2);
+                continue;
+            }
+            const unsigned tg = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((cap + kBlock - 1) / kBlock, 128));
+            DevBuf partials(c, sizeof(Cand) * K * tg), out(c, sizeof(Cand) * K);
+            hipLaunchKernelGGL(k_hist_topk<T>, dim3(tg), dim3(kBlock), 0, c->stream, keys.get<unsigned long long>(),
+                               counts.get<unsigned long long>(), cap, K, partials.get<Cand>(), &c->control->counters[2],
+                               ctl.get<HistCtl>(), out.get<Cand>());
+            HIPC(hipGetLastError());
+            Cand res[kMaxK];
+            d2h(c, res, out.get(), sizeof(Cand) * K);
+            int nd = (int)h.distinct + (h.special ? 1 : 0);
+            if (n_distinct_host) *n_distinct_host = nd;
+            // fewer than K distinct values: repeat the last real key
+            int real = std::min(K, nd);
+            for (int i = 0; i < K; i++) {
+                const Cand& r = res[i < real ? i : real - 1];
+                uint64_t kb = r.key;
+                if (std::is_floating_point<T>::value && kb == 0 && h.negzero) kb = (sizeof(T) == 8) ? (1ull << 63) : (1ull << 31);
+                keys_bits_host[i] = kb;
+            }
+            return;
+        }
+    });
+}
+
+}  // namespace ddj

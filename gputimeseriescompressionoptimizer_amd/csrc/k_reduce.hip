@@ -1,0 +1,366 @@
+// Single-pass reductions: min/max (FL bit width, scale, patch thresholds)
+// and the fused statistics pass the planner reads.
+//
+// One launch each: every workgroup reduces a grid-stride share with 16-byte
+// loads, writes a partial, and the last workgroup to finish (ticket on a
+// control counter that it resets) folds the partials.
+#include "dispatch.h"
+#include "k_common.h"
+#include "kernels.h"
+
+namespace ddj {
+
+namespace {
+
+constexpr int kMaxReduceBlocks = 2048;
+
+__device__ __forceinline__ uint64_t block_reduce_min(uint64_t v, uint64_t* sm) {
+    v = wave_min(v);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    if (lane == 0) sm[wid] = v;
+    __syncthreads();
+    uint64_t r = sm[0];
+    for (int w = 1; w < kBlock / kWave; w++) r = sm[w] < r ? sm[w] : r;
+    __syncthreads();
+    return r;
+}
+__device__ __forceinline__ uint64_t block_reduce_max(uint64_t v, uint64_t* sm) {
+    v = wave_max(v);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    if (lane == 0) sm[wid] = v;
+    __syncthreads();
+    uint64_t r = sm[0];
+    for (int w = 1; w < kBlock / kWave; w++) r = sm[w] > r ? sm[w] : r;
+    __syncthreads();
+    return r;
+}
+template <typename A> __device__ __forceinline__ A block_reduce_sum(A v, A* sm) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    if (lane == 0) sm[wid] = v;
+    __syncthreads();
+    A r = 0;
+    for (int w = 0; w < kBlock / kWave; w++) r += sm[w];
+    __syncthreads();
+    return r;
+}
+
+__device__ __forceinline__ bool last_block_arrives(uint32_t* counter, bool* sm_flag) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        *sm_flag = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    return *sm_flag;
+}
+
+__device__ __forceinline__ uint64_t rd(const uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t rd32(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------------------------ minmax
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size_t n, uint64_t* __restrict__ partials,
+                                                   uint32_t* counter, MinMaxResult* out) {
+    __shared__ uint64_t sm[kBlock / kWave];
+    __shared__ bool last;
+    uint64_t mn = ~0ull, mx = 0;
+    const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x, stride = (size_t)gridDim.x * kBlock;
+    constexpr int V = 16 / sizeof(T);
+    size_t done = 0;
+    if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+        const u32x4* xv = reinterpret_cast<const u32x4*>(x);
+        const size_t nv = n / V;
+        for (size_t i = tid; i < nv; i += stride) {
+            u32x4 v = __builtin_nontemporal_load(xv + i);
+            T vals[V];
+            __builtin_memcpy(vals, &v, 16);
+#pragma unroll
+            for (int k = 0; k < V; k++) {
+                uint64_t key = order_key(vals[k]);
+                mn = key < mn ? key : mn;
+                mx = key > mx ? key : mx;
+            }
+        }
+        done = nv * V;
+    }
+    for (size_t i = done + tid; i < n; i += stride) {
+        uint64_t key = order_key(x[i]);
+        mn = key < mn ? key : mn;
+        mx = key > mx ? key : mx;
+    }
+    mn = block_reduce_min(mn, sm);
+    mx = block_reduce_max(mx, sm);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = mn;
+        partials[2 * blockIdx.x + 1] = mx;
+    }
+    if (!last_block_arrives(counter, &last)) return;
+    mn = ~0ull;
+    mx = 0;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+        uint64_t a = rd(partials + 2 * b), z = rd(partials + 2 * b + 1);
+        mn = a < mn ? a : mn;
+        mx = z > mx ? z : mx;
+    }
+    mn = block_reduce_min(mn, sm);
+    mx = block_reduce_max(mx, sm);
+    if (threadIdx.x == 0) {
+        out->min_key = mn;
+        out->max_key = mx;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ------------------------------------------------------------------- stats
+// _getFloatPrecision (reference util/statistics/cuda_array_statistics.cu:37-44)
+// as the loop count k of round(x*10^k)/10^k != x in float arithmetic.
+__device__ __forceinline__ uint32_t float_precision(float number) {
+    float e = 1.0f;
+    uint32_t k = 0;
+    while (k < 10) {
+        float r = roundf(number * e);
+        if (__fdiv_rn(r, e) == number) break;
+        e *= 10.0f;
+        k++;
+    }
+    return k;
+}
+
+template <typename T> __device__ __forceinline__ int32_t to_int32_sat(T v) {
+    if constexpr (std::is_floating_point<T>::value) {
+        double d = (double)v;
+        if (d != d) return 0;
+        if (d >= 2147483647.0) return 2147483647;
+        if (d <= -2147483648.0) return (-2147483647 - 1);
+        return (int32_t)d;
+    } else {
+        return (int32_t)(int64_t)v;
+    }
+}
+
+template <typename T> __device__ __forceinline__ bool rl_equal(T a, T b) {
+    // _rlMetricKernel (cuda_array_statistics.cu:97-115): int value = data[idx]; value == data[idx+1]
+    int32_t v = to_int32_sat(a);
+    if constexpr (std::is_floating_point<T>::value)
+        return (T)v == b;
+    else
+        return (int64_t)v == (int64_t)b;
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+template <typename T> __device__ __forceinline__ uint64_t zigzag_bits(T v) {
+    if constexpr (std::is_floating_point<T>::value) {
+        return ubits(v);
+    } else {
+        int64_t s = (int64_t)v;
+        return ((uint64_t)s << 1) ^ (uint64_t)(s >> 63);
+    }
+}
+__device__ __forceinline__ uint32_t bitwidth64(uint64_t v) { return v ? 64 - __clzll(v) : 0; }
+
+struct StatsAcc {
+    uint64_t mn, mx, unsorted, rl_eq, runs, sum_bits;
+    double sum_f;
+    uint32_t prec, dmax;
+};
+
+template <typename T>
+__device__ __forceinline__ void stats_elem(StatsAcc& a, T v, T next, bool has_next, bool has_next2, uint32_t* sm_hist,
+                                           uint32_t* sm_hll) {
+    uint64_t key = order_key(v);
+    a.mn = key < a.mn ? key : a.mn;
+    a.mx = key > a.mx ? key : a.mx;
+    if constexpr (std::is_floating_point<T>::value) {
+        a.sum_f += (double)v;
+        uint32_t p = float_precision((float)v);
+        a.prec = p > a.prec ? p : a.prec;
+    } else {
+        a.sum_bits += (uint64_t)(int64_t)v;
+    }
+    if (has_next) {
+        if (v > next) a.unsorted = 1;
+        if (!(next == v)) a.runs++;
+        if constexpr (!std::is_floating_point<T>::value) {
+            constexpr int W = 8 * sizeof(T);
+            const uint64_t d = ubits(next) - ubits(v);
+            const int64_t sd = (int64_t)(d << (64 - W)) >> (64 - W);  // wrap to W bits, sign-extend
+            const uint64_t z = ((uint64_t)sd << 1) ^ (uint64_t)(sd >> 63);
+            const uint32_t bw = bitwidth64(z);
+            a.dmax = bw > a.dmax ? bw : a.dmax;
+        }
+    }
+    if (has_next2 && rl_equal(v, next)) a.rl_eq++;
+    atomicAdd(&sm_hist[bitwidth64(zigzag_bits(v))], 1u);
+    const uint64_t h = mix64(ubits(v) + 0x9e3779b97f4a7c15ull);
+    const uint32_t reg = (uint32_t)(h >> 58);
+    const uint32_t rank = (uint32_t)__clzll((h << 6) | (1ull << 5)) + 1;
+    atomicMax(&sm_hll[reg], rank);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_stats(const T* __restrict__ x, size_t n, DevStats* __restrict__ partials,
+                                                  uint32_t* counter, DevStats* out) {
+    __shared__ uint64_t sm[kBlock / kWave];
+    __shared__ double smd[kBlock / kWave];
+    __shared__ uint32_t sm_hist[65];
+    __shared__ uint32_t sm_hll[64];
+    __shared__ uint32_t fin_hist[65];
+    __shared__ uint32_t fin_hll[64];
+    __shared__ bool last;
+    for (int i = threadIdx.x; i < 65; i += kBlock) sm_hist[i] = 0;
+    for (int i = threadIdx.x; i < 64; i += kBlock) sm_hll[i] = 0;
+    __syncthreads();
+    StatsAcc a{~0ull, 0, 0, 0, 0, 0, 0.0, 0, 0};
+    const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x, stride = (size_t)gridDim.x * kBlock;
+    constexpr int V = 16 / sizeof(T);
+    size_t done = 0;
+    if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+        const u32x4* xv = reinterpret_cast<const u32x4*>(x);
+        const size_t nv = n / V;
+        for (size_t i = tid; i < nv; i += stride) {
+            u32x4 v = __builtin_nontemporal_load(xv + i);
+            T vals[V + 1];
+            __builtin_memcpy(vals, &v, 16);
+            const size_t base = i * V;
+            vals[V] = (base + V < n) ? x[base + V] : T(0);
+#pragma unroll
+            for (int k = 0; k < V; k++)
+                stats_elem(a, vals[k], vals[k + 1], base + k + 1 < n, base + k + 2 < n, sm_hist, sm_hll);
+        }
+        done = nv * V;
+    }
+    for (size_t i = done + tid; i < n; i += stride) {
+        T v = x[i];
+        T nx = (i + 1 < n) ? x[i + 1] : T(0);
+        stats_elem(a, v, nx, i + 1 < n, i + 2 < n, sm_hist, sm_hll);
+    }
+    uint64_t mn = block_reduce_min(a.mn, sm);
+    uint64_t mx = block_reduce_max(a.mx, sm);
+    uint64_t uns = block_reduce_max(a.unsorted, sm);
+    uint64_t rl = block_reduce_sum<uint64_t>(a.rl_eq, sm);
+    uint64_t runs = block_reduce_sum<uint64_t>(a.runs, sm);
+    uint64_t sb = block_reduce_sum<uint64_t>(a.sum_bits, sm);
+    double sf = block_reduce_sum<double>(a.sum_f, smd);
+    uint64_t pr = block_reduce_max((uint64_t)a.prec, sm);
+    uint64_t dm = block_reduce_max((uint64_t)a.dmax, sm);
+    DevStats* p = partials + blockIdx.x;
+    if (threadIdx.x == 0) {
+        p->min_key = mn;
+        p->max_key = mx;
+        p->unsorted = uns;
+        p->rl_eq = rl;
+        p->runs = runs;
+        p->sum_bits = sb;
+        p->sum_f = sf;
+        p->precision = (uint32_t)pr;
+        p->delta_bits_max = (uint32_t)dm;
+    }
+    for (int i = threadIdx.x; i < 65; i += kBlock) p->bit_hist[i] = sm_hist[i];
+    for (int i = threadIdx.x; i < 64; i += kBlock) p->hll[i] = sm_hll[i];
+    __threadfence();
+    if (!last_block_arrives(counter, &last)) return;
+    StatsAcc f{~0ull, 0, 0, 0, 0, 0, 0.0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+        const DevStats* q = partials + b;
+        const uint64_t v0 = rd(&q->min_key), v1 = rd(&q->max_key);
+        f.mn = v0 < f.mn ? v0 : f.mn;
+        f.mx = v1 > f.mx ? v1 : f.mx;
+        f.unsorted |= rd(&q->unsorted);
+        f.rl_eq += rd(&q->rl_eq);
+        f.runs += rd(&q->runs);
+        f.sum_bits += rd(&q->sum_bits);
+        const uint64_t sfb = rd(reinterpret_cast<const uint64_t*>(&q->sum_f));
+        double sfd;
+        __builtin_memcpy(&sfd, &sfb, 8);
+        f.sum_f += sfd;
+        const uint32_t pp = rd32(&q->precision), dd = rd32(&q->delta_bits_max);
+        f.prec = pp > f.prec ? pp : f.prec;
+        f.dmax = dd > f.dmax ? dd : f.dmax;
+    }
+    mn = block_reduce_min(f.mn, sm);
+    mx = block_reduce_max(f.mx, sm);
+    uns = block_reduce_max(f.unsorted, sm);
+    rl = block_reduce_sum<uint64_t>(f.rl_eq, sm);
+    runs = block_reduce_sum<uint64_t>(f.runs, sm);
+    sb = block_reduce_sum<uint64_t>(f.sum_bits, sm);
+    sf = block_reduce_sum<double>(f.sum_f, smd);
+    pr = block_reduce_max((uint64_t)f.prec, sm);
+    dm = block_reduce_max((uint64_t)f.dmax, sm);
+    for (int i = threadIdx.x; i < 65; i += kBlock) fin_hist[i] = 0;
+    for (int i = threadIdx.x; i < 64; i += kBlock) fin_hll[i] = 0;
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+        const DevStats* q = partials + b;
+        for (int e = 0; e < 65; e++) atomicAdd(&fin_hist[e], rd32(&q->bit_hist[e]));
+        for (int e = 0; e < 64; e++) atomicMax(&fin_hll[e], rd32(&q->hll[e]));
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 65; i += kBlock) out->bit_hist[i] = fin_hist[i];
+    for (int i = threadIdx.x; i < 64; i += kBlock) out->hll[i] = fin_hll[i];
+    if (threadIdx.x == 0) {
+        out->min_key = mn;
+        out->max_key = mx;
+        out->unsorted = uns;
+        out->rl_eq = rl;
+        out->runs = runs + (n > 0 ? 1 : 0);
+        out->sum_bits = sb;
+        out->sum_f = sf;
+        out->precision = (uint32_t)pr;
+        out->delta_bits_max = (uint32_t)dm;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+int reduce_grid(Ctx* c, size_t n, int elem) {
+    const size_t per_block = (size_t)kBlock * (16 / elem) * 4;
+    const size_t g = (n + per_block - 1) / per_block;
+    const size_t cap = std::min<size_t>(kMaxReduceBlocks, (size_t)c->cu_count * 4);
+    return (int)std::max<size_t>(1, std::min(g, cap));
+}
+
+}  // namespace
+
+void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_result) {
+    if (n == 0) fail(DDJ_EINVAL, "minmax of an empty array");
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        const int grid = reduce_grid(c, n, sizeof(T));
+        DevBuf partials(c, 16 * (size_t)grid);
+        hipLaunchKernelGGL(k_minmax<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
+                           partials.get<uint64_t>(), &c->control->counters[0], d_result);
+        HIPC(hipGetLastError());
+    });
+}
+
+void decode_minmax(int dtype, const MinMaxResult& r, void* mn, void* mx) {
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        T a = from_order_key<T>(r.min_key), b = from_order_key<T>(r.max_key);
+        memcpy(mn, &a, sizeof(T));
+        memcpy(mx, &b, sizeof(T));
+    });
+}
+
+void launch_stats(Ctx* c, int dtype, const void* x, size_t n, DevStats* d_out) {
+    if (n == 0) fail(DDJ_EINVAL, "stats of an empty array");
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        const int grid = std::min(reduce_grid(c, n, sizeof(T)), 2 * c->cu_count);
+        DevBuf partials(c, sizeof(DevStats) * (size_t)grid);
+        hipLaunchKernelGGL(k_stats<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
+                           partials.get<DevStats>(), &c->control->counters[1], d_out);
+        HIPC(hipGetLastError());
+    });
+}
+
+}  // namespace ddj

@@ -1,0 +1,477 @@
+// Variable-length outputs with single-pass decoupled look-back scans:
+// stable split by a predicate with the packed stencil (patch / const /
+// dict), its inverse merge, RLE encode/decode, dict code packing.
+//
+// Tiles hold 4096 rows; thread t of a workgroup owns the 16 consecutive rows
+// [16t, 16t+16) of the tile, i.e. exactly two stencil bytes.  Inputs are
+// staged through LDS with striped (coalesced) loads, outputs are compacted in
+// LDS and written back striped.
+#include "dispatch.h"
+#include "k_common.h"
+#include "kernels.h"
+
+namespace ddj {
+
+namespace {
+
+constexpr int kItems = 16;
+constexpr int kTile = kBlock * kItems;
+
+struct LB {
+    DevBuf flags, agg, inc;
+    LookbackState<uint64_t> st;
+    LB(Ctx* c, size_t tiles) : flags(c, 4 * tiles + 4), agg(c, 8 * tiles), inc(c, 8 * tiles) {
+        launch_fill_zero(c, flags.get(), 4 * tiles + 4);
+        st = LookbackState<uint64_t>{flags.get<uint32_t>(), agg.get<uint64_t>(), inc.get<uint64_t>(),
+                                     flags.get<uint32_t>() + tiles, &c->control->error_flag};
+    }
+};
+
+template <typename T> struct Pred {
+    int kind;
+    T a, b;
+    T set[4];
+    int set_n;
+    __device__ __forceinline__ bool operator()(T v, uint32_t& code) const {
+        code = 0;
+        switch (kind) {
+        case PRED_LOWER: return v < a;                // LowerOperator (util/stencil/stencil_operators.hpp)
+        case PRED_INSIDE: return !(v > b || v < a);   // OutsideOperator: true inside [low, high]
+        case PRED_NOT_EQUAL: return v != a;           // _constEncodeStencilKernel (const_encoding.cu:19-25)
+        default: {                                    // getMostFrequentStencilKernel (dict_encoding.cu:15-34)
+            bool r = false;
+            for (int j = 0; j < set_n; j++)
+                if (v == set[j]) {
+                    r = true;
+                    code |= (uint32_t)j;              // _compressUniqueKernel ORs every matching index
+                }
+            return r;
+        }
+        }
+    }
+};
+
+template <typename T> T bits_to(uint64_t b) {
+    T v;
+    using U = typename UnsignedOf<T>::type;
+    U u = (U)b;
+    memcpy(&v, &u, sizeof(T));
+    return v;
+}
+
+template <typename T, bool CODES, bool WRITE_NO>
+__global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_t n, Pred<T> p,
+                                                  uint8_t* __restrict__ stencil, T* __restrict__ yes,
+                                                  T* __restrict__ no, uint8_t* __restrict__ codes,
+                                                  LookbackState<uint64_t> st, uint64_t* d_count) {
+    __shared__ T buf[kTile];
+    __shared__ uint8_t cbuf[CODES ? kTile : 1];
+    __shared__ uint64_t red[kBlock / kWave];
+    __shared__ uint32_t tk;
+    __shared__ uint64_t s_prefix;
+    const uint32_t tile = take_ticket(st, &tk);
+    const size_t base = (size_t)tile * kTile;
+    const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
+    const T padv = x[n - 1];  // stencil padding repeats the last row (reference stencil.cu:26-27)
+    for (int i = threadIdx.x; i < kTile; i += kBlock) buf[i] = i < cnt ? x[base + i] : padv;
+    __syncthreads();
+    T v[kItems];
+    uint32_t mask = 0;
+    uint64_t cpack = 0;
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        v[k] = buf[threadIdx.x * kItems + k];
+        uint32_t code;
+        bool f = p(v[k], code);
+        mask |= (uint32_t)f << k;
+        cpack |= (uint64_t)code << (4 * k);
+        if (threadIdx.x * kItems + k < cnt) c += f;
+    }
+    const size_t g0 = base + (size_t)threadIdx.x * kItems;
+    if (g0 < n) stencil[1 + g0 / 8] = (uint8_t)(mask & 0xff);
+    if (g0 + 8 < n) stencil[2 + g0 / 8] = (uint8_t)(mask >> 8);
+    if (tile == 0 && threadIdx.x == 0) stencil[0] = (uint8_t)(n % 8);
+    uint64_t total;
+    const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)c, &total, red);
+    if (threadIdx.x == 0) s_prefix = lookback_prefix<uint64_t>(st, tile, total);
+    __syncthreads();
+    const uint64_t Y0 = s_prefix;
+    int yr = (int)excl, nr = threadIdx.x * kItems - (int)excl;
+    const int nbase = CODES ? 0 : (int)total;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        if (threadIdx.x * kItems + k >= cnt) break;
+        if ((mask >> k) & 1) {
+            if (CODES)
+                cbuf[yr] = (uint8_t)((cpack >> (4 * k)) & 15);
+            else
+                buf[yr] = v[k];
+            yr++;
+        } else {
+            buf[nbase + nr] = v[k];
+            nr++;
+        }
+    }
+    __syncthreads();
+    if (CODES || yes)
+        for (int i = threadIdx.x; i < (int)total; i += kBlock) {
+            if (CODES)
+                codes[Y0 + i] = cbuf[i];
+            else
+                yes[Y0 + i] = buf[i];
+        }
+    if (WRITE_NO) {
+        const size_t N0 = base - Y0;
+        for (int i = threadIdx.x; i < cnt - (int)total; i += kBlock) no[N0 + i] = buf[nbase + i];
+    }
+    if (threadIdx.x == 0 && base + cnt == n) d_count[0] = Y0 + total;
+}
+
+// Splitter::Merge (reference util/splitter/splitter.cu:140-184) driven by a
+// packed stencil: row i takes the next `yes` value when bit i is set.
+template <typename T, bool CONST_NO>
+__global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bits, size_t n, const T* __restrict__ yes,
+                                                  const T* __restrict__ no, T no_const, T* __restrict__ out,
+                                                  LookbackState<uint64_t> st) {
+    __shared__ T buf[kTile];
+    __shared__ uint64_t red[kBlock / kWave];
+    __shared__ uint32_t tk;
+    __shared__ uint64_t s_prefix;
+    const uint32_t tile = take_ticket(st, &tk);
+    const size_t base = (size_t)tile * kTile;
+    const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
+    const size_t g0 = base + (size_t)threadIdx.x * kItems;
+    uint32_t mask = 0;
+    if (g0 < n) mask = bits[g0 / 8];
+    if (g0 + 8 < n) mask |= (uint32_t)bits[g0 / 8 + 1] << 8;
+    const int mine = cnt - threadIdx.x * kItems;
+    if (mine <= 0)
+        mask = 0;
+    else if (mine < kItems)
+        mask &= (1u << mine) - 1u;
+    uint64_t total;
+    const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &total, red);
+    if (threadIdx.x == 0) s_prefix = lookback_prefix<uint64_t>(st, tile, total);
+    __syncthreads();
+    const uint64_t Y0 = s_prefix;
+    const size_t N0 = base - Y0;
+    for (int i = threadIdx.x; i < cnt; i += kBlock) {
+        if (i < (int)total)
+            buf[i] = yes[Y0 + i];
+        else
+            buf[i] = CONST_NO ? no_const : no[N0 + (i - (int)total)];
+    }
+    __syncthreads();
+    T v[kItems];
+    int yr = (int)excl, nr = (int)total + threadIdx.x * kItems - (int)excl;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        if (threadIdx.x * kItems + k < cnt) v[k] = ((mask >> k) & 1) ? buf[yr++] : buf[nr++];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kItems; k++)
+        if (threadIdx.x * kItems + k < cnt) buf[threadIdx.x * kItems + k] = v[k];
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += kBlock) out[base + i] = buf[i];
+}
+
+// _compressUniqueKernel word layout (reference compression/unique/unique_encoding.cu:48-76):
+// code j of the m coded rows sits at bits [(j % P) * b, ...) of word j / P.
+__global__ __launch_bounds__(kBlock) void k_pack_codes(const uint8_t* __restrict__ codes, size_t m, int b,
+                                                       uint32_t* __restrict__ words) {
+    const int P = 32 / b;
+    const size_t nw = (m + P - 1) / P;
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t w = (size_t)blockIdx.x * kBlock + threadIdx.x; w < nw; w += stride) {
+        uint32_t r = 0;
+        for (int i = 0; i < P; i++) {
+            size_t j = w * P + i;
+            if (j < m) r |= (uint32_t)codes[j] << (b * i);
+        }
+        words[w] = r;
+    }
+}
+
+// _decompressUniqueKernel (unique_encoding.cu:183-204); words and the
+// dictionary may sit at any byte offset.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_unpack_codes(const uint8_t* __restrict__ words, size_t m, int b,
+                                                         const uint8_t* __restrict__ dict, T* __restrict__ out) {
+    const int P = 32 / b;
+    const uint32_t mask = b >= 32 ? ~0u : ((1u << b) - 1u);
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t j = (size_t)blockIdx.x * kBlock + threadIdx.x; j < m; j += stride) {
+        uint32_t w;
+        __builtin_memcpy(&w, words + 4 * (j / P), 4);
+        uint32_t code = (w >> (b * (j % P))) & mask;
+        T v;
+        __builtin_memcpy(&v, dict + sizeof(T) * code, sizeof(T));
+        out[j] = v;
+    }
+}
+
+// RleEncoding (reference compression/rle/rle_encode.cu:18-62): a run starts
+// at row 0 and wherever !(x[i] == x[i-1]); the run's value is its first row.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rle_encode(const T* __restrict__ x, size_t n, T* __restrict__ values,
+                                                       uint32_t* __restrict__ starts, LookbackState<uint64_t> st,
+                                                       uint64_t* d_runs) {
+    __shared__ T buf[kTile + 1];
+    __shared__ uint64_t red[kBlock / kWave];
+    __shared__ uint32_t tk;
+    __shared__ uint64_t s_prefix;
+    const uint32_t tile = take_ticket(st, &tk);
+    const size_t base = (size_t)tile * kTile;
+    const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
+    for (int i = threadIdx.x; i < kTile + 1; i += kBlock) {
+        const long long g = (long long)base + i - 1;
+        buf[i] = (g >= 0 && i <= cnt) ? x[g] : T(0);
+    }
+    __syncthreads();
+    uint32_t mask = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        const int idx = threadIdx.x * kItems + k;
+        if (idx < cnt) {
+            const bool f = (base + idx == 0) || !(buf[idx + 1] == buf[idx]);
+            mask |= (uint32_t)f << k;
+        }
+    }
+    uint64_t total;
+    const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &total, red);
+    if (threadIdx.x == 0) s_prefix = lookback_prefix<uint64_t>(st, tile, total);
+    __syncthreads();
+    uint64_t r = s_prefix + excl;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        if ((mask >> k) & 1) {
+            const int idx = threadIdx.x * kItems + k;
+            values[r] = buf[idx + 1];
+            starts[r] = (uint32_t)(base + idx);
+            r++;
+        }
+    }
+    if (threadIdx.x == 0 && base + cnt == n) d_runs[0] = s_prefix + total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rle_lengths(const uint32_t* __restrict__ starts, size_t runs, size_t n,
+                                                        int32_t* __restrict__ lengths) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t r = (size_t)blockIdx.x * kBlock + threadIdx.x; r < runs; r += stride) {
+        const uint64_t e = r + 1 < runs ? starts[r + 1] : n;
+        lengths[r] = (int32_t)(e - starts[r]);
+    }
+}
+
+// Exclusive scan of int32 run lengths into uint64 offsets[R+1].
+__global__ __launch_bounds__(kBlock) void k_scan_lengths(const uint8_t* __restrict__ lens, size_t R,
+                                                         uint64_t* __restrict__ offsets, LookbackState<uint64_t> st) {
+    __shared__ uint64_t buf[kTile + kTile / 32];
+    __shared__ uint64_t red[kBlock / kWave];
+    __shared__ uint32_t tk;
+    __shared__ uint64_t s_prefix;
+    auto pad = [](int i) { return i + (i >> 5); };
+    const uint32_t tile = take_ticket(st, &tk);
+    const size_t base = (size_t)tile * kTile;
+    const int cnt = (int)((R - base) < (size_t)kTile ? (R - base) : (size_t)kTile);
+    const bool aligned = (reinterpret_cast<uintptr_t>(lens) & 3) == 0;
+    for (int i = threadIdx.x; i < kTile; i += kBlock) {
+        int32_t l = 0;
+        if (i < cnt) {
+            if (aligned)
+                l = reinterpret_cast<const int32_t*>(lens)[base + i];
+            else
+                __builtin_memcpy(&l, lens + 4 * (base + i), 4);
+        }
+        buf[pad(i)] = (uint64_t)(int64_t)l;
+    }
+    __syncthreads();
+    uint64_t loc[kItems], s = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        loc[k] = buf[pad(threadIdx.x * kItems + k)];
+        s += loc[k];
+    }
+    uint64_t total;
+    const uint64_t excl = block_exclusive_scan<uint64_t>(s, &total, red);
+    if (threadIdx.x == 0) s_prefix = lookback_prefix<uint64_t>(st, tile, total);
+    __syncthreads();
+    uint64_t run = s_prefix + excl;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        buf[pad(threadIdx.x * kItems + k)] = run;
+        run += loc[k];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < cnt; i += kBlock) offsets[base + i] = buf[pad(i)];
+    if (threadIdx.x == 0 && base + cnt == R) offsets[R] = s_prefix + total;
+}
+
+// Run expansion: a workgroup owns 4096 output rows, finds the first run
+// overlapping them by binary search, stages the overlapping runs in LDS and
+// gives every row its run by a binary search in LDS.
+constexpr int kExpTile = 2048;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rle_expand(const uint64_t* __restrict__ offsets, size_t R,
+                                                       const uint8_t* __restrict__ values, size_t total,
+                                                       T* __restrict__ out) {
+    __shared__ uint64_t soff[kExpTile + 2];
+    __shared__ T sval[kExpTile + 1];
+    __shared__ size_t s_r0;
+    __shared__ int s_nr;
+    const size_t o0 = (size_t)blockIdx.x * kExpTile;
+    const int cnt = (int)((total - o0) < (size_t)kExpTile ? (total - o0) : (size_t)kExpTile);
+    if (threadIdx.x == 0) {
+        // last r with offsets[r] <= o0
+        size_t lo = 0, hi = R;  // offsets[R] = total > o0
+        while (hi - lo > 1) {
+            size_t mid = (lo + hi) / 2;
+            if (offsets[mid] <= o0)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        s_r0 = lo;
+    }
+    __syncthreads();
+    const size_t r0 = s_r0;
+    size_t avail = R - r0;  // runs r0 .. R-1
+    int nr = (int)(avail < (size_t)(kExpTile + 1) ? avail : (size_t)(kExpTile + 1));
+    for (int i = threadIdx.x; i <= nr; i += kBlock) soff[i] = offsets[r0 + i];
+    for (int i = threadIdx.x; i < nr; i += kBlock) __builtin_memcpy(&sval[i], values + sizeof(T) * (r0 + i), sizeof(T));
+    __syncthreads();
+    for (int k = threadIdx.x; k < cnt; k += kBlock) {
+        const uint64_t pos = o0 + k;
+        int lo = 0, hi = nr;  // soff[lo] <= pos < soff[hi]
+        while (hi - lo > 1) {
+            int mid = (lo + hi) / 2;
+            if (soff[mid] <= pos)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        out[pos] = sval[lo];
+    }
+}
+
+inline unsigned grid_for(Ctx* c, size_t items) {
+    size_t g = (items + kBlock - 1) / kBlock;
+    size_t cap = (size_t)c->cu_count * 32;
+    if (g > cap) g = cap;
+    return (unsigned)(g ? g : 1);
+}
+
+}  // namespace
+
+void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa, uint8_t* stencil, void* yes,
+                  void* no, uint8_t* codes, uint64_t* d_counts) {
+    if (n == 0) return;
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        Pred<T> p;
+        p.kind = pa.kind;
+        p.a = bits_to<T>(pa.a_bits);
+        p.b = bits_to<T>(pa.b_bits);
+        for (int j = 0; j < 4; j++) p.set[j] = bits_to<T>(pa.set_bits[j]);
+        p.set_n = pa.set_n;
+        const size_t tiles = (n + kTile - 1) / kTile;
+        LB lb(c, tiles);
+        auto xs = static_cast<const T*>(x);
+        auto ys = static_cast<T*>(yes);
+        auto ns = static_cast<T*>(no);
+        if (codes) {
+            hipLaunchKernelGGL((k_split<T, true, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p,
+                               stencil, ys, ns, codes, lb.st, d_counts);
+        } else if (no) {
+            hipLaunchKernelGGL((k_split<T, false, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p,
+                               stencil, ys, ns, codes, lb.st, d_counts);
+        } else {
+            hipLaunchKernelGGL((k_split<T, false, false>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n,
+                               p, stencil, ys, ns, codes, lb.st, d_counts);
+        }
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_merge(Ctx* c, int dtype, const uint8_t* bits, size_t n, const uint8_t* yes, const uint8_t* no,
+                  uint64_t no_const, void* out) {
+    if (n == 0) return;
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        const size_t tiles = (n + kTile - 1) / kTile;
+        LB lb(c, tiles);
+        auto ys = reinterpret_cast<const T*>(yes);
+        auto ns = reinterpret_cast<const T*>(no);
+        if (no)
+            hipLaunchKernelGGL((k_merge<T, false>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, bits, n, ys,
+                               ns, T(0), static_cast<T*>(out), lb.st);
+        else
+            hipLaunchKernelGGL((k_merge<T, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, bits, n, ys,
+                               ns, bits_to<T>(no_const), static_cast<T*>(out), lb.st);
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_pack_codes(Ctx* c, const uint8_t* codes, size_t m, int bits, uint32_t* words) {
+    if (!m) return;
+    const size_t nw = (m + (32 / bits) - 1) / (32 / bits);
+    hipLaunchKernelGGL(k_pack_codes, dim3(grid_for(c, nw)), dim3(kBlock), 0, c->stream, codes, m, bits, words);
+    HIPC(hipGetLastError());
+}
+
+void launch_unpack_codes(Ctx* c, int dtype, const uint8_t* words, size_t m, int bits, const uint8_t* dict, void* out) {
+    if (!m) return;
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        hipLaunchKernelGGL(k_unpack_codes<T>, dim3(grid_for(c, m)), dim3(kBlock), 0, c->stream, words, m, bits, dict,
+                           static_cast<T*>(out));
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_rle_encode(Ctx* c, int dtype, const void* x, size_t n, int32_t* lengths, void* values, uint64_t* d_runs) {
+    if (n == 0) return;
+    if (n >= (1ull << 31)) fail(DDJ_EUNSUPPORTED, "RLE run lengths are int32 (reference rle_encoding.hpp:35)");
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        const size_t tiles = (n + kTile - 1) / kTile;
+        LB lb(c, tiles);
+        DevBuf starts(c, 4 * n);
+        hipLaunchKernelGGL(k_rle_encode<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                           static_cast<const T*>(x), n, static_cast<T*>(values), starts.get<uint32_t>(), lb.st, d_runs);
+        HIPC(hipGetLastError());
+        uint64_t runs = 0;
+        d2h(c, &runs, d_runs, 8);
+        hipLaunchKernelGGL(k_rle_lengths, dim3(grid_for(c, runs)), dim3(kBlock), 0, c->stream, starts.get<uint32_t>(),
+                           runs, n, lengths);
+        HIPC(hipGetLastError());
+    });
+}
+
+DevBuf rle_decode_dev(Ctx* c, int dtype, const uint8_t* lengths, const uint8_t* values, size_t runs) {
+    // rle_decode.cu:46-67: scan of the lengths sizes the output (one read-back), then runs expand.
+    if (runs == 0) return DevBuf();
+    return dispatch_native(dtype, [&](auto tag) -> DevBuf {
+        using T = decltype(tag);
+        const size_t tiles = (runs + kTile - 1) / kTile;
+        LB lb(c, tiles);
+        DevBuf offsets(c, 8 * (runs + 1));
+        hipLaunchKernelGGL(k_scan_lengths, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, lengths, runs,
+                           offsets.get<uint64_t>(), lb.st);
+        HIPC(hipGetLastError());
+        uint64_t total = 0;
+        d2h(c, &total, offsets.get<uint64_t>() + runs, 8);
+        DevBuf out(c, total * sizeof(T));
+        if (total == 0) return out;
+        const size_t etiles = (total + kExpTile - 1) / kExpTile;
+        hipLaunchKernelGGL(k_rle_expand<T>, dim3((unsigned)etiles), dim3(kBlock), 0, c->stream,
+                           offsets.get<uint64_t>(), runs, values, (size_t)total, out.get<T>());
+        HIPC(hipGetLastError());
+        return out;
+    });
+}
+
+}  // namespace ddj

@@ -1,0 +1,95 @@
+// Host-side launchers of the HIP kernels.  Every launcher enqueues work on
+// ctx->stream and returns without synchronising unless stated otherwise.
+#pragma once
+
+#include "ddj_internal.h"
+
+namespace ddj {
+
+// Scratch of a single-pass reduction (per-block partials + result).
+struct MinMaxResult {
+    uint64_t min_key;  // order_key of the minimum
+    uint64_t max_key;  // order_key of the maximum
+};
+
+// min/max of n values of `dtype` into d_result (device).  One launch.
+void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_result);
+// Converts order keys back to the typed min/max (host).
+void decode_minmax(int dtype, const MinMaxResult& r, void* mn, void* mx);
+
+// Device statistics record filled by launch_stats.
+struct DevStats {
+    uint64_t min_key, max_key;
+    uint64_t unsorted;     // any x[i] > x[i+1]
+    uint64_t rl_eq;        // #{i < n-2 : int(x[i]) == x[i+1]}
+    uint64_t runs;         // #{i : i == 0 || x[i] != x[i-1]}
+    uint64_t sum_bits;     // integer types: wrapping sum
+    double sum_f;          // float types: sum (order not pinned)
+    uint32_t precision;    // max _getFloatPrecision
+    uint32_t delta_bits_max;
+    uint32_t bit_hist[65];
+    uint32_t hll[64];
+};
+void launch_stats(Ctx* c, int dtype, const void* x, size_t n, DevStats* d_out);
+
+// FL pack: bits from `mm` (device min/max) when bits == 0, otherwise fixed.
+// Writes ceil(n*b/W)*sizeof(T) bytes rounded up to 4 into out (32-bit words).
+void launch_fl_pack(Ctx* c, int dtype, const void* x, size_t n, const MinMaxResult* mm, int bits, void* out);
+// FL unpack of n values of b bits from an arbitrary byte address.
+void launch_fl_unpack(Ctx* c, int dtype, const uint8_t* in, size_t in_bytes, size_t n, int bits, void* out);
+
+// Elementwise kernels.
+void launch_delta_encode(Ctx* c, int elem_size, const void* x, size_t n, void* out /* n-1 */);
+void launch_delta_decode(Ctx* c, int elem_size, const void* first /* device, 1 elem */, const uint8_t* d,
+                         size_t m /* deltas */, void* out /* m+1 */);
+void launch_add_scalar(Ctx* c, int elem_size, const void* x, size_t n, uint64_t s, void* out);  // wrapping
+void launch_sub_scalar(Ctx* c, int elem_size, const void* x, size_t n, uint64_t s, void* out);
+void launch_float_split(Ctx* c, const float* x, size_t n, int32_t* mant, int32_t* expo, int32_t* sign);
+void launch_float_compose(Ctx* c, const int32_t* mant, const int32_t* expo, const uint8_t* stencil_bits,
+                          int sign_const, size_t n, float* out);
+void launch_precision(Ctx* c, int dtype, const void* x, size_t n, uint32_t* d_max);
+void launch_float_to_int(Ctx* c, int dtype, const void* x, size_t n, int precision, int32_t* out);
+void launch_int_to_float(Ctx* c, int dtype, const uint8_t* x, size_t n, int precision, void* out);
+
+// Stable partition by a predicate (patch/const/dict) with the packed
+// stencil.  counts[0] = #yes (device).
+enum PredKind { PRED_LOWER = 0, PRED_INSIDE = 1, PRED_NOT_EQUAL = 2, PRED_IN_SET = 3 };
+struct PredArgs {
+    int kind;
+    uint64_t a_bits, b_bits;  // thresholds / constant, as T bit patterns
+    uint64_t set_bits[4];     // PRED_IN_SET keys
+    int set_n;
+};
+void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& p, uint8_t* stencil /* 1+ceil(n/8) */,
+                  void* yes, void* no, uint8_t* codes /* dict codes or null */, uint64_t* d_counts);
+// Merge: out[i] = bit i ? yes[k++] : no[j++] (no == null: `no_const`).
+void launch_merge(Ctx* c, int dtype, const uint8_t* stencil_bits /* packed bits, no header */, size_t n,
+                  const uint8_t* yes, const uint8_t* no, uint64_t no_const, void* out);
+// dict codes: pack bytes (2-bit codes) into uint32 words of 16 / unpack.
+void launch_pack_codes(Ctx* c, const uint8_t* codes, size_t m, int bits, uint32_t* words);
+void launch_unpack_codes(Ctx* c, int dtype, const uint8_t* words, size_t m, int bits, const uint8_t* dict, void* out);
+
+// RLE.
+void launch_rle_encode(Ctx* c, int dtype, const void* x, size_t n, int32_t* lengths, void* values, uint64_t* d_runs);
+DevBuf rle_decode_dev(Ctx* c, int dtype, const uint8_t* lengths, const uint8_t* values, size_t runs);
+
+// Top-K most frequent keys (count desc, key asc), K <= 4.
+void topk_most_frequent(Ctx* c, int dtype, const void* x, size_t n, int K, uint64_t* keys_bits_host,
+                        int* n_distinct_host);
+
+// GFC (float: 32-bit words, double: 64-bit words).
+void gfc_encode(Ctx* c, bool dbl, const void* x, size_t n, DevBuf& meta, DevBuf& offsets, DevBuf& payload);
+void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const uint8_t* offsets, size_t off_bytes,
+                const uint8_t* payload, size_t payload_bytes, DevBuf& out);
+
+// Zero-fill of device memory by a kernel (graph-capture safe).
+void launch_fill_zero(Ctx* c, void* p, size_t bytes);
+// Gather of several device ranges into one buffer (tree blob assembly).
+struct CopyRange {
+    const uint8_t* src;
+    uint64_t dst_off;
+    uint64_t bytes;
+};
+void launch_gather_copy(Ctx* c, const CopyRange* d_ranges, int n_ranges, uint8_t* dst, uint64_t max_bytes);
+
+}  // namespace ddj

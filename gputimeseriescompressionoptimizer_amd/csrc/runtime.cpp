@@ -1,0 +1,115 @@
+// Context, stream, caching allocator and copies.
+#include "ddj_internal.h"
+
+#include <algorithm>
+#include <cstdio>
+
+namespace ddj {
+
+[[noreturn]] void fail(ddj_status st, const std::string& msg) { throw Error{st, msg}; }
+
+void hip_check(hipError_t e, const char* what, const char* file, int line) {
+    if (e == hipSuccess) return;
+    char buf[512];
+    snprintf(buf, sizeof buf, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) fail(DDJ_ENOMEM, buf);
+    fail(DDJ_EHIP, buf);
+}
+
+size_t dtype_size(int dtype) {
+    switch (dtype) {
+    case DDJ_D_INT: return 4;
+    case DDJ_D_TIME: return 8;
+    case DDJ_D_FLOAT: return 4;
+    case DDJ_D_DOUBLE: return 8;
+    case DDJ_D_SHORT: return 2;
+    case DDJ_D_CHAR: return 1;
+    default: return 0;
+    }
+}
+
+// Size classes: powers of two up to 1 MiB, then multiples of 2 MiB.  Every
+// block also has 64 spare bytes so kernels may touch whole 16-byte vectors
+// past a logical end.
+size_t Allocator::round_size(size_t bytes) {
+    bytes += 64;
+    if (bytes <= (1u << 20)) {
+        size_t s = 256;
+        while (s < bytes) s <<= 1;
+        return s;
+    }
+    const size_t g = 2u << 20;
+    return (bytes + g - 1) / g * g;
+}
+
+void* Allocator::alloc(size_t bytes) {
+    size_t s = round_size(bytes);
+    auto it = free_.find(s);
+    if (it != free_.end() && !it->second.empty()) {
+        void* p = it->second.back();
+        it->second.pop_back();
+        live_[p] = s;
+        in_use_ += s;
+        return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        trim();  // give cached blocks back and retry once
+        e = hipMalloc(&p, s);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            fail(DDJ_ENOMEM, "device allocation of " + std::to_string(s) + " bytes failed");
+        }
+    }
+    live_[p] = s;
+    in_use_ += s;
+    return p;
+}
+
+void Allocator::free(void* p) {
+    if (!p) return;
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    free_[it->second].push_back(p);
+    in_use_ -= it->second;
+    live_.erase(it);
+}
+
+void Allocator::trim() {
+    (void)hipDeviceSynchronize();
+    for (auto& kv : free_)
+        for (void* p : kv.second) (void)hipFree(p);
+    free_.clear();
+}
+
+Allocator::~Allocator() {
+    trim();
+    for (auto& kv : live_) (void)hipFree(kv.first);
+}
+
+void d2h(Ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+}
+
+void h2d(Ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));  // the source may be a temporary
+}
+
+void d2d(Ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+}
+
+DevBuf make_buf(Ctx* c, const void* host, size_t bytes) {
+    DevBuf b(c, bytes);
+    h2d(c, b.get(), host, bytes);
+    return b;
+}
+
+}  // namespace ddj

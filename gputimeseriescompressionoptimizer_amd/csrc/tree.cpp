@@ -1,0 +1,241 @@
+// The compression cascade: node records, blob assembly, parse and decode.
+#include "tree.h"
+
+#include <functional>
+#include <sstream>
+
+namespace ddj {
+
+namespace {
+
+const char* enc_name(int e) {
+    switch (e) {
+    case DDJ_ENC_DELTA: return "delta";
+    case DDJ_ENC_AFL: return "afl";
+    case DDJ_ENC_SCALE: return "scale";
+    case DDJ_ENC_RLE: return "rle";
+    case DDJ_ENC_DICT: return "dict";
+    case DDJ_ENC_UNIQUE: return "unique";
+    case DDJ_ENC_PATCH: return "patch";
+    case DDJ_ENC_NONE: return "none";
+    case DDJ_ENC_FLOAT_TO_INT: return "floatToInt";
+    case DDJ_ENC_CONST: return "constData";
+    case DDJ_ENC_GFC: return "gfc";
+    case DDJ_ENC_OUTSIDE_PATCH: return "patch(outside)";
+    default: return "?";
+    }
+}
+
+// Encoding::GetCompressionRatio (compression/encoding.hpp:78-84)
+double ratio_of(size_t in, size_t out) {
+    if (out == 0 || out > in) return 1.0;
+    return (double)in / (double)out;
+}
+
+// One piece of a serialized blob: an 8-byte header (host) or device bytes.
+struct Piece {
+    bool is_header = false;
+    uint8_t hdr[8];
+    Part part;
+    size_t size() const { return is_header ? 8 : part.n; }
+};
+
+// CompressionNode::Compress (compression_node.cpp:103-138) and PrepareMetadata (:84-101)
+size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& out) {
+    std::vector<Part> res;
+    if (node.enc == DDJ_ENC_NONE) {
+        size_t len = data.n;  // the leaf refers to its input instead of copying it
+        res.push_back(own(make_buf(c, &len, sizeof len)));
+        res.push_back(data);
+    } else {
+        res = encode(c, node.enc, node.dtype, data.view());
+    }
+    Piece h;
+    h.is_header = true;
+    const int16_t e = (int16_t)node.enc, d = (int16_t)node.dtype;
+    const int32_t ml = (int32_t)res[0].n;
+    memcpy(h.hdr, &e, 2);
+    memcpy(h.hdr + 2, &d, 2);
+    memcpy(h.hdr + 4, &ml, 4);
+    size_t total = 8 + res[0].n;
+    out.push_back(h);
+    Piece m;
+    m.part = res[0];
+    out.push_back(m);
+    if (node.ch.empty()) {
+        Piece p;
+        p.part = res.size() > 1 ? res[1] : Part{};
+        total += p.part.n;
+        out.push_back(p);
+    } else {
+        for (size_t i = 0; i < node.ch.size(); i++) {
+            if (i + 1 >= res.size()) fail(DDJ_EINVAL, "tree node has more children than its encoding has outputs");
+            total += compress_node(c, *node.ch[i], res[i + 1], out);
+        }
+    }
+    node.ratio = ratio_of(data.n, total);
+    return total;
+}
+
+}  // namespace
+
+NodeP make_node(int enc, int dtype) {
+    auto n = std::make_shared<TNode>();
+    n->enc = enc;
+    n->dtype = dtype;
+    return n;
+}
+
+NodeP copy_tree(const NodeP& n) {
+    auto r = std::make_shared<TNode>();
+    r->enc = n->enc;
+    r->dtype = n->dtype;
+    r->ratio = n->ratio;
+    for (auto& c : n->ch) r->ch.push_back(copy_tree(c));
+    return r;
+}
+
+NodeP tree_from_path(const int* path, size_t len, int dtype) {
+    if (len == 0) fail(DDJ_EINVAL, "empty tree path");
+    size_t pos = 0;
+    std::function<NodeP(int, int)> make = [&](int dt, int depth) -> NodeP {
+        if (depth > kMaxTreeDepth) fail(DDJ_EINVAL, "tree path too deep");
+        const int et = path[pos];
+        if (!enc_ok(et) || et == DDJ_ENC_UNIQUE) fail(DDJ_EINVAL, "bad encoding in tree path");
+        NodeP node = make_node(et, dt);
+        if (et == DDJ_ENC_NONE) return node;
+        int rt[2];
+        return_types(et, dt, rt);
+        const int k = n_results(et);
+        for (int i = 0; i < k; i++) {
+            if (pos + 1 >= len) break;  // path ended: the node stays a leaf
+            pos++;
+            node->ch.push_back(make(rt[i], depth + 1));
+        }
+        return node;
+    };
+    NodeP root = make(dtype, 0);
+    if (pos + 1 != len) fail(DDJ_EINVAL, "tree path longer than the tree it describes");
+    return root;
+}
+
+void fix_tree(const NodeP& n) {
+    if (n->ch.empty()) {
+        if (n->enc == DDJ_ENC_NONE) return;
+        const int k = n_results(n->enc);
+        for (int i = 0; i < k; i++) {
+            NodeP leaf = make_node(DDJ_ENC_NONE, n->dtype);
+            leaf->ratio = 1.0;
+            n->ch.push_back(leaf);
+        }
+    } else {
+        for (auto& c : n->ch) fix_tree(c);
+    }
+}
+
+std::vector<int> preorder(const NodeP& n) {
+    std::vector<int> out{n->enc};
+    for (auto& c : n->ch) {
+        auto s = preorder(c);
+        out.insert(out.end(), s.begin(), s.end());
+    }
+    return out;
+}
+
+std::string tree_string(const NodeP& n) {
+    // CompressionNode::Print (compression_node.cpp:180-186)
+    std::ostringstream os;
+    std::function<void(const NodeP&)> rec = [&](const NodeP& x) {
+        os << enc_name(x->enc) << "[" << x->ratio << "],";
+        for (auto& c : x->ch) rec(c);
+    };
+    rec(n);
+    return os.str();
+}
+
+Part compress_tree(Ctx* c, const NodeP& root, DView data) {
+    std::vector<Piece> pieces;
+    size_t total = compress_node(c, *root, borrow(data), pieces);
+    // Concatenate (util/copy/cuda_array_copy.cu:14-33) as one gather kernel.
+    std::vector<uint8_t> headers;
+    std::vector<CopyRange> ranges;
+    size_t off = 0, hdr_count = 0;
+    for (auto& p : pieces)
+        if (p.is_header) hdr_count++;
+    DevBuf hdr_dev(c, 8 * hdr_count);
+    size_t hi = 0;
+    uint64_t max_bytes = 0;
+    for (auto& p : pieces) {
+        if (p.is_header) {
+            headers.insert(headers.end(), p.hdr, p.hdr + 8);
+            ranges.push_back(CopyRange{hdr_dev.bytes_ptr() + 8 * hi, off, 8});
+            hi++;
+            off += 8;
+        } else if (p.part.n) {
+            ranges.push_back(CopyRange{p.part.p, off, p.part.n});
+            max_bytes = std::max<uint64_t>(max_bytes, p.part.n);
+            off += p.part.n;
+        }
+    }
+    if (off != total) fail(DDJ_EHIP, "blob size mismatch");
+    h2d(c, hdr_dev.get(), headers.data(), headers.size());
+    DevBuf blob(c, total);
+    DevBuf rdev(c, sizeof(CopyRange) * ranges.size());
+    h2d(c, rdev.get(), ranges.data(), sizeof(CopyRange) * ranges.size());
+    launch_gather_copy(c, rdev.get<CopyRange>(), (int)ranges.size(), blob.bytes_ptr(), std::max<uint64_t>(max_bytes, 8));
+    return own(std::move(blob));
+}
+
+Part decompress_tree(Ctx* c, DView blob, int* dtype_out, NodeP* tree_out) {
+    if (blob.n < 8) fail(DDJ_ECORRUPT, "blob shorter than one node header");
+    DevBuf table(c, sizeof(ParsedNode) * kMaxTreeNodes), status(c, 16);
+    launch_parse_tree(c, blob.p, blob.n, table.get<ParsedNode>(), kMaxTreeNodes, status.get<int>());
+    int st[3];
+    d2h(c, st, status.get(), sizeof st);
+    if (st[0] != 0) fail(DDJ_ECORRUPT, "malformed tree blob (parse status " + std::to_string(st[0]) + ")");
+    const int count = st[1];
+    std::vector<ParsedNode> nodes((size_t)count);
+    d2h(c, nodes.data(), table.get(), sizeof(ParsedNode) * count);
+    std::vector<std::vector<int>> kids((size_t)count);
+    for (int i = 1; i < count; i++) kids[(size_t)nodes[i].parent].push_back(i);
+    if (tree_out) {
+        std::vector<NodeP> ns((size_t)count);
+        for (int i = 0; i < count; i++) ns[i] = make_node(nodes[i].enc, nodes[i].dtype);
+        for (int i = 1; i < count; i++) ns[(size_t)nodes[i].parent]->ch.push_back(ns[i]);
+        *tree_out = ns[0];
+    }
+    // CompressionNode::Decompress (compression_node.cpp:140-161): children first.
+    std::function<Part(int)> dec = [&](int i) -> Part {
+        const ParsedNode& nd = nodes[(size_t)i];
+        const DView meta{blob.p + nd.meta_off, (size_t)nd.meta_len};
+        if (nd.enc == DDJ_ENC_NONE) return borrow(DView{blob.p + nd.data_off, (size_t)nd.data_len});
+        if ((int)kids[(size_t)i].size() != nd.nchild) fail(DDJ_ECORRUPT, "node child count mismatch");
+        std::vector<Part> parts;
+        std::vector<DView> in{meta};
+        for (int k : kids[(size_t)i]) {
+            parts.push_back(dec(k));
+            in.push_back(parts.back().view());
+        }
+        return decode(c, nd.enc, nd.dtype, in);
+    };
+    if (dtype_out) *dtype_out = nodes[0].dtype;
+    Part r = dec(0);
+    if (nodes[0].enc == DDJ_ENC_NONE) {  // a bare leaf: hand out a copy
+        DevBuf b(c, r.n);
+        d2d(c, b.get(), r.p, r.n);
+        r = own(std::move(b));
+    }
+    return r;
+}
+
+size_t predict_tree(Ctx* c, const NodeP& n, DView data) {
+    size_t meta = 0, dsz = 0;
+    predict_sizes(c, n->enc, n->dtype, data, &meta, &dsz);
+    size_t size = meta + 8;
+    if (n->ch.empty()) return size + dsz;
+    auto outs = encode(c, n->enc, n->dtype, data);
+    for (size_t i = 0; i < n->ch.size(); i++) size += predict_tree(c, n->ch[i], outs[i + 1].view());
+    return size;
+}
+
+}  // namespace ddj

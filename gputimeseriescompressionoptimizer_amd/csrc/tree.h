@@ -1,0 +1,51 @@
+// CompressionNode / CompressionTree: the cascade, its pre-order serialized
+// format and size prediction (reference tree/compression_node.cpp,
+// tree/compression_tree.cpp).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "codecs.h"
+
+namespace ddj {
+
+constexpr int kMaxTreeDepth = 64;
+constexpr int kMaxTreeNodes = 1024;
+
+struct TNode;
+using NodeP = std::shared_ptr<TNode>;
+
+struct TNode {
+    int enc = DDJ_ENC_NONE;
+    int dtype = DDJ_D_INT;
+    std::vector<NodeP> ch;
+    double ratio = 1.0;  // CompressionNode::_compressionRatio
+};
+
+NodeP make_node(int enc, int dtype);
+NodeP copy_tree(const NodeP& n);
+// PathGenerator::GenerateTree / CreateTree (optimizer/path_generator.cpp:70-91);
+// a path that ends on a non-none node leaves it a leaf.
+NodeP tree_from_path(const int* path, size_t len, int dtype);
+// CompressionNode::Fix (compression_node.cpp:209-223)
+void fix_tree(const NodeP& n);
+std::vector<int> preorder(const NodeP& n);
+std::string tree_string(const NodeP& n);
+
+// CompressionTree::Compress (compression_tree.cpp:98-103): updates the
+// nodes' ratios and returns the blob.
+Part compress_tree(Ctx* c, const NodeP& root, DView data);
+// CompressionTree::Decompress (compression_tree.cpp:148-155)
+Part decompress_tree(Ctx* c, DView blob, int* dtype_out, NodeP* tree_out = nullptr);
+// CompressionNode::PredictCompressionSize (compression_node.cpp:163-177)
+size_t predict_tree(Ctx* c, const NodeP& n, DView data);
+
+struct ParsedNode {
+    int32_t enc, dtype, nchild, parent;
+    uint64_t meta_off, meta_len, data_off, data_len;
+};
+void launch_parse_tree(Ctx* c, const uint8_t* blob, uint64_t bytes, ParsedNode* d_out, int max_nodes, int* d_status);
+
+}  // namespace ddj

@@ -1,0 +1,153 @@
+/*
+ * ddj.h -- C ABI of the MI355X column-compression engine (libddj_hip.so).
+ *
+ * This is the drop-in boundary for the reference's lightweight lossless
+ * column-compression path (dzitkowskik/GpuTimeSeriesCompressionOptimizer,
+ * src/).  Every entry point takes plain pointers and sizes; device buffers
+ * are ddj_buf handles allocated from the context's caching allocator.  All
+ * GPU work of a context runs on its own HIP stream; calls on one context
+ * are serialised by the caller, separate contexts are independent.
+ *
+ * Status codes replace the reference's exit(1) / print-and-continue error
+ * handling (core/macros.h:8-17).
+ */
+#ifndef DDJ_H_
+#define DDJ_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    DDJ_OK = 0,
+    DDJ_EINVAL = 1,       /* bad argument (unknown encoding, type, size) */
+    DDJ_EUNSUPPORTED = 2, /* the reference throws NotImplementedException or returns nothing */
+    DDJ_ECORRUPT = 3,     /* encoded input fails validation */
+    DDJ_ENOMEM = 4,       /* device allocation failed */
+    DDJ_EHIP = 5,         /* HIP runtime error */
+    DDJ_ETIMEOUT = 6      /* a bounded device-side wait expired */
+} ddj_status;
+
+/* EncodingType (reference: compression/encoding_type.hpp:17-30) */
+enum {
+    DDJ_ENC_DELTA = 0,
+    DDJ_ENC_AFL = 1,
+    DDJ_ENC_SCALE = 2,
+    DDJ_ENC_RLE = 3,
+    DDJ_ENC_DICT = 4,
+    DDJ_ENC_UNIQUE = 5,
+    DDJ_ENC_PATCH = 6,
+    DDJ_ENC_NONE = 7,
+    DDJ_ENC_FLOAT_TO_INT = 8,
+    DDJ_ENC_CONST = 9,
+    DDJ_ENC_GFC = 10,
+    DDJ_ENC_OUTSIDE_PATCH = 100 /* OutsidePatchEncoding (compression/patch/outside_patch_encoding.cu) */
+};
+
+/* DataType.  The reference's enum lives in its absent data-manager
+ * submodule; these values follow the order of Encoding::Encode's switch
+ * (compression/encoding.hpp:30-46) and are what tree headers carry. */
+enum { DDJ_D_INT = 0, DDJ_D_TIME = 1, DDJ_D_FLOAT = 2, DDJ_D_DOUBLE = 3, DDJ_D_SHORT = 4, DDJ_D_CHAR = 5 };
+
+typedef struct ddj_ctx ddj_ctx;
+
+/* A device buffer: `dev` points to at least `bytes` bytes (0 bytes -> NULL). */
+typedef struct {
+    void* dev;
+    size_t bytes;
+} ddj_buf;
+
+/* DataStatistics (util/statistics/cuda_array_statistics.hpp:24-34) plus the
+ * planner features the reference does not have. */
+typedef struct {
+    double min;
+    double max;
+    int32_t minBitCnt;   /* BITLEN(double, double): 64 for n > 0, as the reference */
+    int32_t precision;   /* max _getFloatPrecision over rows */
+    int32_t sorted;      /* non-decreasing */
+    float rlMetric;      /* RlMetric<T,2> */
+    double mean;         /* Mean<T>: sum in T / n, converted to T */
+    uint64_t size;       /* rows */
+    /* features beyond the reference (learned planner) */
+    uint64_t runs;            /* number of maximal runs of equal neighbours */
+    uint64_t distinct_est;    /* distinct-value estimate (64-register HyperLogLog) */
+    uint32_t bit_hist[65];    /* rows per bit width 0..64 of the zig-zag value */
+    uint32_t delta_bits_max;  /* bit width of the largest |x[i+1]-x[i]| */
+} ddj_stats_t;
+
+/* ---------------------------------------------------------------- context */
+ddj_status ddj_ctx_create(ddj_ctx** out, int device);
+void ddj_ctx_destroy(ddj_ctx* ctx);
+ddj_status ddj_ctx_sync(ddj_ctx* ctx);
+const char* ddj_last_error(ddj_ctx* ctx);
+/* Returns cached device memory to the driver. */
+void ddj_ctx_trim(ddj_ctx* ctx);
+/* The HIP stream of the context (hipStream_t), for callers that order their own work. */
+void* ddj_ctx_stream(ddj_ctx* ctx);
+
+/* ----------------------------------------------------------------- memory */
+ddj_status ddj_buf_alloc(ddj_ctx* ctx, size_t bytes, ddj_buf* out);
+void ddj_buf_free(ddj_ctx* ctx, ddj_buf* buf);
+ddj_status ddj_memcpy_h2d(ddj_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
+ddj_status ddj_memcpy_d2h(ddj_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes);
+ddj_status ddj_memcpy_d2d(ddj_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes);
+
+/* -------------------------------------------------------------- encodings */
+/* Encoding::GetNumberOfResults / GetReturnTypes (compression/encoding.hpp:74-75) */
+ddj_status ddj_encoding_info(int enc, int dtype, uint32_t* n_results, int* return_dtypes /* [2] */);
+
+/* Encoding::Encode(SharedCudaPtr<char>, DataType) (compression/encoding.hpp:28-47).
+ * outs[0] = metadata, outs[1..n_results] = outputs; the caller frees them. */
+ddj_status ddj_encode(ddj_ctx* ctx, int enc, int dtype, const void* d_in, size_t in_bytes, ddj_buf* outs);
+
+/* Encoding::Decode(SharedCudaPtrVector<char>, DataType) (compression/encoding.hpp:49-67).
+ * ins[0] = metadata, ins[1..] = decoded children outputs. */
+ddj_status ddj_decode(ddj_ctx* ctx, int enc, int dtype, const ddj_buf* ins, uint32_t n_ins, ddj_buf* out);
+
+/* Encoding::GetMetadataSize / GetCompressedSize (compression/encoding.hpp:78-79) */
+ddj_status ddj_predict_sizes(ddj_ctx* ctx, int enc, int dtype, const void* d_in, size_t in_bytes,
+                             size_t* meta_bytes, size_t* data_bytes);
+
+/* CudaArrayStatistics::GenerateStatistics (util/statistics/cuda_array_statistics.cu:159-179) */
+ddj_status ddj_stats(ddj_ctx* ctx, int dtype, const void* d_in, size_t in_bytes, ddj_stats_t* out);
+
+/* FL bit-packing alone, with a given bit width (run_afl_compress_gpu<T,1> /
+ * run_afl_decompress_gpu<T,1>, compression/afl/afl_gpu.cu:67-81). */
+ddj_status ddj_fl_pack(ddj_ctx* ctx, int dtype, int bits, const void* d_in, size_t n, void* d_out);
+ddj_status ddj_fl_unpack(ddj_ctx* ctx, int dtype, int bits, const void* d_in, size_t in_bytes, size_t n, void* d_out);
+
+/* ------------------------------------------------------------------- tree */
+/* CompressionTree built from a pre-order path of encodings
+ * (PathGenerator::GenerateTree, optimizer/path_generator.cpp:70-91); leaves
+ * missing from the path are completed with `none` (CompressionNode::Fix). */
+ddj_status ddj_tree_compress(ddj_ctx* ctx, const int* preorder, size_t path_len, int dtype, const void* d_in,
+                             size_t in_bytes, ddj_buf* blob, double* node_ratios /* [path_len] or NULL */);
+/* CompressionTree::Decompress (tree/compression_tree.cpp:148-155) */
+ddj_status ddj_tree_decompress(ddj_ctx* ctx, const void* d_blob, size_t blob_bytes, ddj_buf* out, int* dtype);
+/* CompressionTree::GetPredictedSizeAfterCompression (tree/compression_tree.cpp:157-160) */
+ddj_status ddj_tree_predict_size(ddj_ctx* ctx, const int* preorder, size_t path_len, int dtype, const void* d_in,
+                                 size_t in_bytes, size_t* size);
+
+/* ---------------------------------------------------------------- planner */
+enum { DDJ_PLAN_EXHAUSTIVE = 0, DDJ_PLAN_OPTIMIZE30 = 1, DDJ_PLAN_LEARNED = 2 };
+/* EXHAUSTIVE: phase 1 of CompressionOptimizer::CompressData (optimizer/compression_optimizer.cpp:156-182);
+ * OPTIMIZE30: CompressionOptimizer::OptimizeTree (:16-36).  The chosen tree
+ * is written as a pre-order path (fixed, leaves included). */
+ddj_status ddj_plan(ddj_ctx* ctx, int mode, int dtype, const void* d_in, size_t in_bytes, int* path_out,
+                    size_t path_cap, size_t* path_len, double* ratio, uint64_t* n_candidates);
+
+/* Stateful optimizer: CompressionOptimizer::CompressData over successive
+ * parts of one column (re-plans on the first part and every 10th). */
+typedef struct ddj_optimizer ddj_optimizer;
+ddj_status ddj_optimizer_create(ddj_ctx* ctx, ddj_optimizer** out);
+void ddj_optimizer_destroy(ddj_optimizer* opt);
+ddj_status ddj_optimizer_compress(ddj_optimizer* opt, int dtype, const void* d_in, size_t in_bytes, ddj_buf* blob);
+ddj_status ddj_optimizer_tree(ddj_optimizer* opt, int* path_out, size_t path_cap, size_t* path_len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,44 @@
+"""The C ABI library loads and exports every entry point include/ddj.h declares
+(no GPU calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "ddj.h")
+LIB = os.path.join(ROOT, "gputimeseriescompressionoptimizer_amd", "libddj_hip.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ddj_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ["ddj_encode", "ddj_decode", "ddj_tree_compress", "ddj_tree_decompress", "ddj_plan", "ddj_stats"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(LIB):
+        pytest.fail("libddj_hip.so not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_encoding_info_without_gpu():
+    import gputimeseriescompressionoptimizer_amd as g
+
+    # pure host function: arity and return types of every encoding
+    assert g.encoding_info(g.RLE, g.D_DOUBLE) == (2, [g.D_INT, g.D_DOUBLE])
+    assert g.encoding_info(g.FLOAT_TO_INT, g.D_FLOAT) == (1, [g.D_INT])
+    assert g.encoding_info(g.NONE, g.D_CHAR) == (0, [])
+    assert g.encoding_info(g.DICT, g.D_SHORT) == (2, [g.D_SHORT, g.D_SHORT])
+    assert g.encoding_info(g.GFC, g.D_FLOAT) == (2, [g.D_FLOAT, g.D_FLOAT])
+    with pytest.raises(g.DdjError):
+        g.encoding_info(42, g.D_INT)
