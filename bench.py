@@ -1,0 +1,160 @@
+"""Benchmark: compress / decompress input GB/s of the single-scheme GPU
+workload (four 64M-row int32 columns, each through its fixed tree), one GPU.
+
+A step is one compression pass over the four resident columns (every tree
+node's kernels, size read-backs and blob assembly included).  Decompression
+of the four blobs is timed separately with the same step count.  The
+dominant kernel of the compress step is timed live with HIP events on the
+library's stream and set against the 8 TB/s HBM peak; the CPU oracle (the
+reference's codecs restated on the host, one thread) compresses a bounded
+prefix of the same columns as the CPU baseline.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "compress/decompress input GB/s at 1 GPU + ratio gap vs exhaustive planner"
+HBM_PEAK_GBPS = 8000.0
+
+
+def columns(rows: int):
+    from gputimeseriescompressionoptimizer_amd import datasets as D
+    import gputimeseriescompressionoptimizer_amd as g
+
+    return [
+        ("timestamps delta+afl", D.timestamps(rows, seed=11), [g.DELTA, g.AFL, g.NONE]),
+        ("step function rle+afl", D.step_function(rows, seed=12), [g.RLE, g.AFL, g.NONE, g.AFL, g.NONE]),
+        ("zipf(1.2) over 50 codes dict", D.zipf_codes(rows, seed=13), [g.DICT, g.NONE, g.NONE]),
+        ("N(1000,50)+1% outliers patch+afl", D.normal_with_outliers(rows, seed=14),
+         [g.PATCH, g.AFL, g.NONE, g.AFL, g.NONE]),
+    ]
+
+
+def cpu_baseline(cols, sample_rows: int, budget_s: float = 15.0):
+    """The oracle (reference codecs restated in C++, one thread) compressing the
+    first `sample_rows` rows of each column with the same trees."""
+    from oracle import oracle as O
+
+    trees = [(O.tree_from_path(path, O.D_INT), x[:sample_rows]) for _, x, path in cols]
+    nbytes = sum(x.nbytes for _, x in trees)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        for t, x in trees:
+            O.compress(t, x)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return {"value": round(nbytes * reps / el / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"first {sample_rows} rows of each of the 4 columns, {reps} passes, {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--rows", type=int, default=1 << 26)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 21)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (columns would shard "
+              "across GPUs with a host-side scan of blob sizes; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import gputimeseriescompressionoptimizer_amd as g
+
+    cols = columns(args.rows)
+    ctx = g.Context(0)
+    dev = [(name, ctx.upload(x), path, x) for name, x, path in cols]
+    in_bytes = sum(x.nbytes for _, x, _ in cols)
+    trees = [g.CompressionTree(ctx, path, g.D_INT) for _, _, path, _ in dev]
+
+    def compress_all():
+        return [t.Compress(buf) for t, (_, buf, _, _) in zip(trees, dev)]
+
+    for _ in range(args.warmup):
+        blobs = compress_all()
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        blobs = compress_all()
+    ctx.sync()
+    t_comp = (time.perf_counter() - t0) / args.steps
+    ktab = ctx.kernel_table()
+    ctx.profile(False)
+
+    # decompression of the same blobs, verified once
+    outs = [t.Decompress(b) for t, b in zip(trees, blobs)]
+    for (_, _, _, x), o in zip(dev, outs):
+        if not np.array_equal(o.to_numpy(np.int32), x):
+            raise SystemExit("round trip mismatch")
+    del outs
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        outs = [t.Decompress(b) for t, b in zip(trees, blobs)]
+        del outs
+    ctx.sync()
+    t_dec = (time.perf_counter() - t0) / args.steps
+
+    ratios = {name: round(x.nbytes / b.nbytes, 3) for (name, _, _, x), b in zip(dev, blobs)}
+    out_bytes = sum(b.nbytes for b in blobs)
+
+    # dominant kernel of the compress step (largest total device time)
+    dom = max(ktab.items(), key=lambda kv: kv[1][0])
+    dname, (dms, dlaunch, dbytes) = dom
+    achieved = dbytes / (dms / 1e3) / 1e9 if dms > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "avg_launch_us": round(dms / dlaunch * 1e3, 2), "launches_per_step": dlaunch / args.steps,
+                "algorithmic_bytes_per_launch": int(dbytes / dlaunch)}
+    kernels = {k: {"ms_per_step": round(v[0] / args.steps, 4), "launches_per_step": v[1] / args.steps,
+                   "gbps": round(v[2] / (v[0] / 1e3) / 1e9, 1) if v[0] > 0 and v[2] > 0 else None}
+               for k, v in sorted(ktab.items(), key=lambda kv: -kv[1][0])}
+
+    res = {
+        "metric": METRIC,
+        "value": round(in_bytes / t_comp / 1e9, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_comp * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic: seeded numpy generators of the four single-scheme columns",
+        "config": {"workload": "single-scheme GPU kernels, 4 int32 columns x %d rows: timestamps delta+afl, "
+                               "step function rle+afl, zipf(1.2)/50 codes dict, N(1000,50)+1%% outliers "
+                               "patch+afl" % args.rows,
+                   "rows_per_column": args.rows, "input_bytes": in_bytes},
+        "decompress_gbps": round(in_bytes / t_dec / 1e9, 3),
+        "decompress_ms_per_step": round(t_dec * 1e3, 3),
+        "compression_ratio": round(in_bytes / out_bytes, 3),
+        "ratios": ratios,
+        "roofline": roofline,
+        "kernels": kernels,
+    }
+    ctx.close()
+    if not args.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(cols, min(args.cpu_sample, args.rows))
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -103,6 +103,9 @@ def load_library(path: str = LIB_PATH):
         "ddj_optimizer_destroy": (None, [vp]),
         "ddj_optimizer_compress": (i32, [vp, i32, vp, sz, P(_Buf)]),
         "ddj_optimizer_tree": (i32, [vp, P(i32), sz, P(sz)]),
+        "ddj_profile_enable": (i32, [vp, i32]),
+        "ddj_profile_kernel": (i32, [vp, ctypes.c_char_p, P(ctypes.c_double), P(u64), P(ctypes.c_double)]),
+        "ddj_profile_reset": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -191,6 +194,37 @@ class Context:
     def sync(self):
         self.check(self._L.ddj_ctx_sync(self._h), "sync")
 
+    def profile(self, on: bool = True):
+        """Record HIP events around every kernel launch of this context."""
+        self.check(self._L.ddj_profile_enable(self._h, 1 if on else 0), "profile")
+
+    def profile_reset(self):
+        self.check(self._L.ddj_profile_reset(self._h), "profile_reset")
+
+    def kernel_time(self, name: str) -> Tuple[float, int, float]:
+        """(total milliseconds, launches, algorithmic bytes) of one kernel since the last reset."""
+        ms = ctypes.c_double()
+        n = ctypes.c_uint64()
+        b = ctypes.c_double()
+        self.check(self._L.ddj_profile_kernel(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(n),
+                                              ctypes.byref(b)), "kernel_time")
+        return ms.value, n.value, b.value
+
+    KERNELS = ["k_fl_pack", "k_fl_unpack", "k_minmax", "k_stats", "k_delta_encode", "k_delta_decode",
+               "k_add_scalar", "k_split", "k_merge", "k_rle_encode", "k_rle_lengths", "k_scan_lengths",
+               "k_rle_expand", "k_hist_count", "k_hist_topk", "k_fill_u64", "k_fill_zero", "k_pack_codes",
+               "k_unpack_codes", "k_gfc_encode", "k_gfc_decode", "k_float_split", "k_float_compose",
+               "k_float_to_int", "k_int_to_float", "k_gather_copy", "k_parse_tree"]
+
+    def kernel_table(self):
+        """{kernel: (ms, launches, algorithmic bytes)} for every kernel that ran."""
+        out = {}
+        for k in self.KERNELS:
+            t = self.kernel_time(k)
+            if t[1]:
+                out[k] = t
+        return out
+
     def stream(self) -> int:
         return self._L.ddj_ctx_stream(self._h) or 0
 
@@ -258,10 +292,11 @@ class Encoding:
 
     def Encode(self, data: DeviceBuffer, dtype: int) -> List[DeviceBuffer]:
         n, _ = encoding_info(self.encoding_type, dtype)
-        outs = (_Buf * (n + 1))()
+        k = 1 + max(n, 1)  # `none` returns its data copy as outs[1] (none_encoding.cu:13-30)
+        outs = (_Buf * k)()
         self.ctx.check(self.ctx._L.ddj_encode(self.ctx._h, self.encoding_type, dtype, data.dev or None, data.nbytes,
                                               outs), f"encode {ENCODING_NAMES.get(self.encoding_type)}")
-        return [self.ctx._wrap(outs[i]) for i in range(n + 1)]
+        return [self.ctx._wrap(outs[i]) for i in range(k)]
 
     def Decode(self, parts: Sequence[DeviceBuffer], dtype: int) -> DeviceBuffer:
         ins = (_Buf * len(parts))(*[p._as_c() for p in parts])

@@ -110,6 +110,47 @@ void ddj_ctx_trim(ddj_ctx* ctx) {
 
 void* ddj_ctx_stream(ddj_ctx* ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
 
+static void drain_timers(Ctx* c) {
+    HIPC(hipStreamSynchronize(c->stream));
+    for (auto& kv : c->ktimes) {
+        for (auto& ev : kv.second.pending) {
+            float ms = 0;
+            HIPC(hipEventElapsedTime(&ms, ev.first, ev.second));
+            kv.second.total_ms += ms;
+            kv.second.launches++;
+            (void)hipEventDestroy(ev.first);
+            (void)hipEventDestroy(ev.second);
+        }
+        kv.second.pending.clear();
+    }
+}
+
+ddj_status ddj_profile_enable(ddj_ctx* ctx, int on) {
+    return guarded(ctx, [&] {
+        drain_timers(&ctx->c);
+        ctx->c.profiling = on != 0;
+    });
+}
+
+ddj_status ddj_profile_kernel(ddj_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches,
+                              double* algo_bytes) {
+    return guarded(ctx, [&] {
+        if (!kernel || !total_ms || !launches) fail(DDJ_EINVAL, "null argument");
+        drain_timers(&ctx->c);
+        auto it = ctx->c.ktimes.find(kernel);
+        *total_ms = it == ctx->c.ktimes.end() ? 0.0 : it->second.total_ms;
+        *launches = it == ctx->c.ktimes.end() ? 0 : it->second.launches;
+        if (algo_bytes) *algo_bytes = it == ctx->c.ktimes.end() ? 0.0 : it->second.bytes;
+    });
+}
+
+ddj_status ddj_profile_reset(ddj_ctx* ctx) {
+    return guarded(ctx, [&] {
+        drain_timers(&ctx->c);
+        ctx->c.ktimes.clear();
+    });
+}
+
 ddj_status ddj_buf_alloc(ddj_ctx* ctx, size_t bytes, ddj_buf* out) {
     return guarded(ctx, [&] {
         if (!out) fail(DDJ_EINVAL, "null output");

@@ -100,6 +100,7 @@ std::vector<Part> delta_encode(Ctx* c, int dtype, DView in) {
     DevBuf meta(c, es), data(c, (n - 1) * es);
     d2d(c, meta.get(), x.p, es);
     launch_delta_encode(c, (int)es, x.p, n, data.get());
+    kernel_bytes(c, "k_delta_encode", (double)((2 * n - 1) * es));
     return {own(std::move(meta)), own(std::move(data))};
 }
 Part delta_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
@@ -109,6 +110,7 @@ Part delta_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     const size_t m = in[1].n / es;
     DevBuf out(c, (m + 1) * es);
     launch_delta_decode(c, (int)es, in[0].p, in[1].p, m, out.get());
+    kernel_bytes(c, "k_delta_decode", (double)((2 * m + 1) * es));
     return own(std::move(out));
 }
 
@@ -181,6 +183,8 @@ std::vector<Part> afl_encode_int(Ctx* c, int dtype, DView in) {
     });
     const uint64_t bytes = (b * (uint64_t)n + W - 1) / W * es;
     data.set_size((size_t)bytes);
+    kernel_bytes(c, "k_minmax", (double)(n * es));
+    kernel_bytes(c, "k_fl_pack", (double)(n * es + bytes));
     uint8_t meta[4] = {(uint8_t)b, (uint8_t)((bytes * 8 - (uint64_t)n * b) & 0xff), 0, 0};
     return {upload(c, meta, 4), own(std::move(data))};
 }
@@ -193,6 +197,7 @@ Part fl_decode_raw(Ctx* c, int dtype, const uint8_t* data, size_t bytes, int b, 
     const size_t n = (size_t)(bits / b);
     DevBuf out(c, n * dtype_size(dtype));
     launch_fl_unpack(c, dtype, data, bytes, n, b, out.get());
+    kernel_bytes(c, "k_fl_unpack", (double)((n * b + 7) / 8 + n * dtype_size(dtype)));
     return own(std::move(out));
 }
 
@@ -285,6 +290,7 @@ std::vector<Part> rle_encode(Ctx* c, int dtype, DView in) {
     uint64_t r = 0;
     d2h(c, &r, runs.get(), 8);
     R = (int32_t)r;
+    kernel_bytes(c, "k_rle_encode", (double)(n * es + r * (4 + es)));
     lens.set_size(4 * r);
     vals.set_size(es * r);
     return {upload(c, &R, 4), own(std::move(lens)), own(std::move(vals))};
@@ -343,6 +349,8 @@ std::vector<Part> patch_encode(Ctx* c, int dtype, DView in, bool outside) {
     d2h(c, &m, cnt.get(), 8);
     yes.set_size(m * es);
     no.set_size((n - m) * es);
+    kernel_bytes(c, "k_minmax", (double)(n * es));
+    kernel_bytes(c, "k_split", (double)(2 * n * es + 1 + (n + 7) / 8));
     return {own(std::move(st)), own(std::move(yes)), own(std::move(no))};
 }
 Part patch_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
@@ -409,6 +417,9 @@ std::vector<Part> dict_encode(Ctx* c, int dtype, DView in) {
     uint64_t m = 0;
     d2h(c, &m, cnt.get(), 8);
     others.set_size((n - m) * es);
+    kernel_bytes(c, "k_hist_count", (double)(n * es));
+    kernel_bytes(c, "k_split", (double)(n * es + 1 + (n + 7) / 8 + m + (n - m) * es));
+    kernel_bytes(c, "k_pack_codes", (double)(m + (m + 15) / 16 * 4));
     Part freq = empty();
     if (m > 0) {
         const int bits = 2;  // ALT_BITLEN(K - 1)

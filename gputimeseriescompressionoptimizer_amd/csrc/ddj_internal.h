@@ -59,6 +59,15 @@ struct Control {
     uint32_t pad[63];
 };
 
+// Per-kernel HIP-event timings, recorded on the context stream around each
+// launch while profiling is enabled.
+struct KernelTimes {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    double total_ms = 0;
+    uint64_t launches = 0;
+    double bytes = 0;  // algorithmic bytes (one read of the inputs, one write of the outputs)
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -67,6 +76,36 @@ struct Ctx {
     int cu_count = 256;
     std::string last_error;
     std::mutex mu;
+    bool profiling = false;
+    std::map<std::string, KernelTimes> ktimes;
+};
+
+// Adds the algorithmic bytes of a launch to its kernel's tally.
+inline void kernel_bytes(Ctx* c, const char* name, double b) {
+    if (c->profiling) c->ktimes[name].bytes += b;
+}
+
+// Brackets one kernel launch with events when profiling is on.
+class KernelTimer {
+public:
+    KernelTimer(Ctx* c, const char* name) : c_(c), name_(name) {
+        if (!c->profiling) return;
+        if (hipEventCreate(&a_) != hipSuccess || hipEventCreate(&b_) != hipSuccess) {
+            a_ = b_ = nullptr;
+            return;
+        }
+        (void)hipEventRecord(a_, c->stream);
+    }
+    ~KernelTimer() {
+        if (!a_) return;
+        (void)hipEventRecord(b_, c_->stream);
+        c_->ktimes[name_].pending.push_back({a_, b_});
+    }
+
+private:
+    Ctx* c_;
+    const char* name_;
+    hipEvent_t a_ = nullptr, b_ = nullptr;
 };
 
 // RAII device buffer from the context allocator.

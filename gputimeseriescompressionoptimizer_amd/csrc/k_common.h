@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <type_traits>
 
+#include "kernels.h"
+
 namespace ddj {
 
 constexpr int kWave = 64;
@@ -130,69 +132,119 @@ template <typename T> __device__ __forceinline__ T block_exclusive_scan(T v, T* 
 }
 
 // ------------------------------------------------ decoupled look-back scan
-// Tiles take tickets in start order, publish their aggregate, and sum the
-// predecessors' aggregates until one with an inclusive prefix is found.
-// Every wait is bounded: on expiry the error flag is raised and the tile
-// treats the missing prefix as zero, so a kernel always finishes.
-constexpr uint32_t kFlagEmpty = 0, kFlagAggregate = 1, kFlagInclusive = 2;
+// Tiles take tickets in start order, publish their aggregate, and add up the
+// predecessors' values until one with an inclusive prefix is found.
+//
+// A tile's state and value travel together in one 64-bit status word
+// (bits 63..62 state, 61..0 value), so a relaxed agent-scope atomic load sees
+// a consistent pair and no acquire/release fence is needed: such fences make
+// the compiler write back and invalidate the whole L2 of the XCD
+// (buffer_wbl2 / buffer_inv), which stalls every tile of a streaming kernel.
+// Wide values (full 64-bit wrapping sums) use two status words carrying the
+// 32-bit halves; a pair is taken only when both words show the same state.
+//
+// One wave inspects 64 predecessors per step.  Every wait is bounded: on
+// expiry the error flag is raised and the missing prefix counts as zero, so
+// a kernel always finishes.
+constexpr uint64_t kStAggregate = 1ull << 62, kStInclusive = 2ull << 62;
+constexpr uint64_t kStValue = (1ull << 62) - 1;
 constexpr uint32_t kSpinLimit = 1u << 22;
 
-template <typename V> struct LookbackState {
-    uint32_t* flags;    // [tiles], zeroed before launch
-    V* aggregate;       // [tiles]
-    V* inclusive;       // [tiles]
-    uint32_t* ticket;   // zeroed before launch
-    uint32_t* error;    // set to 1 on an expired wait
-};
-
-__device__ __forceinline__ uint32_t atomic_load_acq(uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ uint64_t ld_status(uint64_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void atomic_store_rel(uint32_t* p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Called by thread 0 only.  Returns the exclusive prefix of `tile`.
-template <typename V> __device__ V lookback_prefix(LookbackState<V> st, uint32_t tile, V aggregate) {
+template <bool WIDE> __device__ __forceinline__ void lb_publish(const Lookback& lb, uint32_t tile, uint64_t state,
+                                                                uint64_t v) {
+    if constexpr (WIDE) {
+        st_status(&lb.status[2 * (size_t)tile], state | (v & 0xffffffffull));
+        st_status(&lb.status[2 * (size_t)tile + 1], state | (v >> 32));
+    } else {
+        st_status(&lb.status[tile], state | (v & kStValue));
+    }
+}
+
+// Reads predecessor `idx`; returns false while it has published nothing.
+template <bool WIDE> __device__ __forceinline__ bool lb_read(const Lookback& lb, size_t idx, uint64_t& state,
+                                                             uint64_t& v) {
+    if constexpr (WIDE) {
+        const uint64_t a = ld_status(&lb.status[2 * idx]);
+        const uint64_t b = ld_status(&lb.status[2 * idx + 1]);
+        state = a & ~kStValue;
+        v = (b << 32) | (a & 0xffffffffull);
+        return state != 0 && state == (b & ~kStValue);
+    } else {
+        const uint64_t a = ld_status(&lb.status[idx]);
+        state = a & ~kStValue;
+        v = a & kStValue;
+        return state != 0;
+    }
+}
+
+// Exclusive prefix of `tile`.  Called by all 64 lanes of one wave with the
+// same tile and aggregate.  Narrow values must stay below 2^62 (counts), or
+// only their low bits matter (sums that wrap at 32 bits or fewer).
+template <bool WIDE> __device__ uint64_t lookback_wave(const Lookback& lb, uint32_t tile, uint64_t aggregate) {
+    const int lane = threadIdx.x & (kWave - 1);
     if (tile == 0) {
-        st.inclusive[0] = aggregate;
-        atomic_store_rel(&st.flags[0], kFlagInclusive);
-        return V(0);
+        if (lane == 0) lb_publish<WIDE>(lb, 0, kStInclusive, aggregate);
+        return 0;
     }
-    st.aggregate[tile] = aggregate;
-    atomic_store_rel(&st.flags[tile], kFlagAggregate);
-    V excl = V(0);
-    int64_t t = (int64_t)tile - 1;
-    while (t >= 0) {
-        uint32_t f;
+    if (lane == 0) lb_publish<WIDE>(lb, tile, kStAggregate, aggregate);
+    uint64_t excl = 0;
+    int64_t end = tile;  // window [end - 64, end)
+    for (;;) {
+        const int64_t idx = end - 1 - lane;
+        uint64_t state = kStInclusive, v = 0;  // before tile 0: an inclusive zero
+        bool ready = idx < 0;
         uint32_t spins = 0;
-        while ((f = atomic_load_acq(&st.flags[t])) == kFlagEmpty) {
+        for (;;) {
+            if (!ready) ready = lb_read<WIDE>(lb, (size_t)idx, state, v);
+            if (__all(ready)) break;
+            if (++spins > kSpinLimit) {
+                if (!ready) {
+                    __hip_atomic_store(lb.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    state = kStInclusive;
+                    v = 0;
+                }
+                break;
+            }
             __builtin_amdgcn_s_sleep(1);
-            if (++spins > kSpinLimit) break;
         }
-        if (f == kFlagEmpty) {  // expired: report and stop waiting
-            __hip_atomic_store(st.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t inc = __ballot(state == kStInclusive);
+        if (inc) {
+            const int first = __builtin_ctzll(inc);
+            excl += wave_sum<uint64_t>(lane <= first ? v : 0);
             break;
         }
-        if (f == kFlagInclusive) {
-            excl += __hip_atomic_load(&st.inclusive[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-        }
-        excl += __hip_atomic_load(&st.aggregate[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        t--;
+        excl += wave_sum<uint64_t>(v);
+        end -= kWave;
     }
-    st.inclusive[tile] = excl + aggregate;
-    atomic_store_rel(&st.flags[tile], kFlagInclusive);
+    if (lane == 0) lb_publish<WIDE>(lb, tile, kStInclusive, excl + aggregate);
     return excl;
 }
 
 // Ticket of the calling workgroup (all threads get it).
-template <typename V> __device__ __forceinline__ uint32_t take_ticket(LookbackState<V> st, uint32_t* smem) {
-    if (threadIdx.x == 0) *smem = atomicAdd(st.ticket, 1u);
+__device__ __forceinline__ uint32_t take_ticket(const Lookback& lb, uint32_t* smem) {
+    if (threadIdx.x == 0) *smem = atomicAdd(lb.ticket, 1u);
     __syncthreads();
     uint32_t t = *smem;
     __syncthreads();
     return t;
+}
+
+// Block-wide look-back: wave 0 resolves the prefix, every thread gets it.
+template <bool WIDE> __device__ __forceinline__ uint64_t block_lookback(const Lookback& lb, uint32_t tile,
+                                                                       uint64_t aggregate, uint64_t* smem) {
+    if (threadIdx.x < kWave) {
+        const uint64_t p = lookback_wave<WIDE>(lb, tile, aggregate);
+        if (threadIdx.x == 0) *smem = p;
+    }
+    __syncthreads();
+    return *smem;
 }
 
 // -------------------------------------------------------------- bit helpers

@@ -139,14 +139,14 @@ constexpr int kScanTile = kBlock * kScanItems;
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_delta_decode(const uint8_t* first_bytes, const uint8_t* __restrict__ din,
-                                                         size_t m, T* __restrict__ out, LookbackState<uint64_t> st) {
+                                                         size_t m, T* __restrict__ out, Lookback lb) {
     T first;
     __builtin_memcpy(&first, first_bytes, sizeof(T));
     __shared__ uint64_t buf[kScanTile + kScanTile / 32];
     __shared__ uint64_t red[kBlock / kWave];
     __shared__ uint32_t tk;
     __shared__ uint64_t tile_prefix;
-    const uint32_t tile = take_ticket(st, &tk);
+    const uint32_t tile = take_ticket(lb, &tk);
     const size_t base = (size_t)tile * kScanTile;
     const int cnt = (int)((m - base) < (size_t)kScanTile ? (m - base) : (size_t)kScanTile);
     auto pad = [](int i) { return i + (i >> 5); };
@@ -175,9 +175,9 @@ __global__ __launch_bounds__(kBlock) void k_delta_decode(const uint8_t* first_by
     }
     uint64_t total;
     uint64_t excl = block_exclusive_scan<uint64_t>(sum, &total, red);
-    if (threadIdx.x == 0) tile_prefix = lookback_prefix<uint64_t>(st, tile, total);
-    __syncthreads();
-    uint64_t run = ubits(first) + tile_prefix + excl;
+    // sums wrap at the width of T: 8-byte types need the wide status words
+    const uint64_t prefix = block_lookback<sizeof(T) == 8>(lb, tile, total, &tile_prefix);
+    uint64_t run = ubits(first) + prefix + excl;
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
         run += local[k];
@@ -194,8 +194,8 @@ void launch_delta_encode(Ctx* c, int elem_size, const void* x, size_t n, void* o
     if (n < 2) return;
     dispatch_int(elem_size, [&](auto tag) {
         using T = decltype(tag);
-        hipLaunchKernelGGL(k_delta_encode<T>, dim3(grid_for(c, n, 16 / sizeof(T))), dim3(kBlock), 0, c->stream,
-                           static_cast<const T*>(x), n, static_cast<T*>(out));
+        { KernelTimer kt_(c, "k_delta_encode"); hipLaunchKernelGGL(k_delta_encode<T>, dim3(grid_for(c, n, 16 / sizeof(T))), dim3(kBlock), 0, c->stream,
+                           static_cast<const T*>(x), n, static_cast<T*>(out)); }
         HIPC(hipGetLastError());
     });
 }
@@ -204,8 +204,8 @@ void launch_add_scalar(Ctx* c, int elem_size, const void* x, size_t n, uint64_t 
     if (!n) return;
     dispatch_int(elem_size, [&](auto tag) {
         using T = decltype(tag);
-        hipLaunchKernelGGL((k_add_scalar<T, true>), dim3(grid_for(c, n, 16 / sizeof(T))), dim3(kBlock), 0, c->stream,
-                           static_cast<const T*>(x), n, s, static_cast<T*>(out));
+        { KernelTimer kt_(c, "k_add_scalar"); hipLaunchKernelGGL((k_add_scalar<T, true>), dim3(grid_for(c, n, 16 / sizeof(T))), dim3(kBlock), 0, c->stream,
+                           static_cast<const T*>(x), n, s, static_cast<T*>(out)); }
         HIPC(hipGetLastError());
     });
 }
@@ -214,23 +214,23 @@ void launch_sub_scalar(Ctx* c, int elem_size, const void* x, size_t n, uint64_t 
     if (!n) return;
     dispatch_int(elem_size, [&](auto tag) {
         using T = decltype(tag);
-        hipLaunchKernelGGL((k_add_scalar<T, false>), dim3(grid_for(c, n, 16 / sizeof(T))), dim3(kBlock), 0,
-                           c->stream, static_cast<const T*>(x), n, s, static_cast<T*>(out));
+        { KernelTimer kt_(c, "k_add_scalar"); hipLaunchKernelGGL((k_add_scalar<T, false>), dim3(grid_for(c, n, 16 / sizeof(T))), dim3(kBlock), 0,
+                           c->stream, static_cast<const T*>(x), n, s, static_cast<T*>(out)); }
         HIPC(hipGetLastError());
     });
 }
 
 void launch_float_split(Ctx* c, const float* x, size_t n, int32_t* mant, int32_t* expo, int32_t* sign) {
     if (!n) return;
-    hipLaunchKernelGGL(k_float_split, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream, x, n, mant, expo, sign);
+    { KernelTimer kt_(c, "k_float_split"); hipLaunchKernelGGL(k_float_split, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream, x, n, mant, expo, sign); }
     HIPC(hipGetLastError());
 }
 
 void launch_float_compose(Ctx* c, const int32_t* mant, const int32_t* expo, const uint8_t* bits, int sign_const,
                           size_t n, float* out) {
     if (!n) return;
-    hipLaunchKernelGGL(k_float_compose, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream, mant, expo, bits,
-                       sign_const, n, out);
+    { KernelTimer kt_(c, "k_float_compose"); hipLaunchKernelGGL(k_float_compose, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream, mant, expo, bits,
+                       sign_const, n, out); }
     HIPC(hipGetLastError());
 }
 
@@ -238,8 +238,8 @@ void launch_float_to_int(Ctx* c, int dtype, const void* x, size_t n, int precisi
     if (!n) return;
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
-        hipLaunchKernelGGL(k_float_to_int<T>, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream,
-                           static_cast<const T*>(x), n, precision, out);
+        { KernelTimer kt_(c, "k_float_to_int"); hipLaunchKernelGGL(k_float_to_int<T>, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream,
+                           static_cast<const T*>(x), n, precision, out); }
         HIPC(hipGetLastError());
     });
 }
@@ -248,17 +248,24 @@ void launch_int_to_float(Ctx* c, int dtype, const uint8_t* x, size_t n, int prec
     if (!n) return;
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
-        hipLaunchKernelGGL(k_int_to_float<T>, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream,
-                           reinterpret_cast<const int32_t*>(x), n, precision, static_cast<T*>(out));
+        { KernelTimer kt_(c, "k_int_to_float"); hipLaunchKernelGGL(k_int_to_float<T>, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream,
+                           reinterpret_cast<const int32_t*>(x), n, precision, static_cast<T*>(out)); }
         HIPC(hipGetLastError());
     });
+}
+
+LookbackBufs::LookbackBufs(Ctx* c, size_t tiles, bool wide)
+    : mem(c, 8 * tiles * (wide ? 2 : 1) + 8) {
+    launch_fill_zero(c, mem.get(), mem.size());
+    lb = Lookback{mem.get<uint64_t>(), reinterpret_cast<uint32_t*>(mem.get<uint64_t>() + tiles * (wide ? 2 : 1)),
+                  &c->control->error_flag};
 }
 
 void launch_fill_zero(Ctx* c, void* p, size_t bytes) {
     if (!bytes) return;
     size_t words = (bytes + 3) / 4;  // allocations are padded, whole words are safe
-    hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(c, words)), dim3(kBlock), 0, c->stream, static_cast<uint32_t*>(p),
-                       words);
+    { KernelTimer kt_(c, "k_fill_zero"); hipLaunchKernelGGL(k_fill_zero, dim3(grid_for(c, words)), dim3(kBlock), 0, c->stream, static_cast<uint32_t*>(p),
+                       words); }
     HIPC(hipGetLastError());
 }
 
@@ -266,7 +273,7 @@ void launch_gather_copy(Ctx* c, const CopyRange* d_ranges, int n_ranges, uint8_t
     if (n_ranges <= 0 || max_bytes == 0) return;
     unsigned gx = grid_for(c, max_bytes);
     gx = gx > 1024 ? 1024 : gx;
-    hipLaunchKernelGGL(k_gather_copy, dim3(gx, n_ranges), dim3(kBlock), 0, c->stream, d_ranges, dst);
+    { KernelTimer kt_(c, "k_gather_copy"); hipLaunchKernelGGL(k_gather_copy, dim3(gx, n_ranges), dim3(kBlock), 0, c->stream, d_ranges, dst); }
     HIPC(hipGetLastError());
 }
 
@@ -278,13 +285,9 @@ void launch_delta_decode(Ctx* c, int elem_size, const void* first, const uint8_t
             return;
         }
         const size_t tiles = (m + kScanTile - 1) / kScanTile;
-        DevBuf flags(c, 4 * tiles + 4);
-        DevBuf agg(c, 8 * tiles), inc(c, 8 * tiles);
-        launch_fill_zero(c, flags.get(), 4 * tiles + 4);
-        LookbackState<uint64_t> st{flags.get<uint32_t>(), agg.get<uint64_t>(), inc.get<uint64_t>(),
-                                   flags.get<uint32_t>() + tiles, &c->control->error_flag};
-        hipLaunchKernelGGL(k_delta_decode<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                           static_cast<const uint8_t*>(first), d, m, static_cast<T*>(out), st);
+        LookbackBufs lbb(c, tiles, sizeof(T) == 8);
+        { KernelTimer kt_(c, "k_delta_decode"); hipLaunchKernelGGL(k_delta_decode<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                           static_cast<const uint8_t*>(first), d, m, static_cast<T*>(out), lbb.lb); }
         HIPC(hipGetLastError());
     });
 }

@@ -119,8 +119,8 @@ void launch_fl_pack(Ctx* c, int dtype, const void* x, size_t n, const MinMaxResu
         const size_t out_bytes = (size_t)((maxb * (uint64_t)n + W - 1) / W) * sizeof(T);
         const size_t out_words = (out_bytes + 3) / 4;
         const size_t tiles = (n + kFlTile - 1) / kFlTile;
-        hipLaunchKernelGGL(k_fl_pack<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, static_cast<const T*>(x),
-                           n, mm, bits, static_cast<uint32_t*>(out), out_words);
+        { KernelTimer kt_(c, "k_fl_pack"); hipLaunchKernelGGL(k_fl_pack<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, static_cast<const T*>(x),
+                           n, mm, bits, static_cast<uint32_t*>(out), out_words); }
         HIPC(hipGetLastError());
     });
 }
@@ -132,8 +132,8 @@ void launch_fl_unpack(Ctx* c, int dtype, const uint8_t* in, size_t in_bytes, siz
         using T = decltype(tag);
         if (bits <= 0 || bits > 8 * (int)sizeof(T)) fail(DDJ_ECORRUPT, "FL bit width out of range");
         const size_t tiles = (n + kFlTile - 1) / kFlTile;
-        hipLaunchKernelGGL(k_fl_unpack<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, in, in_bytes, n, bits,
-                           static_cast<T*>(out));
+        { KernelTimer kt_(c, "k_fl_unpack"); hipLaunchKernelGGL(k_fl_unpack<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, in, in_bytes, n, bits,
+                           static_cast<T*>(out)); }
         HIPC(hipGetLastError());
     });
 }

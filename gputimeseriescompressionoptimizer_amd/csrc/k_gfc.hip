@@ -46,7 +46,7 @@ GfcGeom gfc_geom(bool dbl, long long n) {
 template <typename U>
 __global__ __launch_bounds__(kBlock) void k_gfc_encode(const U* __restrict__ x, long long n, long long N, long long per,
                                                        int S, uint8_t* __restrict__ payload,
-                                                       int32_t* __restrict__ offsets, LookbackState<uint64_t> st,
+                                                       int32_t* __restrict__ offsets, Lookback lb,
                                                        uint64_t* d_total, uint32_t n_chunks) {
     constexpr int RPT = kGfcMaxPer / kBlock;  // rows per thread (8)
     constexpr bool dbl = sizeof(U) == 8;
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kBlock) void k_gfc_encode(const U* __restrict__ x, 
     __shared__ uint64_t red[kBlock / kWave];
     __shared__ uint32_t tk;
     __shared__ uint64_t s_prefix;
-    const uint32_t chunk = take_ticket(st, &tk);
+    const uint32_t chunk = take_ticket(lb, &tk);
     const long long c0 = (long long)chunk * per;
     long long c1 = c0 + per;
     c1 = c1 < N ? c1 : N;
@@ -120,8 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_gfc_encode(const U* __restrict__ x, 
             pos += nbs[k];
         }
     }
-    if (threadIdx.x == 0) s_prefix = lookback_prefix<uint64_t>(st, chunk, chunk_bytes);
-    __syncthreads();
+    block_lookback<false>(lb, chunk, chunk_bytes, &s_prefix);
     const uint64_t O = s_prefix;
     for (uint32_t i = threadIdx.x; i < (uint32_t)chunk_bytes; i += kBlock) payload[O + i] = out[i];
     if (threadIdx.x == 0) {
@@ -134,7 +133,7 @@ template <typename U>
 __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict__ payload, size_t payload_bytes,
                                                        const uint8_t* __restrict__ offsets, long long n, long long N,
                                                        long long per, int S, U* __restrict__ out,
-                                                       LookbackState<uint64_t> st) {
+                                                       Lookback lb) {
     constexpr int RPT = kGfcMaxPer / kBlock;
     constexpr bool dbl = sizeof(U) == 8;
     constexpr int kMaxBytes = kGfcMaxPer / 32 * (16 + 32 * (int)sizeof(U));
@@ -145,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     __shared__ uint32_t tk;
     __shared__ uint64_t s_prefix;
     __shared__ int s_bad;
-    const uint32_t chunk = take_ticket(st, &tk);
+    const uint32_t chunk = take_ticket(lb, &tk);
     const long long c0 = (long long)chunk * per;
     long long c1 = c0 + per;
     c1 = c1 < N ? c1 : N;
@@ -157,11 +156,8 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     long long cb = (long long)end - start_scr;
     if (cb < 0 || cb > kMaxBytes) cb = 0;  // corrupt: decode as empty (the host validates totals)
     const uint64_t chunk_bytes = (uint64_t)cb;
-    if (threadIdx.x == 0) {
-        s_prefix = lookback_prefix<uint64_t>(st, chunk, chunk_bytes);
-        s_bad = 0;
-    }
-    __syncthreads();
+    if (threadIdx.x == 0) s_bad = 0;
+    block_lookback<false>(lb, chunk, chunk_bytes, &s_prefix);
     const uint64_t P = s_prefix;
     for (uint32_t i = threadIdx.x; i < (uint32_t)chunk_bytes; i += kBlock)
         buf[i] = (P + i < payload_bytes) ? payload[P + i] : 0;
@@ -227,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     __syncthreads();
     for (int i = threadIdx.x; i < rows; i += kBlock)
         if (c0 + i < n) out[c0 + i] = vals[i];
-    if (threadIdx.x == 0 && s_bad) __hip_atomic_store(st.error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0 && s_bad) __hip_atomic_store(lb.error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace
@@ -235,21 +231,19 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
 void gfc_encode(Ctx* c, bool dbl, const void* x, size_t n, DevBuf& meta, DevBuf& offsets, DevBuf& payload) {
     const GfcGeom g = gfc_geom(dbl, (long long)n);
     const size_t tiles = (size_t)g.Wc;
-    DevBuf flags(c, 4 * tiles + 4), agg(c, 8 * tiles), inc(c, 8 * tiles), total(c, 8);
-    launch_fill_zero(c, flags.get(), 4 * tiles + 4);
-    LookbackState<uint64_t> st{flags.get<uint32_t>(), agg.get<uint64_t>(), inc.get<uint64_t>(),
-                               flags.get<uint32_t>() + tiles, &c->control->error_flag};
+    LookbackBufs lbb(c, tiles);
+    DevBuf total(c, 8);
     offsets = DevBuf(c, 4 * tiles);
     const size_t worst = (size_t)(g.N / 32) * (16 + 32 * (dbl ? 8 : 4));
     payload = DevBuf(c, worst);
     if (dbl)
-        hipLaunchKernelGGL(k_gfc_encode<uint64_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+        { KernelTimer kt_(c, "k_gfc_encode"); hipLaunchKernelGGL(k_gfc_encode<uint64_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
                            static_cast<const uint64_t*>(x), g.n, g.N, g.per, (int)g.S, payload.get<uint8_t>(),
-                           offsets.get<int32_t>(), st, total.get<uint64_t>(), (uint32_t)tiles);
+                           offsets.get<int32_t>(), lbb.lb, total.get<uint64_t>(), (uint32_t)tiles); }
     else
-        hipLaunchKernelGGL(k_gfc_encode<uint32_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+        { KernelTimer kt_(c, "k_gfc_encode"); hipLaunchKernelGGL(k_gfc_encode<uint32_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
                            static_cast<const uint32_t*>(x), g.n, g.N, g.per, (int)g.S, payload.get<uint8_t>(),
-                           offsets.get<int32_t>(), st, total.get<uint64_t>(), (uint32_t)tiles);
+                           offsets.get<int32_t>(), lbb.lb, total.get<uint64_t>(), (uint32_t)tiles); }
     HIPC(hipGetLastError());
     int32_t h_meta[3] = {(int32_t)g.blocks, (int32_t)g.wpb, (int32_t)n};
     meta = make_buf(c, h_meta, 12);
@@ -278,19 +272,16 @@ void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const 
     }
     const size_t tiles = (size_t)g.Wc;
     if (off_bytes < 4 * tiles) fail(DDJ_ECORRUPT, "GFC offsets shorter than the chunk count");
-    DevBuf flags(c, 4 * tiles + 4), agg(c, 8 * tiles), inc(c, 8 * tiles);
-    launch_fill_zero(c, flags.get(), 4 * tiles + 4);
+    LookbackBufs lbb(c, tiles);
     uint32_t zero = 0;
     h2d(c, &c->control->error_flag, &zero, 4);
-    LookbackState<uint64_t> st{flags.get<uint32_t>(), agg.get<uint64_t>(), inc.get<uint64_t>(),
-                               flags.get<uint32_t>() + tiles, &c->control->error_flag};
     out = DevBuf(c, (size_t)n * (dbl ? 8 : 4));
     if (dbl)
-        hipLaunchKernelGGL(k_gfc_decode<uint64_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
-                           payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<uint64_t>(), st);
+        { KernelTimer kt_(c, "k_gfc_decode"); hipLaunchKernelGGL(k_gfc_decode<uint64_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
+                           payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<uint64_t>(), lbb.lb); }
     else
-        hipLaunchKernelGGL(k_gfc_decode<uint32_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
-                           payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<uint32_t>(), st);
+        { KernelTimer kt_(c, "k_gfc_decode"); hipLaunchKernelGGL(k_gfc_decode<uint32_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
+                           payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<uint32_t>(), lbb.lb); }
     HIPC(hipGetLastError());
     uint32_t err = 0;
     d2h(c, &err, &c->control->error_flag, 4);

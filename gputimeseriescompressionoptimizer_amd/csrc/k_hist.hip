@@ -9,8 +9,11 @@
 // rows that find no LDS slot go straight to the global table.  A 64-bit
 // value equal to the empty marker is counted on the side.  Floats count
 // -0.0 with +0.0 (operator==), reporting -0.0 when one was seen (it sorts
-// first in the reference).  If the global table fills up the host retries
-// with a table sized for every row distinct.
+// first in the reference).  A NaN equals nothing, so each NaN row is a bin
+// of count 1: they stay out of the table, and every workgroup keeps the four
+// smallest (radix order) NaN rows, which is all the top-K can ever take from
+// them.  If the global table fills up the host retries with a table sized
+// for every row distinct.
 #include "dispatch.h"
 #include "k_common.h"
 #include "kernels.h"
@@ -26,6 +29,7 @@ constexpr int kMaxK = 4;
 struct HistCtl {
     unsigned long long distinct;
     unsigned long long special;  // rows whose 64-bit pattern equals kEmpty
+    unsigned long long nans;     // NaN rows (floats)
     uint32_t overflow;
     uint32_t negzero;
 };
@@ -72,11 +76,71 @@ __device__ bool global_insert(unsigned long long* keys, unsigned long long* coun
     return false;
 }
 
+// Ascending list of the (up to) four smallest keys seen, with multiplicity.
+struct Small4 {
+    uint64_t l[kMaxK];
+    int c = 0;
+    __device__ __forceinline__ void insert(uint64_t k) {
+#pragma unroll
+        for (int j = 0; j < kMaxK; j++) {
+            if (j < c) {
+                if (k < l[j]) {
+                    const uint64_t t = l[j];
+                    l[j] = k;
+                    k = t;
+                }
+            } else if (j == c) {
+                l[j] = k;
+            }
+        }
+        c = c < kMaxK ? c + 1 : kMaxK;
+    }
+    __device__ __forceinline__ void pop() {
+#pragma unroll
+        for (int j = 0; j + 1 < kMaxK; j++) l[j] = l[j + 1];
+        c--;
+    }
+};
+
+// Four smallest keys of the workgroup (each thread holds a Small4) into
+// out_keys[0..*out_n).  Wave rounds take the smallest head, then thread 0
+// merges the waves' lists.
+__device__ void block_small4(Small4& s, uint64_t* sm_keys /* kBlock/kWave * kMaxK */, int* sm_cnt /* kBlock/kWave */,
+                             uint64_t* out_keys, uint32_t* out_n) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    int got = 0;
+    for (int r = 0; r < kMaxK; r++) {
+        const bool has = s.c > 0;
+        const uint64_t h = has ? s.l[0] : ~0ull;
+        const uint64_t m = wave_min(h);
+        const uint64_t who = __ballot(has && h == m);
+        if (who == 0) break;
+        if (lane == __builtin_ctzll(who)) s.pop();
+        if (lane == 0) sm_keys[wid * kMaxK + r] = m;
+        got++;
+    }
+    if (lane == 0) sm_cnt[wid] = got;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        Small4 b;
+        for (int w = 0; w < kBlock / kWave; w++)
+            for (int r = 0; r < sm_cnt[w]; r++) b.insert(sm_keys[w * kMaxK + r]);
+        for (int r = 0; r < b.c; r++) out_keys[r] = b.l[r];
+        *out_n = (uint32_t)b.c;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, size_t n, unsigned long long* keys,
-                                                       unsigned long long* counts, uint64_t cap_mask, HistCtl* ctl) {
+                                                       unsigned long long* counts, uint64_t cap_mask, HistCtl* ctl,
+                                                       uint64_t* nan_keys /* [grid * 4] */,
+                                                       uint32_t* nan_n /* [grid] */) {
     __shared__ unsigned long long skey[kLdsSlots];
     __shared__ uint32_t scnt[kLdsSlots];
+    __shared__ uint64_t s_nk[kBlock / kWave * kMaxK];
+    __shared__ int s_nc[kBlock / kWave];
+    Small4 nan4;
+    unsigned long long nans = 0;
     for (int i = threadIdx.x; i < kLdsSlots; i += kBlock) {
         skey[i] = kEmpty;
         scnt[i] = 0;
@@ -86,7 +150,15 @@ __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, 
     unsigned long long special = 0;
     const size_t stride = (size_t)gridDim.x * kBlock;
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const uint64_t key = hist_key(x[i], negzero);
+        const T v = x[i];
+        if constexpr (std::is_floating_point<T>::value) {
+            if (v != v) {
+                nan4.insert(order_key(v));
+                nans++;
+                continue;
+            }
+        }
+        const uint64_t key = hist_key(v, negzero);
         if (key == kEmpty) {
             special++;
             continue;
@@ -109,10 +181,14 @@ __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, 
         if (skey[s] != kEmpty) global_insert(keys, counts, cap_mask, skey[s], scnt[s], ctl);
     const uint64_t nz = wave_or((uint64_t)negzero);
     const unsigned long long sp = wave_sum(special);
+    const unsigned long long nn = wave_sum(nans);
     if ((threadIdx.x & (kWave - 1)) == 0) {
         if (nz) atomicOr(&ctl->negzero, 1u);
         if (sp) atomicAdd(&ctl->special, sp);
+        if (nn) atomicAdd(&ctl->nans, nn);
     }
+    if constexpr (std::is_floating_point<T>::value)
+        block_small4(nan4, s_nk, s_nc, nan_keys + (size_t)blockIdx.x * kMaxK, nan_n + blockIdx.x);
 }
 
 struct Cand {
@@ -144,7 +220,8 @@ template <typename T> __device__ __forceinline__ void push(Cand* top, int K, Can
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_hist_topk(const unsigned long long* keys, const unsigned long long* counts,
                                                       uint64_t cap, int K, Cand* partials, uint32_t* counter,
-                                                      const HistCtl* ctl, Cand* out) {
+                                                      const HistCtl* ctl, const uint64_t* nan_keys,
+                                                      const uint32_t* nan_n, uint32_t nan_blocks, Cand* out) {
     __shared__ Cand sc[kBlock * kMaxK];
     __shared__ bool last;
     Cand top[kMaxK];
@@ -183,6 +260,14 @@ __global__ __launch_bounds__(kBlock) void k_hist_topk(const unsigned long long* 
         push<T>(b, K, c);
     }
     if (ctl->special) push<T>(b, K, Cand{ctl->special, kEmpty});
+    if constexpr (std::is_floating_point<T>::value) {
+        if (ctl->nans)  // every NaN row is a bin of count 1
+            for (uint32_t g = 0; g < nan_blocks; g++)
+                for (uint32_t r = 0; r < nan_n[g]; r++) {
+                    const T v = from_order_key<T>(nan_keys[(size_t)g * kMaxK + r]);
+                    push<T>(b, K, Cand{1ull, ubits(v)});
+                }
+    }
     for (int i = 0; i < K; i++) out[i] = b[i];
     __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -204,15 +289,17 @@ void topk_most_frequent(Ctx* c, int dtype, const void* x, size_t n, int K, uint6
         for (int attempt = 0; attempt < 2; attempt++) {
             DevBuf keys(c, 8 * cap), counts(c, 8 * cap), ctl(c, sizeof(HistCtl));
             const unsigned fg = (unsigned)std::min<uint64_t>((cap + kBlock - 1) / kBlock, (uint64_t)c->cu_count * 16);
-            hipLaunchKernelGGL(k_fill_u64, dim3(fg), dim3(kBlock), 0, c->stream, keys.get<unsigned long long>(), cap,
-                               (unsigned long long)kEmpty);
+            { KernelTimer kt_(c, "k_fill_u64"); hipLaunchKernelGGL(k_fill_u64, dim3(fg), dim3(kBlock), 0, c->stream, keys.get<unsigned long long>(), cap,
+                               (unsigned long long)kEmpty); }
             launch_fill_zero(c, counts.get(), 8 * cap);
             launch_fill_zero(c, ctl.get(), sizeof(HistCtl));
             size_t g = (n + kBlock * 16 - 1) / (kBlock * 16);
             g = std::max<size_t>(1, std::min<size_t>(g, (size_t)c->cu_count * 4));
-            hipLaunchKernelGGL(k_hist_count<T>, dim3((unsigned)g), dim3(kBlock), 0, c->stream,
+            DevBuf nan_keys(c, 8 * kMaxK * g), nan_n(c, 4 * g);
+            { KernelTimer kt_(c, "k_hist_count"); hipLaunchKernelGGL(k_hist_count<T>, dim3((unsigned)g), dim3(kBlock), 0, c->stream,
                                static_cast<const T*>(x), n, keys.get<unsigned long long>(),
-                               counts.get<unsigned long long>(), cap - 1, ctl.get<HistCtl>());
+                               counts.get<unsigned long long>(), cap - 1, ctl.get<HistCtl>(),
+                               nan_keys.get<uint64_t>(), nan_n.get<uint32_t>()); }
             HIPC(hipGetLastError());
             HistCtl h;
             d2h(c, &h, ctl.get(), sizeof h);
@@ -225,13 +312,14 @@ void topk_most_frequent(Ctx* c, int dtype, const void* x, size_t n, int K, uint6
             }
             const unsigned tg = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((cap + kBlock - 1) / kBlock, 128));
             DevBuf partials(c, sizeof(Cand) * K * tg), out(c, sizeof(Cand) * K);
-            hipLaunchKernelGGL(k_hist_topk<T>, dim3(tg), dim3(kBlock), 0, c->stream, keys.get<unsigned long long>(),
+            { KernelTimer kt_(c, "k_hist_topk"); hipLaunchKernelGGL(k_hist_topk<T>, dim3(tg), dim3(kBlock), 0, c->stream, keys.get<unsigned long long>(),
                                counts.get<unsigned long long>(), cap, K, partials.get<Cand>(), &c->control->counters[2],
-                               ctl.get<HistCtl>(), out.get<Cand>());
+                               ctl.get<HistCtl>(), nan_keys.get<uint64_t>(), nan_n.get<uint32_t>(), (uint32_t)g,
+                               out.get<Cand>()); }
             HIPC(hipGetLastError());
             Cand res[kMaxK];
             d2h(c, res, out.get(), sizeof(Cand) * K);
-            int nd = (int)h.distinct + (h.special ? 1 : 0);
+            int nd = (int)(h.distinct + (h.special ? 1 : 0) + h.nans);
             if (n_distinct_host) *n_distinct_host = nd;
             // fewer than K distinct values: repeat the last real key
             int real = std::min(K, nd);

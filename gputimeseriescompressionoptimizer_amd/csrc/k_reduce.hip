@@ -63,12 +63,51 @@ __device__ __forceinline__ uint32_t rd32(const uint32_t* p) {
 }
 
 // ------------------------------------------------------------------ minmax
+// Keys of the reference's minimum and maximum: thrust::minmax_element
+// (util/statistics/cuda_array_statistics.cu:20-28) under operator<, taken as
+// the sequential fold m = x[0]; m = x[i] < m ? x[i] : m.  For floats that
+// means: NaN rows never win, so a NaN in row 0 is both the minimum and the
+// maximum; -0.0 and +0.0 tie, and the first zero in row order gives the sign.
+// Rows feed add(); `zf` keeps (row << 1 | sign) of the first zero.
+template <typename T> struct FoldKeys {
+    uint64_t mn = ~0ull, mx = 0, zf = ~0ull;
+    __device__ __forceinline__ void add(T v, size_t row) {
+        if constexpr (std::is_floating_point<T>::value) {
+            if (v != v) return;
+            if (v == T(0)) {
+                const uint64_t z = ((uint64_t)row << 1) | (__builtin_signbit(v) ? 1ull : 0ull);
+                zf = z < zf ? z : zf;
+                v = T(0);
+            }
+        }
+        const uint64_t key = order_key(v);
+        mn = key < mn ? key : mn;
+        mx = key > mx ? key : mx;
+    }
+};
+
+// Final keys from the reduced ones and row 0.
+template <typename T> __device__ __forceinline__ void fold_fixup(const T* x, uint64_t& mn, uint64_t& mx, uint64_t zf) {
+    if constexpr (std::is_floating_point<T>::value) {
+        const T x0 = x[0];
+        if (x0 != x0) {
+            mn = mx = order_key(x0);
+            return;
+        }
+        if (zf != ~0ull && (zf & 1)) {
+            const uint64_t kz = order_key(T(0)), knz = order_key(-T(0));
+            if (mn == kz) mn = knz;
+            if (mx == kz) mx = knz;
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size_t n, uint64_t* __restrict__ partials,
                                                    uint32_t* counter, MinMaxResult* out) {
     __shared__ uint64_t sm[kBlock / kWave];
     __shared__ bool last;
-    uint64_t mn = ~0ull, mx = 0;
+    FoldKeys<T> f;
     const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x, stride = (size_t)gridDim.x * kBlock;
     constexpr int V = 16 / sizeof(T);
     size_t done = 0;
@@ -80,36 +119,34 @@ __global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size
             T vals[V];
             __builtin_memcpy(vals, &v, 16);
 #pragma unroll
-            for (int k = 0; k < V; k++) {
-                uint64_t key = order_key(vals[k]);
-                mn = key < mn ? key : mn;
-                mx = key > mx ? key : mx;
-            }
+            for (int k = 0; k < V; k++) f.add(vals[k], i * V + k);
         }
         done = nv * V;
     }
-    for (size_t i = done + tid; i < n; i += stride) {
-        uint64_t key = order_key(x[i]);
-        mn = key < mn ? key : mn;
-        mx = key > mx ? key : mx;
-    }
-    mn = block_reduce_min(mn, sm);
-    mx = block_reduce_max(mx, sm);
+    for (size_t i = done + tid; i < n; i += stride) f.add(x[i], i);
+    uint64_t mn = block_reduce_min(f.mn, sm);
+    uint64_t mx = block_reduce_max(f.mx, sm);
+    uint64_t zf = block_reduce_min(f.zf, sm);
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = mn;
-        partials[2 * blockIdx.x + 1] = mx;
+        partials[3 * blockIdx.x] = mn;
+        partials[3 * blockIdx.x + 1] = mx;
+        partials[3 * blockIdx.x + 2] = zf;
     }
     if (!last_block_arrives(counter, &last)) return;
     mn = ~0ull;
     mx = 0;
+    zf = ~0ull;
     for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
-        uint64_t a = rd(partials + 2 * b), z = rd(partials + 2 * b + 1);
+        const uint64_t a = rd(partials + 3 * b), z = rd(partials + 3 * b + 1), q = rd(partials + 3 * b + 2);
         mn = a < mn ? a : mn;
         mx = z > mx ? z : mx;
+        zf = q < zf ? q : zf;
     }
     mn = block_reduce_min(mn, sm);
     mx = block_reduce_max(mx, sm);
+    zf = block_reduce_min(zf, sm);
     if (threadIdx.x == 0) {
+        fold_fixup(x, mn, mx, zf);
         out->min_key = mn;
         out->max_key = mx;
         __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -169,17 +206,19 @@ template <typename T> __device__ __forceinline__ uint64_t zigzag_bits(T v) {
 __device__ __forceinline__ uint32_t bitwidth64(uint64_t v) { return v ? 64 - __clzll(v) : 0; }
 
 struct StatsAcc {
-    uint64_t mn, mx, unsorted, rl_eq, runs, sum_bits;
+    uint64_t mn, mx, zf, unsorted, rl_eq, runs, sum_bits;
     double sum_f;
     uint32_t prec, dmax;
 };
 
 template <typename T>
-__device__ __forceinline__ void stats_elem(StatsAcc& a, T v, T next, bool has_next, bool has_next2, uint32_t* sm_hist,
-                                           uint32_t* sm_hll) {
-    uint64_t key = order_key(v);
-    a.mn = key < a.mn ? key : a.mn;
-    a.mx = key > a.mx ? key : a.mx;
+__device__ __forceinline__ void stats_elem(StatsAcc& a, T v, T next, bool has_next, bool has_next2, size_t row,
+                                           uint32_t* sm_hist, uint32_t* sm_hll) {
+    FoldKeys<T> fk{a.mn, a.mx, a.zf};
+    fk.add(v, row);
+    a.mn = fk.mn;
+    a.mx = fk.mx;
+    a.zf = fk.zf;
     if constexpr (std::is_floating_point<T>::value) {
         a.sum_f += (double)v;
         uint32_t p = float_precision((float)v);
@@ -220,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const T* __restrict__ x, size_
     for (int i = threadIdx.x; i < 65; i += kBlock) sm_hist[i] = 0;
     for (int i = threadIdx.x; i < 64; i += kBlock) sm_hll[i] = 0;
     __syncthreads();
-    StatsAcc a{~0ull, 0, 0, 0, 0, 0, 0.0, 0, 0};
+    StatsAcc a{~0ull, 0, ~0ull, 0, 0, 0, 0, 0.0, 0, 0};
     const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x, stride = (size_t)gridDim.x * kBlock;
     constexpr int V = 16 / sizeof(T);
     size_t done = 0;
@@ -235,17 +274,18 @@ __global__ __launch_bounds__(kBlock) void k_stats(const T* __restrict__ x, size_
             vals[V] = (base + V < n) ? x[base + V] : T(0);
 #pragma unroll
             for (int k = 0; k < V; k++)
-                stats_elem(a, vals[k], vals[k + 1], base + k + 1 < n, base + k + 2 < n, sm_hist, sm_hll);
+                stats_elem(a, vals[k], vals[k + 1], base + k + 1 < n, base + k + 2 < n, base + k, sm_hist, sm_hll);
         }
         done = nv * V;
     }
     for (size_t i = done + tid; i < n; i += stride) {
         T v = x[i];
         T nx = (i + 1 < n) ? x[i + 1] : T(0);
-        stats_elem(a, v, nx, i + 1 < n, i + 2 < n, sm_hist, sm_hll);
+        stats_elem(a, v, nx, i + 1 < n, i + 2 < n, i, sm_hist, sm_hll);
     }
     uint64_t mn = block_reduce_min(a.mn, sm);
     uint64_t mx = block_reduce_max(a.mx, sm);
+    uint64_t zf = block_reduce_min(a.zf, sm);
     uint64_t uns = block_reduce_max(a.unsorted, sm);
     uint64_t rl = block_reduce_sum<uint64_t>(a.rl_eq, sm);
     uint64_t runs = block_reduce_sum<uint64_t>(a.runs, sm);
@@ -257,6 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const T* __restrict__ x, size_
     if (threadIdx.x == 0) {
         p->min_key = mn;
         p->max_key = mx;
+        p->first_zero = zf;
         p->unsorted = uns;
         p->rl_eq = rl;
         p->runs = runs;
@@ -269,12 +310,14 @@ __global__ __launch_bounds__(kBlock) void k_stats(const T* __restrict__ x, size_
     for (int i = threadIdx.x; i < 64; i += kBlock) p->hll[i] = sm_hll[i];
     __threadfence();
     if (!last_block_arrives(counter, &last)) return;
-    StatsAcc f{~0ull, 0, 0, 0, 0, 0, 0.0, 0, 0};
+    StatsAcc f{~0ull, 0, ~0ull, 0, 0, 0, 0, 0.0, 0, 0};
     for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
         const DevStats* q = partials + b;
         const uint64_t v0 = rd(&q->min_key), v1 = rd(&q->max_key);
         f.mn = v0 < f.mn ? v0 : f.mn;
         f.mx = v1 > f.mx ? v1 : f.mx;
+        const uint64_t z1 = rd(&q->first_zero);
+        f.zf = z1 < f.zf ? z1 : f.zf;
         f.unsorted |= rd(&q->unsorted);
         f.rl_eq += rd(&q->rl_eq);
         f.runs += rd(&q->runs);
@@ -289,6 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const T* __restrict__ x, size_
     }
     mn = block_reduce_min(f.mn, sm);
     mx = block_reduce_max(f.mx, sm);
+    zf = block_reduce_min(f.zf, sm);
     uns = block_reduce_max(f.unsorted, sm);
     rl = block_reduce_sum<uint64_t>(f.rl_eq, sm);
     runs = block_reduce_sum<uint64_t>(f.runs, sm);
@@ -308,8 +352,10 @@ __global__ __launch_bounds__(kBlock) void k_stats(const T* __restrict__ x, size_
     for (int i = threadIdx.x; i < 65; i += kBlock) out->bit_hist[i] = fin_hist[i];
     for (int i = threadIdx.x; i < 64; i += kBlock) out->hll[i] = fin_hll[i];
     if (threadIdx.x == 0) {
+        fold_fixup(x, mn, mx, zf);
         out->min_key = mn;
         out->max_key = mx;
+        out->first_zero = zf;
         out->unsorted = uns;
         out->rl_eq = rl;
         out->runs = runs + (n > 0 ? 1 : 0);
@@ -335,9 +381,9 @@ void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_r
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
         const int grid = reduce_grid(c, n, sizeof(T));
-        DevBuf partials(c, 16 * (size_t)grid);
-        hipLaunchKernelGGL(k_minmax<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
-                           partials.get<uint64_t>(), &c->control->counters[0], d_result);
+        DevBuf partials(c, 24 * (size_t)grid);
+        { KernelTimer kt_(c, "k_minmax"); hipLaunchKernelGGL(k_minmax<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
+                           partials.get<uint64_t>(), &c->control->counters[0], d_result); }
         HIPC(hipGetLastError());
     });
 }
@@ -357,8 +403,8 @@ void launch_stats(Ctx* c, int dtype, const void* x, size_t n, DevStats* d_out) {
         using T = decltype(tag);
         const int grid = std::min(reduce_grid(c, n, sizeof(T)), 2 * c->cu_count);
         DevBuf partials(c, sizeof(DevStats) * (size_t)grid);
-        hipLaunchKernelGGL(k_stats<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
-                           partials.get<DevStats>(), &c->control->counters[1], d_out);
+        { KernelTimer kt_(c, "k_stats"); hipLaunchKernelGGL(k_stats<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
+                           partials.get<DevStats>(), &c->control->counters[1], d_out); }
         HIPC(hipGetLastError());
     });
 }

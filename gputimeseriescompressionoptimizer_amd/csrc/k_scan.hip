@@ -16,16 +16,10 @@ namespace {
 
 constexpr int kItems = 16;
 constexpr int kTile = kBlock * kItems;
-
-struct LB {
-    DevBuf flags, agg, inc;
-    LookbackState<uint64_t> st;
-    LB(Ctx* c, size_t tiles) : flags(c, 4 * tiles + 4), agg(c, 8 * tiles), inc(c, 8 * tiles) {
-        launch_fill_zero(c, flags.get(), 4 * tiles + 4);
-        st = LookbackState<uint64_t>{flags.get<uint32_t>(), agg.get<uint64_t>(), inc.get<uint64_t>(),
-                                     flags.get<uint32_t>() + tiles, &c->control->error_flag};
-    }
-};
+// LDS index of tile row i: one pad slot every 16 rows spreads the blocked
+// (16 consecutive rows per thread) accesses over the banks.
+constexpr int kTilePad = kTile + kTile / 16;
+__device__ __forceinline__ int P(int i) { return i + (i >> 4); }
 
 template <typename T> struct Pred {
     int kind;
@@ -63,17 +57,17 @@ template <typename T, bool CODES, bool WRITE_NO>
 __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_t n, Pred<T> p,
                                                   uint8_t* __restrict__ stencil, T* __restrict__ yes,
                                                   T* __restrict__ no, uint8_t* __restrict__ codes,
-                                                  LookbackState<uint64_t> st, uint64_t* d_count) {
-    __shared__ T buf[kTile];
+                                                  Lookback lb, uint64_t* d_count) {
+    __shared__ T buf[kTilePad];
     __shared__ uint8_t cbuf[CODES ? kTile : 1];
     __shared__ uint64_t red[kBlock / kWave];
     __shared__ uint32_t tk;
     __shared__ uint64_t s_prefix;
-    const uint32_t tile = take_ticket(st, &tk);
+    const uint32_t tile = take_ticket(lb, &tk);
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     const T padv = x[n - 1];  // stencil padding repeats the last row (reference stencil.cu:26-27)
-    for (int i = threadIdx.x; i < kTile; i += kBlock) buf[i] = i < cnt ? x[base + i] : padv;
+    for (int i = threadIdx.x; i < kTile; i += kBlock) buf[P(i)] = i < cnt ? x[base + i] : padv;
     __syncthreads();
     T v[kItems];
     uint32_t mask = 0;
@@ -81,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     int c = 0;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
-        v[k] = buf[threadIdx.x * kItems + k];
+        v[k] = buf[P(threadIdx.x * kItems + k)];
         uint32_t code;
         bool f = p(v[k], code);
         mask |= (uint32_t)f << k;
@@ -94,8 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     if (tile == 0 && threadIdx.x == 0) stencil[0] = (uint8_t)(n % 8);
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)c, &total, red);
-    if (threadIdx.x == 0) s_prefix = lookback_prefix<uint64_t>(st, tile, total);
-    __syncthreads();
+    block_lookback<false>(lb, tile, total, &s_prefix);
     const uint64_t Y0 = s_prefix;
     int yr = (int)excl, nr = threadIdx.x * kItems - (int)excl;
     const int nbase = CODES ? 0 : (int)total;
@@ -106,10 +99,10 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
             if (CODES)
                 cbuf[yr] = (uint8_t)((cpack >> (4 * k)) & 15);
             else
-                buf[yr] = v[k];
+                buf[P(yr)] = v[k];
             yr++;
         } else {
-            buf[nbase + nr] = v[k];
+            buf[P(nbase + nr)] = v[k];
             nr++;
         }
     }
@@ -119,11 +112,11 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
             if (CODES)
                 codes[Y0 + i] = cbuf[i];
             else
-                yes[Y0 + i] = buf[i];
+                yes[Y0 + i] = buf[P(i)];
         }
     if (WRITE_NO) {
         const size_t N0 = base - Y0;
-        for (int i = threadIdx.x; i < cnt - (int)total; i += kBlock) no[N0 + i] = buf[nbase + i];
+        for (int i = threadIdx.x; i < cnt - (int)total; i += kBlock) no[N0 + i] = buf[P(nbase + i)];
     }
     if (threadIdx.x == 0 && base + cnt == n) d_count[0] = Y0 + total;
 }
@@ -133,12 +126,12 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
 template <typename T, bool CONST_NO>
 __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bits, size_t n, const T* __restrict__ yes,
                                                   const T* __restrict__ no, T no_const, T* __restrict__ out,
-                                                  LookbackState<uint64_t> st) {
-    __shared__ T buf[kTile];
+                                                  Lookback lb) {
+    __shared__ T buf[kTilePad];
     __shared__ uint64_t red[kBlock / kWave];
     __shared__ uint32_t tk;
     __shared__ uint64_t s_prefix;
-    const uint32_t tile = take_ticket(st, &tk);
+    const uint32_t tile = take_ticket(lb, &tk);
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     const size_t g0 = base + (size_t)threadIdx.x * kItems;
@@ -152,29 +145,28 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
         mask &= (1u << mine) - 1u;
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &total, red);
-    if (threadIdx.x == 0) s_prefix = lookback_prefix<uint64_t>(st, tile, total);
-    __syncthreads();
+    block_lookback<false>(lb, tile, total, &s_prefix);
     const uint64_t Y0 = s_prefix;
     const size_t N0 = base - Y0;
     for (int i = threadIdx.x; i < cnt; i += kBlock) {
         if (i < (int)total)
-            buf[i] = yes[Y0 + i];
+            buf[P(i)] = yes[Y0 + i];
         else
-            buf[i] = CONST_NO ? no_const : no[N0 + (i - (int)total)];
+            buf[P(i)] = CONST_NO ? no_const : no[N0 + (i - (int)total)];
     }
     __syncthreads();
     T v[kItems];
     int yr = (int)excl, nr = (int)total + threadIdx.x * kItems - (int)excl;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
-        if (threadIdx.x * kItems + k < cnt) v[k] = ((mask >> k) & 1) ? buf[yr++] : buf[nr++];
+        if (threadIdx.x * kItems + k < cnt) v[k] = ((mask >> k) & 1) ? buf[P(yr++)] : buf[P(nr++)];
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kItems; k++)
-        if (threadIdx.x * kItems + k < cnt) buf[threadIdx.x * kItems + k] = v[k];
+        if (threadIdx.x * kItems + k < cnt) buf[P(threadIdx.x * kItems + k)] = v[k];
     __syncthreads();
-    for (int i = threadIdx.x; i < cnt; i += kBlock) out[base + i] = buf[i];
+    for (int i = threadIdx.x; i < cnt; i += kBlock) out[base + i] = buf[P(i)];
 }
 
 // _compressUniqueKernel word layout (reference compression/unique/unique_encoding.cu:48-76):
@@ -216,18 +208,18 @@ __global__ __launch_bounds__(kBlock) void k_unpack_codes(const uint8_t* __restri
 // at row 0 and wherever !(x[i] == x[i-1]); the run's value is its first row.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_rle_encode(const T* __restrict__ x, size_t n, T* __restrict__ values,
-                                                       uint32_t* __restrict__ starts, LookbackState<uint64_t> st,
+                                                       uint32_t* __restrict__ starts, Lookback lb,
                                                        uint64_t* d_runs) {
-    __shared__ T buf[kTile + 1];
+    __shared__ T buf[kTilePad + 1];
     __shared__ uint64_t red[kBlock / kWave];
     __shared__ uint32_t tk;
     __shared__ uint64_t s_prefix;
-    const uint32_t tile = take_ticket(st, &tk);
+    const uint32_t tile = take_ticket(lb, &tk);
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     for (int i = threadIdx.x; i < kTile + 1; i += kBlock) {
         const long long g = (long long)base + i - 1;
-        buf[i] = (g >= 0 && i <= cnt) ? x[g] : T(0);
+        buf[P(i)] = (g >= 0 && i <= cnt) ? x[g] : T(0);
     }
     __syncthreads();
     uint32_t mask = 0;
@@ -235,20 +227,19 @@ __global__ __launch_bounds__(kBlock) void k_rle_encode(const T* __restrict__ x, 
     for (int k = 0; k < kItems; k++) {
         const int idx = threadIdx.x * kItems + k;
         if (idx < cnt) {
-            const bool f = (base + idx == 0) || !(buf[idx + 1] == buf[idx]);
+            const bool f = (base + idx == 0) || !(buf[P(idx + 1)] == buf[P(idx)]);
             mask |= (uint32_t)f << k;
         }
     }
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &total, red);
-    if (threadIdx.x == 0) s_prefix = lookback_prefix<uint64_t>(st, tile, total);
-    __syncthreads();
+    block_lookback<false>(lb, tile, total, &s_prefix);
     uint64_t r = s_prefix + excl;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
         if ((mask >> k) & 1) {
             const int idx = threadIdx.x * kItems + k;
-            values[r] = buf[idx + 1];
+            values[r] = buf[P(idx + 1)];
             starts[r] = (uint32_t)(base + idx);
             r++;
         }
@@ -267,13 +258,13 @@ __global__ __launch_bounds__(kBlock) void k_rle_lengths(const uint32_t* __restri
 
 // Exclusive scan of int32 run lengths into uint64 offsets[R+1].
 __global__ __launch_bounds__(kBlock) void k_scan_lengths(const uint8_t* __restrict__ lens, size_t R,
-                                                         uint64_t* __restrict__ offsets, LookbackState<uint64_t> st) {
+                                                         uint64_t* __restrict__ offsets, Lookback lb) {
     __shared__ uint64_t buf[kTile + kTile / 32];
     __shared__ uint64_t red[kBlock / kWave];
     __shared__ uint32_t tk;
     __shared__ uint64_t s_prefix;
     auto pad = [](int i) { return i + (i >> 5); };
-    const uint32_t tile = take_ticket(st, &tk);
+    const uint32_t tile = take_ticket(lb, &tk);
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((R - base) < (size_t)kTile ? (R - base) : (size_t)kTile);
     const bool aligned = (reinterpret_cast<uintptr_t>(lens) & 3) == 0;
@@ -296,8 +287,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_lengths(const uint8_t* __restri
     }
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>(s, &total, red);
-    if (threadIdx.x == 0) s_prefix = lookback_prefix<uint64_t>(st, tile, total);
-    __syncthreads();
+    block_lookback<true>(lb, tile, total, &s_prefix);
     uint64_t run = s_prefix + excl;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
@@ -378,19 +368,19 @@ void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa
         for (int j = 0; j < 4; j++) p.set[j] = bits_to<T>(pa.set_bits[j]);
         p.set_n = pa.set_n;
         const size_t tiles = (n + kTile - 1) / kTile;
-        LB lb(c, tiles);
+        LookbackBufs lbb(c, tiles);
         auto xs = static_cast<const T*>(x);
         auto ys = static_cast<T*>(yes);
         auto ns = static_cast<T*>(no);
         if (codes) {
-            hipLaunchKernelGGL((k_split<T, true, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p,
-                               stencil, ys, ns, codes, lb.st, d_counts);
+            { KernelTimer kt_(c, "k_split"); hipLaunchKernelGGL((k_split<T, true, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p,
+                               stencil, ys, ns, codes, lbb.lb, d_counts); }
         } else if (no) {
-            hipLaunchKernelGGL((k_split<T, false, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p,
-                               stencil, ys, ns, codes, lb.st, d_counts);
+            { KernelTimer kt_(c, "k_split"); hipLaunchKernelGGL((k_split<T, false, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p,
+                               stencil, ys, ns, codes, lbb.lb, d_counts); }
         } else {
-            hipLaunchKernelGGL((k_split<T, false, false>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n,
-                               p, stencil, ys, ns, codes, lb.st, d_counts);
+            { KernelTimer kt_(c, "k_split"); hipLaunchKernelGGL((k_split<T, false, false>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n,
+                               p, stencil, ys, ns, codes, lbb.lb, d_counts); }
         }
         HIPC(hipGetLastError());
     });
@@ -402,15 +392,15 @@ void launch_merge(Ctx* c, int dtype, const uint8_t* bits, size_t n, const uint8_
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
         const size_t tiles = (n + kTile - 1) / kTile;
-        LB lb(c, tiles);
+        LookbackBufs lbb(c, tiles);
         auto ys = reinterpret_cast<const T*>(yes);
         auto ns = reinterpret_cast<const T*>(no);
         if (no)
-            hipLaunchKernelGGL((k_merge<T, false>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, bits, n, ys,
-                               ns, T(0), static_cast<T*>(out), lb.st);
+            { KernelTimer kt_(c, "k_merge"); hipLaunchKernelGGL((k_merge<T, false>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, bits, n, ys,
+                               ns, T(0), static_cast<T*>(out), lbb.lb); }
         else
-            hipLaunchKernelGGL((k_merge<T, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, bits, n, ys,
-                               ns, bits_to<T>(no_const), static_cast<T*>(out), lb.st);
+            { KernelTimer kt_(c, "k_merge"); hipLaunchKernelGGL((k_merge<T, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, bits, n, ys,
+                               ns, bits_to<T>(no_const), static_cast<T*>(out), lbb.lb); }
         HIPC(hipGetLastError());
     });
 }
@@ -418,7 +408,7 @@ void launch_merge(Ctx* c, int dtype, const uint8_t* bits, size_t n, const uint8_
 void launch_pack_codes(Ctx* c, const uint8_t* codes, size_t m, int bits, uint32_t* words) {
     if (!m) return;
     const size_t nw = (m + (32 / bits) - 1) / (32 / bits);
-    hipLaunchKernelGGL(k_pack_codes, dim3(grid_for(c, nw)), dim3(kBlock), 0, c->stream, codes, m, bits, words);
+    { KernelTimer kt_(c, "k_pack_codes"); hipLaunchKernelGGL(k_pack_codes, dim3(grid_for(c, nw)), dim3(kBlock), 0, c->stream, codes, m, bits, words); }
     HIPC(hipGetLastError());
 }
 
@@ -426,8 +416,8 @@ void launch_unpack_codes(Ctx* c, int dtype, const uint8_t* words, size_t m, int 
     if (!m) return;
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
-        hipLaunchKernelGGL(k_unpack_codes<T>, dim3(grid_for(c, m)), dim3(kBlock), 0, c->stream, words, m, bits, dict,
-                           static_cast<T*>(out));
+        { KernelTimer kt_(c, "k_unpack_codes"); hipLaunchKernelGGL(k_unpack_codes<T>, dim3(grid_for(c, m)), dim3(kBlock), 0, c->stream, words, m, bits, dict,
+                           static_cast<T*>(out)); }
         HIPC(hipGetLastError());
     });
 }
@@ -438,15 +428,15 @@ void launch_rle_encode(Ctx* c, int dtype, const void* x, size_t n, int32_t* leng
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
         const size_t tiles = (n + kTile - 1) / kTile;
-        LB lb(c, tiles);
+        LookbackBufs lbb(c, tiles);
         DevBuf starts(c, 4 * n);
-        hipLaunchKernelGGL(k_rle_encode<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                           static_cast<const T*>(x), n, static_cast<T*>(values), starts.get<uint32_t>(), lb.st, d_runs);
+        { KernelTimer kt_(c, "k_rle_encode"); hipLaunchKernelGGL(k_rle_encode<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                           static_cast<const T*>(x), n, static_cast<T*>(values), starts.get<uint32_t>(), lbb.lb, d_runs); }
         HIPC(hipGetLastError());
         uint64_t runs = 0;
         d2h(c, &runs, d_runs, 8);
-        hipLaunchKernelGGL(k_rle_lengths, dim3(grid_for(c, runs)), dim3(kBlock), 0, c->stream, starts.get<uint32_t>(),
-                           runs, n, lengths);
+        { KernelTimer kt_(c, "k_rle_lengths"); hipLaunchKernelGGL(k_rle_lengths, dim3(grid_for(c, runs)), dim3(kBlock), 0, c->stream, starts.get<uint32_t>(),
+                           runs, n, lengths); }
         HIPC(hipGetLastError());
     });
 }
@@ -457,18 +447,18 @@ DevBuf rle_decode_dev(Ctx* c, int dtype, const uint8_t* lengths, const uint8_t* 
     return dispatch_native(dtype, [&](auto tag) -> DevBuf {
         using T = decltype(tag);
         const size_t tiles = (runs + kTile - 1) / kTile;
-        LB lb(c, tiles);
+        LookbackBufs lbb(c, tiles, true);  // int64 sums of int32 lengths
         DevBuf offsets(c, 8 * (runs + 1));
-        hipLaunchKernelGGL(k_scan_lengths, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, lengths, runs,
-                           offsets.get<uint64_t>(), lb.st);
+        { KernelTimer kt_(c, "k_scan_lengths"); hipLaunchKernelGGL(k_scan_lengths, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, lengths, runs,
+                           offsets.get<uint64_t>(), lbb.lb); }
         HIPC(hipGetLastError());
         uint64_t total = 0;
         d2h(c, &total, offsets.get<uint64_t>() + runs, 8);
         DevBuf out(c, total * sizeof(T));
         if (total == 0) return out;
         const size_t etiles = (total + kExpTile - 1) / kExpTile;
-        hipLaunchKernelGGL(k_rle_expand<T>, dim3((unsigned)etiles), dim3(kBlock), 0, c->stream,
-                           offsets.get<uint64_t>(), runs, values, (size_t)total, out.get<T>());
+        { KernelTimer kt_(c, "k_rle_expand"); hipLaunchKernelGGL(k_rle_expand<T>, dim3((unsigned)etiles), dim3(kBlock), 0, c->stream,
+                           offsets.get<uint64_t>(), runs, values, (size_t)total, out.get<T>()); }
         HIPC(hipGetLastError());
         return out;
     });

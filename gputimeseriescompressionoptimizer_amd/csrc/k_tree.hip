@@ -86,7 +86,7 @@ __global__ void k_parse_tree(const uint8_t* blob, uint64_t bytes, ParsedNode* ou
 }  // namespace
 
 void launch_parse_tree(Ctx* c, const uint8_t* blob, uint64_t bytes, ParsedNode* d_out, int max_nodes, int* d_status) {
-    hipLaunchKernelGGL(k_parse_tree, dim3(1), dim3(kWave), 0, c->stream, blob, bytes, d_out, max_nodes, d_status);
+    { KernelTimer kt_(c, "k_parse_tree"); hipLaunchKernelGGL(k_parse_tree, dim3(1), dim3(kWave), 0, c->stream, blob, bytes, d_out, max_nodes, d_status); }
     HIPC(hipGetLastError());
 }
 

@@ -6,6 +6,20 @@
 
 namespace ddj {
 
+// Device state of one decoupled look-back launch: a 64-bit status word per
+// tile (two per tile for wide values), a ticket counter and the context's
+// error flag.  Status words and ticket start at zero.
+struct Lookback {
+    uint64_t* status;
+    uint32_t* ticket;
+    uint32_t* error;
+};
+struct LookbackBufs {
+    DevBuf mem;
+    Lookback lb;
+    LookbackBufs(Ctx* c, size_t tiles, bool wide = false);
+};
+
 // Scratch of a single-pass reduction (per-block partials + result).
 struct MinMaxResult {
     uint64_t min_key;  // order_key of the minimum
@@ -20,6 +34,7 @@ void decode_minmax(int dtype, const MinMaxResult& r, void* mn, void* mx);
 // Device statistics record filled by launch_stats.
 struct DevStats {
     uint64_t min_key, max_key;
+    uint64_t first_zero;   // floats: row << 1 | sign of the first zero (~0: none)
     uint64_t unsorted;     // any x[i] > x[i+1]
     uint64_t rl_eq;        // #{i < n-2 : int(x[i]) == x[i+1]}
     uint64_t runs;         // #{i : i == 0 || x[i] != x[i-1]}

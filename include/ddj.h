@@ -88,6 +88,14 @@ void ddj_ctx_trim(ddj_ctx* ctx);
 /* The HIP stream of the context (hipStream_t), for callers that order their own work. */
 void* ddj_ctx_stream(ddj_ctx* ctx);
 
+/* Per-kernel timing with HIP events recorded on the context stream around
+ * every launch while enabled (no counterpart in the reference; used by the
+ * benchmark to time one kernel inside a timed region). */
+ddj_status ddj_profile_enable(ddj_ctx* ctx, int on);
+ddj_status ddj_profile_kernel(ddj_ctx* ctx, const char* kernel, double* total_ms, uint64_t* launches,
+                              double* algo_bytes);
+ddj_status ddj_profile_reset(ddj_ctx* ctx);
+
 /* ----------------------------------------------------------------- memory */
 ddj_status ddj_buf_alloc(ddj_ctx* ctx, size_t bytes, ddj_buf* out);
 void ddj_buf_free(ddj_ctx* ctx, ddj_buf* buf);
@@ -100,7 +108,9 @@ ddj_status ddj_memcpy_d2d(ddj_ctx* ctx, void* dst_dev, const void* src_dev, size
 ddj_status ddj_encoding_info(int enc, int dtype, uint32_t* n_results, int* return_dtypes /* [2] */);
 
 /* Encoding::Encode(SharedCudaPtr<char>, DataType) (compression/encoding.hpp:28-47).
- * outs[0] = metadata, outs[1..n_results] = outputs; the caller frees them. */
+ * outs[0] = metadata, outs[1..n_results] = outputs; the caller frees them.
+ * `outs` holds 1 + max(n_results, 1) entries: `none` (0 results) returns its
+ * data copy as outs[1] (compression/none/none_encoding.cu:13-30). */
 ddj_status ddj_encode(ddj_ctx* ctx, int enc, int dtype, const void* d_in, size_t in_bytes, ddj_buf* outs);
 
 /* Encoding::Decode(SharedCudaPtrVector<char>, DataType) (compression/encoding.hpp:49-67).

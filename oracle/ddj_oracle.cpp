@@ -170,14 +170,34 @@ template <typename T> orc_stats stats_of(const T* x, size_t n) {
 
 /* -------------------------------------------------------------- histograms
  * ThrustSparseHistogram (util/histogram/thrust_sparse_histogram.cu:20-69):
- * sort ascending, count equal neighbours.  GetMostFrequentSparse
- * (most_frequent.cu:10-26): sort (count desc) keeping key order for equal
- * counts, take the first K.  When fewer than K distinct values exist the
- * reference leaves the tail uninitialised; this restatement repeats the last
- * real key there (both are outside what its tests pin). */
+ * thrust::sort of the keys, then reduce_by_key with equal_to.  Device thrust
+ * sorts floating-point keys by radix order, i.e. by the bit pattern mapped to
+ * an unsigned key (negatives reversed): -0.0 sorts just before +0.0, NaNs
+ * with the sign bit set before everything, other NaNs after everything.  The
+ * equal_to grouping then merges -0.0 with +0.0 (the bin keeps the first,
+ * -0.0) and never merges a NaN with anything, so every NaN row is a bin of
+ * its own with count 1.  GetMostFrequentSparse (most_frequent.cu:10-26):
+ * sort bins by count, descending; equal counts keep key order (the
+ * reference's sort_by_key is not guaranteed stable, so this tie order is
+ * the documented choice).  When fewer than K bins exist the reference leaves
+ * the tail uninitialised; this restatement repeats the last real key there
+ * (both are outside what its tests pin). */
+template <typename T> uint64_t radix_key(T v) {
+    if (std::is_floating_point<T>::value) {
+        uint64_t b = 0;
+        memcpy(&b, &v, sizeof(T));
+        const int W = 8 * (int)sizeof(T);
+        const uint64_t sign = 1ull << (W - 1);
+        const uint64_t mask = W == 64 ? ~0ull : ((1ull << W) - 1);
+        return (b & sign) ? (~b & mask) : (b | sign);
+    }
+    const int W = 8 * (int)sizeof(T);
+    return W == 64 ? ((uint64_t)(int64_t)v ^ (1ull << 63)) : (uint64_t)((int64_t)v + (1ll << (W - 1)));
+}
+
 template <typename T> std::vector<T> most_frequent(const T* x, size_t n, int K, int* ndistinct) {
     std::vector<T> keys(x, x + n);
-    std::sort(keys.begin(), keys.end());
+    std::stable_sort(keys.begin(), keys.end(), [](T a, T b) { return radix_key(a) < radix_key(b); });
     std::vector<std::pair<T, int32_t>> bins;
     for (size_t i = 0; i < n; i++) {
         if (i > 0 && keys[i] == bins.back().first)

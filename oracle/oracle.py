@@ -535,3 +535,86 @@ def update_statistics(root: Node, st: EdgeStats):
     """CompressionTree::UpdateStatistics (compression_tree.cpp:232-240)."""
     for a, b, no in edges(root, st.height):
         st.update(no, (a.enc, b.enc), (a.ratio + b.ratio) / 2.0)
+
+
+def decodable(root: Node) -> bool:
+    """Every leaf is `none` (a tree the decoder can walk)."""
+    if not root.children:
+        return root.enc == NONE
+    return all(decodable(c) for c in root.children)
+
+
+class Optimizer:
+    """CompressionOptimizer::CompressData (optimizer/compression_optimizer.cpp:156-193)
+    with OptimalTree (optimizer/optimal_tree.cpp:14-93) and the optimizer's
+    edge statistics (tree/compression_statistics.cpp)."""
+
+    MAX_EDGE_NO = (1 << 6) - 2
+
+    def __init__(self):
+        self.tree: Optional[Node] = None
+        self.stats = EdgeStats(5)
+        self.old_stats = EdgeStats(5)
+        self.aliased = False
+        self.old_ratio = 0.0
+        self.parts = 0
+
+    def compress_data(self, data, dt: int) -> bytes:
+        raw = as_bytes(data)
+        if self.tree is None or self.parts % 10 == 9:
+            sample = raw[: sample_size(len(raw), dt)]
+            cands = full_statistics_update(sample, NONE, dt, 0)
+            best, bratio = None, -1.0
+            for t, _ in cands:
+                t.fix()
+                update_statistics(t, self.stats)
+                if t.ratio > bratio:
+                    best, bratio = t, t.ratio
+            self.tree = best
+            self.old_ratio = best.ratio
+            self.old_stats = self.stats.copy()
+            self.aliased = False
+        blob = compress(self.tree, raw)
+        update_statistics(self.tree, self.stats)
+        self.try_correct_tree()
+        self.parts += 1
+        return blob
+
+    def replace(self, node: Node, edge_no: int):
+        """OptimalTree::Replace (optimal_tree.cpp:14-37)."""
+        if edge_no > self.MAX_EDGE_NO:
+            return
+        n, rtypes = encoding_info(node.enc, node.dtype)
+        for i in range(n):
+            t = node.enc
+            (frm, to), value = self.stats.get_best(edge_no + i, (t, t))
+            if value > 1:
+                child = Node(to, rtypes[i])
+                node.children.append(child)
+                self.replace(child, 2 * (edge_no + i) + 2)
+            else:
+                node.children.append(Node(NONE, rtypes[i]))
+
+    def try_correct_tree(self) -> bool:
+        """OptimalTree::TryCorrectTree (optimal_tree.cpp:39-93)."""
+        old_ratio, new_ratio = self.old_ratio, self.tree.ratio
+        if old_ratio <= new_ratio and old_ratio > 1:
+            return False
+        olds = self.stats if self.aliased else self.old_stats
+        for frm, to, no in edges(self.tree, 5):
+            typ = (frm.enc, to.enc)
+            old_value = olds.get(no, typ)
+            new_value = self.stats.get(no, typ)
+            best_edge, best_value = self.stats.get_best(no, typ)
+            if new_value < old_value or best_value > new_value:
+                same_edge, same_value = self.stats.get_best(no, typ, True)
+                if same_value > new_value:
+                    to.children, to.ratio, to.enc = [], 1.0, same_edge[1]
+                    self.replace(to, 2 * (no + 1))
+                else:
+                    frm.children, frm.ratio, frm.enc = [], 1.0, best_edge[0]
+                    self.replace(frm, no - 1 if no % 2 else no)
+                self.old_ratio = self.tree.ratio
+                self.aliased = True
+                return True
+        return False
