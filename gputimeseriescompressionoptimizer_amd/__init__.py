@@ -85,6 +85,7 @@ def load_library(path: str = LIB_PATH):
         "ddj_ctx_stream": (vp, [vp]),
         "ddj_buf_alloc": (i32, [vp, sz, P(_Buf)]),
         "ddj_buf_free": (None, [vp, P(_Buf)]),
+        "ddj_buf_detach": (i32, [vp, P(_Buf)]),
         "ddj_memcpy_h2d": (i32, [vp, vp, vp, sz]),
         "ddj_memcpy_d2h": (i32, [vp, vp, vp, sz]),
         "ddj_memcpy_d2d": (i32, [vp, vp, vp, sz]),

@@ -166,6 +166,13 @@ void ddj_buf_free(ddj_ctx* ctx, ddj_buf* buf) {
     buf->bytes = 0;
 }
 
+ddj_status ddj_buf_detach(ddj_ctx* ctx, ddj_buf* buf) {
+    return guarded(ctx, [&] {
+        if (!buf) fail(DDJ_EINVAL, "null buffer");
+        if (buf->dev && !ctx->c.alloc->detach(buf->dev)) fail(DDJ_EINVAL, "buffer not owned by this context");
+    });
+}
+
 ddj_status ddj_memcpy_h2d(ddj_ctx* ctx, void* dst, const void* src, size_t bytes) {
     return guarded(ctx, [&] { h2d(&ctx->c, dst, src, bytes); });
 }

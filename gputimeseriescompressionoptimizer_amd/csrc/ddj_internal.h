@@ -40,6 +40,8 @@ public:
     ~Allocator();
     void* alloc(size_t bytes);
     void free(void* p);
+    // Hands a live block to the caller (released later with hipFree).
+    bool detach(void* p);
     void trim();
     size_t in_use() const { return in_use_; }
 

@@ -77,6 +77,14 @@ void Allocator::free(void* p) {
     live_.erase(it);
 }
 
+bool Allocator::detach(void* p) {
+    auto it = live_.find(p);
+    if (it == live_.end()) return false;
+    in_use_ -= it->second;
+    live_.erase(it);
+    return true;
+}
+
 void Allocator::trim() {
     (void)hipDeviceSynchronize();
     for (auto& kv : free_)

@@ -99,6 +99,10 @@ ddj_status ddj_profile_reset(ddj_ctx* ctx);
 /* ----------------------------------------------------------------- memory */
 ddj_status ddj_buf_alloc(ddj_ctx* ctx, size_t bytes, ddj_buf* out);
 void ddj_buf_free(ddj_ctx* ctx, ddj_buf* buf);
+/* Transfers ownership of a library buffer to the caller, who releases it
+ * with hipFree (the reference's CudaPtr(T*, size) adopts such pointers,
+ * core/cuda_ptr.hpp:49-56).  The pointer must not be passed to ddj_buf_free. */
+ddj_status ddj_buf_detach(ddj_ctx* ctx, ddj_buf* buf);
 ddj_status ddj_memcpy_h2d(ddj_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
 ddj_status ddj_memcpy_d2h(ddj_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes);
 ddj_status ddj_memcpy_d2d(ddj_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes);

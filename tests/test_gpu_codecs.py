@@ -162,3 +162,22 @@ def test_most_frequent_tie_order(ctx):
     assert got == O.encode(g.DICT, g.D_INT, x)
     keys = np.frombuffer(got[1][16:32], dtype=np.int32)
     assert list(keys) == [3, 5, 7, 9]
+
+
+def test_afl_uniform_1m_matches_reference_digest(ctx):
+    """The reference's CPU-runnable case: 1,000,000 int32 uniform in [0,1024),
+    AFL to 10 bits.  The GPU stream's digest equals the one the reference's
+    own host loop produced (tests/golden/fl_ref_vectors.npz), and decode
+    round-trips."""
+    import hashlib
+    import os
+    from gputimeseriescompressionoptimizer_amd import datasets as D
+
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fl_ref_vectors.npz"))
+    x = D.uniform_int(1_000_000, 1024, seed=0)
+    outs = g.Encoding(ctx, g.AFL).Encode(ctx.upload(x), g.D_INT)
+    meta, stream = outs[0].to_bytes(), outs[1].to_bytes()
+    assert list(meta) == [10, (8 * len(stream) - 10 * x.size) & 0xFF, 0, 0]  # afl_encoding.cu:36-43
+    assert hashlib.sha256(stream).digest() == z["cfg0_stream_sha256"].tobytes()
+    back = g.Encoding(ctx, g.AFL).Decode(outs, g.D_INT).to_numpy(np.int32)
+    assert np.array_equal(back, x)

@@ -2,7 +2,9 @@
 forms and invariants that a plausible mistake in the restatement would break.
 """
 import collections
+import hashlib
 import itertools
+import os
 import struct
 
 import numpy as np
@@ -315,3 +317,37 @@ def test_minmax_fold_semantics():
     assert st.min == 0.0 and not np.signbit(st.min)
     st = O.stats(O.D_DOUBLE, _f64(-0.0, 0.0))
     assert np.signbit(st.min) and np.signbit(st.max)
+
+
+# ------------------------------------------- golden vectors of the reference
+# tests/golden/fl_ref_vectors.npz holds outputs of the reference's own host FL
+# loops (written by tests/golden/make_fl_vectors.py from oracle/_ref).
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fl_ref_vectors.npz")
+
+
+def test_fl_golden_vectors():
+    z = np.load(GOLDEN)
+    npdt = {8: np.int8, 16: np.int16, 32: np.int32, 64: np.int64}
+    dt = {8: O.D_CHAR, 16: O.D_SHORT, 32: O.D_INT, 64: O.D_TIME}
+    seen = 0
+    for W in (8, 16, 32, 64):
+        for b in range(1, W):
+            x = z[f"in_{W}_{b}"]
+            want = z[f"out_{W}_{b}"]
+            assert x.dtype == npdt[W]
+            assert O.fl_pack(dt[W], b, x).tobytes() == want.tobytes(), (W, b)
+            assert np.array_equal(O.fl_unpack(dt[W], b, want, x.size), x), (W, b)
+            seen += 1
+    assert seen == 7 + 15 + 31 + 63
+
+
+def test_fl_golden_uniform_1m_column():
+    """1,000,000 int32 uniform in [0,1024) packed to 10 bits: the stream's
+    digest equals the reference loop's."""
+    z = np.load(GOLDEN)
+    x = D.uniform_int(1_000_000, 1024, seed=0)
+    assert hashlib.sha256(x.tobytes()).digest() == z["cfg0_input_sha256"].tobytes()
+    packed = O.fl_pack(O.D_INT, 10, x)
+    assert hashlib.sha256(packed.tobytes()).digest() == z["cfg0_stream_sha256"].tobytes()
+    assert np.array_equal(packed[:64], z["cfg0_stream_head"])
