@@ -107,6 +107,12 @@ def load_library(path: str = LIB_PATH):
         "ddj_profile_enable": (i32, [vp, i32]),
         "ddj_profile_kernel": (i32, [vp, ctypes.c_char_p, P(ctypes.c_double), P(u64), P(ctypes.c_double)]),
         "ddj_profile_reset": (i32, [vp]),
+        "ddj_ts_create": (i32, [vp, u32, P(i32), sz, P(vp)]),
+        "ddj_ts_destroy": (None, [vp]),
+        "ddj_ts_compress": (i32, [vp, P(vp), sz, ctypes.c_char_p, i32]),
+        "ddj_ts_reader_open": (i32, [vp, ctypes.c_char_p, u32, P(i32), P(vp)]),
+        "ddj_ts_reader_next": (i32, [vp, P(vp), sz, P(sz)]),
+        "ddj_ts_reader_close": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -415,3 +421,69 @@ class CompressionOptimizer:
         n = ctypes.c_size_t()
         self.ctx.check(self.ctx._L.ddj_optimizer_tree(self._h, path, 4096, ctypes.byref(n)), "optimizer tree")
         return list(path[: n.value])
+
+
+class ParallelTSCompressor:
+    """ParallelTSCompressor (parallel_ts_compressor.cpp:17-102) over host
+    columns: rows cut into batches of `batch_size` (SetBatchSize; the
+    reference's default is 1e6), one CompressionOptimizer per column, the
+    file holding [size_t bytes][blob] per column per batch."""
+
+    def __init__(self, ctx: Context, dtypes: Sequence[int], batch_size: int = 1_000_000):
+        self.ctx = ctx
+        self.dtypes = list(dtypes)
+        self.batch_size = int(batch_size)
+        self._h = None
+        self._new()
+
+    def _new(self):
+        self._free()
+        arr = (ctypes.c_int * len(self.dtypes))(*self.dtypes)
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx._L.ddj_ts_create(self.ctx._h, len(self.dtypes), arr, self.batch_size,
+                                                 ctypes.byref(h)), "ts create")
+        self._h = h
+
+    def _free(self):
+        if self._h:
+            self.ctx._L.ddj_ts_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self._free()
+        except Exception:
+            pass
+
+    def SetBatchSize(self, size: int):
+        self.batch_size = int(size)
+        self._new()
+
+    def Compress(self, columns: Sequence[np.ndarray], path: str, append: bool = False):
+        cols = [np.ascontiguousarray(c, dtype=NP_DTYPE[d]) for c, d in zip(columns, self.dtypes)]
+        if len(cols) != len(self.dtypes) or len({c.size for c in cols}) != 1:
+            raise ValueError("one array per column, all of the same length")
+        ptrs = (ctypes.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
+        self.ctx.check(self.ctx._L.ddj_ts_compress(self._h, ptrs, cols[0].size, os.fsencode(path), int(append)),
+                       "ts compress")
+
+    def Decompress(self, path: str) -> List[np.ndarray]:
+        arr = (ctypes.c_int * len(self.dtypes))(*self.dtypes)
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx._L.ddj_ts_reader_open(self.ctx._h, os.fsencode(path), len(self.dtypes), arr,
+                                                      ctypes.byref(h)), "ts open")
+        parts = [[] for _ in self.dtypes]
+        try:
+            bufs = [np.empty(self.batch_size, NP_DTYPE[d]) for d in self.dtypes]
+            ptrs = (ctypes.c_void_p * len(bufs))(*[b.ctypes.data for b in bufs])
+            rows = ctypes.c_size_t()
+            while True:
+                self.ctx.check(self.ctx._L.ddj_ts_reader_next(h, ptrs, self.batch_size, ctypes.byref(rows)),
+                               "ts next")
+                if rows.value == 0:
+                    break
+                for k, b in enumerate(bufs):
+                    parts[k].append(b[: rows.value].copy())
+        finally:
+            self.ctx._L.ddj_ts_reader_close(h)
+        return [np.concatenate(p) if p else np.empty(0, NP_DTYPE[d]) for p, d in zip(parts, self.dtypes)]

@@ -76,10 +76,16 @@ ddj_status ddj_ctx_create(ddj_ctx** out, int device) {
         e = hipMalloc(reinterpret_cast<void**>(&ctx->c.control), sizeof(Control));
     }
     if (e == hipSuccess) e = hipMemset(ctx->c.control, 0, sizeof(Control));
+    if (e == hipSuccess) {
+        ctx->c.staging_size = 4u << 20;
+        e = hipHostMalloc(reinterpret_cast<void**>(&ctx->c.staging), ctx->c.staging_size, hipHostMallocDefault);
+    }
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         fprintf(stderr, "ddj_ctx_create: %s\n", hipGetErrorString(e));
         if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
+        if (ctx->c.control) (void)hipFree(ctx->c.control);
+        if (ctx->c.staging) (void)hipHostFree(ctx->c.staging);
         delete ctx;
         return e == hipErrorOutOfMemory ? DDJ_ENOMEM : DDJ_EHIP;
     }
@@ -94,6 +100,7 @@ void ddj_ctx_destroy(ddj_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->c.stream);
     ctx->c.alloc.reset();
     (void)hipFree(ctx->c.control);
+    if (ctx->c.staging) (void)hipHostFree(ctx->c.staging);
     (void)hipStreamDestroy(ctx->c.stream);
     delete ctx;
 }
@@ -334,6 +341,131 @@ ddj_status ddj_optimizer_tree(ddj_optimizer* opt, int* path_out, size_t path_cap
     return guarded(opt->ctx, [&] {
         if (!opt->opt->tree()) fail(DDJ_EINVAL, "no tree yet");
         write_path(opt->opt->tree(), path_out, path_cap, path_len);
+    });
+}
+
+// ------------------------------------------------------------- column API
+// ParallelTSCompressor (parallel_ts_compressor.cpp:17-102) with its
+// CompressionTask / DecompressionTask (optimizer/compression_task.cpp:35-77,
+// optimizer/decompression_task.cpp:19-46): batches of rows, one stateful
+// optimizer per column, each column's blob written as [size_t bytes][blob]
+// in column order, batch after batch.
+
+}  // extern "C"
+
+struct ddj_ts_compressor {
+    ddj_ctx* ctx;
+    std::vector<int> dtypes;
+    size_t batch_rows;
+    std::vector<std::unique_ptr<Optimizer>> opts;
+};
+
+struct ddj_ts_reader {
+    ddj_ctx* ctx;
+    std::vector<int> dtypes;
+    FILE* f;
+};
+
+extern "C" {
+
+ddj_status ddj_ts_create(ddj_ctx* ctx, uint32_t n_columns, const int* dtypes, size_t batch_rows,
+                         ddj_ts_compressor** out) {
+    return guarded(ctx, [&] {
+        if (!out || !dtypes || n_columns == 0 || batch_rows == 0) fail(DDJ_EINVAL, "bad argument");
+        auto* t = new ddj_ts_compressor;
+        t->ctx = ctx;
+        t->batch_rows = batch_rows;
+        for (uint32_t i = 0; i < n_columns; i++) {
+            if (!dtype_ok(dtypes[i])) {
+                delete t;
+                fail(DDJ_EINVAL, "bad column type");
+            }
+            t->dtypes.push_back(dtypes[i]);
+            t->opts.emplace_back(new Optimizer(&ctx->c));
+        }
+        *out = t;
+    });
+}
+
+void ddj_ts_destroy(ddj_ts_compressor* t) { delete t; }
+
+ddj_status ddj_ts_compress(ddj_ts_compressor* t, const void* const* host_columns, size_t rows, const char* out_path,
+                           int append) {
+    if (!t) return DDJ_EINVAL;
+    return guarded(t->ctx, [&] {
+        if (!host_columns || !out_path) fail(DDJ_EINVAL, "bad argument");
+        FILE* f = fopen(out_path, append ? "ab" : "wb");
+        if (!f) fail(DDJ_EINVAL, std::string("cannot open ") + out_path);
+        std::unique_ptr<FILE, int (*)(FILE*)> guard(f, fclose);
+        Ctx* c = &t->ctx->c;
+        std::vector<uint8_t> host;
+        for (size_t r0 = 0; r0 < rows; r0 += t->batch_rows) {
+            const size_t nr = std::min(t->batch_rows, rows - r0);
+            for (size_t k = 0; k < t->dtypes.size(); k++) {
+                const size_t es = dtype_size(t->dtypes[k]);
+                DevBuf d(c, nr * es);
+                h2d(c, d.get(), static_cast<const uint8_t*>(host_columns[k]) + r0 * es, nr * es);
+                Part blob = t->opts[k]->compress_data(DView{d.bytes_ptr(), nr * es}, t->dtypes[k]);
+                host.resize(blob.n);
+                d2h(c, host.data(), blob.p, blob.n);
+                const size_t sz = blob.n;
+                if (fwrite(&sz, sizeof sz, 1, f) != 1 || (sz && fwrite(host.data(), 1, sz, f) != sz))
+                    fail(DDJ_EINVAL, "write failed");
+            }
+        }
+    });
+}
+
+ddj_status ddj_ts_reader_open(ddj_ctx* ctx, const char* in_path, uint32_t n_columns, const int* dtypes,
+                              ddj_ts_reader** out) {
+    return guarded(ctx, [&] {
+        if (!out || !in_path || !dtypes || n_columns == 0) fail(DDJ_EINVAL, "bad argument");
+        for (uint32_t i = 0; i < n_columns; i++)
+            if (!dtype_ok(dtypes[i])) fail(DDJ_EINVAL, "bad column type");
+        FILE* f = fopen(in_path, "rb");
+        if (!f) fail(DDJ_EINVAL, std::string("cannot open ") + in_path);
+        auto* r = new ddj_ts_reader;
+        r->ctx = ctx;
+        r->dtypes.assign(dtypes, dtypes + n_columns);
+        r->f = f;
+        *out = r;
+    });
+}
+
+void ddj_ts_reader_close(ddj_ts_reader* r) {
+    if (!r) return;
+    if (r->f) fclose(r->f);
+    delete r;
+}
+
+ddj_status ddj_ts_reader_next(ddj_ts_reader* r, void* const* host_columns, size_t cap_rows, size_t* rows) {
+    if (!r) return DDJ_EINVAL;
+    return guarded(r->ctx, [&] {
+        if (!host_columns || !rows) fail(DDJ_EINVAL, "bad argument");
+        *rows = 0;
+        Ctx* c = &r->ctx->c;
+        std::vector<uint8_t> host;
+        size_t batch = 0;
+        for (size_t k = 0; k < r->dtypes.size(); k++) {
+            size_t sz = 0;
+            if (fread(&sz, sizeof sz, 1, r->f) != 1) {
+                if (k == 0 && feof(r->f)) return;  // end of file
+                fail(DDJ_ECORRUPT, "truncated column file");
+            }
+            host.resize(sz);
+            if (sz && fread(host.data(), 1, sz, r->f) != sz) fail(DDJ_ECORRUPT, "truncated blob");
+            DevBuf d(c, sz);
+            h2d(c, d.get(), host.data(), sz);
+            int dt = -1;
+            Part out = decompress_tree(c, DView{d.bytes_ptr(), sz}, &dt);
+            if (dt != r->dtypes[k]) fail(DDJ_ECORRUPT, "column type differs from the blob's");
+            const size_t es = dtype_size(dt), nr = out.n / es;
+            if (k == 0) batch = nr;
+            if (nr != batch) fail(DDJ_ECORRUPT, "columns of one batch differ in length");
+            if (nr > cap_rows) fail(DDJ_EINVAL, "batch larger than the capacity given");
+            d2h(c, host_columns[k], out.p, nr * es);
+        }
+        *rows = batch;
     });
 }
 

@@ -80,6 +80,10 @@ struct Ctx {
     std::mutex mu;
     bool profiling = false;
     std::map<std::string, KernelTimes> ktimes;
+    // Pinned staging ring for small uploads: h2d copies the source here and
+    // returns without waiting; the ring restarts after the stream drains.
+    uint8_t* staging = nullptr;
+    size_t staging_size = 0, staging_off = 0;
 };
 
 // Adds the algorithmic bytes of a launch to its kernel's tally.

@@ -123,11 +123,42 @@ __global__ __launch_bounds__(kBlock) void k_fill_zero(uint32_t* p, size_t words)
     for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < words; i += stride) p[i] = 0u;
 }
 
+// One range per blockIdx.y.  Destinations sit at arbitrary byte offsets of
+// the blob: a byte head brings the destination to 16-byte alignment, the
+// body stores 16 bytes per thread-iteration built from aligned source words
+// by funnel shifts (an aligned word holding a valid byte never crosses a
+// page), and a byte tail finishes.
 __global__ __launch_bounds__(kBlock) void k_gather_copy(const CopyRange* __restrict__ r, uint8_t* __restrict__ dst) {
     const CopyRange cr = r[blockIdx.y];
-    const size_t stride = (size_t)gridDim.x * kBlock;
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < cr.bytes; i += stride)
-        dst[cr.dst_off + i] = cr.src[i];
+    const uint8_t* src = cr.src;
+    uint8_t* d = dst + cr.dst_off;
+    const uint64_t nb = cr.bytes;
+    const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x, stride = (size_t)gridDim.x * kBlock;
+    uint64_t head = (16 - (reinterpret_cast<uintptr_t>(d) & 15)) & 15;
+    head = head < nb ? head : nb;
+    if (tid < head) d[tid] = src[tid];
+    const uint64_t body = (nb - head) / 16;
+    const uintptr_t sa = reinterpret_cast<uintptr_t>(src + head);
+    const uint32_t sh = (uint32_t)(sa & 3) * 8;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(sa & ~(uintptr_t)3);
+    u32x4* d4 = reinterpret_cast<u32x4*>(d + head);
+    if (sh == 0) {
+        for (size_t i = tid; i < body; i += stride) {
+            const uint32_t* p = w + 4 * i;
+            u32x4 o = {p[0], p[1], p[2], p[3]};
+            __builtin_nontemporal_store(o, d4 + i);
+        }
+    } else {
+        for (size_t i = tid; i < body; i += stride) {
+            const uint32_t* p = w + 4 * i;
+            const uint32_t a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3], a4 = p[4];
+            u32x4 o = {__builtin_amdgcn_alignbit(a1, a0, sh), __builtin_amdgcn_alignbit(a2, a1, sh),
+                       __builtin_amdgcn_alignbit(a3, a2, sh), __builtin_amdgcn_alignbit(a4, a3, sh)};
+            __builtin_nontemporal_store(o, d4 + i);
+        }
+    }
+    const uint64_t t0 = head + body * 16;
+    if (tid < nb - t0) d[t0 + tid] = src[t0 + tid];
 }
 
 // ------------------------------------------------------------- delta decode
@@ -271,8 +302,8 @@ void launch_fill_zero(Ctx* c, void* p, size_t bytes) {
 
 void launch_gather_copy(Ctx* c, const CopyRange* d_ranges, int n_ranges, uint8_t* dst, uint64_t max_bytes) {
     if (n_ranges <= 0 || max_bytes == 0) return;
-    unsigned gx = grid_for(c, max_bytes);
-    gx = gx > 1024 ? 1024 : gx;
+    unsigned gx = grid_for(c, max_bytes, 64);  // 4 x 16 bytes per thread
+    gx = gx > 2048 ? 2048 : gx;
     { KernelTimer kt_(c, "k_gather_copy"); hipLaunchKernelGGL(k_gather_copy, dim3(gx, n_ranges), dim3(kBlock), 0, c->stream, d_ranges, dst); }
     HIPC(hipGetLastError());
 }

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
 
 namespace ddj {
 
@@ -101,12 +102,26 @@ void d2h(Ctx* c, void* dst, const void* src, size_t bytes) {
     if (!bytes) return;
     HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
+    c->staging_off = 0;
 }
 
 void h2d(Ctx* c, void* dst, const void* src, size_t bytes) {
     if (!bytes) return;
+    if (c->staging && bytes <= c->staging_size / 4) {
+        const size_t need = (bytes + 63) & ~(size_t)63;
+        if (c->staging_off + need > c->staging_size) {
+            HIPC(hipStreamSynchronize(c->stream));  // every staged copy has landed
+            c->staging_off = 0;
+        }
+        uint8_t* s = c->staging + c->staging_off;
+        memcpy(s, src, bytes);
+        HIPC(hipMemcpyAsync(dst, s, bytes, hipMemcpyHostToDevice, c->stream));
+        c->staging_off += need;
+        return;
+    }
     HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     HIPC(hipStreamSynchronize(c->stream));  // the source may be a temporary
+    c->staging_off = 0;
 }
 
 void d2d(Ctx* c, void* dst, const void* src, size_t bytes) {

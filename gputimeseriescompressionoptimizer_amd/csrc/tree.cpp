@@ -157,32 +157,38 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
     std::vector<Piece> pieces;
     size_t total = compress_node(c, *root, borrow(data), pieces);
     // Concatenate (util/copy/cuda_array_copy.cu:14-33) as one gather kernel.
-    std::vector<uint8_t> headers;
-    std::vector<CopyRange> ranges;
-    size_t off = 0, hdr_count = 0;
-    for (auto& p : pieces)
+    // The copy table and the node headers travel in one upload: the
+    // headers' ranges point into the table buffer itself.
+    size_t n_ranges = 0, hdr_count = 0;
+    for (auto& p : pieces) {
         if (p.is_header) hdr_count++;
-    DevBuf hdr_dev(c, 8 * hdr_count);
-    size_t hi = 0;
-    uint64_t max_bytes = 0;
+        if (p.is_header || p.part.n) n_ranges++;
+    }
+    DevBuf rdev(c, sizeof(CopyRange) * n_ranges + 8 * hdr_count);
+    const uint8_t* hdr_base = rdev.bytes_ptr() + sizeof(CopyRange) * n_ranges;
+    std::vector<uint8_t> table(sizeof(CopyRange) * n_ranges + 8 * hdr_count);
+    CopyRange* ranges = reinterpret_cast<CopyRange*>(table.data());
+    uint8_t* headers = table.data() + sizeof(CopyRange) * n_ranges;
+    size_t off = 0, hi = 0, ri = 0;
+    uint64_t max_bytes = 0, copied = 0;
     for (auto& p : pieces) {
         if (p.is_header) {
-            headers.insert(headers.end(), p.hdr, p.hdr + 8);
-            ranges.push_back(CopyRange{hdr_dev.bytes_ptr() + 8 * hi, off, 8});
+            memcpy(headers + 8 * hi, p.hdr, 8);
+            ranges[ri++] = CopyRange{hdr_base + 8 * hi, off, 8};
             hi++;
             off += 8;
         } else if (p.part.n) {
-            ranges.push_back(CopyRange{p.part.p, off, p.part.n});
+            ranges[ri++] = CopyRange{p.part.p, off, p.part.n};
             max_bytes = std::max<uint64_t>(max_bytes, p.part.n);
             off += p.part.n;
         }
+        copied += p.is_header ? 8 : p.part.n;
     }
     if (off != total) fail(DDJ_EHIP, "blob size mismatch");
-    h2d(c, hdr_dev.get(), headers.data(), headers.size());
+    h2d(c, rdev.get(), table.data(), table.size());
     DevBuf blob(c, total);
-    DevBuf rdev(c, sizeof(CopyRange) * ranges.size());
-    h2d(c, rdev.get(), ranges.data(), sizeof(CopyRange) * ranges.size());
-    launch_gather_copy(c, rdev.get<CopyRange>(), (int)ranges.size(), blob.bytes_ptr(), std::max<uint64_t>(max_bytes, 8));
+    kernel_bytes(c, "k_gather_copy", 2.0 * (double)copied);
+    launch_gather_copy(c, rdev.get<CopyRange>(), (int)n_ranges, blob.bytes_ptr(), std::max<uint64_t>(max_bytes, 8));
     return own(std::move(blob));
 }
 

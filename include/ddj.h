@@ -161,6 +161,28 @@ void ddj_optimizer_destroy(ddj_optimizer* opt);
 ddj_status ddj_optimizer_compress(ddj_optimizer* opt, int dtype, const void* d_in, size_t in_bytes, ddj_buf* blob);
 ddj_status ddj_optimizer_tree(ddj_optimizer* opt, int* path_out, size_t path_cap, size_t* path_len);
 
+/* ------------------------------------------------------------- column API */
+/* ParallelTSCompressor (parallel_ts_compressor.cpp:17-102): rows of several
+ * host columns are cut into batches of `batch_rows`; for every batch, each
+ * column in order goes through its own stateful optimizer
+ * (CompressionOptimizer::CompressData) and is written as [size_t bytes][blob]
+ * (optimizer/compression_task.cpp:35-40).  Successive ddj_ts_compress calls
+ * continue the same per-column optimizers (pass append=1 to add to a file). */
+typedef struct ddj_ts_compressor ddj_ts_compressor;
+ddj_status ddj_ts_create(ddj_ctx* ctx, uint32_t n_columns, const int* dtypes, size_t batch_rows,
+                         ddj_ts_compressor** out);
+void ddj_ts_destroy(ddj_ts_compressor* ts);
+ddj_status ddj_ts_compress(ddj_ts_compressor* ts, const void* const* host_columns, size_t rows, const char* out_path,
+                           int append);
+/* ParallelTSCompressor::Decompress (:63-100) with DecompressionTask
+ * (optimizer/decompression_task.cpp:19-46): one batch per call into
+ * caller buffers of `cap_rows` rows per column; *rows = 0 at end of file. */
+typedef struct ddj_ts_reader ddj_ts_reader;
+ddj_status ddj_ts_reader_open(ddj_ctx* ctx, const char* in_path, uint32_t n_columns, const int* dtypes,
+                              ddj_ts_reader** out);
+ddj_status ddj_ts_reader_next(ddj_ts_reader* reader, void* const* host_columns, size_t cap_rows, size_t* rows);
+void ddj_ts_reader_close(ddj_ts_reader* reader);
+
 #ifdef __cplusplus
 }
 #endif
