@@ -113,6 +113,17 @@ def load_library(path: str = LIB_PATH):
         "ddj_ts_reader_open": (i32, [vp, ctypes.c_char_p, u32, P(i32), P(vp)]),
         "ddj_ts_reader_next": (i32, [vp, P(vp), sz, P(sz)]),
         "ddj_ts_reader_close": (None, [vp]),
+        "ddj_planner_candidates": (i32, [P(i32), sz, P(sz)]),
+        "ddj_planner_features": (i32, [vp, i32, vp, sz, P(ctypes.c_float)]),
+        "ddj_planner_labels": (i32, [vp, i32, vp, sz, P(ctypes.c_float)]),
+        "ddj_planner_create": (i32, [vp, u64, P(vp)]),
+        "ddj_planner_destroy": (None, [vp]),
+        "ddj_planner_params": (i32, [vp, vp, vp]),
+        "ddj_planner_gradient": (i32, [vp, vp, vp, sz, vp, P(ctypes.c_float)]),
+        "ddj_planner_train": (i32, [vp, vp, vp, sz, i32, i32, ctypes.c_float, u64, P(ctypes.c_float)]),
+        "ddj_planner_predict": (i32, [vp, vp, sz, vp]),
+        "ddj_planner_plan": (i32, [vp, i32, vp, sz, ctypes.c_float, P(i32), sz, P(sz), P(i32),
+                                   P(ctypes.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -487,3 +498,112 @@ class ParallelTSCompressor:
         finally:
             self.ctx._L.ddj_ts_reader_close(h)
         return [np.concatenate(p) if p else np.empty(0, NP_DTYPE[d]) for p, d in zip(parts, self.dtypes)]
+
+
+MLP_FEATURES, MLP_HIDDEN, MLP_OUTPUTS, MLP_OUTPUTS_PADDED = 32, 128, 48, 64
+MLP_PARAM_COUNT = 32 * 128 + 128 + 128 * 128 + 128 + 128 * 64 + 64
+
+
+def planner_candidates() -> List[List[int]]:
+    """The 48 candidate trees of the learned planner (pre-order paths)."""
+    L = load_library()
+    offs = (ctypes.c_size_t * (MLP_OUTPUTS + 1))()
+    L.ddj_planner_candidates(None, 0, offs)
+    paths = (ctypes.c_int * offs[MLP_OUTPUTS])()
+    st = L.ddj_planner_candidates(paths, offs[MLP_OUTPUTS], offs)
+    if st:
+        raise DdjError(st, "planner candidates")
+    return [list(paths[offs[i]:offs[i + 1]]) for i in range(MLP_OUTPUTS)]
+
+
+def split_params(flat: np.ndarray):
+    """Views W1 [32][128], b1, W2 [128][128], b2, W3 [128][64], b3 [64] of a flat parameter vector."""
+    o, out = 0, []
+    for shape in ((32, 128), (128,), (128, 128), (128,), (128, 64), (64,)):
+        n = int(np.prod(shape))
+        out.append(flat[o:o + n].reshape(shape))
+        o += n
+    return out
+
+
+class LearnedPlanner:
+    """The learned planner (no counterpart in the reference): features of a
+    block -> predicted log2(compressed / input bytes) of 48 candidate trees,
+    trained with the library's own matrix-core forward/backward kernels."""
+
+    def __init__(self, ctx: Context, seed: int = 0):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(ctx._L.ddj_planner_create(ctx._h, seed, ctypes.byref(h)), "planner create")
+        self._h = h
+
+    def __del__(self):
+        try:
+            if self._h:
+                self.ctx._L.ddj_planner_destroy(self._h)
+        except Exception:
+            pass
+
+    def Features(self, data: DeviceBuffer, dtype: int) -> np.ndarray:
+        out = np.zeros(MLP_FEATURES, np.float32)
+        self.ctx.check(self.ctx._L.ddj_planner_features(self.ctx._h, dtype, data.dev or None, data.nbytes,
+                                                        out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))),
+                       "features")
+        return out
+
+    def Labels(self, data: DeviceBuffer, dtype: int) -> np.ndarray:
+        out = np.zeros(MLP_OUTPUTS, np.float32)
+        self.ctx.check(self.ctx._L.ddj_planner_labels(self.ctx._h, dtype, data.dev or None, data.nbytes,
+                                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "labels")
+        return out
+
+    def Params(self) -> np.ndarray:
+        out = np.zeros(MLP_PARAM_COUNT, np.float32)
+        self.ctx.check(self.ctx._L.ddj_planner_params(self._h, out.ctypes.data, None), "params")
+        return out
+
+    def SetParams(self, flat: np.ndarray):
+        flat = np.ascontiguousarray(flat, np.float32)
+        assert flat.size == MLP_PARAM_COUNT
+        self.ctx.check(self.ctx._L.ddj_planner_params(self._h, None, flat.ctypes.data), "params")
+
+    def Gradient(self, X: np.ndarray, T: np.ndarray) -> Tuple[np.ndarray, float]:
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(MLP_PARAM_COUNT, np.float32)
+        loss = ctypes.c_float()
+        self.ctx.check(self.ctx._L.ddj_planner_gradient(self._h, X.ctypes.data, T.ctypes.data, X.shape[0],
+                                                        g.ctypes.data, ctypes.byref(loss)), "gradient")
+        return g, loss.value
+
+    def Train(self, X: np.ndarray, T: np.ndarray, epochs: int, batch: int = 256, lr: float = 1e-3,
+              seed: int = 0) -> float:
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        loss = ctypes.c_float()
+        self.ctx.check(self.ctx._L.ddj_planner_train(self._h, X.ctypes.data, T.ctypes.data, X.shape[0], epochs, batch,
+                                                     lr, seed, ctypes.byref(loss)), "train")
+        return loss.value
+
+    def Predict(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], MLP_OUTPUTS), np.float32)
+        self.ctx.check(self.ctx._L.ddj_planner_predict(self._h, X.ctypes.data, X.shape[0], Y.ctypes.data),
+                       "predict")
+        return Y
+
+    def Plan(self, data: DeviceBuffer, dtype: int, min_margin: float = 0.0) -> Tuple[List[int], bool, float]:
+        path = (ctypes.c_int * 4096)()
+        n = ctypes.c_size_t()
+        fb = ctypes.c_int()
+        m = ctypes.c_float()
+        self.ctx.check(self.ctx._L.ddj_planner_plan(self._h, dtype, data.dev or None, data.nbytes, min_margin, path,
+                                                    4096, ctypes.byref(n), ctypes.byref(fb), ctypes.byref(m)),
+                       "learned plan")
+        return list(path[: n.value]), bool(fb.value), m.value
+
+    def Save(self, path: str):
+        np.save(path, self.Params(), allow_pickle=False)
+
+    def Load(self, path: str):
+        self.SetParams(np.load(path, allow_pickle=False))

@@ -3,6 +3,7 @@
 #include <cstring>
 
 #include "codecs.h"
+#include "learned.h"
 #include "planner.h"
 #include "tree.h"
 
@@ -19,11 +20,24 @@ struct ddj_optimizer {
 
 namespace {
 
+// The device error flag: 1 = a bounded look-back wait expired, 2 = encoded
+// input failed a device-side check.  Read (and cleared) after every call.
+void check_device_error(Ctx* c) {
+    uint32_t err = 0;
+    d2h(c, &err, &c->control->error_flag, 4);
+    if (!err) return;
+    const uint32_t zero = 0;
+    h2d(c, &c->control->error_flag, &zero, 4);
+    if (err == 2) fail(DDJ_ECORRUPT, "encoded input failed a device-side check");
+    fail(DDJ_ETIMEOUT, "a bounded device-side wait expired");
+}
+
 template <typename F> ddj_status guarded(ddj_ctx* ctx, F&& f) {
     if (!ctx) return DDJ_EINVAL;
     try {
         HIPC(hipSetDevice(ctx->c.device));
         f();
+        check_device_error(&ctx->c);
         return DDJ_OK;
     } catch (const Error& e) {
         ctx->c.last_error = e.msg;
@@ -353,6 +367,11 @@ ddj_status ddj_optimizer_tree(ddj_optimizer* opt, int* path_out, size_t path_cap
 
 }  // extern "C"
 
+struct ddj_planner {
+    ddj_ctx* ctx;
+    std::unique_ptr<PlannerNet> net;
+};
+
 struct ddj_ts_compressor {
     ddj_ctx* ctx;
     std::vector<int> dtypes;
@@ -466,6 +485,122 @@ ddj_status ddj_ts_reader_next(ddj_ts_reader* r, void* const* host_columns, size_
             d2h(c, host_columns[k], out.p, nr * es);
         }
         *rows = batch;
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------- learned planner
+
+extern "C" {
+
+ddj_status ddj_planner_candidates(int* paths, size_t cap, size_t* offsets) {
+    const auto& c = candidate_paths();
+    size_t total = 0;
+    for (auto& p : c) total += p.size();
+    if (!offsets || (paths && cap < total)) return DDJ_EINVAL;
+    size_t off = 0;
+    for (size_t i = 0; i < c.size(); i++) {
+        offsets[i] = off;
+        if (paths) memcpy(paths + off, c[i].data(), sizeof(int) * c[i].size());
+        off += c[i].size();
+    }
+    offsets[c.size()] = off;
+    return DDJ_OK;
+}
+
+ddj_status ddj_planner_features(ddj_ctx* ctx, int dtype, const void* d_in, size_t in_bytes, float* out) {
+    return guarded(ctx, [&] {
+        if (!out || !dtype_ok(dtype)) fail(DDJ_EINVAL, "bad argument");
+        auto f = block_features(&ctx->c, dtype, DView{static_cast<const uint8_t*>(d_in), in_bytes});
+        memcpy(out, f.data(), sizeof(float) * f.size());
+    });
+}
+
+ddj_status ddj_planner_labels(ddj_ctx* ctx, int dtype, const void* d_in, size_t in_bytes, float* out) {
+    return guarded(ctx, [&] {
+        if (!out || !dtype_ok(dtype)) fail(DDJ_EINVAL, "bad argument");
+        auto l = candidate_labels(&ctx->c, dtype, DView{static_cast<const uint8_t*>(d_in), in_bytes});
+        memcpy(out, l.data(), sizeof(float) * l.size());
+    });
+}
+
+ddj_status ddj_planner_create(ddj_ctx* ctx, uint64_t seed, ddj_planner** out) {
+    return guarded(ctx, [&] {
+        if (!out) fail(DDJ_EINVAL, "null output");
+        auto* p = new ddj_planner;
+        p->ctx = ctx;
+        p->net.reset(new PlannerNet(&ctx->c, seed));
+        *out = p;
+    });
+}
+
+void ddj_planner_destroy(ddj_planner* p) { delete p; }
+
+ddj_status ddj_planner_params(ddj_planner* p, float* get_out, const float* set_in) {
+    if (!p) return DDJ_EINVAL;
+    return guarded(p->ctx, [&] {
+        if (set_in) p->net->set(set_in);
+        if (get_out) p->net->get(get_out);
+    });
+}
+
+ddj_status ddj_planner_gradient(ddj_planner* p, const float* X, const float* T, size_t n, float* grad_out,
+                                float* loss) {
+    if (!p) return DDJ_EINVAL;
+    return guarded(p->ctx, [&] {
+        if (!X || !T) fail(DDJ_EINVAL, "bad argument");
+        const float l = p->net->gradient(X, T, n, grad_out);
+        if (loss) *loss = l;
+    });
+}
+
+ddj_status ddj_planner_train(ddj_planner* p, const float* X, const float* T, size_t n, int epochs, int batch,
+                             float lr, uint64_t seed, float* loss) {
+    if (!p) return DDJ_EINVAL;
+    return guarded(p->ctx, [&] {
+        if (!X || !T) fail(DDJ_EINVAL, "bad argument");
+        const float l = p->net->train(X, T, n, epochs, batch, lr, seed);
+        if (loss) *loss = l;
+    });
+}
+
+ddj_status ddj_planner_predict(ddj_planner* p, const float* X, size_t n, float* Y) {
+    if (!p) return DDJ_EINVAL;
+    return guarded(p->ctx, [&] {
+        if (!X || !Y) fail(DDJ_EINVAL, "bad argument");
+        p->net->predict(X, n, Y);
+    });
+}
+
+ddj_status ddj_planner_plan(ddj_planner* p, int dtype, const void* d_in, size_t in_bytes, float min_margin,
+                            int* path_out, size_t path_cap, size_t* path_len, int* used_fallback, float* margin) {
+    if (!p) return DDJ_EINVAL;
+    return guarded(p->ctx, [&] {
+        if (!dtype_ok(dtype) || !path_len) fail(DDJ_EINVAL, "bad argument");
+        Ctx* c = &p->ctx->c;
+        const DView data{static_cast<const uint8_t*>(d_in), in_bytes};
+        auto f = block_features(c, dtype, data);
+        float y[kMlpOutputs];
+        p->net->predict(f.data(), 1, y);
+        int best = 0, second = -1;
+        for (int i = 1; i < kMlpOutputs; i++)
+            if (y[i] < y[best]) best = i;
+        for (int i = 0; i < kMlpOutputs; i++)
+            if (i != best && (second < 0 || y[i] < y[second])) second = i;
+        const float m = y[second] - y[best];
+        if (margin) *margin = m;
+        NodeP tree;
+        if (m < min_margin) {
+            tree = plan_exhaustive(c, data, dtype, nullptr, nullptr);
+            if (used_fallback) *used_fallback = 1;
+        } else {
+            const auto& cp = candidate_paths()[best];
+            tree = tree_from_path(cp.data(), cp.size(), dtype);
+            fix_tree(tree);
+            if (used_fallback) *used_fallback = 0;
+        }
+        write_path(tree, path_out, path_cap, path_len);
     });
 }
 

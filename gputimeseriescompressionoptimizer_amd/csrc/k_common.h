@@ -132,8 +132,8 @@ template <typename T> __device__ __forceinline__ T block_exclusive_scan(T v, T* 
 }
 
 // ------------------------------------------------ decoupled look-back scan
-// Tiles take tickets in start order, publish their aggregate, and add up the
-// predecessors' values until one with an inclusive prefix is found.
+// Tiles publish their aggregate and add up the predecessors' values until
+// one with an inclusive prefix is found.
 //
 // A tile's state and value travel together in one 64-bit status word
 // (bits 63..62 state, 61..0 value), so a relaxed agent-scope atomic load sees
@@ -184,26 +184,33 @@ template <bool WIDE> __device__ __forceinline__ bool lb_read(const Lookback& lb,
     }
 }
 
-// Exclusive prefix of `tile`.  Called by all 64 lanes of one wave with the
-// same tile and aggregate.  Narrow values must stay below 2^62 (counts), or
-// only their low bits matter (sums that wrap at 32 bits or fewer).
-template <bool WIDE> __device__ uint64_t lookback_wave(const Lookback& lb, uint32_t tile, uint64_t aggregate) {
-    const int lane = threadIdx.x & (kWave - 1);
+// Exclusive prefix of `tile`, resolved by the whole workgroup (kBlock
+// threads, all of which call it with the same tile and aggregate): each
+// step inspects the kBlock predecessors before the current window, so a
+// tile that starts while many predecessors are still resolving walks back
+// kBlock tiles per device round trip.  Narrow values must stay below 2^62
+// (counts), or only their low bits matter (sums that wrap at 32 bits or
+// fewer).
+template <bool WIDE> __device__ uint64_t block_lookback(const Lookback& lb, uint32_t tile, uint64_t aggregate) {
+    __shared__ uint64_t red[kBlock / kWave];
+    __shared__ uint32_t first_inc[kBlock / kWave];
+    const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
     if (tile == 0) {
-        if (lane == 0) lb_publish<WIDE>(lb, 0, kStInclusive, aggregate);
+        if (t == 0) lb_publish<WIDE>(lb, 0, kStInclusive, aggregate);
+        __syncthreads();  // the call is a barrier on every path
         return 0;
     }
-    if (lane == 0) lb_publish<WIDE>(lb, tile, kStAggregate, aggregate);
+    if (t == 0) lb_publish<WIDE>(lb, tile, kStAggregate, aggregate);
     uint64_t excl = 0;
-    int64_t end = tile;  // window [end - 64, end)
+    int64_t end = tile;  // window [end - kBlock, end)
     for (;;) {
-        const int64_t idx = end - 1 - lane;
+        const int64_t idx = end - 1 - t;
         uint64_t state = kStInclusive, v = 0;  // before tile 0: an inclusive zero
         bool ready = idx < 0;
         uint32_t spins = 0;
         for (;;) {
             if (!ready) ready = lb_read<WIDE>(lb, (size_t)idx, state, v);
-            if (__all(ready)) break;
+            if (__syncthreads_and(ready)) break;
             if (++spins > kSpinLimit) {
                 if (!ready) {
                     __hip_atomic_store(lb.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -214,37 +221,61 @@ template <bool WIDE> __device__ uint64_t lookback_wave(const Lookback& lb, uint3
             }
             __builtin_amdgcn_s_sleep(1);
         }
+        // the nearest inclusive predecessor is the smallest t holding one
         const uint64_t inc = __ballot(state == kStInclusive);
-        if (inc) {
-            const int first = __builtin_ctzll(inc);
-            excl += wave_sum<uint64_t>(lane <= first ? v : 0);
-            break;
-        }
-        excl += wave_sum<uint64_t>(v);
-        end -= kWave;
+        if (lane == 0) first_inc[wid] = inc ? (uint32_t)(wid * kWave + __builtin_ctzll(inc)) : 0xffffffffu;
+        __syncthreads();
+        uint32_t first = 0xffffffffu;
+#pragma unroll
+        for (int w = 0; w < kBlock / kWave; w++) first = first_inc[w] < first ? first_inc[w] : first;
+        const uint64_t ws = wave_sum<uint64_t>((uint32_t)t <= first ? v : 0);
+        if (lane == 0) red[wid] = ws;
+        __syncthreads();
+        uint64_t sum = 0;
+#pragma unroll
+        for (int w = 0; w < kBlock / kWave; w++) sum += red[w];
+        excl += sum;
+        __syncthreads();  // red / first_inc are rewritten by the next step
+        if (first != 0xffffffffu) break;
+        end -= kBlock;
     }
-    if (lane == 0) lb_publish<WIDE>(lb, tile, kStInclusive, excl + aggregate);
+    if (t == 0) lb_publish<WIDE>(lb, tile, kStInclusive, excl + aggregate);
     return excl;
 }
 
-// Ticket of the calling workgroup (all threads get it).
-__device__ __forceinline__ uint32_t take_ticket(const Lookback& lb, uint32_t* smem) {
-    if (threadIdx.x == 0) *smem = atomicAdd(lb.ticket, 1u);
-    __syncthreads();
-    uint32_t t = *smem;
-    __syncthreads();
-    return t;
+// Tile of the calling workgroup.  Workgroups are dispatched in increasing
+// blockIdx order on every XCD (round-robin over the XCDs), so the lowest
+// unfinished tile is always resident and the look-back always progresses;
+// the bounded waits cover anything else.  A ticket counter (tiles numbered
+// in start order) would cost one device-scope atomic on a single address per
+// tile, and those serialise at roughly 15-20 ns each: 16K tiles of a 64M-row
+// column would spend ~300 us there.
+__device__ __forceinline__ uint32_t tile_index() { return blockIdx.x; }
+
+// Relaxed agent-scope stores and loads for values that one workgroup
+// publishes and another reads within the same launch.
+__device__ __forceinline__ void st_agent(uint64_t* p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Block-wide look-back: wave 0 resolves the prefix, every thread gets it.
-template <bool WIDE> __device__ __forceinline__ uint64_t block_lookback(const Lookback& lb, uint32_t tile,
-                                                                       uint64_t aggregate, uint64_t* smem) {
-    if (threadIdx.x < kWave) {
-        const uint64_t p = lookback_wave<WIDE>(lb, tile, aggregate);
-        if (threadIdx.x == 0) *smem = p;
+// "Last workgroup finishes" without acquire/release fences (which would
+// write back and invalidate the XCD's whole L2 once per workgroup): every
+// thread has written its share of the partial with st_agent; the waitcnt
+// makes those stores complete at the device coherence point before the
+// barrier, and only then does thread 0 take a ticket.  The last workgroup
+// reads the partials with relaxed agent-scope loads, which go to that point.
+__device__ __forceinline__ bool last_block_arrives(uint32_t* counter, bool* sm_flag) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *sm_flag = (prev == gridDim.x - 1);
     }
     __syncthreads();
-    return *smem;
+    return *sm_flag;
 }
 
 // -------------------------------------------------------------- bit helpers

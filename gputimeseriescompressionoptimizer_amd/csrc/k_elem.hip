@@ -175,9 +175,7 @@ __global__ __launch_bounds__(kBlock) void k_delta_decode(const uint8_t* first_by
     __builtin_memcpy(&first, first_bytes, sizeof(T));
     __shared__ uint64_t buf[kScanTile + kScanTile / 32];
     __shared__ uint64_t red[kBlock / kWave];
-    __shared__ uint32_t tk;
-    __shared__ uint64_t tile_prefix;
-    const uint32_t tile = take_ticket(lb, &tk);
+    const uint32_t tile = tile_index();
     const size_t base = (size_t)tile * kScanTile;
     const int cnt = (int)((m - base) < (size_t)kScanTile ? (m - base) : (size_t)kScanTile);
     auto pad = [](int i) { return i + (i >> 5); };
@@ -207,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_delta_decode(const uint8_t* first_by
     uint64_t total;
     uint64_t excl = block_exclusive_scan<uint64_t>(sum, &total, red);
     // sums wrap at the width of T: 8-byte types need the wide status words
-    const uint64_t prefix = block_lookback<sizeof(T) == 8>(lb, tile, total, &tile_prefix);
+    const uint64_t prefix = block_lookback<sizeof(T) == 8>(lb, tile, total);
     uint64_t run = ubits(first) + prefix + excl;
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
@@ -286,10 +284,9 @@ void launch_int_to_float(Ctx* c, int dtype, const uint8_t* x, size_t n, int prec
 }
 
 LookbackBufs::LookbackBufs(Ctx* c, size_t tiles, bool wide)
-    : mem(c, 8 * tiles * (wide ? 2 : 1) + 8) {
+    : mem(c, 8 * tiles * (wide ? 2 : 1)) {
     launch_fill_zero(c, mem.get(), mem.size());
-    lb = Lookback{mem.get<uint64_t>(), reinterpret_cast<uint32_t*>(mem.get<uint64_t>() + tiles * (wide ? 2 : 1)),
-                  &c->control->error_flag};
+    lb = Lookback{mem.get<uint64_t>(), &c->control->error_flag};
 }
 
 void launch_fill_zero(Ctx* c, void* p, size_t bytes) {

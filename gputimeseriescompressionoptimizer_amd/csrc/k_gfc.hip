@@ -53,9 +53,7 @@ __global__ __launch_bounds__(kBlock) void k_gfc_encode(const U* __restrict__ x, 
     __shared__ __attribute__((aligned(16))) uint8_t out[kGfcMaxPer / 32 * (16 + 32 * (int)sizeof(U)) + 16];
     __shared__ U s_last;
     __shared__ uint64_t red[kBlock / kWave];
-    __shared__ uint32_t tk;
-    __shared__ uint64_t s_prefix;
-    const uint32_t chunk = take_ticket(lb, &tk);
+    const uint32_t chunk = tile_index();
     const long long c0 = (long long)chunk * per;
     long long c1 = c0 + per;
     c1 = c1 < N ? c1 : N;
@@ -120,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_gfc_encode(const U* __restrict__ x, 
             pos += nbs[k];
         }
     }
-    block_lookback<false>(lb, chunk, chunk_bytes, &s_prefix);
+    const uint64_t s_prefix = block_lookback<false>(lb, chunk, chunk_bytes);
     const uint64_t O = s_prefix;
     for (uint32_t i = threadIdx.x; i < (uint32_t)chunk_bytes; i += kBlock) payload[O + i] = out[i];
     if (threadIdx.x == 0) {
@@ -141,10 +139,8 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     __shared__ uint32_t sub_off[kGfcMaxPer / 32 + 1];
     __shared__ U vals[kGfcMaxPer];
     __shared__ U s_last;
-    __shared__ uint32_t tk;
-    __shared__ uint64_t s_prefix;
     __shared__ int s_bad;
-    const uint32_t chunk = take_ticket(lb, &tk);
+    const uint32_t chunk = tile_index();
     const long long c0 = (long long)chunk * per;
     long long c1 = c0 + per;
     c1 = c1 < N ? c1 : N;
@@ -157,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     if (cb < 0 || cb > kMaxBytes) cb = 0;  // corrupt: decode as empty (the host validates totals)
     const uint64_t chunk_bytes = (uint64_t)cb;
     if (threadIdx.x == 0) s_bad = 0;
-    block_lookback<false>(lb, chunk, chunk_bytes, &s_prefix);
+    const uint64_t s_prefix = block_lookback<false>(lb, chunk, chunk_bytes);
     const uint64_t P = s_prefix;
     for (uint32_t i = threadIdx.x; i < (uint32_t)chunk_bytes; i += kBlock)
         buf[i] = (P + i < payload_bytes) ? payload[P + i] : 0;

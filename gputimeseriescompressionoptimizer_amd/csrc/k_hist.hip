@@ -245,20 +245,24 @@ __global__ __launch_bounds__(kBlock) void k_hist_topk(const unsigned long long* 
         Cand b[kMaxK];
         for (int i = 0; i < kMaxK; i++) b[i] = Cand{0, 0};
         for (int i = 0; i < kWave * K; i++) push<T>(b, K, sc[i]);
-        for (int i = 0; i < K; i++) partials[blockIdx.x * K + i] = b[i];
-        uint32_t prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == gridDim.x - 1);
+        for (int i = 0; i < K; i++) {
+            st_agent(reinterpret_cast<uint64_t*>(&partials[blockIdx.x * K + i].count), (uint64_t)b[i].count);
+            st_agent(&partials[blockIdx.x * K + i].key, b[i].key);
+        }
     }
-    __syncthreads();
-    if (!last || threadIdx.x != 0) return;
-    Cand b[kMaxK];
-    for (int i = 0; i < kMaxK; i++) b[i] = Cand{0, 0};
-    for (uint32_t i = 0; i < gridDim.x * (uint32_t)K; i++) {
+    if (!last_block_arrives(counter, &last)) return;
+    // the last workgroup gathers every partial list into LDS, thread 0 merges
+    for (uint32_t i = threadIdx.x; i < gridDim.x * (uint32_t)K; i += kBlock) {
         Cand c;
         c.count = __hip_atomic_load(&partials[i].count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         c.key = __hip_atomic_load(&partials[i].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        push<T>(b, K, c);
+        sc[i] = c;
     }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    Cand b[kMaxK];
+    for (int i = 0; i < kMaxK; i++) b[i] = Cand{0, 0};
+    for (uint32_t i = 0; i < gridDim.x * (uint32_t)K; i++) push<T>(b, K, sc[i]);
     if (ctl->special) push<T>(b, K, Cand{ctl->special, kEmpty});
     if constexpr (std::is_floating_point<T>::value) {
         if (ctl->nans)  // every NaN row is a bin of count 1

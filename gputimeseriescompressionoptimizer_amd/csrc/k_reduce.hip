@@ -45,16 +45,6 @@ template <typename A> __device__ __forceinline__ A block_reduce_sum(A v, A* sm) 
     return r;
 }
 
-__device__ __forceinline__ bool last_block_arrives(uint32_t* counter, bool* sm_flag) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        *sm_flag = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    return *sm_flag;
-}
-
 __device__ __forceinline__ uint64_t rd(const uint64_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -114,12 +104,23 @@ __global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size
     if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {
         const u32x4* xv = reinterpret_cast<const u32x4*>(x);
         const size_t nv = n / V;
-        for (size_t i = tid; i < nv; i += stride) {
-            u32x4 v = __builtin_nontemporal_load(xv + i);
-            T vals[V];
-            __builtin_memcpy(vals, &v, 16);
+        constexpr int U = 4;  // loads in flight per thread
+        for (size_t i0 = tid; i0 < nv; i0 += U * stride) {
+            u32x4 v[U];
 #pragma unroll
-            for (int k = 0; k < V; k++) f.add(vals[k], i * V + k);
+            for (int u = 0; u < U; u++) {
+                const size_t i = i0 + u * stride;
+                if (i < nv) v[u] = __builtin_nontemporal_load(xv + i);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const size_t i = i0 + u * stride;
+                if (i >= nv) break;
+                T vals[V];
+                __builtin_memcpy(vals, &v[u], 16);
+#pragma unroll
+                for (int k = 0; k < V; k++) f.add(vals[k], i * V + k);
+            }
         }
         done = nv * V;
     }
@@ -128,9 +129,9 @@ __global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size
     uint64_t mx = block_reduce_max(f.mx, sm);
     uint64_t zf = block_reduce_min(f.zf, sm);
     if (threadIdx.x == 0) {
-        partials[3 * blockIdx.x] = mn;
-        partials[3 * blockIdx.x + 1] = mx;
-        partials[3 * blockIdx.x + 2] = zf;
+        st_agent(partials + 3 * blockIdx.x, mn);
+        st_agent(partials + 3 * blockIdx.x + 1, mx);
+        st_agent(partials + 3 * blockIdx.x + 2, zf);
     }
     if (!last_block_arrives(counter, &last)) return;
     mn = ~0ull;
@@ -295,20 +296,21 @@ __global__ __launch_bounds__(kBlock) void k_stats(const T* __restrict__ x, size_
     uint64_t dm = block_reduce_max((uint64_t)a.dmax, sm);
     DevStats* p = partials + blockIdx.x;
     if (threadIdx.x == 0) {
-        p->min_key = mn;
-        p->max_key = mx;
-        p->first_zero = zf;
-        p->unsorted = uns;
-        p->rl_eq = rl;
-        p->runs = runs;
-        p->sum_bits = sb;
-        p->sum_f = sf;
-        p->precision = (uint32_t)pr;
-        p->delta_bits_max = (uint32_t)dm;
+        st_agent(&p->min_key, mn);
+        st_agent(&p->max_key, mx);
+        st_agent(&p->first_zero, zf);
+        st_agent(&p->unsorted, uns);
+        st_agent(&p->rl_eq, rl);
+        st_agent(&p->runs, runs);
+        st_agent(&p->sum_bits, sb);
+        uint64_t sfb;
+        __builtin_memcpy(&sfb, &sf, 8);
+        st_agent(reinterpret_cast<uint64_t*>(&p->sum_f), sfb);
+        st_agent(&p->precision, (uint32_t)pr);
+        st_agent(&p->delta_bits_max, (uint32_t)dm);
     }
-    for (int i = threadIdx.x; i < 65; i += kBlock) p->bit_hist[i] = sm_hist[i];
-    for (int i = threadIdx.x; i < 64; i += kBlock) p->hll[i] = sm_hll[i];
-    __threadfence();
+    for (int i = threadIdx.x; i < 65; i += kBlock) st_agent(&p->bit_hist[i], sm_hist[i]);
+    for (int i = threadIdx.x; i < 64; i += kBlock) st_agent(&p->hll[i], sm_hll[i]);
     if (!last_block_arrives(counter, &last)) return;
     StatsAcc f{~0ull, 0, ~0ull, 0, 0, 0, 0, 0.0, 0, 0};
     for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {

@@ -61,13 +61,23 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     __shared__ T buf[kTilePad];
     __shared__ uint8_t cbuf[CODES ? kTile : 1];
     __shared__ uint64_t red[kBlock / kWave];
-    __shared__ uint32_t tk;
-    __shared__ uint64_t s_prefix;
-    const uint32_t tile = take_ticket(lb, &tk);
+    const uint32_t tile = tile_index();
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     const T padv = x[n - 1];  // stencil padding repeats the last row (reference stencil.cu:26-27)
-    for (int i = threadIdx.x; i < kTile; i += kBlock) buf[P(i)] = i < cnt ? x[base + i] : padv;
+    if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
+        constexpr int V = 16 / sizeof(T);
+        const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
+        for (int i = threadIdx.x; i < kTile / V; i += kBlock) {
+            const u32x4 w = __builtin_nontemporal_load(s4 + i);
+            T vals[V];
+            __builtin_memcpy(vals, &w, 16);
+#pragma unroll
+            for (int k = 0; k < V; k++) buf[P(V * i + k)] = vals[k];
+        }
+    } else {
+        for (int i = threadIdx.x; i < kTile; i += kBlock) buf[P(i)] = i < cnt ? x[base + i] : padv;
+    }
     __syncthreads();
     T v[kItems];
     uint32_t mask = 0;
@@ -88,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     if (tile == 0 && threadIdx.x == 0) stencil[0] = (uint8_t)(n % 8);
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)c, &total, red);
-    block_lookback<false>(lb, tile, total, &s_prefix);
+    const uint64_t s_prefix = block_lookback<false>(lb, tile, total);
     const uint64_t Y0 = s_prefix;
     int yr = (int)excl, nr = threadIdx.x * kItems - (int)excl;
     const int nbase = CODES ? 0 : (int)total;
@@ -129,9 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
                                                   Lookback lb) {
     __shared__ T buf[kTilePad];
     __shared__ uint64_t red[kBlock / kWave];
-    __shared__ uint32_t tk;
-    __shared__ uint64_t s_prefix;
-    const uint32_t tile = take_ticket(lb, &tk);
+    const uint32_t tile = tile_index();
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     const size_t g0 = base + (size_t)threadIdx.x * kItems;
@@ -145,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
         mask &= (1u << mine) - 1u;
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &total, red);
-    block_lookback<false>(lb, tile, total, &s_prefix);
+    const uint64_t s_prefix = block_lookback<false>(lb, tile, total);
     const uint64_t Y0 = s_prefix;
     const size_t N0 = base - Y0;
     for (int i = threadIdx.x; i < cnt; i += kBlock) {
@@ -212,14 +220,26 @@ __global__ __launch_bounds__(kBlock) void k_rle_encode(const T* __restrict__ x, 
                                                        uint64_t* d_runs) {
     __shared__ T buf[kTilePad + 1];
     __shared__ uint64_t red[kBlock / kWave];
-    __shared__ uint32_t tk;
-    __shared__ uint64_t s_prefix;
-    const uint32_t tile = take_ticket(lb, &tk);
+    const uint32_t tile = tile_index();
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
-    for (int i = threadIdx.x; i < kTile + 1; i += kBlock) {
-        const long long g = (long long)base + i - 1;
-        buf[P(i)] = (g >= 0 && i <= cnt) ? x[g] : T(0);
+    if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
+        // buf[i + 1] = x[base + i]; buf[0] = the row before the tile
+        constexpr int V = 16 / sizeof(T);
+        const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
+        for (int i = threadIdx.x; i < kTile / V; i += kBlock) {
+            const u32x4 w = __builtin_nontemporal_load(s4 + i);
+            T vals[V];
+            __builtin_memcpy(vals, &w, 16);
+#pragma unroll
+            for (int k = 0; k < V; k++) buf[P(V * i + k + 1)] = vals[k];
+        }
+        if (threadIdx.x == 0) buf[P(0)] = base > 0 ? x[base - 1] : T(0);
+    } else {
+        for (int i = threadIdx.x; i < kTile + 1; i += kBlock) {
+            const long long g = (long long)base + i - 1;
+            buf[P(i)] = (g >= 0 && i <= cnt) ? x[g] : T(0);
+        }
     }
     __syncthreads();
     uint32_t mask = 0;
@@ -233,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_rle_encode(const T* __restrict__ x, 
     }
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &total, red);
-    block_lookback<false>(lb, tile, total, &s_prefix);
+    const uint64_t s_prefix = block_lookback<false>(lb, tile, total);
     uint64_t r = s_prefix + excl;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
@@ -261,10 +281,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_lengths(const uint8_t* __restri
                                                          uint64_t* __restrict__ offsets, Lookback lb) {
     __shared__ uint64_t buf[kTile + kTile / 32];
     __shared__ uint64_t red[kBlock / kWave];
-    __shared__ uint32_t tk;
-    __shared__ uint64_t s_prefix;
     auto pad = [](int i) { return i + (i >> 5); };
-    const uint32_t tile = take_ticket(lb, &tk);
+    const uint32_t tile = tile_index();
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((R - base) < (size_t)kTile ? (R - base) : (size_t)kTile);
     const bool aligned = (reinterpret_cast<uintptr_t>(lens) & 3) == 0;
@@ -275,6 +293,10 @@ __global__ __launch_bounds__(kBlock) void k_scan_lengths(const uint8_t* __restri
                 l = reinterpret_cast<const int32_t*>(lens)[base + i];
             else
                 __builtin_memcpy(&l, lens + 4 * (base + i), 4);
+        }
+        if (l < 0) {  // a run of negative length: corrupt input (decoded as empty)
+            __hip_atomic_store(lb.error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            l = 0;
         }
         buf[pad(i)] = (uint64_t)(int64_t)l;
     }
@@ -287,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_lengths(const uint8_t* __restri
     }
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>(s, &total, red);
-    block_lookback<true>(lb, tile, total, &s_prefix);
+    const uint64_t s_prefix = block_lookback<true>(lb, tile, total);
     uint64_t run = s_prefix + excl;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
@@ -304,46 +326,68 @@ __global__ __launch_bounds__(kBlock) void k_scan_lengths(const uint8_t* __restri
 // gives every row its run by a binary search in LDS.
 constexpr int kExpTile = 2048;
 
+// tile_run[t] = the run holding row t * kExpTile (offsets are non-decreasing:
+// lengths were checked non-negative by the scan); tile_run[tiles] = R.
+__global__ __launch_bounds__(kBlock) void k_rle_tile_runs(const uint64_t* __restrict__ offsets, size_t R, size_t tiles,
+                                                          uint32_t* __restrict__ tile_run) {
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t r = (size_t)blockIdx.x * kBlock + threadIdx.x; r < R; r += stride) {
+        const uint64_t s = offsets[r], e = offsets[r + 1];
+        const uint64_t t0 = (s + kExpTile - 1) / kExpTile, t1 = (e + kExpTile - 1) / kExpTile;  // tiles starting in [s, e)
+        for (uint64_t t = t0; t < t1 && t < tiles; t++) tile_run[t] = (uint32_t)r;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) tile_run[tiles] = (uint32_t)R;
+}
+
+// Run expansion: a workgroup owns kExpTile output rows; its runs are
+// tile_run[b] .. tile_run[b + 1] (the run holding the next tile's first row).
+// They are staged in LDS and every row finds its run by binary search there;
+// a tile overlapping more runs than LDS holds (only possible with empty runs)
+// searches the offsets in global memory instead.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_rle_expand(const uint64_t* __restrict__ offsets, size_t R,
                                                        const uint8_t* __restrict__ values, size_t total,
-                                                       T* __restrict__ out) {
+                                                       const uint32_t* __restrict__ tile_run, T* __restrict__ out) {
     __shared__ uint64_t soff[kExpTile + 2];
     __shared__ T sval[kExpTile + 1];
-    __shared__ size_t s_r0;
-    __shared__ int s_nr;
     const size_t o0 = (size_t)blockIdx.x * kExpTile;
     const int cnt = (int)((total - o0) < (size_t)kExpTile ? (total - o0) : (size_t)kExpTile);
-    if (threadIdx.x == 0) {
-        // last r with offsets[r] <= o0
-        size_t lo = 0, hi = R;  // offsets[R] = total > o0
-        while (hi - lo > 1) {
-            size_t mid = (lo + hi) / 2;
-            if (offsets[mid] <= o0)
-                lo = mid;
-            else
-                hi = mid;
+    const size_t r0 = tile_run[blockIdx.x];
+    size_t r1 = tile_run[blockIdx.x + 1];  // last run needed (inclusive)
+    r1 = r1 < R ? r1 : R - 1;
+    r1 = r1 < r0 ? r0 : r1;
+    const size_t nr = r1 - r0 + 1;
+    if (nr <= (size_t)kExpTile + 1) {
+        for (int i = threadIdx.x; i <= (int)nr; i += kBlock) soff[i] = offsets[r0 + i];
+        for (int i = threadIdx.x; i < (int)nr; i += kBlock) __builtin_memcpy(&sval[i], values + sizeof(T) * (r0 + i), sizeof(T));
+        __syncthreads();
+        for (int k = threadIdx.x; k < cnt; k += kBlock) {
+            const uint64_t pos = o0 + k;
+            int lo = 0, hi = (int)nr;  // soff[lo] <= pos < soff[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) / 2;
+                if (soff[mid] <= pos)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            out[pos] = sval[lo];
         }
-        s_r0 = lo;
-    }
-    __syncthreads();
-    const size_t r0 = s_r0;
-    size_t avail = R - r0;  // runs r0 .. R-1
-    int nr = (int)(avail < (size_t)(kExpTile + 1) ? avail : (size_t)(kExpTile + 1));
-    for (int i = threadIdx.x; i <= nr; i += kBlock) soff[i] = offsets[r0 + i];
-    for (int i = threadIdx.x; i < nr; i += kBlock) __builtin_memcpy(&sval[i], values + sizeof(T) * (r0 + i), sizeof(T));
-    __syncthreads();
-    for (int k = threadIdx.x; k < cnt; k += kBlock) {
-        const uint64_t pos = o0 + k;
-        int lo = 0, hi = nr;  // soff[lo] <= pos < soff[hi]
-        while (hi - lo > 1) {
-            int mid = (lo + hi) / 2;
-            if (soff[mid] <= pos)
-                lo = mid;
-            else
-                hi = mid;
+    } else {
+        for (int k = threadIdx.x; k < cnt; k += kBlock) {
+            const uint64_t pos = o0 + k;
+            size_t lo = r0, hi = r1 + 1;  // offsets[lo] <= pos < offsets[hi]
+            while (hi - lo > 1) {
+                const size_t mid = (lo + hi) / 2;
+                if (offsets[mid] <= pos)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            T v;
+            __builtin_memcpy(&v, values + sizeof(T) * lo, sizeof(T));
+            out[pos] = v;
         }
-        out[pos] = sval[lo];
     }
 }
 
@@ -457,8 +501,12 @@ DevBuf rle_decode_dev(Ctx* c, int dtype, const uint8_t* lengths, const uint8_t* 
         DevBuf out(c, total * sizeof(T));
         if (total == 0) return out;
         const size_t etiles = (total + kExpTile - 1) / kExpTile;
+        DevBuf tile_run(c, 4 * (etiles + 1));
+        { KernelTimer kt_(c, "k_rle_tile_runs"); hipLaunchKernelGGL(k_rle_tile_runs, dim3(grid_for(c, runs)), dim3(kBlock), 0, c->stream,
+                           offsets.get<uint64_t>(), runs, etiles, tile_run.get<uint32_t>()); }
+        kernel_bytes(c, "k_rle_expand", (double)total * sizeof(T) + (double)runs * (8 + sizeof(T)));
         { KernelTimer kt_(c, "k_rle_expand"); hipLaunchKernelGGL(k_rle_expand<T>, dim3((unsigned)etiles), dim3(kBlock), 0, c->stream,
-                           offsets.get<uint64_t>(), runs, values, (size_t)total, out.get<T>()); }
+                           offsets.get<uint64_t>(), runs, values, (size_t)total, tile_run.get<uint32_t>(), out.get<T>()); }
         HIPC(hipGetLastError());
         return out;
     });

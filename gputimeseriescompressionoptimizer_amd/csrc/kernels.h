@@ -7,11 +7,10 @@
 namespace ddj {
 
 // Device state of one decoupled look-back launch: a 64-bit status word per
-// tile (two per tile for wide values), a ticket counter and the context's
-// error flag.  Status words and ticket start at zero.
+// tile (two per tile for wide values), zeroed before the launch, and the
+// context's error flag.
 struct Lookback {
     uint64_t* status;
-    uint32_t* ticket;
     uint32_t* error;
 };
 struct LookbackBufs {
@@ -96,6 +95,32 @@ void topk_most_frequent(Ctx* c, int dtype, const void* x, size_t n, int K, uint6
 void gfc_encode(Ctx* c, bool dbl, const void* x, size_t n, DevBuf& meta, DevBuf& offsets, DevBuf& payload);
 void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const uint8_t* offsets, size_t off_bytes,
                 const uint8_t* payload, size_t payload_bytes, DevBuf& out);
+
+// Planner network (32 -> 128 -> 128 -> 48, ReLU), FP32 parameters.
+constexpr int kMlpFeatures = 32, kMlpHidden = 128, kMlpOutputs = 48, kMlpOutputsPadded = 64;
+struct MlpParams {
+    float* W1;  // [32][128]
+    float* b1;  // [128]
+    float* W2;  // [128][128]
+    float* b2;  // [128]
+    float* W3;  // [128][64], columns 48..63 zero
+    float* b3;  // [64]
+};
+constexpr size_t kMlpParamCount = 32 * 128 + 128 + 128 * 128 + 128 + 128 * 64 + 64;
+inline MlpParams mlp_params_at(float* base) {
+    MlpParams p;
+    p.W1 = base;
+    p.b1 = p.W1 + 32 * 128;
+    p.W2 = p.b1 + 128;
+    p.b2 = p.W2 + 128 * 128;
+    p.W3 = p.b2 + 128;
+    p.b3 = p.W3 + 128 * 64;
+    return p;
+}
+void launch_mlp_forward(Ctx* c, const float* X, int B, const MlpParams& p, float* Y, uint16_t* H1s, uint16_t* H2s);
+void launch_mlp_backward(Ctx* c, const float* X, const uint16_t* H1s, const uint16_t* H2s, const float* Y,
+                         const float* T, int B, const MlpParams& p, const MlpParams& g);
+void launch_adam(Ctx* c, float* w, float* grad, float* m, float* v, int n, float lr, int step);
 
 // Zero-fill of device memory by a kernel (graph-capture safe).
 void launch_fill_zero(Ctx* c, void* p, size_t bytes);

@@ -1,0 +1,242 @@
+// Learned planner: features, exact candidate sizes, the network's training
+// and inference loops (kernels in k_mlp.hip).
+#include "learned.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+
+#include "planner.h"
+
+namespace ddj {
+
+const std::vector<std::vector<int>>& candidate_paths() {
+    static const std::vector<std::vector<int>> paths = [] {
+        const int D = DDJ_ENC_DELTA, A = DDJ_ENC_AFL, S = DDJ_ENC_SCALE, R = DDJ_ENC_RLE, K = DDJ_ENC_DICT,
+                  P = DDJ_ENC_PATCH, N = DDJ_ENC_NONE, C = DDJ_ENC_CONST;
+        std::vector<std::vector<int>> v = generate_paths();
+        const std::vector<std::vector<int>> extra = {
+            {N},
+            {D, N},
+            {S, N},
+            {D, D, A, N},
+            {S, D, A, N},
+            {D, S, A, N},
+            {R, N, N},
+            {R, A, N, N},
+            {R, A, N, D, A, N},
+            {R, D, A, N, A, N},
+            {K, N, N},
+            {K, A, N, D, A, N},
+            {P, N, N},
+            {P, D, A, N, A, N},
+            {C, N},
+            {D, R, A, N, D, A, N},
+            {D, K, N, N},
+            {D, P, A, N, N},
+        };
+        v.insert(v.end(), extra.begin(), extra.end());
+        return v;
+    }();
+    return paths;
+}
+
+std::array<float, kMlpFeatures> block_features(Ctx* c, int dtype, DView data) {
+    const ddj_stats_t s = stats(c, dtype, data);
+    std::array<float, kMlpFeatures> f{};
+    const double n = (double)s.size, W = 8.0 * (double)dtype_size(dtype);
+    if (s.size == 0) return f;
+    auto lg = [](double v) { return std::log2(1.0 + std::fabs(v)); };
+    int bl = (int)W;
+    if (s.min >= 0 && dtype != DDJ_D_FLOAT && dtype != DDJ_D_DOUBLE) {
+        uint64_t m = (uint64_t)s.max;
+        bl = 1;
+        while (m >>= 1) bl++;
+    }
+    f[0] = (float)(std::log2(n + 1) / 32);
+    f[1] = (float)(bl / W);
+    f[2] = (float)(s.delta_bits_max / (W + 1));
+    f[3] = (float)(s.runs / n);
+    f[4] = (float)(lg((double)s.runs) / 32);
+    f[5] = (float)std::min(1.0, (double)s.distinct_est / n);
+    f[6] = (float)(lg((double)s.distinct_est) / 32);
+    f[7] = (float)s.sorted;
+    f[8] = s.rlMetric - 1.0f;
+    f[9] = s.min < 0 ? 1.0f : 0.0f;
+    f[10] = (float)(lg(s.min) / 64);
+    f[11] = (float)(lg(s.max) / 64);
+    f[12] = (float)(lg(s.max - s.min) / 64);
+    f[13] = (float)(s.precision / 10.0);
+    // rows per zig-zag bit width, 17 groups over 0..W+1
+    for (int w = 0; w <= 64; w++) {
+        if (!s.bit_hist[w]) continue;
+        int grp = (int)((double)w * 17.0 / (W + 2.0));
+        grp = std::min(grp, 16);
+        f[14 + grp] += (float)(s.bit_hist[w] / n);
+    }
+    f[31] = (float)(lg(s.mean) / 64);
+    return f;
+}
+
+namespace {
+
+// Exact sizes of subtrees over device buffers, encoding each (buffer,
+// encoding) pair once.
+class SizeEval {
+public:
+    explicit SizeEval(Ctx* c) : c_(c) {}
+    size_t size(const NodeP& node, const Part& data) {
+        if (node->enc == DDJ_ENC_NONE) return 8 + 8 + data.n;  // header, size_t length, the data
+        const auto ek = std::make_tuple(data.p, data.n, node->enc, node->dtype);
+        auto it = enc_.find(ek);
+        if (it == enc_.end()) it = enc_.emplace(ek, encode(c_, node->enc, node->dtype, data.view())).first;
+        const std::vector<Part>& out = it->second;
+        size_t total = 8 + out[0].n;
+        if (node->ch.empty()) return total + (out.size() > 1 ? out[1].n : 0);
+        for (size_t i = 0; i < node->ch.size(); i++) total += size(node->ch[i], out[i + 1]);
+        return total;
+    }
+
+private:
+    Ctx* c_;
+    std::map<std::tuple<const uint8_t*, size_t, int, int>, std::vector<Part>> enc_;
+};
+
+uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+std::array<float, kCandidates> candidate_labels(Ctx* c, int dtype, DView data) {
+    if (dtype == DDJ_D_FLOAT || dtype == DDJ_D_DOUBLE) fail(DDJ_EUNSUPPORTED, "the learned planner covers integer columns");
+    std::array<float, kCandidates> out{};
+    SizeEval ev(c);
+    const Part root = borrow(data);
+    const auto& paths = candidate_paths();
+    for (int i = 0; i < kCandidates; i++) {
+        NodeP t = tree_from_path(paths[i].data(), paths[i].size(), dtype);
+        fix_tree(t);
+        const size_t sz = ev.size(t, root);
+        out[i] = (float)std::log2((double)sz / (double)std::max<size_t>(data.n, 1));
+    }
+    return out;
+}
+
+PlannerNet::PlannerNet(Ctx* c, uint64_t seed)
+    : c_(c), params_(c, 4 * kMlpParamCount), grads_(c, 4 * kMlpParamCount), m_(c, 4 * kMlpParamCount),
+      v_(c, 4 * kMlpParamCount) {
+    // He-uniform weights, zero biases, zero padding outputs
+    std::vector<float> h(kMlpParamCount, 0.0f);
+    MlpParams p = mlp_params_at(h.data());
+    uint64_t s = seed;
+    auto fill = [&](float* w, int K, int N, int N_real) {
+        const double a = std::sqrt(6.0 / K);
+        for (int k = 0; k < K; k++)
+            for (int n = 0; n < N; n++) {
+                const double u = (double)(splitmix(s) >> 11) / 9007199254740992.0;
+                w[k * N + n] = n < N_real ? (float)((2 * u - 1) * a) : 0.0f;
+            }
+    };
+    fill(p.W1, kMlpFeatures, kMlpHidden, kMlpHidden);
+    fill(p.W2, kMlpHidden, kMlpHidden, kMlpHidden);
+    fill(p.W3, kMlpHidden, kMlpOutputsPadded, kMlpOutputs);
+    set(h.data());
+    launch_fill_zero(c, grads_.get(), grads_.size());
+    launch_fill_zero(c, m_.get(), m_.size());
+    launch_fill_zero(c, v_.get(), v_.size());
+}
+
+void PlannerNet::get(float* host) const { d2h(c_, host, params_.get(), 4 * kMlpParamCount); }
+
+void PlannerNet::set(const float* host) {
+    std::vector<float> h(host, host + kMlpParamCount);
+    MlpParams p = mlp_params_at(h.data());
+    for (int k = 0; k < kMlpHidden; k++)  // the padding outputs stay zero
+        for (int n = kMlpOutputs; n < kMlpOutputsPadded; n++) p.W3[k * kMlpOutputsPadded + n] = 0.0f;
+    for (int n = kMlpOutputs; n < kMlpOutputsPadded; n++) p.b3[n] = 0.0f;
+    h2d(c_, params_.get(), h.data(), 4 * kMlpParamCount);
+}
+
+float PlannerNet::gradient(const float* X, const float* T, size_t n, float* grad_host) {
+    if (n == 0) fail(DDJ_EINVAL, "no rows");
+    DevBuf dX(c_, 4 * n * kMlpFeatures), dT(c_, 4 * n * kMlpOutputs), dY(c_, 4 * n * kMlpOutputs);
+    DevBuf h1(c_, 2 * n * kMlpHidden), h2(c_, 2 * n * kMlpHidden);
+    h2d(c_, dX.get(), X, 4 * n * kMlpFeatures);
+    h2d(c_, dT.get(), T, 4 * n * kMlpOutputs);
+    MlpParams p = mlp_params_at(params_.get<float>()), g = mlp_params_at(grads_.get<float>());
+    launch_fill_zero(c_, grads_.get(), grads_.size());
+    launch_mlp_forward(c_, dX.get<float>(), (int)n, p, dY.get<float>(), h1.get<uint16_t>(), h2.get<uint16_t>());
+    launch_mlp_backward(c_, dX.get<float>(), h1.get<uint16_t>(), h2.get<uint16_t>(), dY.get<float>(), dT.get<float>(),
+                        (int)n, p, g);
+    if (grad_host) d2h(c_, grad_host, grads_.get(), 4 * kMlpParamCount);
+    std::vector<float> y(n * kMlpOutputs);
+    d2h(c_, y.data(), dY.get(), 4 * y.size());
+    launch_fill_zero(c_, grads_.get(), grads_.size());
+    double l = 0;
+    for (size_t i = 0; i < y.size(); i++) l += (double)(y[i] - T[i]) * (y[i] - T[i]);
+    return (float)(l / (double)y.size());
+}
+
+float PlannerNet::train(const float* X, const float* T, size_t n, int epochs, int batch, float lr, uint64_t seed) {
+    if (n == 0 || batch <= 0 || epochs <= 0) fail(DDJ_EINVAL, "bad training arguments");
+    DevBuf dX(c_, 4 * n * kMlpFeatures), dT(c_, 4 * n * kMlpOutputs), dY(c_, 4 * n * kMlpOutputs);
+    DevBuf h1(c_, 2 * (size_t)batch * kMlpHidden), h2(c_, 2 * (size_t)batch * kMlpHidden);
+    MlpParams p = mlp_params_at(params_.get<float>()), g = mlp_params_at(grads_.get<float>());
+    std::vector<size_t> perm(n);
+    std::vector<float> xs(n * kMlpFeatures), ts(n * kMlpOutputs);
+    uint64_t s = seed;
+    float loss = 0;
+    for (int e = 0; e < epochs; e++) {
+        for (size_t i = 0; i < n; i++) perm[i] = i;
+        for (size_t i = n - 1; i > 0; i--) std::swap(perm[i], perm[splitmix(s) % (i + 1)]);
+        for (size_t i = 0; i < n; i++) {
+            memcpy(&xs[i * kMlpFeatures], X + perm[i] * kMlpFeatures, 4 * kMlpFeatures);
+            memcpy(&ts[i * kMlpOutputs], T + perm[i] * kMlpOutputs, 4 * kMlpOutputs);
+        }
+        HIPC(hipMemcpyAsync(dX.get(), xs.data(), 4 * xs.size(), hipMemcpyHostToDevice, c_->stream));
+        HIPC(hipMemcpyAsync(dT.get(), ts.data(), 4 * ts.size(), hipMemcpyHostToDevice, c_->stream));
+        for (size_t b0 = 0; b0 < n; b0 += (size_t)batch) {
+            const int nb = (int)std::min<size_t>((size_t)batch, n - b0);
+            const float* bx = dX.get<float>() + b0 * kMlpFeatures;
+            const float* bt = dT.get<float>() + b0 * kMlpOutputs;
+            float* by = dY.get<float>() + b0 * kMlpOutputs;
+            launch_mlp_forward(c_, bx, nb, p, by, h1.get<uint16_t>(), h2.get<uint16_t>());
+            launch_mlp_backward(c_, bx, h1.get<uint16_t>(), h2.get<uint16_t>(), by, bt, nb, p, g);
+            launch_adam(c_, params_.get<float>(), grads_.get<float>(), m_.get<float>(), v_.get<float>(),
+                        (int)kMlpParamCount, lr, ++step_);
+        }
+        if (e == epochs - 1) {
+            // loss of the final weights over the whole set
+            launch_mlp_forward(c_, dX.get<float>(), (int)n, p, dY.get<float>(), nullptr, nullptr);
+            std::vector<float> y(n * kMlpOutputs);
+            d2h(c_, y.data(), dY.get(), 4 * y.size());
+            double l = 0;
+            for (size_t i = 0; i < y.size(); i++) l += (double)(y[i] - ts[i]) * (y[i] - ts[i]);
+            loss = (float)(l / (double)y.size());
+        } else {
+            HIPC(hipStreamSynchronize(c_->stream));  // xs / ts are rewritten next epoch
+        }
+    }
+    return loss;
+}
+
+void PlannerNet::predict_dev(const float* dX, int n, float* dY) {
+    launch_mlp_forward(c_, dX, n, mlp_params_at(params_.get<float>()), dY, nullptr, nullptr);
+}
+
+void PlannerNet::predict(const float* X, size_t n, float* Y) {
+    if (n == 0) return;
+    DevBuf dX(c_, 4 * n * kMlpFeatures), dY(c_, 4 * n * kMlpOutputs);
+    h2d(c_, dX.get(), X, 4 * n * kMlpFeatures);
+    predict_dev(dX.get<float>(), (int)n, dY.get<float>());
+    d2h(c_, Y, dY.get(), 4 * n * kMlpOutputs);
+}
+
+}  // namespace ddj

@@ -1,0 +1,51 @@
+// Learned planner: a 32-128-128-48 network over one pass of block
+// statistics predicts log2(compressed bytes / input bytes) for each of 48
+// candidate trees; the planner picks the smallest prediction and falls back
+// to the exhaustive search when the margin to the runner-up is small.
+// Training labels are exact serialized sizes computed by the library's own
+// encoders (the same sizes the exhaustive planner compares).
+#pragma once
+
+#include <array>
+#include <vector>
+
+#include "kernels.h"
+#include "tree.h"
+
+namespace ddj {
+
+constexpr int kCandidates = kMlpOutputs;
+
+// The 48 candidate trees for integer columns (pre-order paths): the
+// reference's 30 generated paths (PathGenerator::GeneratePaths) first, then
+// 18 shallower or alternative cascades.
+const std::vector<std::vector<int>>& candidate_paths();
+
+// 32 features of one block from its statistics pass.
+std::array<float, kMlpFeatures> block_features(Ctx* c, int dtype, DView data);
+
+// Exact blob size of every candidate tree on `data` (shared prefixes are
+// encoded once), as log2(size / input bytes).
+std::array<float, kCandidates> candidate_labels(Ctx* c, int dtype, DView data);
+
+class PlannerNet {
+public:
+    PlannerNet(Ctx* c, uint64_t seed);
+    void get(float* host) const;
+    void set(const float* host);
+    // Mean squared error and its gradient (host copies) on n rows.
+    float gradient(const float* X, const float* T, size_t n, float* grad_host);
+    // Adam over `epochs` passes of shuffled minibatches; returns the last
+    // epoch's mean loss.
+    float train(const float* X, const float* T, size_t n, int epochs, int batch, float lr, uint64_t seed);
+    void predict(const float* X, size_t n, float* Y);
+    // Device-resident variants (rows already on the device).
+    void predict_dev(const float* dX, int n, float* dY);
+
+private:
+    Ctx* c_;
+    DevBuf params_, grads_, m_, v_;
+    int step_ = 0;
+};
+
+}  // namespace ddj

@@ -1,0 +1,140 @@
+"""Trains the learned planner on the GPU and evaluates it.
+
+1. Labels: `--blocks` random int32 blocks of `--rows` rows (planner_data
+   families with random parameters); for each, the 32 features (one
+   statistics pass) and the exact blob size of all 48 candidate trees.
+2. Training: the network's own HIP forward/backward/Adam kernels, MSE on
+   log2(blob bytes / input bytes).
+3. Evaluation on a stream whose family changes every `--regime` blocks:
+   compressed bytes with the learned choice against the best of the 48
+   candidates (every block) and against the exhaustive planner
+   (CompressionOptimizer phase 1, every `--exhaustive-every`-th block), and
+   the planning rate of each.
+
+    python -m gputimeseriescompressionoptimizer_amd.train_planner --out-dir gpurun_out/planner
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import gputimeseriescompressionoptimizer_amd as g  # noqa: E402
+from gputimeseriescompressionoptimizer_amd import planner_data as PD  # noqa: E402
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "planner_weights.npy")
+
+
+def label_blocks(ctx, blocks, log_every=500):
+    net = g.LearnedPlanner(ctx)
+    X, T, fams = [], [], []
+    t0 = time.perf_counter()
+    for i, (fam, x) in enumerate(blocks):
+        buf = ctx.upload(x)
+        X.append(net.Features(buf, g.D_INT))
+        T.append(net.Labels(buf, g.D_INT))
+        fams.append(fam)
+        if log_every and (i + 1) % log_every == 0:
+            print(f"[label] {i + 1} blocks, {time.perf_counter() - t0:.1f} s", flush=True)
+    return np.array(X, np.float32), np.array(T, np.float32), fams
+
+
+def evaluate(ctx, net, stream, exhaustive_every: int, min_margin: float = 0.0):
+    """Per block: learned choice (timed), best-of-48 size, and every k-th
+    block the exhaustive planner's choice (timed) and its size."""
+    paths = g.planner_candidates()
+    learned_bytes = best_bytes = raw = 0.0
+    ex_learned = ex_bytes = 0.0
+    t_learned = t_ex = 0.0
+    n_ex = n = fallbacks = 0
+    lab = g.LearnedPlanner(ctx)
+    for i, (_, fam, x) in enumerate(stream):
+        buf = ctx.upload(x)
+        ctx.sync()
+        t0 = time.perf_counter()
+        path, fb, _ = net.Plan(buf, g.D_INT, min_margin=min_margin)
+        t_learned += time.perf_counter() - t0
+        fallbacks += int(fb)
+        labels = lab.Labels(buf, g.D_INT)
+        sizes = np.exp2(labels.astype(np.float64)) * x.nbytes
+        idx = paths.index(path) if path in paths else -1
+        chosen = sizes[idx] if idx >= 0 else g.CompressionTree(ctx, path, g.D_INT).Compress(buf).nbytes
+        learned_bytes += chosen
+        best_bytes += sizes.min()
+        raw += x.nbytes
+        n += 1
+        if exhaustive_every and i % exhaustive_every == 0:
+            opt = g.CompressionOptimizer(ctx)
+            ctx.sync()
+            t0 = time.perf_counter()
+            tree, _, _ = opt.PlanExhaustive(buf, g.D_INT)
+            t_ex += time.perf_counter() - t0
+            ex_bytes += tree.Compress(buf).nbytes
+            ex_learned += chosen
+            n_ex += 1
+    return {
+        "blocks": n,
+        "learned_bytes": learned_bytes,
+        "best_of_48_bytes": best_bytes,
+        "input_bytes": raw,
+        "learned_vs_best_of_48": learned_bytes / best_bytes,
+        "exhaustive_blocks": n_ex,
+        "learned_vs_exhaustive": ex_learned / ex_bytes if ex_bytes else None,
+        "learned_blocks_per_s": n / t_learned if t_learned else None,
+        "exhaustive_blocks_per_s": n_ex / t_ex if t_ex else None,
+        "planning_speedup": (n / t_learned) / (n_ex / t_ex) if t_ex and t_learned else None,
+        "fallbacks": fallbacks,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--blocks", type=int, default=6000)
+    ap.add_argument("--rows", type=int, default=65536)
+    ap.add_argument("--epochs", type=int, default=80)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--eval-blocks", type=int, default=1000)
+    ap.add_argument("--regime", type=int, default=500)
+    ap.add_argument("--exhaustive-every", type=int, default=20)
+    ap.add_argument("--out-dir", default=".")
+    ap.add_argument("--seed", type=int, default=7)
+    args = ap.parse_args()
+    os.makedirs(args.out_dir, exist_ok=True)
+    ctx = g.Context(0)
+    X, T, fams = label_blocks(ctx, PD.random_blocks(args.blocks, args.rows, args.seed))
+    np.savez_compressed(os.path.join(args.out_dir, "planner_dataset.npz"), X=X, T=T, fams=np.array(fams))
+    n_val = max(1, len(X) // 10)
+    Xtr, Ttr, Xva, Tva = X[n_val:], T[n_val:], X[:n_val], T[:n_val]
+    net = g.LearnedPlanner(ctx, seed=args.seed)
+    t0 = time.perf_counter()
+    history = []
+    for e in range(0, args.epochs, 10):
+        loss = net.Train(Xtr, Ttr, epochs=min(10, args.epochs - e), batch=args.batch, lr=args.lr, seed=args.seed + e)
+        Yva = net.Predict(Xva)
+        vloss = float(np.mean((Yva - Tva) ** 2))
+        hit = float(np.mean(np.argmin(Yva, 1) == np.argmin(Tva, 1)))
+        history.append({"epoch": e + 10, "train_mse": loss, "val_mse": vloss, "val_top1": hit})
+        print(f"[train] epoch {e + 10} train {loss:.4f} val {vloss:.4f} top1 {hit:.3f}", flush=True)
+    train_s = time.perf_counter() - t0
+    weights = os.path.join(args.out_dir, "planner_weights.npy")
+    net.Save(weights)
+    ev = evaluate(ctx, net, PD.regime_stream(args.eval_blocks, args.rows, args.regime, args.seed + 1000),
+                  args.exhaustive_every)
+    res = {"train_blocks": len(Xtr), "rows_per_block": args.rows, "epochs": args.epochs, "train_s": train_s,
+           "history": history, "eval": ev}
+    with open(os.path.join(args.out_dir, "planner_eval.json"), "w") as f:
+        json.dump(res, f, indent=1)
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -161,6 +161,37 @@ void ddj_optimizer_destroy(ddj_optimizer* opt);
 ddj_status ddj_optimizer_compress(ddj_optimizer* opt, int dtype, const void* d_in, size_t in_bytes, ddj_buf* blob);
 ddj_status ddj_optimizer_tree(ddj_optimizer* opt, int* path_out, size_t path_cap, size_t* path_len);
 
+/* -------------------------------------------------------- learned planner */
+/* No counterpart in the reference.  A 32-128-128-48 network (ReLU, BF16
+ * matrix-core products, FP32 accumulation, kernels in k_mlp.hip) maps one
+ * statistics pass over a block to the predicted log2(compressed bytes /
+ * input bytes) of 48 candidate trees for integer columns: the reference's
+ * 30 generated paths (optimizer/path_generator.cpp:20-68) and 18 more.  Its
+ * labels are exact serialized sizes from the library's encoders. */
+typedef struct ddj_planner ddj_planner;
+/* The 48 candidate trees as pre-order paths, concatenated; offsets[49]. */
+ddj_status ddj_planner_candidates(int* paths, size_t cap, size_t* offsets);
+/* 32 features of one block (one statistics pass). */
+ddj_status ddj_planner_features(ddj_ctx* ctx, int dtype, const void* d_in, size_t in_bytes, float* out /* [32] */);
+/* log2(exact blob bytes / input bytes) of every candidate tree on a block. */
+ddj_status ddj_planner_labels(ddj_ctx* ctx, int dtype, const void* d_in, size_t in_bytes, float* out /* [48] */);
+ddj_status ddj_planner_create(ddj_ctx* ctx, uint64_t seed, ddj_planner** out);
+void ddj_planner_destroy(ddj_planner* planner);
+/* Parameters (FP32, W1[32][128] b1[128] W2[128][128] b2[128] W3[128][64] b3[64]):
+ * copied out to get_out and/or in from set_in (either may be NULL). */
+ddj_status ddj_planner_params(ddj_planner* planner, float* get_out, const float* set_in);
+/* Mean squared error over n rows and its gradient (no update). */
+ddj_status ddj_planner_gradient(ddj_planner* planner, const float* X, const float* T, size_t n, float* grad_out,
+                                float* loss);
+/* Adam over `epochs` shuffled passes of minibatches; *loss = final MSE. */
+ddj_status ddj_planner_train(ddj_planner* planner, const float* X, const float* T, size_t n, int epochs, int batch,
+                             float lr, uint64_t seed, float* loss);
+ddj_status ddj_planner_predict(ddj_planner* planner, const float* X, size_t n, float* Y /* [n][48] */);
+/* Chooses a tree for a block: the smallest predicted size, or the exhaustive
+ * planner's choice when the runner-up is within min_margin (log2 units). */
+ddj_status ddj_planner_plan(ddj_planner* planner, int dtype, const void* d_in, size_t in_bytes, float min_margin,
+                            int* path_out, size_t path_cap, size_t* path_len, int* used_fallback, float* margin);
+
 /* ------------------------------------------------------------- column API */
 /* ParallelTSCompressor (parallel_ts_compressor.cpp:17-102): rows of several
  * host columns are cut into batches of `batch_rows`; for every batch, each

@@ -181,3 +181,17 @@ def test_afl_uniform_1m_matches_reference_digest(ctx):
     assert hashlib.sha256(stream).digest() == z["cfg0_stream_sha256"].tobytes()
     back = g.Encoding(ctx, g.AFL).Decode(outs, g.D_INT).to_numpy(np.int32)
     assert np.array_equal(back, x)
+
+
+def test_rle_negative_run_length_is_corrupt(ctx):
+    """A run of negative length cannot come from the encoder: the decoder
+    reports the input as corrupt instead of expanding it."""
+    x = np.repeat(np.arange(50, dtype=np.int32), 7)
+    outs = g.Encoding(ctx, g.RLE).Encode(ctx.upload(x), g.D_INT)
+    lengths = outs[1].to_numpy(np.int32).copy()
+    lengths[3] = -5
+    bad = [outs[0], ctx.upload(lengths), outs[2]]
+    with pytest.raises(g.DdjError):
+        g.Encoding(ctx, g.RLE).Decode(bad, g.D_INT)
+    # the context stays usable afterwards
+    assert g.Encoding(ctx, g.RLE).Decode(outs, g.D_INT).to_bytes() == x.tobytes()

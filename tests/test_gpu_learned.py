@@ -1,0 +1,132 @@
+"""The learned planner's network on the GPU (matrix-core BF16 products, FP32
+accumulation) against a NumPy FP32 reference that rounds the same operands
+to BF16, and its labels against the oracle's exact tree sizes.
+
+Tolerances: the kernels and the reference round the same values to BF16 and
+differ only in FP32 summation order, plus the occasional BF16 rounding of a
+hidden activation that lands on the other side of a rounding boundary.  The
+bounds below (2% of the largest magnitude of each tensor) are far below
+what a wrong layout, a dropped term or a transposed operand produces (errors
+of the order of the values themselves)."""
+import numpy as np
+import pytest
+
+import gputimeseriescompressionoptimizer_amd as g
+from gputimeseriescompressionoptimizer_amd import datasets as D
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16(a):
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return r.view(np.float32)
+
+
+def mm(a, b):
+    return (a.astype(np.float64) @ b.astype(np.float64)).astype(np.float32)
+
+
+def reference(params, X, T):
+    W1, b1, W2, b2, W3, b3 = g.split_params(params)
+    W3, b3 = W3[:, :48], b3[:48]
+    Xb = bf16(X)
+    H1 = bf16(np.maximum(mm(Xb, bf16(W1)) + b1, 0))
+    H2 = bf16(np.maximum(mm(H1, bf16(W2)) + b2, 0))
+    Y = mm(H2, bf16(W3)) + b3
+    B = X.shape[0]
+    dY = (2.0 / (48 * B)) * (Y - T)
+    dYb = bf16(dY)
+    gW3 = mm(H2.T, dYb)
+    gb3 = dY.sum(0)
+    G2 = np.where(H2 != 0, mm(dYb, bf16(W3).T), 0).astype(np.float32)
+    gb2 = G2.sum(0)
+    G2b = bf16(G2)
+    gW2 = mm(H1.T, G2b)
+    G1 = np.where(H1 != 0, mm(G2b, bf16(W2).T), 0).astype(np.float32)
+    gb1 = G1.sum(0)
+    gW1 = mm(Xb.T, bf16(G1))
+    loss = float(np.mean((Y - T).astype(np.float64) ** 2))
+    return Y, [gW1, gb1, gW2, gb2, gW3, gb3], loss
+
+
+def random_problem(n, seed):
+    r = np.random.default_rng(seed)
+    X = r.normal(0, 1, (n, 32)).astype(np.float32)
+    T = r.normal(-1, 2, (n, 48)).astype(np.float32)
+    return X, T
+
+
+def close(a, b, frac=0.02):
+    scale = max(float(np.max(np.abs(b))), 1e-6)
+    return float(np.max(np.abs(a - b))) <= frac * scale
+
+
+@pytest.mark.parametrize("n", [1, 31, 32, 100, 257])
+def test_forward_matches_reference(ctx, n):
+    net = g.LearnedPlanner(ctx, seed=3)
+    X, T = random_problem(n, seed=n)
+    Y = net.Predict(X)
+    Yr, _, _ = reference(net.Params(), X, T)
+    assert Y.shape == (n, 48)
+    assert close(Y, Yr), float(np.max(np.abs(Y - Yr)))
+
+
+@pytest.mark.parametrize("n", [32, 77, 300])
+def test_gradient_matches_reference(ctx, n):
+    net = g.LearnedPlanner(ctx, seed=5)
+    X, T = random_problem(n, seed=100 + n)
+    grad, loss = net.Gradient(X, T)
+    _, gref, lref = reference(net.Params(), X, T)
+    assert abs(loss - lref) <= 0.02 * lref
+    names = ["W1", "b1", "W2", "b2", "W3", "b3"]
+    parts = g.split_params(grad)
+    parts[4] = parts[4][:, :48]
+    parts[5] = parts[5][:48]
+    for name, a, b in zip(names, parts, gref):
+        assert close(a, b), (name, float(np.max(np.abs(a - b))), float(np.max(np.abs(b))))
+    W3 = g.split_params(grad)[4]
+    assert not np.any(W3[:, 48:]), "padding outputs must get no gradient"
+
+
+def test_training_fits_a_smooth_target(ctx):
+    """Adam on the GPU drives the loss of a learnable map well below its start."""
+    r = np.random.default_rng(9)
+    X = r.normal(0, 1, (4096, 32)).astype(np.float32)
+    A = r.normal(0, 0.3, (32, 48)).astype(np.float32)
+    T = np.tanh(X @ A).astype(np.float32)
+    net = g.LearnedPlanner(ctx, seed=1)
+    _, loss0 = net.Gradient(X, T)
+    loss = net.Train(X, T, epochs=30, batch=256, lr=2e-3, seed=2)
+    assert loss < 0.2 * loss0, (loss0, loss)
+
+
+def test_labels_are_exact_tree_sizes(ctx):
+    """Every label is log2(blob bytes / input bytes) of its candidate tree,
+    and the blob sizes equal the oracle's."""
+    x = D.timestamps(20_000, seed=12)
+    net = g.LearnedPlanner(ctx, seed=0)
+    labels = net.Labels(ctx.upload(x), g.D_INT)
+    for i, path in enumerate(g.planner_candidates()):
+        t = O.tree_from_path(path, O.D_INT)
+        t.fix()
+        size = len(O.compress(t, x))
+        assert labels[i] == np.float32(np.log2(size / x.nbytes)), (i, path)
+
+
+def test_plan_follows_the_smallest_prediction(ctx):
+    net = g.LearnedPlanner(ctx, seed=4)
+    x = D.step_function(30_000, seed=2)
+    buf = ctx.upload(x)
+    y = net.Predict(net.Features(buf, g.D_INT)[None, :])[0]
+    path, fallback, margin = net.Plan(buf, g.D_INT, min_margin=0.0)
+    assert not fallback
+    t = O.tree_from_path(g.planner_candidates()[int(np.argmin(y))], O.D_INT)
+    t.fix()
+    assert path == t.preorder()
+    srt = np.sort(y)
+    assert np.isclose(margin, srt[1] - srt[0])
+    # a margin nothing reaches sends the block to the exhaustive planner
+    path2, fallback2, _ = net.Plan(buf, g.D_INT, min_margin=1e9)
+    assert fallback2 and path2 == O.plan_exhaustive(x, O.D_INT)[0].preorder()
