@@ -58,6 +58,68 @@ def cpu_baseline(cols, sample_rows: int, budget_s: float = 15.0):
             "sample": f"first {sample_rows} rows of each of the 4 columns, {reps} passes, {el:.1f} s"}
 
 
+def float_section(ctx, rows: int, steps: int):
+    """The float price columns (64M rows): GFC against floatToInt cascades,
+    compress and decompress GB/s with inputs resident in HBM, ratios, and a
+    bit-exact round-trip check of every tree."""
+    import numpy as np
+    import gputimeseriescompressionoptimizer_amd as g
+    from gputimeseriescompressionoptimizer_amd import datasets as D
+
+    cols = [
+        ("float64 walk", D.price_walk_f64(rows, seed=21), g.D_DOUBLE,
+         {"gfc": [g.GFC, g.NONE, g.NONE], "floatToInt+delta+afl": [g.FLOAT_TO_INT, g.DELTA, g.AFL, g.NONE]}),
+        ("float32 walk", D.price_walk_f32(rows, seed=22), g.D_FLOAT,
+         {"gfc": [g.GFC, g.NONE, g.NONE],
+          "floatToInt+scale+delta+afl": [g.FLOAT_TO_INT, g.SCALE, g.DELTA, g.AFL, g.NONE]}),
+    ]
+    out = {}
+    for name, x, dt, trees in cols:
+        buf = ctx.upload(x)
+        for tname, path in trees.items():
+            tree = g.CompressionTree(ctx, path, dt)
+            blob = tree.Compress(buf)
+            back = tree.Decompress(blob)
+            exact = back.to_bytes() == x.tobytes()
+            del back
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                blob = tree.Compress(buf)
+            ctx.sync()
+            tc = (time.perf_counter() - t0) / steps
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                back = tree.Decompress(blob)
+                del back
+            ctx.sync()
+            td = (time.perf_counter() - t0) / steps
+            out[f"{name} {tname}"] = {"compress_gbps": round(x.nbytes / tc / 1e9, 2),
+                                      "decompress_gbps": round(x.nbytes / td / 1e9, 2),
+                                      "ratio": round(x.nbytes / blob.nbytes, 3), "bit_exact": exact}
+            del blob
+        del buf
+    return out
+
+
+def learned_planner_section(ctx, blocks: int, rows: int, regime: int, exhaustive_every: int):
+    """Learned vs exhaustive planning on a stream of int32 blocks whose
+    generator family changes every `regime` blocks (weights trained by
+    gputimeseriescompressionoptimizer_amd.train_planner)."""
+    import gputimeseriescompressionoptimizer_amd as g
+    from gputimeseriescompressionoptimizer_amd import planner_data as PD
+    from gputimeseriescompressionoptimizer_amd.train_planner import WEIGHTS, evaluate
+
+    if not os.path.exists(WEIGHTS):
+        return None
+    net = g.LearnedPlanner(ctx)
+    net.Load(WEIGHTS)
+    ev = evaluate(ctx, net, PD.regime_stream(blocks, rows, regime, seed=4242), exhaustive_every)
+    ev["stream"] = f"{blocks} blocks x {rows} int32 rows, family changes every {regime} blocks"
+    ev["ratio_gap_vs_exhaustive"] = (ev["learned_vs_exhaustive"] - 1.0) if ev["learned_vs_exhaustive"] else None
+    return ev
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +128,11 @@ def main():
     ap.add_argument("--rows", type=int, default=1 << 26)
     ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--planner-blocks", type=int, default=200)
+    ap.add_argument("--planner-rows", type=int, default=65536)
+    ap.add_argument("--no-planner", action="store_true")
+    ap.add_argument("--float-rows", type=int, default=1 << 26)
+    ap.add_argument("--no-float", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (columns would shard "
@@ -150,6 +217,12 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
     }
+    if not args.no_float:
+        res["float_columns"] = float_section(ctx, args.float_rows, max(1, args.steps // 2))
+    if not args.no_planner:
+        res["learned_planner"] = learned_planner_section(ctx, args.planner_blocks, args.planner_rows,
+                                                         regime=max(1, args.planner_blocks // 4),
+                                                         exhaustive_every=10)
     ctx.close()
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(cols, min(args.cpu_sample, args.rows))

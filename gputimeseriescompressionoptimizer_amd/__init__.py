@@ -113,6 +113,7 @@ def load_library(path: str = LIB_PATH):
         "ddj_ts_reader_open": (i32, [vp, ctypes.c_char_p, u32, P(i32), P(vp)]),
         "ddj_ts_reader_next": (i32, [vp, P(vp), sz, P(sz)]),
         "ddj_ts_reader_close": (None, [vp]),
+        "ddj_batch_compress": (i32, [vp, vp, u32, u32, P(_Buf)]),
         "ddj_planner_candidates": (i32, [P(i32), sz, P(sz)]),
         "ddj_planner_features": (i32, [vp, i32, vp, sz, P(ctypes.c_float)]),
         "ddj_planner_labels": (i32, [vp, i32, vp, sz, P(ctypes.c_float)]),
@@ -607,3 +608,20 @@ class LearnedPlanner:
 
     def Load(self, path: str):
         self.SetParams(np.load(path, allow_pickle=False))
+
+
+class _Column(ctypes.Structure):
+    _fields_ = [("dtype", ctypes.c_int), ("d_in", ctypes.c_void_p), ("in_bytes", ctypes.c_size_t),
+                ("preorder", ctypes.POINTER(ctypes.c_int)), ("path_len", ctypes.c_size_t)]
+
+
+def batch_compress(ctx: Context, columns: Sequence[Tuple[DeviceBuffer, int, Sequence[int]]],
+                   workers: int = 4) -> List[DeviceBuffer]:
+    """Compresses (buffer, dtype, tree path) columns together (ddj_batch_compress)."""
+    keep = [(ctypes.c_int * len(p))(*p) for _, _, p in columns]
+    arr = (_Column * len(columns))()
+    for i, ((buf, dt, path), pa) in enumerate(zip(columns, keep)):
+        arr[i] = _Column(dt, buf.dev or None, buf.nbytes, pa, len(path))
+    outs = (_Buf * max(len(columns), 1))()
+    ctx.check(ctx._L.ddj_batch_compress(ctx._h, arr, len(columns), workers, outs), "batch compress")
+    return [ctx._wrap(outs[i]) for i in range(len(columns))]

@@ -1,6 +1,9 @@
 // The C ABI (include/ddj.h): status codes at the boundary, C++ inside.
+#include <atomic>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
+#include <thread>
 
 #include "codecs.h"
 #include "learned.h"
@@ -9,8 +12,54 @@
 
 using namespace ddj;
 
+// Sets up a stream, control block, pinned staging ring and allocator on
+// `device`; on failure everything made so far is released.
+static void fini_ctx(Ctx& c);
+static hipError_t init_ctx(Ctx& c, int device) {
+    c.device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+            c.cu_count = prop.multiProcessorCount;
+        e = hipMalloc(reinterpret_cast<void**>(&c.control), sizeof(Control));
+    }
+    if (e == hipSuccess) e = hipMemset(c.control, 0, sizeof(Control));
+    if (e == hipSuccess) {
+        c.staging_size = 4u << 20;
+        e = hipHostMalloc(reinterpret_cast<void**>(&c.staging), c.staging_size, hipHostMallocDefault);
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        fini_ctx(c);
+        return e;
+    }
+    c.alloc.reset(new Allocator(device));
+    return hipSuccess;
+}
+
+static void fini_ctx(Ctx& c) {
+    (void)hipSetDevice(c.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    c.alloc.reset();
+    if (c.control) (void)hipFree(c.control);
+    if (c.staging) (void)hipHostFree(c.staging);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c.control = nullptr;
+    c.staging = nullptr;
+    c.stream = nullptr;
+}
+
+// A Ctx that releases its device resources when destroyed.
+struct OwnedCtx {
+    Ctx c;
+    ~OwnedCtx() { fini_ctx(c); }
+};
+
 struct ddj_ctx {
     Ctx c;
+    std::vector<std::unique_ptr<OwnedCtx>> pool;  // worker contexts of ddj_batch_compress
 };
 
 struct ddj_optimizer {
@@ -80,42 +129,20 @@ ddj_status ddj_ctx_create(ddj_ctx** out, int device) {
     *out = nullptr;
     auto* ctx = new (std::nothrow) ddj_ctx;
     if (!ctx) return DDJ_ENOMEM;
-    ctx->c.device = device;
-    hipError_t e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->c.stream, hipStreamNonBlocking);
-    if (e == hipSuccess) {
-        hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-            ctx->c.cu_count = prop.multiProcessorCount;
-        e = hipMalloc(reinterpret_cast<void**>(&ctx->c.control), sizeof(Control));
-    }
-    if (e == hipSuccess) e = hipMemset(ctx->c.control, 0, sizeof(Control));
-    if (e == hipSuccess) {
-        ctx->c.staging_size = 4u << 20;
-        e = hipHostMalloc(reinterpret_cast<void**>(&ctx->c.staging), ctx->c.staging_size, hipHostMallocDefault);
-    }
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+    const hipError_t e = init_ctx(ctx->c, device);
     if (e != hipSuccess) {
         fprintf(stderr, "ddj_ctx_create: %s\n", hipGetErrorString(e));
-        if (ctx->c.stream) (void)hipStreamDestroy(ctx->c.stream);
-        if (ctx->c.control) (void)hipFree(ctx->c.control);
-        if (ctx->c.staging) (void)hipHostFree(ctx->c.staging);
         delete ctx;
         return e == hipErrorOutOfMemory ? DDJ_ENOMEM : DDJ_EHIP;
     }
-    ctx->c.alloc.reset(new Allocator(device));
     *out = ctx;
     return DDJ_OK;
 }
 
 void ddj_ctx_destroy(ddj_ctx* ctx) {
     if (!ctx) return;
-    (void)hipSetDevice(ctx->c.device);
-    (void)hipStreamSynchronize(ctx->c.stream);
-    ctx->c.alloc.reset();
-    (void)hipFree(ctx->c.control);
-    if (ctx->c.staging) (void)hipHostFree(ctx->c.staging);
-    (void)hipStreamDestroy(ctx->c.stream);
+    ctx->pool.clear();
+    fini_ctx(ctx->c);
     delete ctx;
 }
 
@@ -601,6 +628,73 @@ ddj_status ddj_planner_plan(ddj_planner* p, int dtype, const void* d_in, size_t 
             if (used_fallback) *used_fallback = 0;
         }
         write_path(tree, path_out, path_cap, path_len);
+    });
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ batched trees
+
+extern "C" {
+
+ddj_status ddj_batch_compress(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_cols, uint32_t workers,
+                              ddj_buf* blobs) {
+    return guarded(ctx, [&] {
+        if ((!cols || !blobs) && n_cols) fail(DDJ_EINVAL, "bad argument");
+        if (workers == 0) workers = 4;  // the hardware queues one process gets
+        if (workers > 16) workers = 16;
+        while (ctx->pool.size() < workers) {
+            std::unique_ptr<OwnedCtx> w(new OwnedCtx);
+            const hipError_t e = init_ctx(w->c, ctx->c.device);
+            if (e != hipSuccess) fail(e == hipErrorOutOfMemory ? DDJ_ENOMEM : DDJ_EHIP, hipGetErrorString(e));
+            ctx->pool.push_back(std::move(w));
+        }
+        for (uint32_t i = 0; i < n_cols; i++) blobs[i] = ddj_buf{nullptr, 0};
+        HIPC(hipStreamSynchronize(ctx->c.stream));  // inputs written on the caller's stream are in place
+        std::atomic<uint32_t> next{0};
+        std::mutex mu;  // the caller's allocator and the first error
+        ddj_status first_st = DDJ_OK;
+        std::string first_msg;
+        auto work = [&](Ctx* w) {
+            (void)hipSetDevice(w->device);
+            for (;;) {
+                const uint32_t i = next.fetch_add(1);
+                if (i >= n_cols) return;
+                try {
+                    NodeP t = tree_from_path(cols[i].preorder, cols[i].path_len, cols[i].dtype);
+                    fix_tree(t);
+                    Part blob = compress_tree(w, t, DView{static_cast<const uint8_t*>(cols[i].d_in), cols[i].in_bytes});
+                    void* dst = nullptr;
+                    {
+                        std::lock_guard<std::mutex> lk(mu);
+                        dst = ctx->c.alloc->alloc(blob.n);
+                    }
+                    d2d(w, dst, blob.p, blob.n);
+                    check_device_error(w);  // synchronises the worker's stream
+                    blobs[i] = ddj_buf{dst, blob.n};
+                } catch (const Error& e) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (first_st == DDJ_OK) {
+                        first_st = e.status;
+                        first_msg = "column " + std::to_string(i) + ": " + e.msg;
+                    }
+                } catch (const std::exception& e) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (first_st == DDJ_OK) {
+                        first_st = DDJ_EINVAL;
+                        first_msg = "column " + std::to_string(i) + ": " + e.what();
+                    }
+                }
+            }
+        };
+        std::vector<std::thread> th;
+        for (uint32_t k = 0; k < workers; k++) th.emplace_back(work, &ctx->pool[k]->c);
+        for (auto& t : th) t.join();
+        if (first_st != DDJ_OK) {
+            for (uint32_t i = 0; i < n_cols; i++)
+                if (blobs[i].dev) ctx->c.alloc->free(blobs[i].dev), blobs[i] = ddj_buf{nullptr, 0};
+            fail(first_st, first_msg);
+        }
     });
 }
 

@@ -47,7 +47,7 @@ def label_blocks(ctx, blocks, log_every=500):
     return np.array(X, np.float32), np.array(T, np.float32), fams
 
 
-def evaluate(ctx, net, stream, exhaustive_every: int, min_margin: float = 0.0):
+def evaluate(ctx, net, stream, exhaustive_every: int, min_margin: float = 0.0, log_every: int = 0):
     """Per block: learned choice (timed), best-of-48 size, and every k-th
     block the exhaustive planner's choice (timed) and its size."""
     paths = g.planner_candidates()
@@ -80,6 +80,8 @@ def evaluate(ctx, net, stream, exhaustive_every: int, min_margin: float = 0.0):
             ex_bytes += tree.Compress(buf).nbytes
             ex_learned += chosen
             n_ex += 1
+        if log_every and (i + 1) % log_every == 0:
+            print(f"[eval] {i + 1} blocks", flush=True)
     return {
         "blocks": n,
         "learned_bytes": learned_bytes,
@@ -122,13 +124,17 @@ def main():
         Yva = net.Predict(Xva)
         vloss = float(np.mean((Yva - Tva) ** 2))
         hit = float(np.mean(np.argmin(Yva, 1) == np.argmin(Tva, 1)))
-        history.append({"epoch": e + 10, "train_mse": loss, "val_mse": vloss, "val_top1": hit})
-        print(f"[train] epoch {e + 10} train {loss:.4f} val {vloss:.4f} top1 {hit:.3f}", flush=True)
+        pick = np.exp2(Tva[np.arange(len(Tva)), np.argmin(Yva, 1)].astype(np.float64))
+        size_ratio = float(pick.sum() / np.exp2(Tva.min(1).astype(np.float64)).sum())
+        history.append({"epoch": e + 10, "train_mse": loss, "val_mse": vloss, "val_top1": hit,
+                        "val_bytes_vs_best_of_48": size_ratio})
+        print(f"[train] epoch {e + 10} train {loss:.4f} val {vloss:.4f} top1 {hit:.3f} "
+              f"bytes/best {size_ratio:.4f}", flush=True)
     train_s = time.perf_counter() - t0
     weights = os.path.join(args.out_dir, "planner_weights.npy")
     net.Save(weights)
     ev = evaluate(ctx, net, PD.regime_stream(args.eval_blocks, args.rows, args.regime, args.seed + 1000),
-                  args.exhaustive_every)
+                  args.exhaustive_every, log_every=50)
     res = {"train_blocks": len(Xtr), "rows_per_block": args.rows, "epochs": args.epochs, "train_s": train_s,
            "history": history, "eval": ev}
     with open(os.path.join(args.out_dir, "planner_eval.json"), "w") as f:

@@ -161,6 +161,23 @@ void ddj_optimizer_destroy(ddj_optimizer* opt);
 ddj_status ddj_optimizer_compress(ddj_optimizer* opt, int dtype, const void* d_in, size_t in_bytes, ddj_buf* blob);
 ddj_status ddj_optimizer_tree(ddj_optimizer* opt, int* path_out, size_t path_cap, size_t* path_len);
 
+/* --------------------------------------------------------- batched trees */
+/* Many columns, each with its own tree, compressed together: worker threads
+ * (one per worker context, each with its own HIP stream) take columns from a
+ * shared queue, so one column's host round trips (size read-backs) overlap
+ * other columns' kernels.  Blobs are owned by `ctx` (ddj_buf_free).  No
+ * counterpart in the reference, whose column API compresses one column at
+ * a time on one worker (parallel_ts_compressor.cpp:31-36). */
+typedef struct {
+    int dtype;
+    const void* d_in;      /* device data */
+    size_t in_bytes;
+    const int* preorder;   /* the column's tree, as for ddj_tree_compress */
+    size_t path_len;
+} ddj_column;
+ddj_status ddj_batch_compress(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_cols, uint32_t workers /* 0: 4 */,
+                              ddj_buf* blobs);
+
 /* -------------------------------------------------------- learned planner */
 /* No counterpart in the reference.  A 32-128-128-48 network (ReLU, BF16
  * matrix-core products, FP32 accumulation, kernels in k_mlp.hip) maps one
