@@ -102,6 +102,71 @@ def float_section(ctx, rows: int, steps: int):
     return out
 
 
+def batched_section(ctx, n_cols: int, min_rows: int, max_rows: int, workers: int):
+    """Many columns of log-uniform lengths, mixed int32 timestamps, float64
+    walk prices, int32 Zipf volumes and int32 50-code categoricals, each type
+    with the tree the exhaustive planner picks on a 10,000-row sample:
+    compressed one column after another (ddj_tree_compress per column)
+    against together (ddj_batch_compress)."""
+    import numpy as np
+    import gputimeseriescompressionoptimizer_amd as g
+    from gputimeseriescompressionoptimizer_amd import datasets as D
+
+    r = np.random.default_rng(33)
+    lengths = np.floor(np.exp(r.uniform(np.log(min_rows), np.log(max_rows), n_cols))).astype(np.int64)
+    kinds = ["timestamps", "walk_f64", "zipf_volumes", "categorical50"]
+
+    def make(kind, n, seed):
+        rr = np.random.default_rng(seed)
+        if kind == "timestamps":
+            return D.timestamps(n, seed=seed), g.D_INT
+        if kind == "walk_f64":
+            return D.price_walk_f64(n, seed=seed), g.D_DOUBLE
+        if kind == "zipf_volumes":
+            return ((rr.zipf(1.5, n) % 100000) * 100).astype(np.int32), g.D_INT
+        return rr.integers(0, 50, n).astype(np.int32), g.D_INT
+
+    trees = {}
+    for k in kinds:
+        x, dt = make(k, 10_000, 1)
+        tree, _, _ = g.CompressionOptimizer(ctx).PlanExhaustive(ctx.upload(x), dt)
+        trees[k] = tree.path
+    cols, raw = [], 0
+    for i, n in enumerate(lengths):
+        k = kinds[i % 4]
+        x, dt = make(k, int(n), 100 + i)
+        cols.append((ctx.upload(x), dt, trees[k]))
+        raw += x.nbytes
+        del x
+    ctx.sync()
+
+    def per_column():
+        return [g.CompressionTree(ctx, path, dt).Compress(buf) for buf, dt, path in cols]
+
+    def together():
+        return g.batch_compress(ctx, cols, workers=workers)
+
+    a = per_column()
+    b = together()
+    same = all(x.to_bytes() == y.to_bytes() for x, y in zip(a, b))
+    out_bytes = sum(x.nbytes for x in a)
+    del a, b
+    times = {}
+    for name, fn in (("per_column", per_column), ("batched", together)):
+        ctx.sync()
+        t0 = time.perf_counter()
+        res = fn()
+        ctx.sync()
+        times[name] = time.perf_counter() - t0
+        del res
+    return {"columns": n_cols, "rows": int(lengths.sum()), "input_bytes": raw,
+            "rows_min_max": [int(lengths.min()), int(lengths.max())], "trees": trees,
+            "per_column_gbps": round(raw / times["per_column"] / 1e9, 2),
+            "batched_gbps": round(raw / times["batched"] / 1e9, 2),
+            "speedup": round(times["per_column"] / times["batched"], 2), "workers": workers,
+            "ratio": round(raw / out_bytes, 3), "blobs_identical": same}
+
+
 def learned_planner_section(ctx, blocks: int, rows: int, regime: int, exhaustive_every: int):
     """Learned vs exhaustive planning on a stream of int32 blocks whose
     generator family changes every `regime` blocks (weights trained by
@@ -132,6 +197,10 @@ def main():
     ap.add_argument("--planner-rows", type=int, default=65536)
     ap.add_argument("--no-planner", action="store_true")
     ap.add_argument("--float-rows", type=int, default=1 << 26)
+    ap.add_argument("--batch-columns", type=int, default=256)
+    ap.add_argument("--batch-max-rows", type=int, default=16 << 20)
+    ap.add_argument("--batch-workers", type=int, default=4)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-float", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -219,6 +288,9 @@ def main():
     }
     if not args.no_float:
         res["float_columns"] = float_section(ctx, args.float_rows, max(1, args.steps // 2))
+    if not args.no_batch:
+        res["batched_columns"] = batched_section(ctx, args.batch_columns, 4096, args.batch_max_rows,
+                                                 args.batch_workers)
     if not args.no_planner:
         res["learned_planner"] = learned_planner_section(ctx, args.planner_blocks, args.planner_rows,
                                                          regime=max(1, args.planner_blocks // 4),

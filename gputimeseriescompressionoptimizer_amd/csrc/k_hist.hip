@@ -125,7 +125,9 @@ __device__ void block_small4(Small4& s, uint64_t* sm_keys /* kBlock/kWave * kMax
         Small4 b;
         for (int w = 0; w < kBlock / kWave; w++)
             for (int r = 0; r < sm_cnt[w]; r++) b.insert(sm_keys[w * kMaxK + r]);
-        for (int r = 0; r < b.c; r++) out_keys[r] = b.l[r];
+#pragma unroll
+        for (int r = 0; r < kMaxK; r++)
+            if (r < b.c) out_keys[r] = b.l[r];
         *out_n = (uint32_t)b.c;
     }
 }
@@ -148,21 +150,55 @@ __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, 
     __syncthreads();
     bool negzero = false;
     unsigned long long special = 0;
+    const int lane = threadIdx.x & (kWave - 1);
     const size_t stride = (size_t)gridDim.x * kBlock;
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const T v = x[i];
-        if constexpr (std::is_floating_point<T>::value) {
-            if (v != v) {
-                nan4.insert(order_key(v));
-                nans++;
-                continue;
+    // Whole waves step through the rows together so that equal keys of a
+    // wave can be merged: a few rounds each take the first pending lane's
+    // key and add the number of lanes holding it with one LDS atomic (a
+    // skewed column's frequent keys would otherwise serialise the wave's
+    // atomics on one address); lanes still pending afterwards add 1 each.
+    const size_t n_rounds = (n + stride - 1) / stride;
+    for (size_t it = 0; it < n_rounds; it++) {
+        const size_t i = it * stride + (size_t)blockIdx.x * kBlock + threadIdx.x;
+        bool pending = i < n;
+        uint64_t key = 0;
+        if (pending) {
+            const T v = x[i];
+            bool skip = false;
+            if constexpr (std::is_floating_point<T>::value) {
+                if (v != v) {
+                    nan4.insert(order_key(v));
+                    nans++;
+                    skip = true;
+                }
             }
+            if (!skip) {
+                key = hist_key(v, negzero);
+                if (key == kEmpty) {
+                    special++;
+                    skip = true;
+                }
+            }
+            pending = !skip;
         }
-        const uint64_t key = hist_key(v, negzero);
-        if (key == kEmpty) {
-            special++;
-            continue;
+        uint32_t add = 1;
+        for (int r = 0; r < 4; r++) {
+            const uint64_t pm = __ballot(pending);
+            if (pm == 0) break;
+            const int leader = __builtin_ctzll(pm);
+            const uint64_t kl = __shfl(key, leader, kWave);
+            const bool same = pending && key == kl;
+            const uint64_t sm = __ballot(same);
+            if (same) {
+                if (lane == leader) {
+                    add = (uint32_t)__popcll(sm);
+                } else {
+                    pending = false;  // counted by the leader
+                }
+            }
+            if (lane == leader) break;  // the leader adds below
         }
+        if (!pending) continue;
         const uint32_t h = (uint32_t)hmix(key) & (kLdsSlots - 1);
         bool done = false;
         for (int p = 0; p < 16 && !done; p++) {
@@ -170,11 +206,11 @@ __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, 
             unsigned long long k = skey[s];
             if (k == kEmpty) k = atomicCAS(&skey[s], kEmpty, (unsigned long long)key);
             if (k == kEmpty || k == key) {
-                atomicAdd(&scnt[s], 1u);
+                atomicAdd(&scnt[s], add);
                 done = true;
             }
         }
-        if (!done) global_insert(keys, counts, cap_mask, key, 1ull, ctl);
+        if (!done) global_insert(keys, counts, cap_mask, key, (unsigned long long)add, ctl);
     }
     __syncthreads();
     for (int s = threadIdx.x; s < kLdsSlots; s += kBlock)
@@ -206,15 +242,51 @@ template <typename T> __device__ __forceinline__ bool better(const Cand& a, cons
     return order_key(va) < order_key(vb);
 }
 
+// Inserts c into the sorted list top[0..K) (K <= kMaxK).  Loops run to
+// kMaxK with guards so that a register-resident list keeps constant indices.
 template <typename T> __device__ __forceinline__ void push(Cand* top, int K, Cand c) {
     if (c.count == 0) return;
-    for (int i = 0; i < K; i++) {
-        if (top[i].count == 0 || better<T>(c, top[i])) {
-            for (int j = K - 1; j > i; j--) top[j] = top[j - 1];
-            top[i] = c;
-            return;
-        }
+    int pos = 0;  // entries that stay ahead of c
+#pragma unroll
+    for (int i = 0; i < kMaxK; i++)
+        if (i < K && top[i].count != 0 && !better<T>(c, top[i])) pos = i + 1;
+    if (pos >= K) return;
+#pragma unroll
+    for (int j = kMaxK - 1; j > 0; j--)
+        if (j < K && j > pos) top[j] = top[j - 1];
+#pragma unroll
+    for (int i = 0; i < kMaxK; i++)
+        if (i == pos) top[i] = c;
+}
+
+// a = the best K of the sorted lists a and b (empty entries have count 0).
+template <typename T> __device__ __forceinline__ void merge_lists(Cand* a, const Cand* b, int K) {
+    Cand o[kMaxK];
+    int i = 0, j = 0;
+#pragma unroll
+    for (int k = 0; k < kMaxK; k++) {
+        if (k >= K) break;
+        const bool ta = i < K && a[i].count != 0, tb = j < K && b[j].count != 0;
+        if (ta && (!tb || better<T>(a[i], b[j])))
+            o[k] = a[i++];
+        else if (tb)
+            o[k] = b[j++];
+        else
+            o[k] = Cand{0, 0};
     }
+#pragma unroll
+    for (int k = 0; k < kMaxK; k++)
+        if (k < K) a[k] = o[k];
+}
+
+// Pairwise tree merge of nl sorted lists of K in LDS into list 0.
+template <typename T> __device__ void block_merge(Cand* sc, int nl, int K) {
+    for (int s = 1; s < nl; s <<= 1) {
+        __syncthreads();
+        const int t = threadIdx.x;
+        if (t < nl && t % (2 * s) == 0 && t + s < nl) merge_lists<T>(sc + t * K, sc + (t + s) * K, K);
+    }
+    __syncthreads();
 }
 
 template <typename T>
@@ -231,38 +303,27 @@ __global__ __launch_bounds__(kBlock) void k_hist_topk(const unsigned long long* 
         const uint64_t k = keys[s];
         if (k != kEmpty) push<T>(top, K, Cand{counts[s], k});
     }
-    for (int i = 0; i < K; i++) sc[threadIdx.x * K + i] = top[i];
-    __syncthreads();
-    if (threadIdx.x < kWave) {  // 256 lists -> 64 lists
-        for (int r = 1; r < kBlock / kWave; r++)
-            for (int i = 0; i < K; i++) push<T>(top, K, sc[(threadIdx.x + r * kWave) * K + i]);
-    }
-    __syncthreads();
-    if (threadIdx.x < kWave)
-        for (int i = 0; i < K; i++) sc[threadIdx.x * K + i] = top[i];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        Cand b[kMaxK];
-        for (int i = 0; i < kMaxK; i++) b[i] = Cand{0, 0};
-        for (int i = 0; i < kWave * K; i++) push<T>(b, K, sc[i]);
+#pragma unroll
+    for (int i = 0; i < kMaxK; i++)
+        if (i < K) sc[threadIdx.x * K + i] = top[i];
+    block_merge<T>(sc, kBlock, K);
+    if (threadIdx.x == 0)
         for (int i = 0; i < K; i++) {
-            st_agent(reinterpret_cast<uint64_t*>(&partials[blockIdx.x * K + i].count), (uint64_t)b[i].count);
-            st_agent(&partials[blockIdx.x * K + i].key, b[i].key);
+            st_agent(reinterpret_cast<uint64_t*>(&partials[blockIdx.x * K + i].count), (uint64_t)sc[i].count);
+            st_agent(&partials[blockIdx.x * K + i].key, sc[i].key);
         }
-    }
     if (!last_block_arrives(counter, &last)) return;
-    // the last workgroup gathers every partial list into LDS, thread 0 merges
+    // the last workgroup gathers every workgroup's list into LDS and merges
     for (uint32_t i = threadIdx.x; i < gridDim.x * (uint32_t)K; i += kBlock) {
         Cand c;
         c.count = __hip_atomic_load(&partials[i].count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         c.key = __hip_atomic_load(&partials[i].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         sc[i] = c;
     }
-    __syncthreads();
+    block_merge<T>(sc, (int)gridDim.x, K);
     if (threadIdx.x != 0) return;
     Cand b[kMaxK];
-    for (int i = 0; i < kMaxK; i++) b[i] = Cand{0, 0};
-    for (uint32_t i = 0; i < gridDim.x * (uint32_t)K; i++) push<T>(b, K, sc[i]);
+    for (int i = 0; i < kMaxK; i++) b[i] = i < K ? sc[i] : Cand{0, 0};
     if (ctl->special) push<T>(b, K, Cand{ctl->special, kEmpty});
     if constexpr (std::is_floating_point<T>::value) {
         if (ctl->nans)  // every NaN row is a bin of count 1
@@ -272,7 +333,9 @@ __global__ __launch_bounds__(kBlock) void k_hist_topk(const unsigned long long* 
                     push<T>(b, K, Cand{1ull, ubits(v)});
                 }
     }
-    for (int i = 0; i < K; i++) out[i] = b[i];
+#pragma unroll
+    for (int i = 0; i < kMaxK; i++)
+        if (i < K) out[i] = b[i];
     __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 

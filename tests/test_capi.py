@@ -42,3 +42,22 @@ def test_encoding_info_without_gpu():
     assert g.encoding_info(g.GFC, g.D_FLOAT) == (2, [g.D_FLOAT, g.D_FLOAT])
     with pytest.raises(g.DdjError):
         g.encoding_info(42, g.D_INT)
+
+
+def _build_cpp(out):
+    """Compiles tests/cpp/roundtrip.cpp against include/ddj.hpp and the library."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    libdir = os.path.join(root, "gputimeseriescompressionoptimizer_amd")
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", f"-I{os.path.join(root, 'include')}",
+           os.path.join(root, "tests", "cpp", "roundtrip.cpp"), f"-L{libdir}", "-lddj_hip", f"-Wl,-rpath,{libdir}",
+           "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    return out
+
+
+def test_cpp_mirror_compiles_and_links(tmp_path):
+    """include/ddj.hpp (the reference's class names over the C ABI) builds
+    with a plain host compiler and links against the library."""
+    assert os.path.exists(_build_cpp(str(tmp_path / "roundtrip")))
