@@ -566,6 +566,34 @@ Part gfc_decode_any(Ctx* c, int dtype, const std::vector<DView>& in) {
 
 }  // namespace
 
+std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in) {
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    if (n < 2) fail(DDJ_EINVAL, "fused delta/FL needs two rows");
+    Part x = aligned_view(c, in, es);
+    DevBuf dmeta(c, es);
+    d2d(c, dmeta.get(), x.p, es);
+    const size_t m = n - 1;
+    DevBuf mmb(c, sizeof(MinMaxResult));
+    launch_minmax_delta(c, (int)es, x.p, n, mmb.get<MinMaxResult>());
+    const uint64_t W = 8 * es;
+    DevBuf data(c, (size_t)((W * m + W - 1) / W) * es);  // worst case b = W
+    launch_fl_pack_delta(c, (int)es, x.p, m, mmb.get<MinMaxResult>(), data.get());
+    MinMaxResult h;
+    d2h(c, &h, mmb.get(), sizeof h);
+    const int b = dispatch_int((int)es, [&](auto tag) {
+        using T = decltype(tag);
+        T mn, mx;
+        decode_minmax(dtype, h, &mn, &mx);
+        return bitlen_of<T>(mn, mx);
+    });
+    const uint64_t bytes = (b * (uint64_t)m + W - 1) / W * es;
+    data.set_size((size_t)bytes);
+    kernel_bytes(c, "k_minmax", (double)(n * es));
+    kernel_bytes(c, "k_fl_pack", (double)(n * es + bytes));
+    uint8_t meta[4] = {(uint8_t)b, (uint8_t)((bytes * 8 - (uint64_t)m * b) & 0xff), 0, 0};
+    return {own(std::move(dmeta)), upload(c, meta, 4), own(std::move(data))};
+}
+
 bool enc_ok(int enc) {
     return (enc >= DDJ_ENC_DELTA && enc <= DDJ_ENC_GFC) || enc == DDJ_ENC_OUTSIDE_PATCH;
 }

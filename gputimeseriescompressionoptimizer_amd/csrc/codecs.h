@@ -34,4 +34,9 @@ void predict_sizes(Ctx* c, int enc, int dtype, DView in, size_t* meta, size_t* d
 // CudaArrayStatistics::GenerateStatistics plus planner features.
 ddj_stats_t stats(Ctx* c, int dtype, DView in);
 
+// DELTA followed by AFL on an integer column of >= 2 rows, without writing
+// the deltas: {delta metadata, AFL metadata, AFL data}, byte-identical to
+// delta_encode followed by the AFL encoder on its output.
+std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in);
+
 }  // namespace ddj

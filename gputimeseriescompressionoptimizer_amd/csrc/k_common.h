@@ -185,12 +185,18 @@ template <bool WIDE> __device__ __forceinline__ bool lb_read(const Lookback& lb,
 }
 
 // Exclusive prefix of `tile`, resolved by the whole workgroup (kBlock
-// threads, all of which call it with the same tile and aggregate): each
-// step inspects the kBlock predecessors before the current window, so a
-// tile that starts while many predecessors are still resolving walks back
-// kBlock tiles per device round trip.  Narrow values must stay below 2^62
-// (counts), or only their low bits matter (sums that wrap at 32 bits or
-// fewer).
+// threads, all of which call it with the same tile and aggregate).  Each
+// step inspects the kLbWindow = kBlock * kLbPerThread predecessors before the
+// current window (thread t, entry j: tile end - 1 - t - kBlock * j), all
+// loads in flight together.  The window must be wide: a tile only turns
+// inclusive after its own look-back, so in steady state the nearest
+// inclusive predecessor lies about (tiles started per microsecond) x (one
+// device round trip) tiles back, and tiles resolve at most one window per
+// round trip.  Narrow values must stay below 2^62 (counts), or only their low
+// bits matter (sums that wrap at 32 bits or fewer).
+constexpr int kLbPerThread = 8;
+constexpr int kLbWindow = kBlock * kLbPerThread;
+
 template <bool WIDE> __device__ uint64_t block_lookback(const Lookback& lb, uint32_t tile, uint64_t aggregate) {
     __shared__ uint64_t red[kBlock / kWave];
     __shared__ uint32_t first_inc[kBlock / kWave];
@@ -202,33 +208,58 @@ template <bool WIDE> __device__ uint64_t block_lookback(const Lookback& lb, uint
     }
     if (t == 0) lb_publish<WIDE>(lb, tile, kStAggregate, aggregate);
     uint64_t excl = 0;
-    int64_t end = tile;  // window [end - kBlock, end)
+    int64_t end = tile;  // window [end - kLbWindow, end)
     for (;;) {
-        const int64_t idx = end - 1 - t;
-        uint64_t state = kStInclusive, v = 0;  // before tile 0: an inclusive zero
-        bool ready = idx < 0;
+        uint64_t state[kLbPerThread], v[kLbPerThread];
+        bool ready[kLbPerThread];
+#pragma unroll
+        for (int j = 0; j < kLbPerThread; j++) {
+            state[j] = kStInclusive;  // before tile 0: an inclusive zero
+            v[j] = 0;
+            ready[j] = end - 1 - t - (int64_t)kBlock * j < 0;
+        }
         uint32_t spins = 0;
         for (;;) {
-            if (!ready) ready = lb_read<WIDE>(lb, (size_t)idx, state, v);
-            if (__syncthreads_and(ready)) break;
+            bool all = true;
+#pragma unroll
+            for (int j = 0; j < kLbPerThread; j++) {
+                if (!ready[j]) ready[j] = lb_read<WIDE>(lb, (size_t)(end - 1 - t - (int64_t)kBlock * j), state[j], v[j]);
+                all = all && ready[j];
+            }
+            if (__syncthreads_and(all)) break;
             if (++spins > kSpinLimit) {
-                if (!ready) {
-                    __hip_atomic_store(lb.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    state = kStInclusive;
-                    v = 0;
-                }
+#pragma unroll
+                for (int j = 0; j < kLbPerThread; j++)
+                    if (!ready[j]) {
+                        __hip_atomic_store(lb.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        state[j] = kStInclusive;
+                        v[j] = 0;
+                    }
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        // the nearest inclusive predecessor is the smallest t holding one
-        const uint64_t inc = __ballot(state == kStInclusive);
-        if (lane == 0) first_inc[wid] = inc ? (uint32_t)(wid * kWave + __builtin_ctzll(inc)) : 0xffffffffu;
+        // the nearest inclusive predecessor has the smallest distance t + kBlock * j
+        uint32_t mine = 0xffffffffu;
+#pragma unroll
+        for (int j = kLbPerThread - 1; j >= 0; j--)
+            if (state[j] == kStInclusive) mine = (uint32_t)(t + kBlock * j);
+        uint32_t wmin = mine;
+#pragma unroll
+        for (int o = kWave / 2; o > 0; o >>= 1) {
+            const uint32_t w = __shfl_xor(wmin, o, kWave);
+            wmin = w < wmin ? w : wmin;
+        }
+        if (lane == 0) first_inc[wid] = wmin;
         __syncthreads();
         uint32_t first = 0xffffffffu;
 #pragma unroll
         for (int w = 0; w < kBlock / kWave; w++) first = first_inc[w] < first ? first_inc[w] : first;
-        const uint64_t ws = wave_sum<uint64_t>((uint32_t)t <= first ? v : 0);
+        uint64_t part = 0;
+#pragma unroll
+        for (int j = 0; j < kLbPerThread; j++)
+            if ((uint32_t)(t + kBlock * j) <= first) part += v[j];
+        const uint64_t ws = wave_sum<uint64_t>(part);
         if (lane == 0) red[wid] = ws;
         __syncthreads();
         uint64_t sum = 0;
@@ -237,7 +268,7 @@ template <bool WIDE> __device__ uint64_t block_lookback(const Lookback& lb, uint
         excl += sum;
         __syncthreads();  // red / first_inc are rewritten by the next step
         if (first != 0xffffffffu) break;
-        end -= kBlock;
+        end -= kLbWindow;
     }
     if (t == 0) lb_publish<WIDE>(lb, tile, kStInclusive, excl + aggregate);
     return excl;

@@ -28,10 +28,11 @@ namespace {
 
 constexpr int kFlTile = 4096;
 
-template <typename T>
+// DELTA: value i is x[i+1] - x[i] (wrapping), read straight from x (n + 1 rows).
+template <typename T, bool DELTA = false>
 __global__ __launch_bounds__(kBlock) void k_fl_pack(const T* __restrict__ x, size_t n, const MinMaxResult* mm,
                                                     int fixed_bits, uint32_t* __restrict__ out, size_t out_words) {
-    __shared__ __attribute__((aligned(16))) T vals[kFlTile];
+    __shared__ __attribute__((aligned(16))) T vals[kFlTile + (DELTA ? 16 / sizeof(T) : 0)];
     int b = fixed_bits;
     if (mm) b = bitlen_of<T>(from_order_key<T>(mm->min_key), from_order_key<T>(mm->max_key));
     const size_t tile = blockIdx.x;
@@ -44,10 +45,24 @@ __global__ __launch_bounds__(kBlock) void k_fl_pack(const T* __restrict__ x, siz
         constexpr int N4 = kFlTile * (int)sizeof(T) / 16;
 #pragma unroll
         for (int i = threadIdx.x; i < N4; i += kBlock) d4[i] = __builtin_nontemporal_load(s4 + i);
+        if (DELTA && threadIdx.x == 0) vals[kFlTile] = src[kFlTile];  // x has n + 1 rows
     } else {
-        for (int i = threadIdx.x; i < kFlTile; i += kBlock) vals[i] = i < nv ? src[i] : T(0);
+        for (int i = threadIdx.x; i < kFlTile + (DELTA ? 1 : 0); i += kBlock) vals[i] = i < nv + (DELTA ? 1 : 0) ? src[i] : T(0);
     }
     __syncthreads();
+    if (DELTA) {  // in place: vals[i] = vals[i+1] - vals[i]
+        using U = typename UnsignedOf<T>::type;
+        T d[kFlTile / kBlock];
+#pragma unroll
+        for (int k = 0; k < kFlTile / kBlock; k++) {
+            const int i = threadIdx.x + k * kBlock;
+            d[k] = i < nv ? (T)(U)((U)vals[i + 1] - (U)vals[i]) : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kFlTile / kBlock; k++) vals[threadIdx.x + k * kBlock] = d[k];
+        __syncthreads();
+    }
     const int tile_words = kFlTile / 32 * b;  // 128 * b
     const size_t w0 = tile * (size_t)tile_words;
     const uint64_t mask = lowmask64(b);
@@ -121,6 +136,20 @@ void launch_fl_pack(Ctx* c, int dtype, const void* x, size_t n, const MinMaxResu
         const size_t tiles = (n + kFlTile - 1) / kFlTile;
         { KernelTimer kt_(c, "k_fl_pack"); hipLaunchKernelGGL(k_fl_pack<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, static_cast<const T*>(x),
                            n, mm, bits, static_cast<uint32_t*>(out), out_words); }
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_fl_pack_delta(Ctx* c, int elem_size, const void* x, size_t m, const MinMaxResult* mm, void* out) {
+    if (m == 0) return;
+    dispatch_int(elem_size, [&](auto tag) {
+        using T = decltype(tag);
+        const uint64_t W = 8 * sizeof(T);
+        const size_t out_bytes = (size_t)((W * m + W - 1) / W) * sizeof(T);
+        const size_t out_words = (out_bytes + 3) / 4;
+        const size_t tiles = (m + kFlTile - 1) / kFlTile;
+        { KernelTimer kt_(c, "k_fl_pack"); hipLaunchKernelGGL((k_fl_pack<T, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                           static_cast<const T*>(x), m, mm, 0, static_cast<uint32_t*>(out), out_words); }
         HIPC(hipGetLastError());
     });
 }

@@ -154,6 +154,78 @@ __global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size
     }
 }
 
+// Min/max of the wrapping differences d[i] = x[i+1] - x[i], i < n - 1, read
+// straight from x (the delta encoder's output is never written when an FL
+// node consumes it).  Integer types only.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_minmax_delta(const T* __restrict__ x, size_t n, uint64_t* __restrict__ partials,
+                                                         uint32_t* counter, MinMaxResult* out) {
+    __shared__ uint64_t sm[kBlock / kWave];
+    __shared__ bool last;
+    using U = typename UnsignedOf<T>::type;
+    uint64_t mn = ~0ull, mx = 0;
+    auto add = [&](T a, T b) {  // d = b - a, wrapping in T
+        const T d = (T)(U)((U)b - (U)a);
+        const uint64_t key = order_key(d);
+        mn = key < mn ? key : mn;
+        mx = key > mx ? key : mx;
+    };
+    const size_t m = n - 1;
+    const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x, stride = (size_t)gridDim.x * kBlock;
+    constexpr int V = 16 / sizeof(T);
+    size_t done = 0;
+    if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+        const u32x4* xv = reinterpret_cast<const u32x4*>(x);
+        const size_t nv = m / V;  // vectors whose V deltas are all in range (x[i*V+V] exists)
+        constexpr int Uf = 4;
+        for (size_t i0 = tid; i0 < nv; i0 += Uf * stride) {
+            u32x4 v[Uf];
+            T nx[Uf];
+#pragma unroll
+            for (int u = 0; u < Uf; u++) {
+                const size_t i = i0 + u * stride;
+                if (i < nv) {
+                    v[u] = __builtin_nontemporal_load(xv + i);
+                    nx[u] = x[(i + 1) * V];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < Uf; u++) {
+                const size_t i = i0 + u * stride;
+                if (i >= nv) break;
+                T vals[V + 1];
+                __builtin_memcpy(vals, &v[u], 16);
+                vals[V] = nx[u];
+#pragma unroll
+                for (int k = 0; k < V; k++) add(vals[k], vals[k + 1]);
+            }
+        }
+        done = nv * V;
+    }
+    for (size_t i = done + tid; i < m; i += stride) add(x[i], x[i + 1]);
+    mn = block_reduce_min(mn, sm);
+    mx = block_reduce_max(mx, sm);
+    if (threadIdx.x == 0) {
+        st_agent(partials + 2 * blockIdx.x, mn);
+        st_agent(partials + 2 * blockIdx.x + 1, mx);
+    }
+    if (!last_block_arrives(counter, &last)) return;
+    mn = ~0ull;
+    mx = 0;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
+        const uint64_t a = rd(partials + 2 * b), z = rd(partials + 2 * b + 1);
+        mn = a < mn ? a : mn;
+        mx = z > mx ? z : mx;
+    }
+    mn = block_reduce_min(mn, sm);
+    mx = block_reduce_max(mx, sm);
+    if (threadIdx.x == 0) {
+        out->min_key = mn;
+        out->max_key = mx;
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // ------------------------------------------------------------------- stats
 // _getFloatPrecision (reference util/statistics/cuda_array_statistics.cu:37-44)
 // as the loop count k of round(x*10^k)/10^k != x in float arithmetic.
@@ -385,6 +457,18 @@ void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_r
         const int grid = reduce_grid(c, n, sizeof(T));
         DevBuf partials(c, 24 * (size_t)grid);
         { KernelTimer kt_(c, "k_minmax"); hipLaunchKernelGGL(k_minmax<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
+                           partials.get<uint64_t>(), &c->control->counters[0], d_result); }
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_minmax_delta(Ctx* c, int elem_size, const void* x, size_t n, MinMaxResult* d_result) {
+    if (n < 2) fail(DDJ_EINVAL, "delta min/max needs two rows");
+    dispatch_int(elem_size, [&](auto tag) {
+        using T = decltype(tag);
+        const int grid = reduce_grid(c, n, sizeof(T));
+        DevBuf partials(c, 16 * (size_t)grid);
+        { KernelTimer kt_(c, "k_minmax"); hipLaunchKernelGGL(k_minmax_delta<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
                            partials.get<uint64_t>(), &c->control->counters[0], d_result); }
         HIPC(hipGetLastError());
     });

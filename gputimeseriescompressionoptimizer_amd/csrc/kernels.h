@@ -27,6 +27,10 @@ struct MinMaxResult {
 
 // min/max of n values of `dtype` into d_result (device).  One launch.
 void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_result);
+// min/max of the wrapping differences x[i+1] - x[i] of n integer rows.
+void launch_minmax_delta(Ctx* c, int elem_size, const void* x, size_t n, MinMaxResult* d_result);
+// FL pack of the m = n - 1 differences of n integer rows (bits from mm).
+void launch_fl_pack_delta(Ctx* c, int elem_size, const void* x, size_t m, const MinMaxResult* mm, void* out);
 // Converts order keys back to the typed min/max (host).
 void decode_minmax(int dtype, const MinMaxResult& r, void* mn, void* mx);
 

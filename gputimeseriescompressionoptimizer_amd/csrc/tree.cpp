@@ -40,13 +40,37 @@ struct Piece {
     size_t size() const { return is_header ? 8 : part.n; }
 };
 
-// CompressionNode::Compress (compression_node.cpp:103-138) and PrepareMetadata (:84-101)
-size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& out) {
+bool is_int_type(int dt) { return dt == DDJ_D_INT || dt == DDJ_D_TIME || dt == DDJ_D_SHORT || dt == DDJ_D_CHAR; }
+
+// DELTA -> AFL (-> none) on an integer column: the deltas feed the FL pass
+// straight from the column (delta_afl_encode) and are never written.
+bool fusable_delta_afl(const TNode& node, const Part& data) {
+    if (node.enc != DDJ_ENC_DELTA || !is_int_type(node.dtype) || node.ch.size() != 1) return false;
+    const TNode& a = *node.ch[0];
+    if (a.enc != DDJ_ENC_AFL || a.dtype != node.dtype) return false;
+    if (!(a.ch.empty() || (a.ch.size() == 1 && a.ch[0]->enc == DDJ_ENC_NONE && a.ch[0]->ch.empty()))) return false;
+    return data.n / dtype_size(node.dtype) >= 2;
+}
+
+// CompressionNode::Compress (compression_node.cpp:103-138) and PrepareMetadata (:84-101).
+// `pre` holds this node's encoder outputs when a parent already produced them.
+size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& out,
+                     const std::vector<Part>* pre = nullptr) {
     std::vector<Part> res;
-    if (node.enc == DDJ_ENC_NONE) {
+    std::vector<Part> child_pre;
+    if (pre) {
+        res = *pre;
+    } else if (node.enc == DDJ_ENC_NONE) {
         size_t len = data.n;  // the leaf refers to its input instead of copying it
         res.push_back(own(make_buf(c, &len, sizeof len)));
         res.push_back(data);
+    } else if (fusable_delta_afl(node, data)) {
+        auto f = delta_afl_encode(c, node.dtype, data.view());
+        const size_t es = dtype_size(node.dtype);
+        Part deltas;  // never materialised: only its size is read
+        deltas.n = (data.n / es - 1) * es;
+        res = {f[0], deltas};
+        child_pre = {f[1], f[2]};
     } else {
         res = encode(c, node.enc, node.dtype, data.view());
     }
@@ -70,7 +94,7 @@ size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& 
     } else {
         for (size_t i = 0; i < node.ch.size(); i++) {
             if (i + 1 >= res.size()) fail(DDJ_EINVAL, "tree node has more children than its encoding has outputs");
-            total += compress_node(c, *node.ch[i], res[i + 1], out);
+            total += compress_node(c, *node.ch[i], res[i + 1], out, child_pre.empty() ? nullptr : &child_pre);
         }
     }
     node.ratio = ratio_of(data.n, total);
