@@ -566,6 +566,24 @@ Part gfc_decode_any(Ctx* c, int dtype, const std::vector<DView>& in) {
 
 }  // namespace
 
+Part delta_afl_decode(Ctx* c, int dtype, DView dmeta, DView ameta, DView data) {
+    const size_t es = dtype_size(dtype);
+    if (dmeta.n < es) fail(DDJ_ECORRUPT, "delta metadata shorter than one value");
+    if (ameta.n < 2) fail(DDJ_ECORRUPT, "AFL metadata shorter than 2 bytes");
+    int8_t mb[2];
+    d2h(c, mb, ameta.p, 2);
+    const int b = mb[0], rest = mb[1];
+    const int W = 8 * (int)es;
+    if (b <= 0 || b > W) fail(DDJ_ECORRUPT, "AFL bit width out of range");
+    const long long bits = (long long)data.n * 8 - rest;
+    if (bits < 0) fail(DDJ_ECORRUPT, "AFL rest larger than the data");
+    const size_t m = (size_t)(bits / b);
+    DevBuf out(c, (m + 1) * es);
+    launch_fl_unpack_delta(c, (int)es, dmeta.p, data.p, data.n, m, b, out.get());
+    kernel_bytes(c, "k_fl_unpack_delta_scan", (double)((m * b + 7) / 8 + (m + 1) * es));
+    return own(std::move(out));
+}
+
 std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in) {
     const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
     if (n < 2) fail(DDJ_EINVAL, "fused delta/FL needs two rows");

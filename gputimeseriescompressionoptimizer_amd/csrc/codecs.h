@@ -38,5 +38,7 @@ ddj_stats_t stats(Ctx* c, int dtype, DView in);
 // the deltas: {delta metadata, AFL metadata, AFL data}, byte-identical to
 // delta_encode followed by the AFL encoder on its output.
 std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in);
+// The inverse: AFL data (non-empty) decoded and prefix-summed in one pass.
+Part delta_afl_decode(Ctx* c, int dtype, DView delta_meta, DView afl_meta, DView afl_data);
 
 }  // namespace ddj

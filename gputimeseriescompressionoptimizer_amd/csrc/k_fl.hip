@@ -82,14 +82,13 @@ __global__ __launch_bounds__(kBlock) void k_fl_pack(const T* __restrict__ x, siz
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_fl_unpack(const uint8_t* __restrict__ in, size_t in_bytes, size_t n,
-                                                      int b, T* __restrict__ out) {
-    constexpr int kMaxWords = kFlTile * 64 / 32 + 8;
-    __shared__ uint32_t w[kMaxWords];
-    const size_t tile = blockIdx.x;
-    const size_t v0 = tile * kFlTile;
-    const int nv = (int)((n - v0) < (size_t)kFlTile ? (n - v0) : (size_t)kFlTile);
+constexpr int kMaxWords = kFlTile * 64 / 32 + 8;
+
+// Stages the aligned 32-bit words holding the b-bit values [v0, v0 + nv) of
+// the stream `in` (any byte address) in LDS; returns the byte offset of the
+// first value's byte inside w[0].  Bytes outside the stream are zero.
+__device__ __forceinline__ int stage_fl_words(const uint8_t* __restrict__ in, size_t in_bytes, size_t v0, int nv, int b,
+                                              uint32_t* w) {
     const size_t B0 = v0 * (size_t)b / 8;  // exact: v0 is a multiple of 8
     const size_t need = ((size_t)nv * b + 7) / 8;
     const size_t avail_total = in_bytes > B0 ? in_bytes - B0 : 0;
@@ -111,15 +110,69 @@ __global__ __launch_bounds__(kBlock) void k_fl_unpack(const uint8_t* __restrict_
         }
         w[i] = v;
     }
+    return a;
+}
+
+// Value k (0-based in the tile) from staged words.
+__device__ __forceinline__ uint64_t fl_value(const uint32_t* w, int k, int b, int a, uint64_t mask) {
+    const uint64_t p = (uint64_t)k * b + 8 * a;
+    const uint32_t q = (uint32_t)(p >> 5), s = (uint32_t)(p & 31);
+    uint64_t val = ((uint64_t)w[q] | ((uint64_t)w[q + 1] << 32)) >> s;
+    if (s + b > 64) val |= (uint64_t)w[q + 2] << (64 - s);
+    return val & mask;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fl_unpack(const uint8_t* __restrict__ in, size_t in_bytes, size_t n,
+                                                      int b, T* __restrict__ out) {
+    __shared__ uint32_t w[kMaxWords];
+    const size_t v0 = (size_t)blockIdx.x * kFlTile;
+    const int nv = (int)((n - v0) < (size_t)kFlTile ? (n - v0) : (size_t)kFlTile);
+    const int a = stage_fl_words(in, in_bytes, v0, nv, b, w);
     __syncthreads();
     const uint64_t mask = lowmask64(b);
-    for (int k = threadIdx.x; k < nv; k += kBlock) {
-        const uint64_t p = (uint64_t)k * b + 8 * a;
-        const uint32_t q = (uint32_t)(p >> 5), s = (uint32_t)(p & 31);
-        uint64_t val = ((uint64_t)w[q] | ((uint64_t)w[q + 1] << 32)) >> s;
-        if (s + b > 64) val |= (uint64_t)w[q + 2] << (64 - s);
-        out[v0 + k] = (T)(val & mask);
+    for (int k = threadIdx.x; k < nv; k += kBlock) out[v0 + k] = (T)fl_value(w, k, b, a, mask);
+}
+
+// DeltaEncoding::Decode over AflEncoding::Decode in one pass
+// (delta_encoding.cu:74-102 on afl_encoding.cu:74-113): the m b-bit deltas
+// are unpacked and prefix-summed (wrapping in T) on top of the first row,
+// without writing the deltas.  out has m + 1 rows.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fl_unpack_delta_scan(const uint8_t* first_bytes, const uint8_t* __restrict__ in,
+                                                                 size_t in_bytes, size_t m, int b, T* __restrict__ out,
+                                                                 Lookback lb) {
+    __shared__ uint32_t w[kMaxWords];  // the staged words, then the tile's results
+    __shared__ uint64_t red[kBlock / kWave];
+    constexpr int kPer = kFlTile / kBlock;  // 16 consecutive values per thread
+    T first;
+    __builtin_memcpy(&first, first_bytes, sizeof(T));
+    const uint32_t tile = tile_index();
+    const size_t v0 = (size_t)tile * kFlTile;
+    const int nv = (int)((m - v0) < (size_t)kFlTile ? (m - v0) : (size_t)kFlTile);
+    const int a = stage_fl_words(in, in_bytes, v0, nv, b, w);
+    __syncthreads();
+    const uint64_t mask = lowmask64(b);
+    uint64_t d[kPer], sum = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        const int i = threadIdx.x * kPer + k;
+        d[k] = i < nv ? fl_value(w, i, b, a, mask) : 0;
+        sum += d[k];
     }
+    uint64_t total;
+    const uint64_t excl = block_exclusive_scan<uint64_t>(sum, &total, red);
+    const uint64_t prefix = block_lookback<sizeof(T) == 8>(lb, tile, total);  // a barrier: w is free again
+    T* res = reinterpret_cast<T*>(w);
+    uint64_t run = ubits(first) + prefix + excl;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        run += d[k];
+        res[threadIdx.x * kPer + k] = (T)run;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nv; i += kBlock) out[v0 + 1 + i] = res[i];
+    if (tile == 0 && threadIdx.x == 0) out[0] = first;
 }
 
 }  // namespace
@@ -150,6 +203,23 @@ void launch_fl_pack_delta(Ctx* c, int elem_size, const void* x, size_t m, const 
         const size_t tiles = (m + kFlTile - 1) / kFlTile;
         { KernelTimer kt_(c, "k_fl_pack"); hipLaunchKernelGGL((k_fl_pack<T, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
                            static_cast<const T*>(x), m, mm, 0, static_cast<uint32_t*>(out), out_words); }
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_fl_unpack_delta(Ctx* c, int elem_size, const uint8_t* first, const uint8_t* in, size_t in_bytes, size_t m,
+                           int bits, void* out) {
+    dispatch_int(elem_size, [&](auto tag) {
+        using T = decltype(tag);
+        if (bits <= 0 || bits > 8 * (int)sizeof(T)) fail(DDJ_ECORRUPT, "FL bit width out of range");
+        if (m == 0) {
+            d2d(c, out, first, sizeof(T));
+            return;
+        }
+        const size_t tiles = (m + kFlTile - 1) / kFlTile;
+        LookbackBufs lbb(c, tiles, sizeof(T) == 8);
+        { KernelTimer kt_(c, "k_fl_unpack_delta_scan"); hipLaunchKernelGGL(k_fl_unpack_delta_scan<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                           first, in, in_bytes, m, bits, static_cast<T*>(out), lbb.lb); }
         HIPC(hipGetLastError());
     });
 }

@@ -53,6 +53,10 @@ void launch_stats(Ctx* c, int dtype, const void* x, size_t n, DevStats* d_out);
 // FL pack: bits from `mm` (device min/max) when bits == 0, otherwise fixed.
 // Writes ceil(n*b/W)*sizeof(T) bytes rounded up to 4 into out (32-bit words).
 void launch_fl_pack(Ctx* c, int dtype, const void* x, size_t n, const MinMaxResult* mm, int bits, void* out);
+// FL unpack of m b-bit deltas fused with the delta decode: out[0] = first
+// (device, one value), out[i+1] = out[i] + d[i] (wrapping).
+void launch_fl_unpack_delta(Ctx* c, int elem_size, const uint8_t* first, const uint8_t* in, size_t in_bytes, size_t m,
+                           int bits, void* out);
 // FL unpack of n values of b bits from an arbitrary byte address.
 void launch_fl_unpack(Ctx* c, int dtype, const uint8_t* in, size_t in_bytes, size_t n, int bits, void* out);
 

@@ -1,6 +1,7 @@
 // The compression cascade: node records, blob assembly, parse and decode.
 #include "tree.h"
 
+#include <cstring>
 #include <functional>
 #include <sstream>
 
@@ -218,14 +219,25 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
 
 Part decompress_tree(Ctx* c, DView blob, int* dtype_out, NodeP* tree_out) {
     if (blob.n < 8) fail(DDJ_ECORRUPT, "blob shorter than one node header");
-    DevBuf table(c, sizeof(ParsedNode) * kMaxTreeNodes), status(c, 16);
-    launch_parse_tree(c, blob.p, blob.n, table.get<ParsedNode>(), kMaxTreeNodes, status.get<int>());
+    // status words and node table in one buffer, read back with one copy
+    // (trees have a few nodes; 64 table entries cover every planner tree)
+    constexpr int kHead = 64;  // bytes before the table
+    DevBuf parsed(c, kHead + sizeof(ParsedNode) * kMaxTreeNodes);
+    ParsedNode* table = reinterpret_cast<ParsedNode*>(parsed.bytes_ptr() + kHead);
+    launch_parse_tree(c, blob.p, blob.n, table, kMaxTreeNodes, parsed.get<int>());
+    constexpr int kFirst = 64;
+    std::vector<uint8_t> host(kHead + sizeof(ParsedNode) * kFirst);
+    d2h(c, host.data(), parsed.get(), host.size());
     int st[3];
-    d2h(c, st, status.get(), sizeof st);
+    memcpy(st, host.data(), sizeof st);
     if (st[0] != 0) fail(DDJ_ECORRUPT, "malformed tree blob (parse status " + std::to_string(st[0]) + ")");
     const int count = st[1];
+    if (count <= 0 || count > kMaxTreeNodes) fail(DDJ_ECORRUPT, "malformed tree blob (node count)");
     std::vector<ParsedNode> nodes((size_t)count);
-    d2h(c, nodes.data(), table.get(), sizeof(ParsedNode) * count);
+    if (count <= kFirst)
+        memcpy(nodes.data(), host.data() + kHead, sizeof(ParsedNode) * count);
+    else
+        d2h(c, nodes.data(), table, sizeof(ParsedNode) * count);
     std::vector<std::vector<int>> kids((size_t)count);
     for (int i = 1; i < count; i++) kids[(size_t)nodes[i].parent].push_back(i);
     if (tree_out) {
@@ -240,6 +252,17 @@ Part decompress_tree(Ctx* c, DView blob, int* dtype_out, NodeP* tree_out) {
         const DView meta{blob.p + nd.meta_off, (size_t)nd.meta_len};
         if (nd.enc == DDJ_ENC_NONE) return borrow(DView{blob.p + nd.data_off, (size_t)nd.data_len});
         if ((int)kids[(size_t)i].size() != nd.nchild) fail(DDJ_ECORRUPT, "node child count mismatch");
+        // DELTA over AFL over none (integer): unpack and prefix-sum in one pass
+        if (nd.enc == DDJ_ENC_DELTA && is_int_type(nd.dtype) && nd.nchild == 1) {
+            const ParsedNode& an = nodes[(size_t)kids[(size_t)i][0]];
+            const auto& ak = kids[(size_t)kids[(size_t)i][0]];
+            if (an.enc == DDJ_ENC_AFL && an.dtype == nd.dtype && an.nchild == 1 && ak.size() == 1) {
+                const ParsedNode& ln = nodes[(size_t)ak[0]];
+                if (ln.enc == DDJ_ENC_NONE && ln.data_len > 0 && meta.n >= dtype_size(nd.dtype))
+                    return delta_afl_decode(c, nd.dtype, meta, DView{blob.p + an.meta_off, (size_t)an.meta_len},
+                                            DView{blob.p + ln.data_off, (size_t)ln.data_len});
+            }
+        }
         std::vector<Part> parts;
         std::vector<DView> in{meta};
         for (int k : kids[(size_t)i]) {
