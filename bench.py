@@ -24,6 +24,23 @@ sys.path.insert(0, ROOT)
 
 METRIC = "compress/decompress input GB/s at 1 GPU + ratio gap vs exhaustive planner"
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes per launch of each kernel on this workload, measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and scaled by a copy kernel of known
+# size (the summary and its commands are in this file).
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_int_columns_64M.txt")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed PMC summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == kernel:
+                    return int(parts[2])
+    except OSError:
+        return None
+    return None
 
 
 def columns(rows: int):
@@ -255,7 +272,7 @@ def main():
     dname, (dms, dlaunch, dbytes) = dom
     achieved = dbytes / (dms / 1e3) / 1e9 if dms > 0 else 0.0
     roofline = {"bound": "hbm", "kernel": dname, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(dname),
                 "avg_launch_us": round(dms / dlaunch * 1e3, 2), "launches_per_step": dlaunch / args.steps,
                 "algorithmic_bytes_per_launch": int(dbytes / dlaunch)}
     kernels = {k: {"ms_per_step": round(v[0] / args.steps, 4), "launches_per_step": v[1] / args.steps,

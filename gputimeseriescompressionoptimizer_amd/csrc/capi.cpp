@@ -57,9 +57,24 @@ struct OwnedCtx {
     ~OwnedCtx() { fini_ctx(c); }
 };
 
+// The captured graph of the last batch of FL-leaf columns, replayed while
+// the batch layout (pointers, sizes, types, trees) stays the same.
+struct BatchGraph {
+    std::vector<std::tuple<const void*, size_t, int, int>> key;
+    std::unique_ptr<DevBuf> cols, ranges, tiles, arena;
+    std::vector<BatchColumn> host_cols;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~BatchGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
 struct ddj_ctx {
     Ctx c;
     std::vector<std::unique_ptr<OwnedCtx>> pool;  // worker contexts of ddj_batch_compress
+    std::unique_ptr<BatchGraph> bgraph;
 };
 
 struct ddj_optimizer {
@@ -141,6 +156,9 @@ ddj_status ddj_ctx_create(ddj_ctx** out, int device) {
 
 void ddj_ctx_destroy(ddj_ctx* ctx) {
     if (!ctx) return;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    ctx->bgraph.reset();
     ctx->pool.clear();
     fini_ctx(ctx->c);
     delete ctx;
@@ -635,6 +653,86 @@ ddj_status ddj_planner_plan(ddj_planner* p, int dtype, const void* d_in, size_t 
 
 // ------------------------------------------------------------ batched trees
 
+namespace {
+
+// 1: afl -> none, 2: delta -> afl -> none on an aligned int32 / int64 column
+// (the graph-captured batch path); 0: anything else.
+int fl_leaf_kind(const ddj_column& col) {
+    if (col.dtype != DDJ_D_INT && col.dtype != DDJ_D_TIME) return 0;
+    if ((reinterpret_cast<uintptr_t>(col.d_in) & 15) != 0 || !col.preorder) return 0;
+    NodeP t = tree_from_path(col.preorder, col.path_len, col.dtype);
+    fix_tree(t);
+    const std::vector<int> p = preorder(t);
+    const size_t n = col.in_bytes / dtype_size(col.dtype);
+    if (p == std::vector<int>{DDJ_ENC_AFL, DDJ_ENC_NONE} && n >= 1) return 1;
+    if (p == std::vector<int>{DDJ_ENC_DELTA, DDJ_ENC_AFL, DDJ_ENC_NONE} && n >= 2) return 2;
+    return 0;
+}
+
+// Compresses the FL-leaf columns `idx` of `cols` through the captured graph
+// (captured on first use of a layout) into blobs owned by ctx.
+void batch_fl_graph(ddj_ctx* ctx, const ddj_column* cols, const std::vector<uint32_t>& idx, ddj_buf* blobs) {
+    Ctx* c = &ctx->c;
+    std::vector<std::tuple<const void*, size_t, int, int>> key;
+    for (uint32_t i : idx) key.emplace_back(cols[i].d_in, cols[i].in_bytes, cols[i].dtype, fl_leaf_kind(cols[i]));
+    if (!ctx->bgraph || ctx->bgraph->key != key) {
+        ctx->bgraph.reset();
+        std::unique_ptr<BatchGraph> g(new BatchGraph);
+        g->key = key;
+        std::vector<BatchItem> ranges, tiles;
+        uint64_t arena = 16;
+        constexpr uint64_t kRange = 1ull << 18;  // rows per min/max item
+        for (size_t k = 0; k < idx.size(); k++) {
+            const ddj_column& col = cols[idx[k]];
+            BatchColumn d{};
+            d.x = static_cast<const uint8_t*>(col.d_in);
+            d.es = (uint32_t)dtype_size(col.dtype);
+            d.n = col.in_bytes / d.es;
+            d.delta = std::get<3>(key[k]) == 2 ? 1 : 0;
+            d.dtype = (uint32_t)col.dtype;
+            g->host_cols.push_back(d);
+            const uint64_t m = d.delta ? d.n - 1 : d.n;
+            for (uint64_t s0 = 0; s0 < m; s0 += kRange)
+                ranges.push_back(BatchItem{(uint32_t)k, 0, s0, std::min(m, s0 + kRange)});
+            for (uint64_t t = 0; t * kFlTileRows < m; t++) tiles.push_back(BatchItem{(uint32_t)k, 0, t, 0});
+            arena += ((d.delta ? 8 + d.es : 0) + 28 + m * d.es + 15) & ~15ull;
+        }
+        g->cols.reset(new DevBuf(c, sizeof(BatchColumn) * g->host_cols.size()));
+        g->ranges.reset(new DevBuf(c, sizeof(BatchItem) * std::max<size_t>(ranges.size(), 1)));
+        g->tiles.reset(new DevBuf(c, sizeof(BatchItem) * std::max<size_t>(tiles.size(), 1)));
+        g->arena.reset(new DevBuf(c, arena));
+        h2d(c, g->cols->get(), g->host_cols.data(), sizeof(BatchColumn) * g->host_cols.size());
+        if (!ranges.empty()) h2d(c, g->ranges->get(), ranges.data(), sizeof(BatchItem) * ranges.size());
+        if (!tiles.empty()) h2d(c, g->tiles->get(), tiles.data(), sizeof(BatchItem) * tiles.size());
+        HIPC(hipStreamSynchronize(c->stream));
+        HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        launch_batch_fl(c, g->cols->get<BatchColumn>(), (int)g->host_cols.size(), g->ranges->get<BatchItem>(),
+                        (int)ranges.size(), g->tiles->get<BatchItem>(), (int)tiles.size(), g->arena->bytes_ptr());
+        HIPC(hipStreamEndCapture(c->stream, &g->graph));
+        HIPC(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        ctx->bgraph = std::move(g);
+    }
+    BatchGraph* g = ctx->bgraph.get();
+    HIPC(hipGraphLaunch(g->exec, c->stream));
+    std::vector<BatchColumn> out(g->host_cols.size());
+    d2h(c, out.data(), g->cols->get(), sizeof(BatchColumn) * out.size());
+    // every blob into its own allocation, one copy launch for all of them
+    std::vector<CopyPair> pairs;
+    uint64_t max_bytes = 8;
+    for (size_t k = 0; k < idx.size(); k++) {
+        DevBuf b(c, out[k].blob_bytes);
+        pairs.push_back(CopyPair{g->arena->bytes_ptr() + out[k].out_off, b.bytes_ptr(), out[k].blob_bytes});
+        max_bytes = std::max<uint64_t>(max_bytes, out[k].blob_bytes);
+        blobs[idx[k]] = b.release();
+    }
+    DevBuf dp(c, sizeof(CopyPair) * pairs.size());
+    h2d(c, dp.get(), pairs.data(), sizeof(CopyPair) * pairs.size());
+    launch_copy_pairs(c, dp.get<CopyPair>(), (int)pairs.size(), max_bytes);
+    HIPC(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace
+
 extern "C" {
 
 ddj_status ddj_batch_compress(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_cols, uint32_t workers,
@@ -651,6 +749,10 @@ ddj_status ddj_batch_compress(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_c
         }
         for (uint32_t i = 0; i < n_cols; i++) blobs[i] = ddj_buf{nullptr, 0};
         HIPC(hipStreamSynchronize(ctx->c.stream));  // inputs written on the caller's stream are in place
+        // FL-leaf columns go through the captured graph on the caller's
+        // stream; the others through the worker contexts, concurrently
+        std::vector<uint32_t> graph_cols, other_cols;
+        for (uint32_t i = 0; i < n_cols; i++) (fl_leaf_kind(cols[i]) ? graph_cols : other_cols).push_back(i);
         std::atomic<uint32_t> next{0};
         std::mutex mu;  // the caller's allocator and the first error
         ddj_status first_st = DDJ_OK;
@@ -658,8 +760,9 @@ ddj_status ddj_batch_compress(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_c
         auto work = [&](Ctx* w) {
             (void)hipSetDevice(w->device);
             for (;;) {
-                const uint32_t i = next.fetch_add(1);
-                if (i >= n_cols) return;
+                const uint32_t q = next.fetch_add(1);
+                if (q >= other_cols.size()) return;
+                const uint32_t i = other_cols[q];
                 try {
                     NodeP t = tree_from_path(cols[i].preorder, cols[i].path_len, cols[i].dtype);
                     fix_tree(t);
@@ -688,7 +791,18 @@ ddj_status ddj_batch_compress(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_c
             }
         };
         std::vector<std::thread> th;
-        for (uint32_t k = 0; k < workers; k++) th.emplace_back(work, &ctx->pool[k]->c);
+        for (uint32_t k = 0; k < workers && k < other_cols.size(); k++) th.emplace_back(work, &ctx->pool[k]->c);
+        if (!graph_cols.empty()) {
+            try {
+                batch_fl_graph(ctx, cols, graph_cols, blobs);
+            } catch (const Error& e) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (first_st == DDJ_OK) {
+                    first_st = e.status;
+                    first_msg = "graph batch: " + e.msg;
+                }
+            }
+        }
         for (auto& t : th) t.join();
         if (first_st != DDJ_OK) {
             for (uint32_t i = 0; i < n_cols; i++)

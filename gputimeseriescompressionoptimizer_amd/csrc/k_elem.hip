@@ -161,6 +161,12 @@ __global__ __launch_bounds__(kBlock) void k_gather_copy(const CopyRange* __restr
     if (tid < nb - t0) d[t0 + tid] = src[t0 + tid];
 }
 
+__global__ __launch_bounds__(kBlock) void k_copy_pairs(const CopyPair* __restrict__ r) {
+    const CopyPair cp = r[blockIdx.y];
+    const size_t stride = (size_t)gridDim.x * kBlock;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < cp.bytes; i += stride) cp.dst[i] = cp.src[i];
+}
+
 // ------------------------------------------------------------- delta decode
 // out[0] = x0; out[i+1] = x0 + d[0] + ... + d[i] (wrapping).  Tiles of 4096
 // deltas; each thread sums 16 consecutive deltas staged through LDS, the
@@ -302,6 +308,14 @@ void launch_gather_copy(Ctx* c, const CopyRange* d_ranges, int n_ranges, uint8_t
     unsigned gx = grid_for(c, max_bytes, 64);  // 4 x 16 bytes per thread
     gx = gx > 2048 ? 2048 : gx;
     { KernelTimer kt_(c, "k_gather_copy"); hipLaunchKernelGGL(k_gather_copy, dim3(gx, n_ranges), dim3(kBlock), 0, c->stream, d_ranges, dst); }
+    HIPC(hipGetLastError());
+}
+
+void launch_copy_pairs(Ctx* c, const CopyPair* d_pairs, int n_pairs, uint64_t max_bytes) {
+    if (n_pairs <= 0 || max_bytes == 0) return;
+    unsigned gx = grid_for(c, max_bytes, 4);
+    gx = gx > 1024 ? 1024 : gx;
+    { KernelTimer kt_(c, "k_copy_pairs"); hipLaunchKernelGGL(k_copy_pairs, dim3(gx, n_pairs), dim3(kBlock), 0, c->stream, d_pairs); }
     HIPC(hipGetLastError());
 }
 

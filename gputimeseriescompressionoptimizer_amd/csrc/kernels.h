@@ -130,6 +130,30 @@ void launch_mlp_backward(Ctx* c, const float* X, const uint16_t* H1s, const uint
                          const float* T, int B, const MlpParams& p, const MlpParams& g);
 void launch_adam(Ctx* c, float* w, float* grad, float* m, float* v, int n, float lr, int step);
 
+// Batched FL-leaf trees (afl -> none, delta -> afl -> none; int32 / int64):
+// one descriptor per column, work items of (column, row range) for the
+// min/max pass and (column, FL tile) for the pack pass.
+struct BatchColumn {
+    const uint8_t* x;  // 16-byte aligned column
+    uint64_t n;        // rows
+    uint32_t es;       // 4 or 8
+    uint32_t delta;    // 1: delta -> afl -> none
+    uint32_t dtype;
+    uint32_t bits;     // written by the layout pass
+    uint64_t min_key, max_key;
+    uint64_t out_off, blob_bytes, data_off, data_bytes;  // written by the layout pass
+};
+struct BatchItem {
+    uint32_t col;
+    uint32_t pad;
+    uint64_t start, end;  // row range, or (tile, unused) for the pack pass
+};
+constexpr uint64_t kFlTileRows = 4096;  // values per FL pack tile
+// The four launches (init, min/max, layout, pack); no host round trip, so
+// the sequence can be captured into a graph.
+void launch_batch_fl(Ctx* c, BatchColumn* d_cols, int ncol, const BatchItem* d_ranges, int nranges,
+                     const BatchItem* d_tiles, int ntiles, uint8_t* arena);
+
 // Zero-fill of device memory by a kernel (graph-capture safe).
 void launch_fill_zero(Ctx* c, void* p, size_t bytes);
 // Gather of several device ranges into one buffer (tree blob assembly).
@@ -139,5 +163,12 @@ struct CopyRange {
     uint64_t bytes;
 };
 void launch_gather_copy(Ctx* c, const CopyRange* d_ranges, int n_ranges, uint8_t* dst, uint64_t max_bytes);
+// Copies of several device ranges, each to its own destination.
+struct CopyPair {
+    const uint8_t* src;
+    uint8_t* dst;
+    uint64_t bytes;
+};
+void launch_copy_pairs(Ctx* c, const CopyPair* d_pairs, int n_pairs, uint64_t max_bytes);
 
 }  // namespace ddj
