@@ -184,6 +184,46 @@ def batched_section(ctx, n_cols: int, min_rows: int, max_rows: int, workers: int
             "ratio": round(raw / out_bytes, 3), "blobs_identical": same}
 
 
+def fl_plumbing_section(ctx):
+    """The reference's CPU-runnable case: 1,000,000 int32 uniform in [0,1024)
+    FL-packed to 10 bits and unpacked, on the GPU and with the reference's
+    own host loops (run_afl_compress_cpu<int,1>, compiled from its sources
+    into oracle/_ref when they were available at build time), one thread."""
+    import ctypes
+    import numpy as np
+    import gputimeseriescompressionoptimizer_amd as g
+    from gputimeseriescompressionoptimizer_amd import datasets as D
+
+    x = D.uniform_int(1_000_000, 1024, seed=0)
+    buf = ctx.upload(x)
+    packed = ctx.fl_pack(buf, g.D_INT, 10, x.size)
+    exact = np.array_equal(ctx.fl_unpack(packed, g.D_INT, 10, x.size).to_numpy(np.int32), x)
+    reps = 200
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        packed = ctx.fl_pack(buf, g.D_INT, 10, x.size)
+    ctx.sync()
+    t_gpu = (time.perf_counter() - t0) / reps
+    out = {"rows": int(x.size), "bits": 10, "gpu_pack_gbps": round(x.nbytes / t_gpu / 1e9, 2), "round_trip_exact": exact}
+    ref = os.path.join(ROOT, "oracle", "_ref", "libddj_ref.so")
+    if os.path.exists(ref):
+        L = ctypes.CDLL(ref)
+        L.ref_fl_pack.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_ulong, ctypes.c_void_p,
+                                  ctypes.c_ulong]
+        words = (10 * x.size + 31) // 32
+        o = np.zeros(words, np.int32)
+        n_ref, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 3.0:
+            L.ref_fl_pack(4, 10, x.ctypes.data, x.size, o.ctypes.data, words)
+            n_ref += 1
+        t_ref = (time.perf_counter() - t0) / n_ref
+        out["reference_cpu_pack_gbps"] = round(x.nbytes / t_ref / 1e9, 3)
+        out["reference_cpu_cores"] = 1
+        out["stream_matches_reference"] = o.tobytes() == packed.to_bytes()
+    return out
+
+
 def learned_planner_section(ctx, blocks: int, rows: int, regime: int, exhaustive_every: int):
     """Learned vs exhaustive planning on a stream of int32 blocks whose
     generator family changes every `regime` blocks (weights trained by
@@ -303,6 +343,7 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
     }
+    res["fl_uniform_1m"] = fl_plumbing_section(ctx)
     if not args.no_float:
         res["float_columns"] = float_section(ctx, args.float_rows, max(1, args.steps // 2))
     if not args.no_batch:

@@ -87,6 +87,8 @@ namespace {
 // The device error flag: 1 = a bounded look-back wait expired, 2 = encoded
 // input failed a device-side check.  Read (and cleared) after every call.
 void check_device_error(Ctx* c) {
+    if (!c->error_flag_armed) return;
+    c->error_flag_armed = false;
     uint32_t err = 0;
     d2h(c, &err, &c->control->error_flag, 4);
     if (!err) return;
@@ -773,7 +775,8 @@ ddj_status ddj_batch_compress(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_c
                         dst = ctx->c.alloc->alloc(blob.n);
                     }
                     d2d(w, dst, blob.p, blob.n);
-                    check_device_error(w);  // synchronises the worker's stream
+                    HIPC(hipStreamSynchronize(w->stream));
+                    check_device_error(w);
                     blobs[i] = ddj_buf{dst, blob.n};
                 } catch (const Error& e) {
                     std::lock_guard<std::mutex> lk(mu);

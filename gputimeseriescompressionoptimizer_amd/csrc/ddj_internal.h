@@ -84,6 +84,9 @@ struct Ctx {
     // returns without waiting; the ring restarts after the stream drains.
     uint8_t* staging = nullptr;
     size_t staging_size = 0, staging_off = 0;
+    // set when a launch may raise the device error flag (bounded waits,
+    // device-side input checks); the C ABI reads the flag back only then
+    bool error_flag_armed = false;
 };
 
 // Adds the algorithmic bytes of a launch to its kernel's tally.

@@ -291,6 +291,7 @@ void launch_int_to_float(Ctx* c, int dtype, const uint8_t* x, size_t n, int prec
 
 LookbackBufs::LookbackBufs(Ctx* c, size_t tiles, bool wide)
     : mem(c, 8 * tiles * (wide ? 2 : 1)) {
+    c->error_flag_armed = true;
     launch_fill_zero(c, mem.get(), mem.size());
     lb = Lookback{mem.get<uint64_t>(), &c->control->error_flag};
 }
