@@ -262,7 +262,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (columns would shard "
-              "across GPUs with a host-side scan of blob sizes; see DESIGN.md)", file=sys.stderr)
+              "across GPUs with a host-side scan of blob sizes)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

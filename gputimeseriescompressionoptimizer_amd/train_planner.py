@@ -11,7 +11,7 @@
    (CompressionOptimizer phase 1, every `--exhaustive-every`-th block), and
    the planning rate of each.
 
-    python -m gputimeseriescompressionoptimizer_amd.train_planner --out-dir gpurun_out/planner
+    python -m gputimeseriescompressionoptimizer_amd.train_planner --out-dir /tmp/planner
 """
 from __future__ import annotations
 
