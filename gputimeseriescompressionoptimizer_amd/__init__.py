@@ -19,6 +19,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 from typing import Iterable, List, Optional, Sequence, Tuple
 
 import numpy as np
@@ -193,14 +194,22 @@ class Context:
         if st:
             raise DdjError(st, f"ddj_ctx_create(device={device})")
         self._h = h
+        self._deps = weakref.WeakSet()  # objects holding native state allocated from this context
 
     def check(self, st: int, what: str):
         if st:
             msg = self._L.ddj_last_error(self._h)
             raise DdjError(st, f"{what}: {msg.decode() if msg else ''}")
 
+    def _track(self, obj) -> None:
+        self._deps.add(obj)
+
     def close(self):
+        """Releases every optimizer, planner and column compressor made from
+        this context, then the context itself."""
         if getattr(self, "_h", None):
+            for d in list(self._deps):
+                d._release()
             self._L.ddj_ctx_destroy(self._h)
             self._h = None
 
@@ -229,11 +238,14 @@ class Context:
                                               ctypes.byref(b)), "kernel_time")
         return ms.value, n.value, b.value
 
-    KERNELS = ["k_fl_pack", "k_fl_unpack", "k_minmax", "k_stats", "k_delta_encode", "k_delta_decode",
-               "k_add_scalar", "k_split", "k_merge", "k_rle_encode", "k_rle_lengths", "k_scan_lengths",
-               "k_rle_expand", "k_hist_count", "k_hist_topk", "k_fill_u64", "k_fill_zero", "k_pack_codes",
-               "k_unpack_codes", "k_gfc_encode", "k_gfc_decode", "k_float_split", "k_float_compose",
-               "k_float_to_int", "k_int_to_float", "k_gather_copy", "k_parse_tree"]
+    KERNELS = ["k_adam", "k_add_scalar", "k_copy_pairs", "k_delta_decode", "k_delta_encode", "k_delta_sums",
+               "k_fill_u64", "k_fill_zero", "k_fl_delta_sums", "k_fl_pack", "k_fl_unpack", "k_fl_unpack_delta_scan",
+               "k_float_compose", "k_float_split", "k_float_to_int", "k_gather_copy", "k_gfc_chunk_bytes",
+               "k_gfc_decode", "k_gfc_encode", "k_gfc_sizes", "k_hist_count", "k_hist_topk", "k_int_to_float",
+               "k_length_sums", "k_merge", "k_minmax", "k_mlp_backward", "k_mlp_forward", "k_pack_codes",
+               "k_parse_tree", "k_rle_count", "k_rle_encode", "k_rle_expand", "k_rle_lengths", "k_rle_tile_runs",
+               "k_scan_lengths", "k_scan_tiles", "k_split", "k_split_count", "k_stats", "k_stencil_count",
+               "k_unpack_codes"]
 
     def kernel_table(self):
         """{kernel: (ms, launches, algorithmic bytes)} for every kernel that ran."""
@@ -395,11 +407,16 @@ class CompressionOptimizer:
         h = ctypes.c_void_p()
         ctx.check(ctx._L.ddj_optimizer_create(ctx._h, ctypes.byref(h)), "optimizer create")
         self._h = h
+        ctx._track(self)
+
+    def _release(self):
+        if self._h and self.ctx._h:
+            self.ctx._L.ddj_optimizer_destroy(self._h)
+        self._h = None
 
     def __del__(self):
         try:
-            if self._h:
-                self.ctx._L.ddj_optimizer_destroy(self._h)
+            self._release()
         except Exception:
             pass
 
@@ -455,11 +472,15 @@ class ParallelTSCompressor:
         self.ctx.check(self.ctx._L.ddj_ts_create(self.ctx._h, len(self.dtypes), arr, self.batch_size,
                                                  ctypes.byref(h)), "ts create")
         self._h = h
+        self.ctx._track(self)
 
     def _free(self):
-        if self._h:
+        if self._h and self.ctx._h:
             self.ctx._L.ddj_ts_destroy(self._h)
-            self._h = None
+        self._h = None
+
+    def _release(self):
+        self._free()
 
     def __del__(self):
         try:
@@ -537,11 +558,16 @@ class LearnedPlanner:
         h = ctypes.c_void_p()
         ctx.check(ctx._L.ddj_planner_create(ctx._h, seed, ctypes.byref(h)), "planner create")
         self._h = h
+        ctx._track(self)
+
+    def _release(self):
+        if self._h and self.ctx._h:
+            self.ctx._L.ddj_planner_destroy(self._h)
+        self._h = None
 
     def __del__(self):
         try:
-            if self._h:
-                self.ctx._L.ddj_planner_destroy(self._h)
+            self._release()
         except Exception:
             pass
 

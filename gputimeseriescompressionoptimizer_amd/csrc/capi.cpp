@@ -84,9 +84,10 @@ struct ddj_optimizer {
 
 namespace {
 
-// The device error flag: 1 = a bounded look-back wait expired, 2 = encoded
-// input failed a device-side check.  Read (and cleared) after every call.
-void check_device_error(Ctx* c) {
+// The device error flag (2 = encoded input failed a device-side check) is
+// zero between calls.  A call whose kernels may raise it arms it; the flag
+// is then read and cleared when the call ends, also when it fails.
+void check_device_error(Ctx* c, bool report = true) {
     if (!c->error_flag_armed) return;
     c->error_flag_armed = false;
     uint32_t err = 0;
@@ -94,8 +95,7 @@ void check_device_error(Ctx* c) {
     if (!err) return;
     const uint32_t zero = 0;
     h2d(c, &c->control->error_flag, &zero, 4);
-    if (err == 2) fail(DDJ_ECORRUPT, "encoded input failed a device-side check");
-    fail(DDJ_ETIMEOUT, "a bounded device-side wait expired");
+    if (report) fail(DDJ_ECORRUPT, "encoded input failed a device-side check");
 }
 
 template <typename F> ddj_status guarded(ddj_ctx* ctx, F&& f) {
@@ -107,6 +107,10 @@ template <typename F> ddj_status guarded(ddj_ctx* ctx, F&& f) {
         return DDJ_OK;
     } catch (const Error& e) {
         ctx->c.last_error = e.msg;
+        try {
+            check_device_error(&ctx->c, false);
+        } catch (const Error&) {
+        }
         return e.status;
     } catch (const std::bad_alloc&) {
         ctx->c.last_error = "host allocation failed";
@@ -117,10 +121,11 @@ template <typename F> ddj_status guarded(ddj_ctx* ctx, F&& f) {
     }
 }
 
-ddj_buf to_buf(Ctx* c, const Part& p) {
+ddj_buf to_buf(Ctx* c, const Part& hp) {
     // hand out a buffer the caller owns: reuse the block when the part owns
     // all of it, otherwise copy
-    if (p.n == 0) return ddj_buf{nullptr, 0};
+    if (hp.n == 0) return ddj_buf{nullptr, 0};
+    const Part p = on_device(c, hp);
     if (p.owner && p.owner.use_count() == 1 && p.owner->bytes_ptr() == p.p) {
         DevBuf& b = *p.owner;
         b.set_size(p.n);

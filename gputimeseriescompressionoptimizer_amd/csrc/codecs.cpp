@@ -23,6 +23,20 @@ Part own(DevBuf&& b) {
     return p;
 }
 
+Part host_part(const void* h, size_t n) {
+    Part p;
+    auto v = std::make_shared<std::vector<uint8_t>>(n);
+    if (n) memcpy(v->data(), h, n);
+    p.host = std::move(v);
+    p.n = n;
+    return p;
+}
+
+Part on_device(Ctx* c, const Part& p) {
+    if (!p.on_host()) return p;
+    return own(make_buf(c, p.host->data(), p.n));
+}
+
 Part borrow(DView v, std::shared_ptr<DevBuf> keep) {
     Part p;
     p.owner = std::move(keep);
@@ -35,7 +49,6 @@ namespace {
 
 Part empty() { return Part{}; }
 
-Part upload(Ctx* c, const void* h, size_t n) { return own(make_buf(c, h, n)); }
 
 template <typename T> T read_dev(Ctx* c, const uint8_t* p) {
     T v;
@@ -88,7 +101,7 @@ std::vector<Part> none_encode(Ctx* c, DView in) {
     size_t len = in.n;
     DevBuf d(c, in.n);
     d2d(c, d.get(), in.p, in.n);
-    return {upload(c, &len, sizeof len), own(std::move(d))};
+    return {host_part(&len, sizeof len), own(std::move(d))};
 }
 
 // ------------------------------------------------------------------ delta
@@ -97,11 +110,10 @@ std::vector<Part> delta_encode(Ctx* c, int dtype, DView in) {
     const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
     if (n == 0) return {empty(), empty()};
     Part x = aligned_view(c, in, es);
-    DevBuf meta(c, es), data(c, (n - 1) * es);
-    d2d(c, meta.get(), x.p, es);
+    DevBuf data(c, (n - 1) * es);
     launch_delta_encode(c, (int)es, x.p, n, data.get());
     kernel_bytes(c, "k_delta_encode", (double)((2 * n - 1) * es));
-    return {own(std::move(meta)), own(std::move(data))};
+    return {borrow(DView{x.p, es}, x.owner), own(std::move(data))};  // metadata: the first row, in place
 }
 Part delta_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     const size_t es = dtype_size(dtype);
@@ -145,7 +157,7 @@ std::vector<Part> scale_encode(Ctx* c, int dtype, DView in) {
         T m = scale_min(mn, mx);
         DevBuf data(c, n * es);
         launch_sub_scalar(c, (int)es, x.p, n, tbits(m), data.get());
-        return {upload(c, &m, sizeof m), own(std::move(data))};
+        return {host_part(&m, sizeof m), own(std::move(data))};
     });
 }
 Part scale_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
@@ -186,7 +198,7 @@ std::vector<Part> afl_encode_int(Ctx* c, int dtype, DView in) {
     kernel_bytes(c, "k_minmax", (double)(n * es));
     kernel_bytes(c, "k_fl_pack", (double)(n * es + bytes));
     uint8_t meta[4] = {(uint8_t)b, (uint8_t)((bytes * 8 - (uint64_t)n * b) & 0xff), 0, 0};
-    return {upload(c, meta, 4), own(std::move(data))};
+    return {host_part(meta, 4), own(std::move(data))};
 }
 
 Part fl_decode_raw(Ctx* c, int dtype, const uint8_t* data, size_t bytes, int b, int rest) {
@@ -242,7 +254,10 @@ std::vector<Part> afl_encode_float(Ctx* c, DView in) {
     DevBuf data(c, a[0].n + a[1].n + b[0].n + b[1].n);
     size_t off = 0;
     for (const Part* q : {&a[0], &a[1], &b[0], &b[1]}) {
-        d2d(c, data.bytes_ptr() + off, q->p, q->n);
+        if (q->on_host())
+            h2d(c, data.bytes_ptr() + off, q->host->data(), q->n);
+        else
+            d2d(c, data.bytes_ptr() + off, q->p, q->n);
         off += q->n;
     }
     return {own(std::move(meta)), own(std::move(data))};
@@ -283,7 +298,7 @@ Part afl_decode_float(Ctx* c, const std::vector<DView>& in) {
 std::vector<Part> rle_encode(Ctx* c, int dtype, DView in) {
     const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
     int32_t R = 0;
-    if (n == 0) return {upload(c, &R, 4), empty(), empty()};
+    if (n == 0) return {host_part(&R, 4), empty(), empty()};
     Part x = aligned_view(c, in, es);
     DevBuf lens(c, 4 * n), vals(c, es * n), runs(c, 8);
     launch_rle_encode(c, dtype, x.p, n, lens.get<int32_t>(), vals.get(), runs.get<uint64_t>());
@@ -293,7 +308,7 @@ std::vector<Part> rle_encode(Ctx* c, int dtype, DView in) {
     kernel_bytes(c, "k_rle_encode", (double)(n * es + r * (4 + es)));
     lens.set_size(4 * r);
     vals.set_size(es * r);
-    return {upload(c, &R, 4), own(std::move(lens)), own(std::move(vals))};
+    return {host_part(&R, 4), own(std::move(lens)), own(std::move(vals))};
 }
 // rle_decode.cu:19-73
 Part rle_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
@@ -519,7 +534,7 @@ std::vector<Part> float_encode(Ctx* c, int dtype, DView in) {
         data = own(std::move(y));
     }
     int32_t p32 = precision;
-    return {upload(c, &p32, 4), data};
+    return {host_part(&p32, 4), data};
 }
 Part float_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     const size_t es = dtype_size(dtype);
@@ -588,8 +603,6 @@ std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in) {
     const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
     if (n < 2) fail(DDJ_EINVAL, "fused delta/FL needs two rows");
     Part x = aligned_view(c, in, es);
-    DevBuf dmeta(c, es);
-    d2d(c, dmeta.get(), x.p, es);
     const size_t m = n - 1;
     DevBuf mmb(c, sizeof(MinMaxResult));
     launch_minmax_delta(c, (int)es, x.p, n, mmb.get<MinMaxResult>());
@@ -609,7 +622,7 @@ std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in) {
     kernel_bytes(c, "k_minmax", (double)(n * es));
     kernel_bytes(c, "k_fl_pack", (double)(n * es + bytes));
     uint8_t meta[4] = {(uint8_t)b, (uint8_t)((bytes * 8 - (uint64_t)m * b) & 0xff), 0, 0};
-    return {own(std::move(dmeta)), upload(c, meta, 4), own(std::move(data))};
+    return {borrow(DView{x.p, es}, x.owner), host_part(meta, 4), own(std::move(data))};
 }
 
 bool enc_ok(int enc) {

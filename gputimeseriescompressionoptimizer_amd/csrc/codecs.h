@@ -11,14 +11,22 @@
 namespace ddj {
 
 // A device byte range that keeps its storage alive.
+// Device bytes, or a few host bytes (metadata computed on the host, which
+// blob assembly uploads together with the node headers instead of one copy
+// per node).  Only encoder metadata (out[0]) is ever host-resident.
 struct Part {
     std::shared_ptr<DevBuf> owner;  // may be null for caller-owned memory
     const uint8_t* p = nullptr;
     size_t n = 0;
+    std::shared_ptr<const std::vector<uint8_t>> host;  // set: the bytes live here, p is null
+    bool on_host() const { return host != nullptr; }
     DView view() const { return DView{p, n}; }
 };
 Part own(DevBuf&& b);
 Part borrow(DView v, std::shared_ptr<DevBuf> keep = nullptr);
+Part host_part(const void* h, size_t n);
+// The part's bytes in device memory (host parts are uploaded).
+Part on_device(Ctx* c, const Part& p);
 
 int n_results(int enc);
 // GetReturnTypes (each encoding's .hpp)

@@ -54,7 +54,7 @@ private:
 };
 
 // Device-side control block: counters that must start at zero for the
-// "last workgroup finishes the reduction" pattern and for look-back scans.
+// "last workgroup finishes the reduction" pattern, and the error flag.
 struct Control {
     uint32_t counters[64];
     uint32_t error_flag;

@@ -111,9 +111,10 @@ template <typename T> __device__ __forceinline__ T wave_inclusive_scan(T v) {
     return v;
 }
 
-// Block-wide exclusive scan of one value per thread (kBlock threads).
+// Block-wide exclusive scan of one value per thread (NT threads).
 // Returns the exclusive prefix; *total receives the block sum.
-template <typename T> __device__ __forceinline__ T block_exclusive_scan(T v, T* total, T* smem /* kBlock/kWave */) {
+template <typename T, int NT = kBlock>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* total, T* smem /* NT/kWave */) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = threadIdx.x / kWave;
     T inc = wave_inclusive_scan(v);
@@ -121,7 +122,7 @@ template <typename T> __device__ __forceinline__ T block_exclusive_scan(T v, T* 
     __syncthreads();
     T wave_off = 0, tot = 0;
 #pragma unroll
-    for (int w = 0; w < kBlock / kWave; w++) {
+    for (int w = 0; w < NT / kWave; w++) {
         T s = smem[w];
         if (w < wid) wave_off += s;
         tot += s;
@@ -131,157 +132,14 @@ template <typename T> __device__ __forceinline__ T block_exclusive_scan(T v, T* 
     return wave_off + inc - v;
 }
 
-// ------------------------------------------------ decoupled look-back scan
-// Tiles publish their aggregate and add up the predecessors' values until
-// one with an inclusive prefix is found.
-//
-// A tile's state and value travel together in one 64-bit status word
-// (bits 63..62 state, 61..0 value), so a relaxed agent-scope atomic load sees
-// a consistent pair and no acquire/release fence is needed: such fences make
-// the compiler write back and invalidate the whole L2 of the XCD
-// (buffer_wbl2 / buffer_inv), which stalls every tile of a streaming kernel.
-// Wide values (full 64-bit wrapping sums) use two status words carrying the
-// 32-bit halves; a pair is taken only when both words show the same state.
-//
-// One wave inspects 64 predecessors per step.  Every wait is bounded: on
-// expiry the error flag is raised and the missing prefix counts as zero, so
-// a kernel always finishes.
-constexpr uint64_t kStAggregate = 1ull << 62, kStInclusive = 2ull << 62;
-constexpr uint64_t kStValue = (1ull << 62) - 1;
-constexpr uint32_t kSpinLimit = 1u << 22;
-
-__device__ __forceinline__ uint64_t ld_status(uint64_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_status(uint64_t* p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <bool WIDE> __device__ __forceinline__ void lb_publish(const Lookback& lb, uint32_t tile, uint64_t state,
-                                                                uint64_t v) {
-    if constexpr (WIDE) {
-        st_status(&lb.status[2 * (size_t)tile], state | (v & 0xffffffffull));
-        st_status(&lb.status[2 * (size_t)tile + 1], state | (v >> 32));
-    } else {
-        st_status(&lb.status[tile], state | (v & kStValue));
-    }
-}
-
-// Reads predecessor `idx`; returns false while it has published nothing.
-template <bool WIDE> __device__ __forceinline__ bool lb_read(const Lookback& lb, size_t idx, uint64_t& state,
-                                                             uint64_t& v) {
-    if constexpr (WIDE) {
-        const uint64_t a = ld_status(&lb.status[2 * idx]);
-        const uint64_t b = ld_status(&lb.status[2 * idx + 1]);
-        state = a & ~kStValue;
-        v = (b << 32) | (a & 0xffffffffull);
-        return state != 0 && state == (b & ~kStValue);
-    } else {
-        const uint64_t a = ld_status(&lb.status[idx]);
-        state = a & ~kStValue;
-        v = a & kStValue;
-        return state != 0;
-    }
-}
-
-// Exclusive prefix of `tile`, resolved by the whole workgroup (kBlock
-// threads, all of which call it with the same tile and aggregate).  Each
-// step inspects the kLbWindow = kBlock * kLbPerThread predecessors before the
-// current window (thread t, entry j: tile end - 1 - t - kBlock * j), all
-// loads in flight together.  The window must be wide: a tile only turns
-// inclusive after its own look-back, so in steady state the nearest
-// inclusive predecessor lies about (tiles started per microsecond) x (one
-// device round trip) tiles back, and tiles resolve at most one window per
-// round trip.  Narrow values must stay below 2^62 (counts), or only their low
-// bits matter (sums that wrap at 32 bits or fewer).
-constexpr int kLbPerThread = 8;
-constexpr int kLbWindow = kBlock * kLbPerThread;
-
-template <bool WIDE> __device__ uint64_t block_lookback(const Lookback& lb, uint32_t tile, uint64_t aggregate) {
-    __shared__ uint64_t red[kBlock / kWave];
-    __shared__ uint32_t first_inc[kBlock / kWave];
-    const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
-    if (tile == 0) {
-        if (t == 0) lb_publish<WIDE>(lb, 0, kStInclusive, aggregate);
-        __syncthreads();  // the call is a barrier on every path
-        return 0;
-    }
-    if (t == 0) lb_publish<WIDE>(lb, tile, kStAggregate, aggregate);
-    uint64_t excl = 0;
-    int64_t end = tile;  // window [end - kLbWindow, end)
-    for (;;) {
-        uint64_t state[kLbPerThread], v[kLbPerThread];
-        bool ready[kLbPerThread];
-#pragma unroll
-        for (int j = 0; j < kLbPerThread; j++) {
-            state[j] = kStInclusive;  // before tile 0: an inclusive zero
-            v[j] = 0;
-            ready[j] = end - 1 - t - (int64_t)kBlock * j < 0;
-        }
-        uint32_t spins = 0;
-        for (;;) {
-            bool all = true;
-#pragma unroll
-            for (int j = 0; j < kLbPerThread; j++) {
-                if (!ready[j]) ready[j] = lb_read<WIDE>(lb, (size_t)(end - 1 - t - (int64_t)kBlock * j), state[j], v[j]);
-                all = all && ready[j];
-            }
-            if (__syncthreads_and(all)) break;
-            if (++spins > kSpinLimit) {
-#pragma unroll
-                for (int j = 0; j < kLbPerThread; j++)
-                    if (!ready[j]) {
-                        __hip_atomic_store(lb.error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        state[j] = kStInclusive;
-                        v[j] = 0;
-                    }
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        // the nearest inclusive predecessor has the smallest distance t + kBlock * j
-        uint32_t mine = 0xffffffffu;
-#pragma unroll
-        for (int j = kLbPerThread - 1; j >= 0; j--)
-            if (state[j] == kStInclusive) mine = (uint32_t)(t + kBlock * j);
-        uint32_t wmin = mine;
-#pragma unroll
-        for (int o = kWave / 2; o > 0; o >>= 1) {
-            const uint32_t w = __shfl_xor(wmin, o, kWave);
-            wmin = w < wmin ? w : wmin;
-        }
-        if (lane == 0) first_inc[wid] = wmin;
-        __syncthreads();
-        uint32_t first = 0xffffffffu;
-#pragma unroll
-        for (int w = 0; w < kBlock / kWave; w++) first = first_inc[w] < first ? first_inc[w] : first;
-        uint64_t part = 0;
-#pragma unroll
-        for (int j = 0; j < kLbPerThread; j++)
-            if ((uint32_t)(t + kBlock * j) <= first) part += v[j];
-        const uint64_t ws = wave_sum<uint64_t>(part);
-        if (lane == 0) red[wid] = ws;
-        __syncthreads();
-        uint64_t sum = 0;
-#pragma unroll
-        for (int w = 0; w < kBlock / kWave; w++) sum += red[w];
-        excl += sum;
-        __syncthreads();  // red / first_inc are rewritten by the next step
-        if (first != 0xffffffffu) break;
-        end -= kLbWindow;
-    }
-    if (t == 0) lb_publish<WIDE>(lb, tile, kStInclusive, excl + aggregate);
-    return excl;
-}
-
-// Tile of the calling workgroup.  Workgroups are dispatched in increasing
-// blockIdx order on every XCD (round-robin over the XCDs), so the lowest
-// unfinished tile is always resident and the look-back always progresses;
-// the bounded waits cover anything else.  A ticket counter (tiles numbered
-// in start order) would cost one device-scope atomic on a single address per
-// tile, and those serialise at roughly 15-20 ns each: 16K tiles of a 64M-row
-// column would spend ~300 us there.
-__device__ __forceinline__ uint32_t tile_index() { return blockIdx.x; }
+// ------------------------------------------------------ reduce, then scan
+// Variable-length outputs (split, merge, RLE, delta decode, GFC) take their
+// tile offsets from two cheap launches ahead of the output pass: a counting
+// pass writes one value per tile, k_scan_tiles (one workgroup) turns them
+// into exclusive prefixes, and the output pass reads prefix[tile].  No
+// workgroup waits on another, so nothing spins.
+constexpr int kScanThreads = 1024;
+constexpr int kScanPer = 16;  // values per thread per step of k_scan_tiles
 
 // Relaxed agent-scope stores and loads for values that one workgroup
 // publishes and another reads within the same launch.

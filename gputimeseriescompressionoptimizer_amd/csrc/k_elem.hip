@@ -169,36 +169,74 @@ __global__ __launch_bounds__(kBlock) void k_copy_pairs(const CopyPair* __restric
 
 // ------------------------------------------------------------- delta decode
 // out[0] = x0; out[i+1] = x0 + d[0] + ... + d[i] (wrapping).  Tiles of 4096
-// deltas; each thread sums 16 consecutive deltas staged through LDS, the
-// workgroup scans, and the tile prefix comes from the decoupled look-back.
+// deltas: k_delta_sums writes each tile's sum, k_scan_tiles their prefixes,
+// and k_delta_decode scans its tile on top of that prefix.  Sums run over
+// the zero-extended bits of T in 64 bits; only their low 8*sizeof(T) bits
+// are kept, which is the wrapping sum in T.
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kBlock * kScanItems;
 
+// Row i of a T stream at any byte address.
+template <typename T> __device__ __forceinline__ T load_row(const uint8_t* p, size_t i) {
+    T t;
+    __builtin_memcpy(&t, p + i * sizeof(T), sizeof(T));
+    return t;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_delta_sums(const uint8_t* __restrict__ din, size_t m,
+                                                       uint64_t* __restrict__ vals) {
+    __shared__ uint64_t red[kBlock / kWave];
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    const int cnt = (int)((m - base) < (size_t)kScanTile ? (m - base) : (size_t)kScanTile);
+    const uint8_t* tp = din + base * sizeof(T);
+    uint64_t s = 0;
+    if (cnt == kScanTile && (reinterpret_cast<uintptr_t>(tp) & 15) == 0) {
+        constexpr int V = 16 / sizeof(T);
+        const u32x4* s4 = reinterpret_cast<const u32x4*>(tp);
+#pragma unroll
+        for (int q = 0; q < kScanTile / V / kBlock; q++) {
+            const u32x4 w = __builtin_nontemporal_load(s4 + threadIdx.x + kBlock * q);
+            T v[V];
+            __builtin_memcpy(v, &w, 16);
+#pragma unroll
+            for (int k = 0; k < V; k++) s += ubits(v[k]);
+        }
+    } else {
+        for (int i = threadIdx.x; i < cnt; i += kBlock) s += ubits(load_row<T>(tp, i));
+    }
+    uint64_t total;
+    block_exclusive_scan<uint64_t>(s, &total, red);
+    if (threadIdx.x == 0) vals[blockIdx.x] = total;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_delta_decode(const uint8_t* first_bytes, const uint8_t* __restrict__ din,
-                                                         size_t m, T* __restrict__ out, Lookback lb) {
+                                                         size_t m, T* __restrict__ out,
+                                                         const uint64_t* __restrict__ prefix) {
     T first;
     __builtin_memcpy(&first, first_bytes, sizeof(T));
     __shared__ uint64_t buf[kScanTile + kScanTile / 32];
     __shared__ uint64_t red[kBlock / kWave];
-    const uint32_t tile = tile_index();
+    const uint32_t tile = blockIdx.x;
     const size_t base = (size_t)tile * kScanTile;
     const int cnt = (int)((m - base) < (size_t)kScanTile ? (m - base) : (size_t)kScanTile);
     auto pad = [](int i) { return i + (i >> 5); };
-    // striped, coalesced load (input may sit at any byte offset)
-    const bool aligned = (reinterpret_cast<uintptr_t>(din) % sizeof(T)) == 0;
-    const T* dt = reinterpret_cast<const T*>(din);
-    for (int i = threadIdx.x; i < kScanTile; i += kBlock) {
-        uint64_t v = 0;
-        if (i < cnt) {
-            T t;
-            if (aligned)
-                t = dt[base + i];
-            else
-                __builtin_memcpy(&t, din + (base + i) * sizeof(T), sizeof(T));
-            v = ubits(t);
+    const uint8_t* tp = din + base * sizeof(T);
+    if (cnt == kScanTile && (reinterpret_cast<uintptr_t>(tp) & 15) == 0) {
+        constexpr int V = 16 / sizeof(T);
+        const u32x4* s4 = reinterpret_cast<const u32x4*>(tp);
+#pragma unroll
+        for (int q = 0; q < kScanTile / V / kBlock; q++) {
+            const int i = threadIdx.x + kBlock * q;
+            const u32x4 w = __builtin_nontemporal_load(s4 + i);
+            T v[V];
+            __builtin_memcpy(v, &w, 16);
+#pragma unroll
+            for (int k = 0; k < V; k++) buf[pad(V * i + k)] = ubits(v[k]);
         }
-        buf[pad(i)] = v;
+    } else {
+        for (int i = threadIdx.x; i < kScanTile; i += kBlock) buf[pad(i)] = i < cnt ? ubits(load_row<T>(tp, i)) : 0;
     }
     __syncthreads();
     uint64_t local[kScanItems];
@@ -209,10 +247,8 @@ __global__ __launch_bounds__(kBlock) void k_delta_decode(const uint8_t* first_by
         sum += local[k];
     }
     uint64_t total;
-    uint64_t excl = block_exclusive_scan<uint64_t>(sum, &total, red);
-    // sums wrap at the width of T: 8-byte types need the wide status words
-    const uint64_t prefix = block_lookback<sizeof(T) == 8>(lb, tile, total);
-    uint64_t run = ubits(first) + prefix + excl;
+    const uint64_t excl = block_exclusive_scan<uint64_t>(sum, &total, red);  // a barrier: buf is free again
+    uint64_t run = ubits(first) + prefix[tile] + excl;
 #pragma unroll
     for (int k = 0; k < kScanItems; k++) {
         run += local[k];
@@ -221,6 +257,36 @@ __global__ __launch_bounds__(kBlock) void k_delta_decode(const uint8_t* first_by
     __syncthreads();
     for (int i = threadIdx.x; i < cnt; i += kBlock) out[base + i + 1] = (T)buf[pad(i)];
     if (tile == 0 && threadIdx.x == 0) out[0] = first;
+}
+
+// Exclusive prefixes of per-tile values; one workgroup of kScanThreads, each
+// thread taking kScanPer consecutive values per step.
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const uint64_t* __restrict__ vals, size_t tiles,
+                                                             uint64_t* __restrict__ prefix) {
+    __shared__ uint64_t red[kScanThreads / kWave];
+    uint64_t carry = 0;
+    for (size_t c0 = 0; c0 < tiles; c0 += (size_t)kScanThreads * kScanPer) {  // uniform
+        const size_t b = c0 + (size_t)threadIdx.x * kScanPer;
+        uint64_t v[kScanPer], s = 0;
+        if (b + kScanPer <= tiles) {
+#pragma unroll
+            for (int k = 0; k < kScanPer; k++) v[k] = vals[b + k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < kScanPer; k++) v[k] = b + k < tiles ? vals[b + k] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < kScanPer; k++) s += v[k];
+        uint64_t total;
+        uint64_t run = carry + block_exclusive_scan<uint64_t, kScanThreads>(s, &total, red);
+#pragma unroll
+        for (int k = 0; k < kScanPer; k++) {
+            if (b + k < tiles) prefix[b + k] = run;
+            run += v[k];
+        }
+        carry += total;
+    }
+    if (threadIdx.x == 0) prefix[tiles] = carry;
 }
 
 }  // namespace
@@ -289,12 +355,18 @@ void launch_int_to_float(Ctx* c, int dtype, const uint8_t* x, size_t n, int prec
     });
 }
 
-LookbackBufs::LookbackBufs(Ctx* c, size_t tiles, bool wide)
-    : mem(c, 8 * tiles * (wide ? 2 : 1)) {
-    c->error_flag_armed = true;
-    launch_fill_zero(c, mem.get(), mem.size());
-    lb = Lookback{mem.get<uint64_t>(), &c->control->error_flag};
+TileScan::TileScan(Ctx* c, size_t n_tiles) : mem(c, 8 * (2 * n_tiles + 1)), tiles(n_tiles) {
+    vals = mem.get<uint64_t>();
+    prefix = vals + n_tiles;
 }
+
+void TileScan::scan(Ctx* c) {
+    kernel_bytes(c, "k_scan_tiles", 8.0 * (double)(2 * tiles + 1));
+    { KernelTimer kt_(c, "k_scan_tiles"); hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, c->stream, vals, tiles, prefix); }
+    HIPC(hipGetLastError());
+}
+
+void arm_error_flag(Ctx* c) { c->error_flag_armed = true; }
 
 void launch_fill_zero(Ctx* c, void* p, size_t bytes) {
     if (!bytes) return;
@@ -328,9 +400,13 @@ void launch_delta_decode(Ctx* c, int elem_size, const void* first, const uint8_t
             return;
         }
         const size_t tiles = (m + kScanTile - 1) / kScanTile;
-        LookbackBufs lbb(c, tiles, sizeof(T) == 8);
+        TileScan ts(c, tiles);
+        kernel_bytes(c, "k_delta_sums", (double)m * sizeof(T));
+        { KernelTimer kt_(c, "k_delta_sums"); hipLaunchKernelGGL(k_delta_sums<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, d, m, ts.vals); }
+        HIPC(hipGetLastError());
+        ts.scan(c);
         { KernelTimer kt_(c, "k_delta_decode"); hipLaunchKernelGGL(k_delta_decode<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                           static_cast<const uint8_t*>(first), d, m, static_cast<T*>(out), lbb.lb); }
+                           static_cast<const uint8_t*>(first), d, m, static_cast<T*>(out), (const uint64_t*)ts.prefix); }
         HIPC(hipGetLastError());
     });
 }

@@ -88,20 +88,39 @@ __global__ __launch_bounds__(kBlock) void k_fl_unpack(const uint8_t* __restrict_
     for (int k = threadIdx.x; k < nv; k += kBlock) out[v0 + k] = (T)fl_value(w, k, b, a, mask);
 }
 
-// DeltaEncoding::Decode over AflEncoding::Decode in one pass
-// (delta_encoding.cu:74-102 on afl_encoding.cu:74-113): the m b-bit deltas
-// are unpacked and prefix-summed (wrapping in T) on top of the first row,
-// without writing the deltas.  out has m + 1 rows.
+// DeltaEncoding::Decode over AflEncoding::Decode without writing the deltas
+// (delta_encoding.cu:74-102 on afl_encoding.cu:74-113): k_fl_delta_sums
+// unpacks each tile of 4096 b-bit deltas and writes their sum, k_scan_tiles
+// turns the sums into tile prefixes, and k_fl_unpack_delta_scan unpacks the
+// tile again and prefix-sums it (wrapping in T) on top of the first row and
+// its prefix.  The packed stream is b/8 bytes a row, so reading it twice
+// costs far less than writing and re-reading the deltas.  out has m + 1 rows.
+__global__ __launch_bounds__(kBlock) void k_fl_delta_sums(const uint8_t* __restrict__ in, size_t in_bytes, size_t m,
+                                                          int b, uint64_t* __restrict__ vals) {
+    __shared__ uint32_t w[kMaxWords];
+    __shared__ uint64_t red[kBlock / kWave];
+    const size_t v0 = (size_t)blockIdx.x * kFlTile;
+    const int nv = (int)((m - v0) < (size_t)kFlTile ? (m - v0) : (size_t)kFlTile);
+    const int a = stage_fl_words(in, in_bytes, v0, nv, b, w);
+    __syncthreads();
+    const uint64_t mask = lowmask64(b);
+    uint64_t sum = 0;
+    for (int k = threadIdx.x; k < nv; k += kBlock) sum += fl_value(w, k, b, a, mask);
+    uint64_t total;
+    block_exclusive_scan<uint64_t>(sum, &total, red);
+    if (threadIdx.x == 0) vals[blockIdx.x] = total;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fl_unpack_delta_scan(const uint8_t* first_bytes, const uint8_t* __restrict__ in,
                                                                  size_t in_bytes, size_t m, int b, T* __restrict__ out,
-                                                                 Lookback lb) {
+                                                                 const uint64_t* __restrict__ prefix) {
     __shared__ uint32_t w[kMaxWords];  // the staged words, then the tile's results
     __shared__ uint64_t red[kBlock / kWave];
     constexpr int kPer = kFlTile / kBlock;  // 16 consecutive values per thread
     T first;
     __builtin_memcpy(&first, first_bytes, sizeof(T));
-    const uint32_t tile = tile_index();
+    const uint32_t tile = blockIdx.x;
     const size_t v0 = (size_t)tile * kFlTile;
     const int nv = (int)((m - v0) < (size_t)kFlTile ? (m - v0) : (size_t)kFlTile);
     const int a = stage_fl_words(in, in_bytes, v0, nv, b, w);
@@ -115,10 +134,9 @@ __global__ __launch_bounds__(kBlock) void k_fl_unpack_delta_scan(const uint8_t* 
         sum += d[k];
     }
     uint64_t total;
-    const uint64_t excl = block_exclusive_scan<uint64_t>(sum, &total, red);
-    const uint64_t prefix = block_lookback<sizeof(T) == 8>(lb, tile, total);  // a barrier: w is free again
+    const uint64_t excl = block_exclusive_scan<uint64_t>(sum, &total, red);  // a barrier: w is free again
     T* res = reinterpret_cast<T*>(w);
-    uint64_t run = ubits(first) + prefix + excl;
+    uint64_t run = ubits(first) + prefix[tile] + excl;
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
         run += d[k];
@@ -171,9 +189,14 @@ void launch_fl_unpack_delta(Ctx* c, int elem_size, const uint8_t* first, const u
             return;
         }
         const size_t tiles = (m + kFlTile - 1) / kFlTile;
-        LookbackBufs lbb(c, tiles, sizeof(T) == 8);
+        TileScan ts(c, tiles);
+        kernel_bytes(c, "k_fl_delta_sums", (double)((m * (uint64_t)bits + 7) / 8));
+        { KernelTimer kt_(c, "k_fl_delta_sums"); hipLaunchKernelGGL(k_fl_delta_sums, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                           in, in_bytes, m, bits, ts.vals); }
+        HIPC(hipGetLastError());
+        ts.scan(c);
         { KernelTimer kt_(c, "k_fl_unpack_delta_scan"); hipLaunchKernelGGL(k_fl_unpack_delta_scan<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                           first, in, in_bytes, m, bits, static_cast<T*>(out), lbb.lb); }
+                           first, in, in_bytes, m, bits, static_cast<T*>(out), (const uint64_t*)ts.prefix); }
         HIPC(hipGetLastError());
     });
 }

@@ -9,12 +9,14 @@
 // nb bytes of |diff|.  out[1] holds, per chunk, the end offset in the
 // reference's scratch layout; out[2] is the concatenated chunk payloads.
 //
-// MI355X mapping: one workgroup per chunk (<= 2048 rows).  Encode computes
-// all byte counts, scans them in the workgroup, assembles the chunk payload
-// in LDS and gets its output position from a decoupled look-back over chunk
-// sizes.  Decode stages the chunk payload in LDS, walks the subchunk headers
-// with one wave (the only sequential step of the format) and then decodes
-// all rows in parallel.
+// MI355X mapping: one workgroup per chunk (<= 2048 rows).  A size pass
+// computes every chunk's byte count (and the reference's offsets table),
+// k_scan_tiles places the chunks, and the encode pass recomputes the byte
+// counts, scans them in the workgroup, assembles the chunk payload in LDS and
+// writes it at its place.  Decode takes the chunk sizes from the offsets
+// table (scanned the same way), stages the chunk payload in LDS, walks the
+// subchunk headers with one wave (the only sequential step of the format)
+// and then decodes all rows in parallel.
 #include "dispatch.h"
 #include "k_common.h"
 #include "kernels.h"
@@ -43,36 +45,29 @@ GfcGeom gfc_geom(bool dbl, long long n) {
     return g;
 }
 
+constexpr int kGfcRpt = kGfcMaxPer / kBlock;  // rows per thread (8)
+
+// Thread t's rows [8t, 8t+8) of the chunk starting at row c0: |diff| against
+// the predictor, byte counts and the sign|code nibbles.
+template <typename U> struct GfcRows {
+    U d[kGfcRpt];
+    int nbs[kGfcRpt];
+    uint32_t nibs = 0;   // 4 bits per row
+    uint32_t nbsum = 0;  // residual bytes of the thread's rows
+};
+
 template <typename U>
-__global__ __launch_bounds__(kBlock) void k_gfc_encode(const U* __restrict__ x, long long n, long long N, long long per,
-                                                       int S, uint8_t* __restrict__ payload,
-                                                       int32_t* __restrict__ offsets, Lookback lb,
-                                                       uint64_t* d_total, uint32_t n_chunks) {
-    constexpr int RPT = kGfcMaxPer / kBlock;  // rows per thread (8)
+__device__ __forceinline__ void gfc_rows(const U* __restrict__ x, long long n, long long c0, int rows, U last_c,
+                                         GfcRows<U>& g) {
     constexpr bool dbl = sizeof(U) == 8;
-    __shared__ __attribute__((aligned(16))) uint8_t out[kGfcMaxPer / 32 * (16 + 32 * (int)sizeof(U)) + 16];
-    __shared__ U s_last;
-    __shared__ uint64_t red[kBlock / kWave];
-    const uint32_t chunk = tile_index();
-    const long long c0 = (long long)chunk * per;
-    long long c1 = c0 + per;
-    c1 = c1 < N ? c1 : N;
-    const int rows = c0 < N ? (int)(c1 - c0) : 0;
-    if (threadIdx.x == 0) s_last = (rows > 0 && c0 + 31 < n) ? x[c0 + 31] : U(0);
-    __syncthreads();
-    const U last_c = s_last;
-    U d[RPT];
-    uint32_t nibs = 0;  // 4 bits per row
-    uint32_t nbsum = 0;
-    int nbs[RPT];
 #pragma unroll
-    for (int k = 0; k < RPT; k++) {
-        const int r = threadIdx.x * RPT + k;
-        nbs[k] = 0;
-        d[k] = 0;
+    for (int k = 0; k < kGfcRpt; k++) {
+        const int r = threadIdx.x * kGfcRpt + k;
+        g.nbs[k] = 0;
+        g.d[k] = 0;
         if (r < rows) {
-            const long long g = c0 + r;
-            const U v = g < n ? x[g] : U(0);
+            const long long gi = c0 + r;
+            const U v = gi < n ? x[gi] : U(0);
             const U last = (r < 32) ? U(0) : last_c;
             U diff = (U)(v - last);
             const uint32_t sign = (uint32_t)(diff >> (8 * sizeof(U) - 1)) & 1u;
@@ -87,12 +82,71 @@ __global__ __launch_bounds__(kBlock) void k_gfc_encode(const U* __restrict__ x, 
                 nb = 4 - z;
                 code = nb;
             }
-            nbs[k] = nb;
-            d[k] = diff;
-            nibs |= (uint32_t)((sign << 3) | code) << (4 * k);
-            nbsum += nb;
+            g.nbs[k] = nb;
+            g.d[k] = diff;
+            g.nibs |= (uint32_t)((sign << 3) | code) << (4 * k);
+            g.nbsum += nb;
         }
     }
+}
+
+// The chunk's rows and its predictor (row 31 of the chunk).
+struct GfcChunk {
+    long long c0;
+    int rows;
+};
+__device__ __forceinline__ GfcChunk gfc_chunk(uint32_t chunk, long long N, long long per) {
+    const long long c0 = (long long)chunk * per;
+    long long c1 = c0 + per;
+    c1 = c1 < N ? c1 : N;
+    return GfcChunk{c0, c0 < N ? (int)(c1 - c0) : 0};
+}
+
+// The reference's offsets-table entry: the chunk's end in its scratch layout.
+__device__ __forceinline__ long long gfc_scratch_start(long long c0, long long N, int S) {
+    return (c0 < N ? (c0 + 1) / 2 : (N + 1) / 2) * (S + 1);
+}
+
+// Size pass: payload bytes of every chunk, and the offsets table.
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_gfc_sizes(const U* __restrict__ x, long long n, long long N, long long per,
+                                                      int S, uint64_t* __restrict__ vals,
+                                                      int32_t* __restrict__ offsets) {
+    __shared__ U s_last;
+    __shared__ uint64_t red[kBlock / kWave];
+    const uint32_t chunk = blockIdx.x;
+    const GfcChunk ch = gfc_chunk(chunk, N, per);
+    if (threadIdx.x == 0) s_last = (ch.rows > 0 && ch.c0 + 31 < n) ? x[ch.c0 + 31] : U(0);
+    __syncthreads();
+    GfcRows<U> g;
+    gfc_rows(x, n, ch.c0, ch.rows, s_last, g);
+    uint64_t total_nb;
+    block_exclusive_scan<uint64_t>(g.nbsum, &total_nb, red);
+    const uint64_t chunk_bytes = 16ull * (ch.rows / 32) + total_nb;
+    if (threadIdx.x == 0) {
+        vals[chunk] = chunk_bytes;
+        offsets[chunk] = (int32_t)(gfc_scratch_start(ch.c0, N, S) + (long long)chunk_bytes);
+    }
+}
+
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_gfc_encode(const U* __restrict__ x, long long n, long long N, long long per,
+                                                       uint8_t* __restrict__ payload,
+                                                       const uint64_t* __restrict__ prefix) {
+    constexpr int RPT = kGfcRpt;
+    __shared__ __attribute__((aligned(16))) uint8_t out[kGfcMaxPer / 32 * (16 + 32 * (int)sizeof(U)) + 16];
+    __shared__ U s_last;
+    __shared__ uint64_t red[kBlock / kWave];
+    const uint32_t chunk = blockIdx.x;
+    const GfcChunk ch = gfc_chunk(chunk, N, per);
+    const int rows = ch.rows;
+    if (threadIdx.x == 0) s_last = (rows > 0 && ch.c0 + 31 < n) ? x[ch.c0 + 31] : U(0);
+    __syncthreads();
+    GfcRows<U> g;
+    gfc_rows(x, n, ch.c0, rows, s_last, g);
+    const uint32_t nibs = g.nibs, nbsum = g.nbsum;
+    const int* nbs = g.nbs;
+    const U* d = g.d;
     uint64_t total_nb;
     const uint64_t excl = block_exclusive_scan<uint64_t>(nbsum, &total_nb, red);
     const int nsub = rows / 32;
@@ -118,20 +172,36 @@ __global__ __launch_bounds__(kBlock) void k_gfc_encode(const U* __restrict__ x, 
             pos += nbs[k];
         }
     }
-    const uint64_t s_prefix = block_lookback<false>(lb, chunk, chunk_bytes);
-    const uint64_t O = s_prefix;
+    __syncthreads();
+    const uint64_t O = prefix[chunk];
     for (uint32_t i = threadIdx.x; i < (uint32_t)chunk_bytes; i += kBlock) payload[O + i] = out[i];
-    if (threadIdx.x == 0) {
-        offsets[chunk] = (int32_t)((c0 < N ? (c0 + 1) / 2 : (N + 1) / 2) * (S + 1) + (long long)chunk_bytes);
-        if (chunk == n_chunks - 1) *d_total = O + chunk_bytes;
-    }
+}
+
+// Payload bytes of a chunk from the offsets table; out-of-range entries
+// (corrupt input) count as empty chunks and are caught by the header walk.
+template <typename U>
+__device__ __forceinline__ uint64_t gfc_chunk_bytes(const uint8_t* offsets, uint32_t chunk, long long N, long long per,
+                                                    int S) {
+    constexpr int kMaxBytes = kGfcMaxPer / 32 * (16 + 32 * (int)sizeof(U));
+    int32_t end;
+    __builtin_memcpy(&end, offsets + 4 * (size_t)chunk, 4);
+    const long long cb = (long long)end - gfc_scratch_start((long long)chunk * per, N, S);
+    return (cb < 0 || cb > kMaxBytes) ? 0 : (uint64_t)cb;
+}
+
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_gfc_chunk_bytes(const uint8_t* __restrict__ offsets, size_t chunks,
+                                                            long long N, long long per, int S,
+                                                            uint64_t* __restrict__ vals) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < chunks) vals[i] = gfc_chunk_bytes<U>(offsets, (uint32_t)i, N, per, S);
 }
 
 template <typename U>
 __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict__ payload, size_t payload_bytes,
                                                        const uint8_t* __restrict__ offsets, long long n, long long N,
                                                        long long per, int S, U* __restrict__ out,
-                                                       Lookback lb) {
+                                                       const uint64_t* __restrict__ prefix, uint32_t* error) {
     constexpr int RPT = kGfcMaxPer / kBlock;
     constexpr bool dbl = sizeof(U) == 8;
     constexpr int kMaxBytes = kGfcMaxPer / 32 * (16 + 32 * (int)sizeof(U));
@@ -140,21 +210,14 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     __shared__ U vals[kGfcMaxPer];
     __shared__ U s_last;
     __shared__ int s_bad;
-    const uint32_t chunk = tile_index();
+    const uint32_t chunk = blockIdx.x;
     const long long c0 = (long long)chunk * per;
-    long long c1 = c0 + per;
-    c1 = c1 < N ? c1 : N;
-    const int rows = c0 < N ? (int)(c1 - c0) : 0;
+    const int rows = gfc_chunk(chunk, N, per).rows;
     const int nsub = rows / 32;
-    int32_t end;
-    __builtin_memcpy(&end, offsets + 4 * (size_t)chunk, 4);
-    const long long start_scr = (c0 < N ? (c0 + 1) / 2 : (N + 1) / 2) * (S + 1);
-    long long cb = (long long)end - start_scr;
-    if (cb < 0 || cb > kMaxBytes) cb = 0;  // corrupt: decode as empty (the host validates totals)
-    const uint64_t chunk_bytes = (uint64_t)cb;
+    const uint64_t chunk_bytes = gfc_chunk_bytes<U>(offsets, chunk, N, per, S);
     if (threadIdx.x == 0) s_bad = 0;
-    const uint64_t s_prefix = block_lookback<false>(lb, chunk, chunk_bytes);
-    const uint64_t P = s_prefix;
+    __syncthreads();
+    const uint64_t P = prefix[chunk];
     for (uint32_t i = threadIdx.x; i < (uint32_t)chunk_bytes; i += kBlock)
         buf[i] = (P + i < payload_bytes) ? payload[P + i] : 0;
     __syncthreads();
@@ -219,7 +282,7 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     __syncthreads();
     for (int i = threadIdx.x; i < rows; i += kBlock)
         if (c0 + i < n) out[c0 + i] = vals[i];
-    if (threadIdx.x == 0 && s_bad) __hip_atomic_store(lb.error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0 && s_bad) __hip_atomic_store(error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 }  // namespace
@@ -227,25 +290,32 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
 void gfc_encode(Ctx* c, bool dbl, const void* x, size_t n, DevBuf& meta, DevBuf& offsets, DevBuf& payload) {
     const GfcGeom g = gfc_geom(dbl, (long long)n);
     const size_t tiles = (size_t)g.Wc;
-    LookbackBufs lbb(c, tiles);
-    DevBuf total(c, 8);
+    TileScan ts(c, tiles);
     offsets = DevBuf(c, 4 * tiles);
     const size_t worst = (size_t)(g.N / 32) * (16 + 32 * (dbl ? 8 : 4));
     payload = DevBuf(c, worst);
+    auto run = [&](auto tag) {
+        using U = decltype(tag);
+        const U* xs = static_cast<const U*>(x);
+        kernel_bytes(c, "k_gfc_sizes", (double)n * sizeof(U) + 12.0 * (double)tiles);
+        { KernelTimer kt_(c, "k_gfc_sizes"); hipLaunchKernelGGL(k_gfc_sizes<U>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                           xs, g.n, g.N, g.per, (int)g.S, ts.vals, offsets.get<int32_t>()); }
+        HIPC(hipGetLastError());
+        ts.scan(c);
+        { KernelTimer kt_(c, "k_gfc_encode"); hipLaunchKernelGGL(k_gfc_encode<U>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                           xs, g.n, g.N, g.per, payload.get<uint8_t>(), (const uint64_t*)ts.prefix); }
+        HIPC(hipGetLastError());
+    };
     if (dbl)
-        { KernelTimer kt_(c, "k_gfc_encode"); hipLaunchKernelGGL(k_gfc_encode<uint64_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                           static_cast<const uint64_t*>(x), g.n, g.N, g.per, (int)g.S, payload.get<uint8_t>(),
-                           offsets.get<int32_t>(), lbb.lb, total.get<uint64_t>(), (uint32_t)tiles); }
+        run(uint64_t());
     else
-        { KernelTimer kt_(c, "k_gfc_encode"); hipLaunchKernelGGL(k_gfc_encode<uint32_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                           static_cast<const uint32_t*>(x), g.n, g.N, g.per, (int)g.S, payload.get<uint8_t>(),
-                           offsets.get<int32_t>(), lbb.lb, total.get<uint64_t>(), (uint32_t)tiles); }
-    HIPC(hipGetLastError());
+        run(uint32_t());
     int32_t h_meta[3] = {(int32_t)g.blocks, (int32_t)g.wpb, (int32_t)n};
     meta = make_buf(c, h_meta, 12);
     uint64_t tot = 0;
-    d2h(c, &tot, total.get(), 8);
+    d2h(c, &tot, ts.prefix + tiles, 8);
     payload.set_size(tot);
+    kernel_bytes(c, "k_gfc_encode", (double)n * (dbl ? 8 : 4) + (double)tot);
 }
 
 void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const uint8_t* offsets, size_t off_bytes,
@@ -268,21 +338,33 @@ void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const 
     }
     const size_t tiles = (size_t)g.Wc;
     if (off_bytes < 4 * tiles) fail(DDJ_ECORRUPT, "GFC offsets shorter than the chunk count");
-    LookbackBufs lbb(c, tiles);
-    uint32_t zero = 0;
-    h2d(c, &c->control->error_flag, &zero, 4);
+    TileScan ts(c, tiles);
     out = DevBuf(c, (size_t)n * (dbl ? 8 : 4));
+    uint32_t* err_flag = &c->control->error_flag;  // zero between calls
+    auto run = [&](auto tag) {
+        using U = decltype(tag);
+        { KernelTimer kt_(c, "k_gfc_chunk_bytes"); hipLaunchKernelGGL(k_gfc_chunk_bytes<U>, dim3((unsigned)((tiles + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                           c->stream, offsets, tiles, g.N, g.per, (int)g.S, ts.vals); }
+        HIPC(hipGetLastError());
+        ts.scan(c);
+        kernel_bytes(c, "k_gfc_chunk_bytes", 12.0 * (double)tiles);
+        kernel_bytes(c, "k_gfc_decode", (double)payload_bytes + 4.0 * (double)tiles + (double)n * sizeof(U));
+        { KernelTimer kt_(c, "k_gfc_decode"); hipLaunchKernelGGL(k_gfc_decode<U>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
+                           payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<U>(), (const uint64_t*)ts.prefix,
+                           err_flag); }
+        HIPC(hipGetLastError());
+    };
     if (dbl)
-        { KernelTimer kt_(c, "k_gfc_decode"); hipLaunchKernelGGL(k_gfc_decode<uint64_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
-                           payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<uint64_t>(), lbb.lb); }
+        run(uint64_t());
     else
-        { KernelTimer kt_(c, "k_gfc_decode"); hipLaunchKernelGGL(k_gfc_decode<uint32_t>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
-                           payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<uint32_t>(), lbb.lb); }
-    HIPC(hipGetLastError());
+        run(uint32_t());
     uint32_t err = 0;
-    d2h(c, &err, &c->control->error_flag, 4);
-    if (err == 2) fail(DDJ_ECORRUPT, "GFC chunk sizes disagree with their headers");
-    if (err) fail(DDJ_ETIMEOUT, "look-back wait expired");
+    d2h(c, &err, err_flag, 4);
+    if (err) {
+        const uint32_t zero = 0;
+        h2d(c, err_flag, &zero, 4);
+        fail(DDJ_ECORRUPT, "GFC chunk sizes disagree with their headers");
+    }
 }
 
 }  // namespace ddj

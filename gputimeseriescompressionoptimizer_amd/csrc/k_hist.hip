@@ -4,8 +4,10 @@
 // largest counts, equal counts ordered by the smaller value (what the
 // reference's stable sort of key-sorted bins yields).
 //
-// Counting: each workgroup aggregates its rows in an LDS hash table and
-// flushes it into a global open-addressing table (64-bit keys, ~0 = empty);
+// Counting: each workgroup aggregates its rows in an LDS hash table (plain
+// LDS atomics; the 16 rows a thread loads at once keep HBM busy while they
+// wait) and flushes it into a global open-addressing table (64-bit keys,
+// ~0 = empty);
 // rows that find no LDS slot go straight to the global table.  A 64-bit
 // value equal to the empty marker is counted on the side.  Floats count
 // -0.0 with +0.0 (operator==), reporting -0.0 when one was seen (it sorts
@@ -132,6 +134,16 @@ __device__ void block_small4(Small4& s, uint64_t* sm_keys /* kBlock/kWave * kMax
     }
 }
 
+// LDS slot of a key: a multiplicative hash of its folded halves (cheaper
+// than hmix's 64-bit multiplies, which the global table keeps).
+__device__ __forceinline__ uint32_t lds_slot(uint64_t key) {
+    const uint32_t a = (uint32_t)key ^ ((uint32_t)(key >> 32) * 0x85ebca6bu);
+    return (a * 0x9E3779B1u) >> (32 - 11);
+}
+static_assert(kLdsSlots == 1 << 11, "lds_slot yields 11 bits");
+
+// Tiles of 4096 rows, each thread taking 16 rows with 16-byte loads issued
+// together; workgroups stride over the tiles.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, size_t n, unsigned long long* keys,
                                                        unsigned long long* counts, uint64_t cap_mask, HistCtl* ctl,
@@ -150,67 +162,54 @@ __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, 
     __syncthreads();
     bool negzero = false;
     unsigned long long special = 0;
-    const int lane = threadIdx.x & (kWave - 1);
-    const size_t stride = (size_t)gridDim.x * kBlock;
-    // Whole waves step through the rows together so that equal keys of a
-    // wave can be merged: a few rounds each take the first pending lane's
-    // key and add the number of lanes holding it with one LDS atomic (a
-    // skewed column's frequent keys would otherwise serialise the wave's
-    // atomics on one address); lanes still pending afterwards add 1 each.
-    const size_t n_rounds = (n + stride - 1) / stride;
-    for (size_t it = 0; it < n_rounds; it++) {
-        const size_t i = it * stride + (size_t)blockIdx.x * kBlock + threadIdx.x;
-        bool pending = i < n;
-        uint64_t key = 0;
-        if (pending) {
-            const T v = x[i];
-            bool skip = false;
-            if constexpr (std::is_floating_point<T>::value) {
-                if (v != v) {
-                    nan4.insert(order_key(v));
-                    nans++;
-                    skip = true;
-                }
+    auto count = [&](T v) {
+        if constexpr (std::is_floating_point<T>::value) {
+            if (v != v) {
+                nan4.insert(order_key(v));
+                nans++;
+                return;
             }
-            if (!skip) {
-                key = hist_key(v, negzero);
-                if (key == kEmpty) {
-                    special++;
-                    skip = true;
-                }
-            }
-            pending = !skip;
         }
-        uint32_t add = 1;
-        for (int r = 0; r < 4; r++) {
-            const uint64_t pm = __ballot(pending);
-            if (pm == 0) break;
-            const int leader = __builtin_ctzll(pm);
-            const uint64_t kl = __shfl(key, leader, kWave);
-            const bool same = pending && key == kl;
-            const uint64_t sm = __ballot(same);
-            if (same) {
-                if (lane == leader) {
-                    add = (uint32_t)__popcll(sm);
-                } else {
-                    pending = false;  // counted by the leader
-                }
-            }
-            if (lane == leader) break;  // the leader adds below
+        const uint64_t key = hist_key(v, negzero);
+        if (key == kEmpty) {
+            special++;
+            return;
         }
-        if (!pending) continue;
-        const uint32_t h = (uint32_t)hmix(key) & (kLdsSlots - 1);
-        bool done = false;
-        for (int p = 0; p < 16 && !done; p++) {
+        const uint32_t h = lds_slot(key);
+        for (int p = 0; p < 16; p++) {
             const uint32_t s = (h + p) & (kLdsSlots - 1);
             unsigned long long k = skey[s];
             if (k == kEmpty) k = atomicCAS(&skey[s], kEmpty, (unsigned long long)key);
             if (k == kEmpty || k == key) {
-                atomicAdd(&scnt[s], add);
-                done = true;
+                atomicAdd(&scnt[s], 1u);
+                return;
             }
         }
-        if (!done) global_insert(keys, counts, cap_mask, key, (unsigned long long)add, ctl);
+        global_insert(keys, counts, cap_mask, key, 1ull, ctl);
+    };
+    constexpr int kRows = 4096;
+    constexpr int V = 16 / sizeof(T);
+    constexpr int Q = kRows / V / kBlock;  // 16-byte loads per thread and tile
+    const size_t tiles = (n + kRows - 1) / kRows;
+    const bool aligned = (reinterpret_cast<uintptr_t>(x) & 15) == 0;
+    for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {  // uniform per workgroup
+        const size_t base = t * kRows;
+        if (aligned && base + kRows <= n) {
+            const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
+            u32x4 w[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) w[q] = __builtin_nontemporal_load(s4 + threadIdx.x + kBlock * q);
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                T v[V];
+                __builtin_memcpy(v, &w[q], 16);
+#pragma unroll
+                for (int k = 0; k < V; k++) count(v[k]);
+            }
+        } else {
+            const int cnt = (int)((n - base) < (size_t)kRows ? (n - base) : (size_t)kRows);
+            for (int i = threadIdx.x; i < cnt; i += kBlock) count(x[base + i]);
+        }
     }
     __syncthreads();
     for (int s = threadIdx.x; s < kLdsSlots; s += kBlock)
@@ -360,7 +359,7 @@ void topk_most_frequent(Ctx* c, int dtype, const void* x, size_t n, int K, uint6
                                (unsigned long long)kEmpty); }
             launch_fill_zero(c, counts.get(), 8 * cap);
             launch_fill_zero(c, ctl.get(), sizeof(HistCtl));
-            size_t g = (n + kBlock * 16 - 1) / (kBlock * 16);
+            size_t g = (n + 4095) / 4096;  // one workgroup per 4096-row tile, at most 4 per CU
             g = std::max<size_t>(1, std::min<size_t>(g, (size_t)c->cu_count * 4));
             DevBuf nan_keys(c, 8 * kMaxK * g), nan_n(c, 4 * g);
             { KernelTimer kt_(c, "k_hist_count"); hipLaunchKernelGGL(k_hist_count<T>, dim3((unsigned)g), dim3(kBlock), 0, c->stream,

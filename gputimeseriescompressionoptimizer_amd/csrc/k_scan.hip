@@ -1,6 +1,10 @@
-// Variable-length outputs with single-pass decoupled look-back scans:
-// stable split by a predicate with the packed stencil (patch / const /
-// dict), its inverse merge, RLE encode/decode, dict code packing.
+// Variable-length outputs: stable split by a predicate with the packed
+// stencil (patch / const / dict), its inverse merge, RLE encode/decode, dict
+// code packing.  Each output pass takes its tile's offset from a counting
+// pass and k_scan_tiles (reduce, then scan): the counting passes read the
+// input once more (split, RLE encode) or only the stencil / lengths (merge,
+// RLE decode), which costs less on MI355X than chaining 16K tiles through a
+// look-back.
 //
 // Tiles hold 4096 rows; thread t of a workgroup owns the 16 consecutive rows
 // [16t, 16t+16) of the tile, i.e. exactly two stencil bytes.  Inputs are
@@ -53,15 +57,42 @@ template <typename T> T bits_to(uint64_t b) {
     return v;
 }
 
+// Rows of each tile that satisfy the predicate (the split's counting pass).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_split_count(const T* __restrict__ x, size_t n, Pred<T> p,
+                                                        uint64_t* __restrict__ vals) {
+    __shared__ uint64_t red[kBlock / kWave];
+    const size_t base = (size_t)blockIdx.x * kTile;
+    const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
+    uint32_t c = 0, code;
+    if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
+        constexpr int V = 16 / sizeof(T);
+        const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
+#pragma unroll
+        for (int q = 0; q < kTile / V / kBlock; q++) {
+            const u32x4 w = s4[threadIdx.x + kBlock * q];
+            T v[V];
+            __builtin_memcpy(v, &w, 16);
+#pragma unroll
+            for (int k = 0; k < V; k++) c += p(v[k], code) ? 1u : 0u;
+        }
+    } else {
+        for (int i = threadIdx.x; i < cnt; i += kBlock) c += p(x[base + i], code) ? 1u : 0u;
+    }
+    uint64_t total;
+    block_exclusive_scan<uint64_t>((uint64_t)c, &total, red);
+    if (threadIdx.x == 0) vals[blockIdx.x] = total;
+}
+
 template <typename T, bool CODES, bool WRITE_NO>
 __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_t n, Pred<T> p,
                                                   uint8_t* __restrict__ stencil, T* __restrict__ yes,
                                                   T* __restrict__ no, uint8_t* __restrict__ codes,
-                                                  Lookback lb, uint64_t* d_count) {
+                                                  const uint64_t* __restrict__ prefix, uint64_t* d_count) {
     __shared__ T buf[kTilePad];
     __shared__ uint8_t cbuf[CODES ? kTile : 1];
     __shared__ uint64_t red[kBlock / kWave];
-    const uint32_t tile = tile_index();
+    const uint32_t tile = blockIdx.x;
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     const T padv = x[n - 1];  // stencil padding repeats the last row (reference stencil.cu:26-27)
@@ -98,8 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     if (tile == 0 && threadIdx.x == 0) stencil[0] = (uint8_t)(n % 8);
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)c, &total, red);
-    const uint64_t s_prefix = block_lookback<false>(lb, tile, total);
-    const uint64_t Y0 = s_prefix;
+    const uint64_t Y0 = prefix[tile];
     int yr = (int)excl, nr = threadIdx.x * kItems - (int)excl;
     const int nbase = CODES ? 0 : (int)total;
 #pragma unroll
@@ -133,13 +163,37 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
 
 // Splitter::Merge (reference util/splitter/splitter.cu:140-184) driven by a
 // packed stencil: row i takes the next `yes` value when bit i is set.
+// Set stencil bits of each tile (the merge's counting pass): one wave per
+// tile, lane l taking the tile's stencil bytes [8l, 8l + 8).  Bits past row
+// n - 1 (the stencil's padding) are not counted.
+__global__ __launch_bounds__(kBlock) void k_stencil_count(const uint8_t* __restrict__ bits, size_t n, size_t tiles,
+                                                          uint64_t* __restrict__ vals) {
+    const size_t tile = (size_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (tile >= tiles) return;  // uniform per wave
+    const int lane = threadIdx.x & (kWave - 1);
+    const size_t r0 = tile * kTile + (size_t)lane * 64;
+    uint32_t c = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const size_t r = r0 + 8 * k;
+        if (r < n) {
+            const size_t valid = n - r;
+            uint32_t byte = bits[r / 8];
+            if (valid < 8) byte &= (1u << valid) - 1u;
+            c += __popc(byte);
+        }
+    }
+    c = wave_sum(c);
+    if (lane == 0) vals[tile] = c;
+}
+
 template <typename T, bool CONST_NO>
 __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bits, size_t n, const T* __restrict__ yes,
                                                   const T* __restrict__ no, T no_const, T* __restrict__ out,
-                                                  Lookback lb) {
+                                                  const uint64_t* __restrict__ prefix) {
     __shared__ T buf[kTilePad];
     __shared__ uint64_t red[kBlock / kWave];
-    const uint32_t tile = tile_index();
+    const uint32_t tile = blockIdx.x;
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     const size_t g0 = base + (size_t)threadIdx.x * kItems;
@@ -153,8 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
         mask &= (1u << mine) - 1u;
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &total, red);
-    const uint64_t s_prefix = block_lookback<false>(lb, tile, total);
-    const uint64_t Y0 = s_prefix;
+    const uint64_t Y0 = prefix[tile];
     const size_t N0 = base - Y0;
     for (int i = threadIdx.x; i < cnt; i += kBlock) {
         if (i < (int)total)
@@ -214,13 +267,46 @@ __global__ __launch_bounds__(kBlock) void k_unpack_codes(const uint8_t* __restri
 
 // RleEncoding (reference compression/rle/rle_encode.cu:18-62): a run starts
 // at row 0 and wherever !(x[i] == x[i-1]); the run's value is its first row.
+// Runs starting in each tile (the RLE encoder's counting pass).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_rle_count(const T* __restrict__ x, size_t n, uint64_t* __restrict__ vals) {
+    __shared__ uint64_t red[kBlock / kWave];
+    const size_t base = (size_t)blockIdx.x * kTile;
+    const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
+    uint32_t c = 0;
+    if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
+        constexpr int V = 16 / sizeof(T);
+        const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
+#pragma unroll
+        for (int q = 0; q < kTile / V / kBlock; q++) {
+            const int i = threadIdx.x + kBlock * q;
+            const u32x4 w = s4[i];
+            T v[V];
+            __builtin_memcpy(v, &w, 16);
+            const size_t g = base + (size_t)V * i;
+            T prev = g > 0 ? x[g - 1] : v[0];
+            c += (g == 0 || !(v[0] == prev)) ? 1u : 0u;
+#pragma unroll
+            for (int k = 1; k < V; k++) c += !(v[k] == v[k - 1]) ? 1u : 0u;
+        }
+    } else {
+        for (int i = threadIdx.x; i < cnt; i += kBlock) {
+            const size_t g = base + i;
+            c += (g == 0 || !(x[g] == x[g - 1])) ? 1u : 0u;
+        }
+    }
+    uint64_t total;
+    block_exclusive_scan<uint64_t>((uint64_t)c, &total, red);
+    if (threadIdx.x == 0) vals[blockIdx.x] = total;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_rle_encode(const T* __restrict__ x, size_t n, T* __restrict__ values,
-                                                       uint32_t* __restrict__ starts, Lookback lb,
-                                                       uint64_t* d_runs) {
+                                                       uint32_t* __restrict__ starts,
+                                                       const uint64_t* __restrict__ prefix, uint64_t* d_runs) {
     __shared__ T buf[kTilePad + 1];
     __shared__ uint64_t red[kBlock / kWave];
-    const uint32_t tile = tile_index();
+    const uint32_t tile = blockIdx.x;
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
@@ -253,7 +339,7 @@ __global__ __launch_bounds__(kBlock) void k_rle_encode(const T* __restrict__ x, 
     }
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &total, red);
-    const uint64_t s_prefix = block_lookback<false>(lb, tile, total);
+    const uint64_t s_prefix = prefix[tile];
     uint64_t r = s_prefix + excl;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
@@ -276,30 +362,51 @@ __global__ __launch_bounds__(kBlock) void k_rle_lengths(const uint32_t* __restri
     }
 }
 
+// Run length r of an RLE stream at any byte address; a negative length is
+// corrupt input and counts as an empty run.
+__device__ __forceinline__ uint64_t run_length(const uint8_t* lens, size_t r, bool aligned, bool& bad) {
+    int32_t l;
+    if (aligned)
+        l = reinterpret_cast<const int32_t*>(lens)[r];
+    else
+        __builtin_memcpy(&l, lens + 4 * r, 4);
+    if (l < 0) {
+        bad = true;
+        return 0;
+    }
+    return (uint64_t)l;
+}
+
+// Rows covered by each tile of 4096 runs (the decoder's counting pass); a
+// negative length raises the error flag (2: corrupt input).
+__global__ __launch_bounds__(kBlock) void k_length_sums(const uint8_t* __restrict__ lens, size_t R,
+                                                        uint64_t* __restrict__ vals, uint32_t* error) {
+    __shared__ uint64_t red[kBlock / kWave];
+    const size_t base = (size_t)blockIdx.x * kTile;
+    const int cnt = (int)((R - base) < (size_t)kTile ? (R - base) : (size_t)kTile);
+    const bool aligned = (reinterpret_cast<uintptr_t>(lens) & 3) == 0;
+    bool bad = false;
+    uint64_t s = 0;
+    for (int i = threadIdx.x; i < cnt; i += kBlock) s += run_length(lens, base + i, aligned, bad);
+    if (bad) __hip_atomic_store(error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t total;
+    block_exclusive_scan<uint64_t>(s, &total, red);
+    if (threadIdx.x == 0) vals[blockIdx.x] = total;
+}
+
 // Exclusive scan of int32 run lengths into uint64 offsets[R+1].
 __global__ __launch_bounds__(kBlock) void k_scan_lengths(const uint8_t* __restrict__ lens, size_t R,
-                                                         uint64_t* __restrict__ offsets, Lookback lb) {
+                                                         uint64_t* __restrict__ offsets,
+                                                         const uint64_t* __restrict__ prefix) {
     __shared__ uint64_t buf[kTile + kTile / 32];
     __shared__ uint64_t red[kBlock / kWave];
     auto pad = [](int i) { return i + (i >> 5); };
-    const uint32_t tile = tile_index();
+    const uint32_t tile = blockIdx.x;
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((R - base) < (size_t)kTile ? (R - base) : (size_t)kTile);
     const bool aligned = (reinterpret_cast<uintptr_t>(lens) & 3) == 0;
-    for (int i = threadIdx.x; i < kTile; i += kBlock) {
-        int32_t l = 0;
-        if (i < cnt) {
-            if (aligned)
-                l = reinterpret_cast<const int32_t*>(lens)[base + i];
-            else
-                __builtin_memcpy(&l, lens + 4 * (base + i), 4);
-        }
-        if (l < 0) {  // a run of negative length: corrupt input (decoded as empty)
-            __hip_atomic_store(lb.error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            l = 0;
-        }
-        buf[pad(i)] = (uint64_t)(int64_t)l;
-    }
+    bool bad = false;
+    for (int i = threadIdx.x; i < kTile; i += kBlock) buf[pad(i)] = i < cnt ? run_length(lens, base + i, aligned, bad) : 0;
     __syncthreads();
     uint64_t loc[kItems], s = 0;
 #pragma unroll
@@ -308,8 +415,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_lengths(const uint8_t* __restri
         s += loc[k];
     }
     uint64_t total;
-    const uint64_t excl = block_exclusive_scan<uint64_t>(s, &total, red);
-    const uint64_t s_prefix = block_lookback<true>(lb, tile, total);
+    const uint64_t excl = block_exclusive_scan<uint64_t>(s, &total, red);  // a barrier: buf is free again
+    const uint64_t s_prefix = prefix[tile];
     uint64_t run = s_prefix + excl;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
@@ -412,19 +519,24 @@ void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa
         for (int j = 0; j < 4; j++) p.set[j] = bits_to<T>(pa.set_bits[j]);
         p.set_n = pa.set_n;
         const size_t tiles = (n + kTile - 1) / kTile;
-        LookbackBufs lbb(c, tiles);
+        TileScan ts(c, tiles);
         auto xs = static_cast<const T*>(x);
         auto ys = static_cast<T*>(yes);
         auto ns = static_cast<T*>(no);
+        kernel_bytes(c, "k_split_count", (double)n * sizeof(T));
+        { KernelTimer kt_(c, "k_split_count"); hipLaunchKernelGGL(k_split_count<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p, ts.vals); }
+        HIPC(hipGetLastError());
+        ts.scan(c);
+        const uint64_t* pre = ts.prefix;
         if (codes) {
             { KernelTimer kt_(c, "k_split"); hipLaunchKernelGGL((k_split<T, true, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p,
-                               stencil, ys, ns, codes, lbb.lb, d_counts); }
+                               stencil, ys, ns, codes, pre, d_counts); }
         } else if (no) {
             { KernelTimer kt_(c, "k_split"); hipLaunchKernelGGL((k_split<T, false, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p,
-                               stencil, ys, ns, codes, lbb.lb, d_counts); }
+                               stencil, ys, ns, codes, pre, d_counts); }
         } else {
             { KernelTimer kt_(c, "k_split"); hipLaunchKernelGGL((k_split<T, false, false>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n,
-                               p, stencil, ys, ns, codes, lbb.lb, d_counts); }
+                               p, stencil, ys, ns, codes, pre, d_counts); }
         }
         HIPC(hipGetLastError());
     });
@@ -436,15 +548,22 @@ void launch_merge(Ctx* c, int dtype, const uint8_t* bits, size_t n, const uint8_
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
         const size_t tiles = (n + kTile - 1) / kTile;
-        LookbackBufs lbb(c, tiles);
+        TileScan ts(c, tiles);
+        const unsigned cg = (unsigned)((tiles + kBlock / kWave - 1) / (kBlock / kWave));
+        kernel_bytes(c, "k_stencil_count", (double)((n + 7) / 8));
+        kernel_bytes(c, "k_merge", (double)((n + 7) / 8) + 2.0 * (double)n * sizeof(T));
+        { KernelTimer kt_(c, "k_stencil_count"); hipLaunchKernelGGL(k_stencil_count, dim3(cg), dim3(kBlock), 0, c->stream, bits, n, tiles, ts.vals); }
+        HIPC(hipGetLastError());
+        ts.scan(c);
+        const uint64_t* pre = ts.prefix;
         auto ys = reinterpret_cast<const T*>(yes);
         auto ns = reinterpret_cast<const T*>(no);
         if (no)
             { KernelTimer kt_(c, "k_merge"); hipLaunchKernelGGL((k_merge<T, false>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, bits, n, ys,
-                               ns, T(0), static_cast<T*>(out), lbb.lb); }
+                               ns, T(0), static_cast<T*>(out), pre); }
         else
             { KernelTimer kt_(c, "k_merge"); hipLaunchKernelGGL((k_merge<T, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, bits, n, ys,
-                               ns, bits_to<T>(no_const), static_cast<T*>(out), lbb.lb); }
+                               ns, bits_to<T>(no_const), static_cast<T*>(out), pre); }
         HIPC(hipGetLastError());
     });
 }
@@ -472,10 +591,16 @@ void launch_rle_encode(Ctx* c, int dtype, const void* x, size_t n, int32_t* leng
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
         const size_t tiles = (n + kTile - 1) / kTile;
-        LookbackBufs lbb(c, tiles);
+        TileScan ts(c, tiles);
+        kernel_bytes(c, "k_rle_count", (double)n * sizeof(T));
+        { KernelTimer kt_(c, "k_rle_count"); hipLaunchKernelGGL(k_rle_count<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                           static_cast<const T*>(x), n, ts.vals); }
+        HIPC(hipGetLastError());
+        ts.scan(c);
         DevBuf starts(c, 4 * n);
         { KernelTimer kt_(c, "k_rle_encode"); hipLaunchKernelGGL(k_rle_encode<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                           static_cast<const T*>(x), n, static_cast<T*>(values), starts.get<uint32_t>(), lbb.lb, d_runs); }
+                           static_cast<const T*>(x), n, static_cast<T*>(values), starts.get<uint32_t>(),
+                           (const uint64_t*)ts.prefix, d_runs); }
         HIPC(hipGetLastError());
         uint64_t runs = 0;
         d2h(c, &runs, d_runs, 8);
@@ -491,10 +616,17 @@ DevBuf rle_decode_dev(Ctx* c, int dtype, const uint8_t* lengths, const uint8_t* 
     return dispatch_native(dtype, [&](auto tag) -> DevBuf {
         using T = decltype(tag);
         const size_t tiles = (runs + kTile - 1) / kTile;
-        LookbackBufs lbb(c, tiles, true);  // int64 sums of int32 lengths
+        TileScan ts(c, tiles);
+        arm_error_flag(c);
+        kernel_bytes(c, "k_length_sums", 4.0 * (double)runs);
+        kernel_bytes(c, "k_scan_lengths", 4.0 * (double)runs + 8.0 * (double)(runs + 1));
+        { KernelTimer kt_(c, "k_length_sums"); hipLaunchKernelGGL(k_length_sums, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, lengths, runs,
+                           ts.vals, &c->control->error_flag); }
+        HIPC(hipGetLastError());
+        ts.scan(c);
         DevBuf offsets(c, 8 * (runs + 1));
         { KernelTimer kt_(c, "k_scan_lengths"); hipLaunchKernelGGL(k_scan_lengths, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, lengths, runs,
-                           offsets.get<uint64_t>(), lbb.lb); }
+                           offsets.get<uint64_t>(), (const uint64_t*)ts.prefix); }
         HIPC(hipGetLastError());
         uint64_t total = 0;
         d2h(c, &total, offsets.get<uint64_t>() + runs, 8);

@@ -6,18 +6,22 @@
 
 namespace ddj {
 
-// Device state of one decoupled look-back launch: a 64-bit status word per
-// tile (two per tile for wide values), zeroed before the launch, and the
-// context's error flag.
-struct Lookback {
-    uint64_t* status;
-    uint32_t* error;
-};
-struct LookbackBufs {
+// Tile offsets of a variable-length output, computed ahead of the output
+// pass: a counting kernel fills vals[0..tiles) (one value per tile), scan()
+// launches k_scan_tiles, which writes prefix[t] = vals[0] + ... + vals[t-1]
+// for t <= tiles (prefix[tiles] is the total).  Sums wrap at 64 bits.
+struct TileScan {
     DevBuf mem;
-    Lookback lb;
-    LookbackBufs(Ctx* c, size_t tiles, bool wide = false);
+    uint64_t* vals = nullptr;
+    uint64_t* prefix = nullptr;
+    size_t tiles = 0;
+    TileScan(Ctx* c, size_t tiles);
+    void scan(Ctx* c);
 };
+
+// Kernels that can raise the device error flag (encoded input failing a
+// device-side check) arm it first; the call then reads and clears it.
+void arm_error_flag(Ctx* c);
 
 // Scratch of a single-pass reduction (per-block partials + result).
 struct MinMaxResult {

@@ -33,12 +33,15 @@ double ratio_of(size_t in, size_t out) {
     return (double)in / (double)out;
 }
 
-// One piece of a serialized blob: an 8-byte header (host) or device bytes.
+// One piece of a serialized blob: an 8-byte header (host) or a part.  A
+// keep-alive piece holds a node's input until the blob is assembled
+// (encoders may hand out views into their input, e.g. delta's first row)
+// and contributes no bytes.
 struct Piece {
     bool is_header = false;
+    bool keep_alive = false;
     uint8_t hdr[8];
     Part part;
-    size_t size() const { return is_header ? 8 : part.n; }
 };
 
 bool is_int_type(int dt) { return dt == DDJ_D_INT || dt == DDJ_D_TIME || dt == DDJ_D_SHORT || dt == DDJ_D_CHAR; }
@@ -59,11 +62,17 @@ size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& 
                      const std::vector<Part>* pre = nullptr) {
     std::vector<Part> res;
     std::vector<Part> child_pre;
+    if (data.owner) {
+        Piece k;
+        k.keep_alive = true;
+        k.part = data;
+        out.push_back(k);
+    }
     if (pre) {
         res = *pre;
     } else if (node.enc == DDJ_ENC_NONE) {
         size_t len = data.n;  // the leaf refers to its input instead of copying it
-        res.push_back(own(make_buf(c, &len, sizeof len)));
+        res.push_back(host_part(&len, sizeof len));
         res.push_back(data);
     } else if (fusable_delta_afl(node, data)) {
         auto f = delta_afl_encode(c, node.dtype, data.view());
@@ -182,28 +191,39 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
     std::vector<Piece> pieces;
     size_t total = compress_node(c, *root, borrow(data), pieces);
     // Concatenate (util/copy/cuda_array_copy.cu:14-33) as one gather kernel.
-    // The copy table and the node headers travel in one upload: the
-    // headers' ranges point into the table buffer itself.
-    size_t n_ranges = 0, hdr_count = 0;
+    // The copy table, the node headers and the host-resident metadata travel
+    // in one upload: their ranges point into the table buffer itself.
+    size_t n_ranges = 0, inline_bytes = 0;
     for (auto& p : pieces) {
-        if (p.is_header) hdr_count++;
+        if (p.keep_alive) continue;
+        if (p.is_header)
+            inline_bytes += 8;
+        else if (p.part.on_host())
+            inline_bytes += p.part.n;
         if (p.is_header || p.part.n) n_ranges++;
     }
-    DevBuf rdev(c, sizeof(CopyRange) * n_ranges + 8 * hdr_count);
-    const uint8_t* hdr_base = rdev.bytes_ptr() + sizeof(CopyRange) * n_ranges;
-    std::vector<uint8_t> table(sizeof(CopyRange) * n_ranges + 8 * hdr_count);
+    DevBuf rdev(c, sizeof(CopyRange) * n_ranges + inline_bytes);
+    const uint8_t* inl_dev = rdev.bytes_ptr() + sizeof(CopyRange) * n_ranges;
+    std::vector<uint8_t> table(sizeof(CopyRange) * n_ranges + inline_bytes);
     CopyRange* ranges = reinterpret_cast<CopyRange*>(table.data());
-    uint8_t* headers = table.data() + sizeof(CopyRange) * n_ranges;
-    size_t off = 0, hi = 0, ri = 0;
+    uint8_t* inl = table.data() + sizeof(CopyRange) * n_ranges;
+    size_t off = 0, io = 0, ri = 0;
     uint64_t max_bytes = 0, copied = 0;
     for (auto& p : pieces) {
+        if (p.keep_alive) continue;
         if (p.is_header) {
-            memcpy(headers + 8 * hi, p.hdr, 8);
-            ranges[ri++] = CopyRange{hdr_base + 8 * hi, off, 8};
-            hi++;
+            memcpy(inl + io, p.hdr, 8);
+            ranges[ri++] = CopyRange{inl_dev + io, off, 8};
+            io += 8;
             off += 8;
         } else if (p.part.n) {
-            ranges[ri++] = CopyRange{p.part.p, off, p.part.n};
+            if (p.part.on_host()) {
+                memcpy(inl + io, p.part.host->data(), p.part.n);
+                ranges[ri++] = CopyRange{inl_dev + io, off, p.part.n};
+                io += p.part.n;
+            } else {
+                ranges[ri++] = CopyRange{p.part.p, off, p.part.n};
+            }
             max_bytes = std::max<uint64_t>(max_bytes, p.part.n);
             off += p.part.n;
         }

@@ -28,7 +28,7 @@ typedef enum {
     DDJ_ECORRUPT = 3,     /* encoded input fails validation */
     DDJ_ENOMEM = 4,       /* device allocation failed */
     DDJ_EHIP = 5,         /* HIP runtime error */
-    DDJ_ETIMEOUT = 6      /* a bounded device-side wait expired */
+    DDJ_ETIMEOUT = 6      /* reserved: no kernel waits on another, so none is raised */
 } ddj_status;
 
 /* EncodingType (reference: compression/encoding_type.hpp:17-30) */

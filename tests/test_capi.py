@@ -61,3 +61,17 @@ def test_cpp_mirror_compiles_and_links(tmp_path):
     """include/ddj.hpp (the reference's class names over the C ABI) builds
     with a plain host compiler and links against the library."""
     assert os.path.exists(_build_cpp(str(tmp_path / "roundtrip")))
+
+
+def test_profiled_kernel_names_are_listed():
+    """Every kernel the library times (KernelTimer names in csrc/) is one the
+    Python profiler asks for, so benchmark tables miss none."""
+    import gputimeseriescompressionoptimizer_amd as g
+
+    csrc = os.path.join(ROOT, "gputimeseriescompressionoptimizer_amd", "csrc")
+    timed = set()
+    for f in os.listdir(csrc):
+        if f.endswith((".hip", ".cpp", ".h")):
+            timed |= set(re.findall(r'KernelTimer kt_\(c, "([a-z_0-9]+)"\)', open(os.path.join(csrc, f)).read()))
+    assert timed, "no timed kernels found"
+    assert timed <= set(g.Context.KERNELS), sorted(timed - set(g.Context.KERNELS))

@@ -58,3 +58,16 @@ def test_column_file_append_continues_the_optimizers(ctx, tmp_path):
     want = [ref.compress_data(x[r:r + 4_000], O.D_INT) for r in range(0, 12_000, 4_000)]
     assert read_blobs(path) == want
     assert np.array_equal(ts.Decompress(path)[0], x)
+
+
+def test_context_close_releases_dependents():
+    """Closing a context first releases the optimizers, planners and column
+    compressors made from it; dropping them afterwards touches nothing."""
+    c = g.Context(0)
+    opt = g.CompressionOptimizer(c)
+    net = g.LearnedPlanner(c)
+    ts = g.ParallelTSCompressor(c, [g.D_INT], 1000)
+    c.close()
+    assert opt._h is None and net._h is None and ts._h is None
+    del opt, net, ts
+    c.close()  # idempotent
