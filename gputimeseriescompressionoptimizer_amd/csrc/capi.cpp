@@ -30,6 +30,7 @@ static hipError_t init_ctx(Ctx& c, int device) {
         c.staging_size = 4u << 20;
         e = hipHostMalloc(reinterpret_cast<void**>(&c.staging), c.staging_size, hipHostMallocDefault);
     }
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c.readback), Ctx::kReadbackSize, hipHostMallocDefault);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         fini_ctx(c);
@@ -45,9 +46,11 @@ static void fini_ctx(Ctx& c) {
     c.alloc.reset();
     if (c.control) (void)hipFree(c.control);
     if (c.staging) (void)hipHostFree(c.staging);
+    if (c.readback) (void)hipHostFree(c.readback);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.control = nullptr;
     c.staging = nullptr;
+    c.readback = nullptr;
     c.stream = nullptr;
 }
 

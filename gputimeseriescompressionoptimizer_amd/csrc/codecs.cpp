@@ -52,7 +52,7 @@ Part empty() { return Part{}; }
 
 template <typename T> T read_dev(Ctx* c, const uint8_t* p) {
     T v;
-    d2h(c, &v, p, sizeof(T));
+    read_small(c, &v, p, sizeof(T));
     return v;
 }
 
@@ -166,7 +166,7 @@ Part scale_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     if (in[0].n < es) fail(DDJ_ECORRUPT, "scale metadata shorter than one value");
     const size_t n = in[1].n / es;
     uint64_t m = 0;
-    d2h(c, &m, in[0].p, es);
+    read_small(c, &m, in[0].p, es);
     Part x = aligned_view(c, in[1], es);
     DevBuf out(c, n * es);
     launch_add_scalar(c, (int)es, x.p, n, m, out.get());
@@ -218,7 +218,7 @@ Part afl_decode_int(Ctx* c, int dtype, const std::vector<DView>& in) {
     if (in[1].n == 0) return empty();
     if (in[0].n < 2) fail(DDJ_ECORRUPT, "AFL metadata shorter than 2 bytes");
     int8_t m[2];
-    d2h(c, m, in[0].p, 2);
+    read_small(c, m, in[0].p, 2);
     return fl_decode_raw(c, dtype, in[1].p, in[1].n, m[0], m[1]);
 }
 
@@ -268,15 +268,15 @@ Part afl_decode_float(Ctx* c, const std::vector<DView>& in) {
     if (in[1].n == 0) return empty();
     if (in[0].n < sizeof(size_t) + 1 || in[1].n < 8) fail(DDJ_ECORRUPT, "AFL float metadata too short");
     uint8_t head[sizeof(size_t) + 1];
-    d2h(c, head, in[0].p, sizeof head);
+    read_small(c, head, in[0].p, sizeof head);
     size_t msize;
     memcpy(&msize, head, sizeof(size_t));
     const int sign = (int8_t)head[sizeof(size_t)];
     if (msize + 8 > in[1].n) fail(DDJ_ECORRUPT, "AFL float mantissa size out of range");
     int8_t ma[2], ea[2];
-    d2h(c, ma, in[1].p, 2);
+    read_small(c, ma, in[1].p, 2);
     const uint8_t* eb = in[1].p + 4 + msize;
-    d2h(c, ea, eb, 2);
+    read_small(c, ea, eb, 2);
     Part mant = fl_decode_raw(c, DDJ_D_INT, in[1].p + 4, msize, ma[0], ma[1]);
     Part expo = fl_decode_raw(c, DDJ_D_INT, eb + 4, in[1].n - msize - 8, ea[0], ea[1]);
     const size_t n = mant.n / 4;
@@ -403,7 +403,7 @@ Part const_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     if (in[0].n == 0) return empty();
     if (in[0].n < es + 1) fail(DDJ_ECORRUPT, "const metadata too short");
     uint64_t cv = 0;
-    d2h(c, &cv, in[0].p, es);
+    read_small(c, &cv, in[0].p, es);
     const size_t n = stencil_rows(c, DView{in[0].p + es, in[0].n - es});
     Part y = aligned_view(c, in[1], es);
     DevBuf out(c, n * es);
@@ -460,7 +460,7 @@ Part dict_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     if (in[1].n > 0) {
         if (in[1].n < 16) fail(DDJ_ECORRUPT, "dict header too short");
         size_t sizes[2];
-        d2h(c, sizes, in[1].p, 16);
+        read_small(c, sizes, in[1].p, 16);
         const size_t K = sizes[0], m = sizes[1];
         if (K == 0 || K > (1u << 30) || 16 + K * es > in[1].n) fail(DDJ_ECORRUPT, "dict key count out of range");
         int bits = 1;
@@ -586,7 +586,7 @@ Part delta_afl_decode(Ctx* c, int dtype, DView dmeta, DView ameta, DView data) {
     if (dmeta.n < es) fail(DDJ_ECORRUPT, "delta metadata shorter than one value");
     if (ameta.n < 2) fail(DDJ_ECORRUPT, "AFL metadata shorter than 2 bytes");
     int8_t mb[2];
-    d2h(c, mb, ameta.p, 2);
+    read_small(c, mb, ameta.p, 2);
     const int b = mb[0], rest = mb[1];
     const int W = 8 * (int)es;
     if (b <= 0 || b > W) fail(DDJ_ECORRUPT, "AFL bit width out of range");

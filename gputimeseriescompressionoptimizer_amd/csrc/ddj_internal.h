@@ -84,9 +84,22 @@ struct Ctx {
     // returns without waiting; the ring restarts after the stream drains.
     uint8_t* staging = nullptr;
     size_t staging_size = 0, staging_off = 0;
-    // set when a launch may raise the device error flag (bounded waits,
-    // device-side input checks); the C ABI reads the flag back only then
+    // Pinned landing area for small read-backs (a copy into pageable memory
+    // is staged by the runtime itself, which costs a further round trip).
+    static constexpr size_t kReadbackSize = 64 << 10;
+    uint8_t* readback = nullptr;
+    // set when a launch may raise the device error flag (device-side input
+    // checks); the C ABI reads the flag back only then
     bool error_flag_armed = false;
+    // Host copies of small device ranges that do not change while they are
+    // registered (the metadata heads of a blob being decoded): read_small
+    // serves reads inside them without a device round trip.
+    struct Shadow {
+        const uint8_t* dev;
+        size_t n;
+        const uint8_t* host;
+    };
+    std::vector<Shadow> shadows;
 };
 
 // Adds the algorithmic bytes of a launch to its kernel's tally.
@@ -173,5 +186,8 @@ void d2h(Ctx* c, void* dst, const void* src, size_t bytes);  // synchronises the
 void h2d(Ctx* c, void* dst, const void* src, size_t bytes);  // stream-ordered
 void d2d(Ctx* c, void* dst, const void* src, size_t bytes);  // stream-ordered
 DevBuf make_buf(Ctx* c, const void* host, size_t bytes);     // stream-ordered upload
+// A few bytes of device memory to the host: from a registered shadow when
+// one covers them, else a synchronous read-back.
+void read_small(Ctx* c, void* dst, const void* src, size_t bytes);
 
 }  // namespace ddj

@@ -139,7 +139,7 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T* total, T* smem /* NT/k
 // into exclusive prefixes, and the output pass reads prefix[tile].  No
 // workgroup waits on another, so nothing spins.
 constexpr int kScanThreads = 1024;
-constexpr int kScanPer = 16;  // values per thread per step of k_scan_tiles
+constexpr int kScanRows = 16;  // rows of kScanThreads values per step of k_scan_tiles
 
 // Relaxed agent-scope stores and loads for values that one workgroup
 // publishes and another reads within the same launch.

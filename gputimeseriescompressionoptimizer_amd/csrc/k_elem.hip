@@ -259,34 +259,61 @@ __global__ __launch_bounds__(kBlock) void k_delta_decode(const uint8_t* first_by
     if (tile == 0 && threadIdx.x == 0) out[0] = first;
 }
 
-// Exclusive prefixes of per-tile values; one workgroup of kScanThreads, each
-// thread taking kScanPer consecutive values per step.
+// Exclusive prefixes of per-tile values; one workgroup of kScanThreads.
+// A step covers kScanRows rows of kScanThreads consecutive values: thread t
+// holds value t of every row (all loads and stores coalesced and in flight
+// together), each row is scanned across the workgroup (wave scans, then the
+// waves' totals in LDS), and the rows are chained by their totals.
 __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const uint64_t* __restrict__ vals, size_t tiles,
                                                              uint64_t* __restrict__ prefix) {
-    __shared__ uint64_t red[kScanThreads / kWave];
+    constexpr int R = kScanRows, NW = kScanThreads / kWave;
+    __shared__ uint64_t wsum[R][NW];  // per row: the waves' totals, then their exclusive offsets
+    __shared__ uint64_t rowtot[R];
+    __shared__ uint64_t rowoff[R + 1];
+    const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
     uint64_t carry = 0;
-    for (size_t c0 = 0; c0 < tiles; c0 += (size_t)kScanThreads * kScanPer) {  // uniform
-        const size_t b = c0 + (size_t)threadIdx.x * kScanPer;
-        uint64_t v[kScanPer], s = 0;
-        if (b + kScanPer <= tiles) {
+    for (size_t c0 = 0; c0 < tiles; c0 += (size_t)R * kScanThreads) {  // uniform
+        uint64_t v[R], inc[R];
 #pragma unroll
-            for (int k = 0; k < kScanPer; k++) v[k] = vals[b + k];
-        } else {
-#pragma unroll
-            for (int k = 0; k < kScanPer; k++) v[k] = b + k < tiles ? vals[b + k] : 0;
+        for (int k = 0; k < R; k++) {
+            const size_t i = c0 + (size_t)k * kScanThreads + t;
+            v[k] = i < tiles ? vals[i] : 0;
         }
 #pragma unroll
-        for (int k = 0; k < kScanPer; k++) s += v[k];
-        uint64_t total;
-        uint64_t run = carry + block_exclusive_scan<uint64_t, kScanThreads>(s, &total, red);
+        for (int k = 0; k < R; k++) inc[k] = wave_inclusive_scan(v[k]);
+        if (lane == kWave - 1) {
 #pragma unroll
-        for (int k = 0; k < kScanPer; k++) {
-            if (b + k < tiles) prefix[b + k] = run;
-            run += v[k];
+            for (int k = 0; k < R; k++) wsum[k][wid] = inc[k];
         }
-        carry += total;
+        __syncthreads();
+        if (t < R) {
+            uint64_t run = 0;
+            for (int w = 0; w < NW; w++) {
+                const uint64_t x = wsum[t][w];
+                wsum[t][w] = run;
+                run += x;
+            }
+            rowtot[t] = run;
+        }
+        __syncthreads();
+        if (t == 0) {
+            uint64_t run = carry;
+            for (int k = 0; k < R; k++) {
+                rowoff[k] = run;
+                run += rowtot[k];
+            }
+            rowoff[R] = run;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < R; k++) {
+            const size_t i = c0 + (size_t)k * kScanThreads + t;
+            if (i < tiles) prefix[i] = rowoff[k] + wsum[k][wid] + inc[k] - v[k];
+        }
+        carry = rowoff[R];
+        __syncthreads();  // wsum / rowoff are rewritten by the next step
     }
-    if (threadIdx.x == 0) prefix[tiles] = carry;
+    if (t == 0) prefix[tiles] = carry;
 }
 
 }  // namespace

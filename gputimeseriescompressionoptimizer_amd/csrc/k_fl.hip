@@ -14,10 +14,10 @@
 // OR-ing the <= ceil(32/b)+1 values that overlap it.  Loads and stores are
 // both fully coalesced.
 //
-// Unpack: a workgroup stages the tile's aligned input words in LDS (an
-// aligned 4-byte word containing a valid byte never crosses a page, so the
-// stream may start at any byte address) and every thread extracts values
-// with a 64/96-bit funnel shift.
+// Unpack: a workgroup stages the tile's input in LDS from the aligned
+// 16-byte chunks that hold it (an aligned chunk containing a valid byte never
+// crosses a page, so the stream may start at any byte address) and every
+// thread extracts values with a 64/96-bit funnel shift.
 #include "dispatch.h"
 #include "k_common.h"
 #include "kernels.h"
@@ -36,33 +36,59 @@ __global__ __launch_bounds__(kBlock) void k_fl_pack(const T* __restrict__ x, siz
     fl_pack_tile<T, DELTA>(x, n, b, out, out_words, blockIdx.x, vals);
 }
 
-constexpr int kMaxWords = kFlTile * 64 / 32 + 8;
+// Staged words of one tile: at most kFlTile * 64 bits, plus the bytes before
+// the stream start in its first 16-byte chunk and one zero chunk after it.
+constexpr int kMaxWords = kFlTile * 64 / 32 + 16;
 
-// Stages the aligned 32-bit words holding the b-bit values [v0, v0 + nv) of
-// the stream `in` (any byte address) in LDS; returns the byte offset of the
-// first value's byte inside w[0].  Bytes outside the stream are zero.
+// Bytes [lo, hi) of a 32-bit word kept, the others cleared.
+__device__ __forceinline__ uint32_t byte_mask(int lo, int hi) {
+    lo = lo < 0 ? 0 : (lo > 4 ? 4 : lo);
+    hi = hi < 0 ? 0 : (hi > 4 ? 4 : hi);
+    if (hi <= lo) return 0u;
+    const uint32_t upto = hi == 4 ? ~0u : ((1u << (8 * hi)) - 1u);
+    const uint32_t from = lo == 4 ? 0u : ~((1u << (8 * lo)) - 1u);
+    return upto & from;
+}
+
+// Stages the b-bit values [v0, v0 + nv) of the stream `in` (any byte
+// address) in LDS as 32-bit words, from the 16-byte chunks that hold them
+// (an aligned 16-byte chunk holding a valid byte never crosses a page), four
+// chunks in flight per thread.  Returns the byte offset of the first value's
+// byte inside w[0]; bytes outside the stream are zero, and one zero chunk
+// follows the last staged one.
 __device__ __forceinline__ int stage_fl_words(const uint8_t* __restrict__ in, size_t in_bytes, size_t v0, int nv, int b,
                                               uint32_t* w) {
     const size_t B0 = v0 * (size_t)b / 8;  // exact: v0 is a multiple of 8
     const size_t need = ((size_t)nv * b + 7) / 8;
     const size_t avail_total = in_bytes > B0 ? in_bytes - B0 : 0;
-    const size_t avail = need < avail_total ? need : avail_total;
+    const int avail = (int)(need < avail_total ? need : avail_total);
     const uint8_t* base = in + B0;
-    const int a = (int)(reinterpret_cast<uintptr_t>(base) & 3);
-    const uint32_t* wp = reinterpret_cast<const uint32_t*>(base - a);
-    const int nw_valid = (int)((a + avail + 3) / 4);
-    int nw_fill = nw_valid + 3;
-    nw_fill = nw_fill < kMaxWords ? nw_fill : kMaxWords;
-    for (int i = threadIdx.x; i < nw_fill; i += kBlock) {
-        uint32_t v = i < nw_valid ? wp[i] : 0u;
-        // bytes before the stream start or after its end are masked out
-        if (i == 0 && a) v &= ~0u << (8 * a);
-        const long long last_byte = (long long)a + (long long)avail;  // exclusive, relative to wp
-        if (i < nw_valid && (long long)(i + 1) * 4 > last_byte) {
-            const int keep = (int)(last_byte - (long long)i * 4);
-            v &= keep >= 4 ? ~0u : ((1u << (8 * keep)) - 1u);
+    const int a = (int)(reinterpret_cast<uintptr_t>(base) & 15);
+    const u32x4* wp4 = reinterpret_cast<const u32x4*>(base - a);
+    const int n4_valid = (a + avail + 15) / 16;
+    const int n4_fill = n4_valid + 1;  // <= kMaxWords / 4
+    u32x4* w4 = reinterpret_cast<u32x4*>(w);
+    constexpr int U = 4;
+    for (int i0 = threadIdx.x; i0 < n4_fill; i0 += U * kBlock) {
+        u32x4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + u * kBlock;
+            v[u] = i < n4_valid ? wp4[i] : u32x4{0u, 0u, 0u, 0u};
         }
-        w[i] = v;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + u * kBlock;
+            if (i < n4_fill) {
+                const int b0 = 16 * i;  // first byte of the chunk, relative to wp4
+                u32x4 m = v[u];
+                m.x &= byte_mask(a - b0, a + avail - b0);
+                m.y &= byte_mask(a - b0 - 4, a + avail - b0 - 4);
+                m.z &= byte_mask(a - b0 - 8, a + avail - b0 - 8);
+                m.w &= byte_mask(a - b0 - 12, a + avail - b0 - 12);
+                w4[i] = m;
+            }
+        }
     }
     return a;
 }
@@ -79,7 +105,7 @@ __device__ __forceinline__ uint64_t fl_value(const uint32_t* w, int k, int b, in
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fl_unpack(const uint8_t* __restrict__ in, size_t in_bytes, size_t n,
                                                       int b, T* __restrict__ out) {
-    __shared__ uint32_t w[kMaxWords];
+    __shared__ __attribute__((aligned(16))) uint32_t w[kMaxWords];
     const size_t v0 = (size_t)blockIdx.x * kFlTile;
     const int nv = (int)((n - v0) < (size_t)kFlTile ? (n - v0) : (size_t)kFlTile);
     const int a = stage_fl_words(in, in_bytes, v0, nv, b, w);
@@ -97,7 +123,7 @@ __global__ __launch_bounds__(kBlock) void k_fl_unpack(const uint8_t* __restrict_
 // costs far less than writing and re-reading the deltas.  out has m + 1 rows.
 __global__ __launch_bounds__(kBlock) void k_fl_delta_sums(const uint8_t* __restrict__ in, size_t in_bytes, size_t m,
                                                           int b, uint64_t* __restrict__ vals) {
-    __shared__ uint32_t w[kMaxWords];
+    __shared__ __attribute__((aligned(16))) uint32_t w[kMaxWords];
     __shared__ uint64_t red[kBlock / kWave];
     const size_t v0 = (size_t)blockIdx.x * kFlTile;
     const int nv = (int)((m - v0) < (size_t)kFlTile ? (m - v0) : (size_t)kFlTile);
@@ -115,7 +141,7 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_fl_unpack_delta_scan(const uint8_t* first_bytes, const uint8_t* __restrict__ in,
                                                                  size_t in_bytes, size_t m, int b, T* __restrict__ out,
                                                                  const uint64_t* __restrict__ prefix) {
-    __shared__ uint32_t w[kMaxWords];  // the staged words, then the tile's results
+    __shared__ __attribute__((aligned(16))) uint32_t w[kMaxWords];  // the staged words, then the tile's results
     __shared__ uint64_t red[kBlock / kWave];
     constexpr int kPer = kFlTile / kBlock;  // 16 consecutive values per thread
     T first;

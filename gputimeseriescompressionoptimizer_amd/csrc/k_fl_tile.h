@@ -44,11 +44,14 @@ __device__ __forceinline__ void fl_pack_tile(const T* __restrict__ x, size_t n, 
     const int tile_words = kFlTile / 32 * b;  // 128 * b
     const size_t w0 = tile * (size_t)tile_words;
     const uint64_t mask = lowmask64(b);
+    // q / b as (q * ceil(2^32 / b)) >> 32: exact for every b <= 64 and
+    // q < 2^18 + 64 (the error term stays below 1/b)
+    const uint64_t rb = ((1ull << 32) + (uint64_t)b - 1) / (uint64_t)b;
     for (int j = threadIdx.x; j < tile_words; j += kBlock) {
         if (w0 + j >= out_words) break;
         const int bit0 = j * 32;
-        const int va = bit0 / b;
-        int vb = (bit0 + 31) / b;
+        const int va = (int)(((uint64_t)bit0 * rb) >> 32);
+        int vb = (int)(((uint64_t)(bit0 + 31) * rb) >> 32);
         vb = vb < kFlTile - 1 ? vb : kFlTile - 1;
         uint64_t word = 0;
         for (int v = va; v <= vb; v++) {

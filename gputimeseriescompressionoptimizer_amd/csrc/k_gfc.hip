@@ -322,7 +322,7 @@ void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const 
                 const uint8_t* payload, size_t payload_bytes, DevBuf& out) {
     if (meta_bytes < 12) fail(DDJ_ECORRUPT, "GFC metadata shorter than 12 bytes");
     int32_t h_meta[3];
-    d2h(c, h_meta, meta, 12);
+    read_small(c, h_meta, meta, 12);
     if (h_meta[0] <= 0 || h_meta[1] <= 0 || h_meta[2] <= 0) fail(DDJ_ECORRUPT, "GFC metadata out of range");
     const long long n = h_meta[2];
     GfcGeom g = gfc_geom(dbl, n);

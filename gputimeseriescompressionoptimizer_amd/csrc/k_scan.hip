@@ -38,8 +38,9 @@ template <typename T> struct Pred {
         case PRED_NOT_EQUAL: return v != a;           // _constEncodeStencilKernel (const_encoding.cu:19-25)
         default: {                                    // getMostFrequentStencilKernel (dict_encoding.cu:15-34)
             bool r = false;
-            for (int j = 0; j < set_n; j++)
-                if (v == set[j]) {
+#pragma unroll
+            for (int j = 0; j < 4; j++)               // the dictionary holds at most 4 keys
+                if (j < set_n && v == set[j]) {
                     r = true;
                     code |= (uint32_t)j;              // _compressUniqueKernel ORs every matching index
                 }
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(kBlock) void k_split_count(const T* __restrict__ x,
         const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
 #pragma unroll
         for (int q = 0; q < kTile / V / kBlock; q++) {
-            const u32x4 w = s4[threadIdx.x + kBlock * q];
+            const u32x4 w = __builtin_nontemporal_load(s4 + threadIdx.x + kBlock * q);
             T v[V];
             __builtin_memcpy(v, &w, 16);
 #pragma unroll
@@ -267,6 +268,22 @@ __global__ __launch_bounds__(kBlock) void k_unpack_codes(const uint8_t* __restri
 
 // RleEncoding (reference compression/rle/rle_encode.cu:18-62): a run starts
 // at row 0 and wherever !(x[i] == x[i-1]); the run's value is its first row.
+// The value of lane - 1 (any 1-, 2-, 4- or 8-byte type); lane 0 gets its own.
+template <typename T> __device__ __forceinline__ T shfl_up_value(T v) {
+    using U = typename UnsignedOf<T>::type;
+    U u;
+    __builtin_memcpy(&u, &v, sizeof(T));
+    U r;
+    if constexpr (sizeof(T) == 8) {
+        r = (U)__shfl_up((unsigned long long)u, 1, kWave);
+    } else {
+        r = (U)__shfl_up((unsigned)u, 1, kWave);
+    }
+    T out;
+    __builtin_memcpy(&out, &r, sizeof(T));
+    return out;
+}
+
 // Runs starting in each tile (the RLE encoder's counting pass).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_rle_count(const T* __restrict__ x, size_t n, uint64_t* __restrict__ vals) {
@@ -280,11 +297,13 @@ __global__ __launch_bounds__(kBlock) void k_rle_count(const T* __restrict__ x, s
 #pragma unroll
         for (int q = 0; q < kTile / V / kBlock; q++) {
             const int i = threadIdx.x + kBlock * q;
-            const u32x4 w = s4[i];
+            const u32x4 w = __builtin_nontemporal_load(s4 + i);
             T v[V];
             __builtin_memcpy(v, &w, 16);
             const size_t g = base + (size_t)V * i;
-            T prev = g > 0 ? x[g - 1] : v[0];
+            // the row before this chunk ends the previous lane's chunk
+            T prev = shfl_up_value(v[V - 1]);
+            if ((threadIdx.x & (kWave - 1)) == 0) prev = g > 0 ? x[g - 1] : v[0];
             c += (g == 0 || !(v[0] == prev)) ? 1u : 0u;
 #pragma unroll
             for (int k = 1; k < V; k++) c += !(v[k] == v[k - 1]) ? 1u : 0u;

@@ -1,8 +1,8 @@
 // Device-side parse of a serialized compression tree (reference
 // DecompressNodes, tree/compression_tree.cpp:105-146): one thread walks the
 // pre-order node records and writes a table of (encoding, type, metadata and
-// leaf-data ranges), so the host needs a single read-back instead of one per
-// node header.
+// leaf-data ranges, plus each node's first metadata bytes), so the host needs
+// a single read-back instead of one per node header or small metadata.
 #include "k_common.h"
 #include "tree.h"
 
@@ -34,8 +34,27 @@ __device__ uint64_t rd_le(const uint8_t* p, int n) {
     return v;
 }
 
-__global__ void k_parse_tree(const uint8_t* blob, uint64_t bytes, ParsedNode* out, int max_nodes, int* status) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ int walk_tree(const uint8_t* blob, uint64_t bytes, ParsedNode* out, int max_nodes, int* status);
+
+// Thread 0 walks the records; then every thread copies one byte of the
+// nodes' metadata / leaf-data heads (kMetaHead bytes each).
+__global__ __launch_bounds__(kBlock) void k_parse_tree(const uint8_t* blob, uint64_t bytes, ParsedNode* out,
+                                                       int max_nodes, int* status) {
+    __shared__ int s_count;
+    if (threadIdx.x == 0) s_count = walk_tree(blob, bytes, out, max_nodes, status);
+    __syncthreads();
+    const int count = s_count;
+    for (int i = threadIdx.x; i < count * 2 * kMetaHead; i += kBlock) {
+        ParsedNode& nd = out[i / (2 * kMetaHead)];
+        const int k = i % (2 * kMetaHead);
+        if (k < kMetaHead)
+            nd.meta_head[k] = (uint64_t)k < nd.meta_len ? blob[nd.meta_off + k] : 0;
+        else
+            nd.data_head[k - kMetaHead] = (uint64_t)(k - kMetaHead) < nd.data_len ? blob[nd.data_off + k - kMetaHead] : 0;
+    }
+}
+
+__device__ int walk_tree(const uint8_t* blob, uint64_t bytes, ParsedNode* out, int max_nodes, int* status) {
     int stack_node[kMaxTreeDepth];
     int stack_left[kMaxTreeDepth];
     int sp = 0, count = 0;
@@ -81,12 +100,13 @@ __global__ void k_parse_tree(const uint8_t* blob, uint64_t bytes, ParsedNode* ou
     status[0] = st;
     status[1] = count;
     status[2] = (int)(pos & 0x7fffffff);
+    return count;
 }
 
 }  // namespace
 
 void launch_parse_tree(Ctx* c, const uint8_t* blob, uint64_t bytes, ParsedNode* d_out, int max_nodes, int* d_status) {
-    { KernelTimer kt_(c, "k_parse_tree"); hipLaunchKernelGGL(k_parse_tree, dim3(1), dim3(kWave), 0, c->stream, blob, bytes, d_out, max_nodes, d_status); }
+    { KernelTimer kt_(c, "k_parse_tree"); hipLaunchKernelGGL(k_parse_tree, dim3(1), dim3(kBlock), 0, c->stream, blob, bytes, d_out, max_nodes, d_status); }
     HIPC(hipGetLastError());
 }
 

@@ -100,9 +100,22 @@ Allocator::~Allocator() {
 
 void d2h(Ctx* c, void* dst, const void* src, size_t bytes) {
     if (!bytes) return;
-    HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    const bool small = c->readback && bytes <= Ctx::kReadbackSize;
+    HIPC(hipMemcpyAsync(small ? c->readback : dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
+    if (small) memcpy(dst, c->readback, bytes);
     c->staging_off = 0;
+}
+
+void read_small(Ctx* c, void* dst, const void* src, size_t bytes) {
+    const uint8_t* s = static_cast<const uint8_t*>(src);
+    for (const auto& sh : c->shadows) {
+        if (s >= sh.dev && s + bytes <= sh.dev + sh.n) {
+            memcpy(dst, sh.host + (s - sh.dev), bytes);
+            return;
+        }
+    }
+    d2h(c, dst, src, bytes);
 }
 
 void h2d(Ctx* c, void* dst, const void* src, size_t bytes) {

@@ -260,6 +260,20 @@ Part decompress_tree(Ctx* c, DView blob, int* dtype_out, NodeP* tree_out) {
         d2h(c, nodes.data(), table, sizeof(ParsedNode) * count);
     std::vector<std::vector<int>> kids((size_t)count);
     for (int i = 1; i < count; i++) kids[(size_t)nodes[i].parent].push_back(i);
+    // The table carries each node's leading metadata (and leaf data) bytes:
+    // decoders read their small metadata from these host copies.
+    struct ShadowScope {
+        Ctx* c;
+        size_t keep;
+        ~ShadowScope() { c->shadows.resize(keep); }
+    } scope{c, c->shadows.size()};
+    for (int i = 0; i < count; i++) {
+        const ParsedNode& nd = nodes[(size_t)i];
+        c->shadows.push_back({blob.p + nd.meta_off, (size_t)std::min<uint64_t>(nd.meta_len, kMetaHead), nd.meta_head});
+        if (nd.enc == DDJ_ENC_NONE)
+            c->shadows.push_back(
+                {blob.p + nd.data_off, (size_t)std::min<uint64_t>(nd.data_len, kMetaHead), nd.data_head});
+    }
     if (tree_out) {
         std::vector<NodeP> ns((size_t)count);
         for (int i = 0; i < count; i++) ns[i] = make_node(nodes[i].enc, nodes[i].dtype);

@@ -42,9 +42,12 @@ Part decompress_tree(Ctx* c, DView blob, int* dtype_out, NodeP* tree_out = nullp
 // CompressionNode::PredictCompressionSize (compression_node.cpp:163-177)
 size_t predict_tree(Ctx* c, const NodeP& n, DView data);
 
+constexpr int kMetaHead = 32;  // leading metadata bytes copied into the parse table
 struct ParsedNode {
     int32_t enc, dtype, nchild, parent;
     uint64_t meta_off, meta_len, data_off, data_len;
+    uint8_t meta_head[kMetaHead];  // metadata bytes [0, min(meta_len, kMetaHead))
+    uint8_t data_head[kMetaHead];  // leaves: data bytes [0, min(data_len, kMetaHead))
 };
 void launch_parse_tree(Ctx* c, const uint8_t* blob, uint64_t bytes, ParsedNode* d_out, int max_nodes, int* d_status);
 

@@ -109,9 +109,22 @@ def main():
     ap.add_argument("--exhaustive-every", type=int, default=20)
     ap.add_argument("--out-dir", default=".")
     ap.add_argument("--seed", type=int, default=7)
+    ap.add_argument("--eval-only", metavar="WEIGHTS", help="skip labelling and training; evaluate these weights")
+    ap.add_argument("--no-eval", action="store_true", help="label and train only")
     args = ap.parse_args()
     os.makedirs(args.out_dir, exist_ok=True)
     ctx = g.Context(0)
+    if args.eval_only:
+        net = g.LearnedPlanner(ctx)
+        net.Load(args.eval_only)
+        ev = evaluate(ctx, net, PD.regime_stream(args.eval_blocks, args.rows, args.regime, args.seed + 1000),
+                      args.exhaustive_every, log_every=500)
+        res = {"weights": os.path.basename(args.eval_only), "rows_per_block": args.rows, "regime": args.regime,
+               "eval": ev}
+        with open(os.path.join(args.out_dir, "planner_eval.json"), "w") as f:
+            json.dump(res, f, indent=1)
+        print(json.dumps(res))
+        return
     X, T, fams = label_blocks(ctx, PD.random_blocks(args.blocks, args.rows, args.seed))
     np.savez_compressed(os.path.join(args.out_dir, "planner_dataset.npz"), X=X, T=T, fams=np.array(fams))
     n_val = max(1, len(X) // 10)
@@ -133,6 +146,11 @@ def main():
     train_s = time.perf_counter() - t0
     weights = os.path.join(args.out_dir, "planner_weights.npy")
     net.Save(weights)
+    if args.no_eval:
+        with open(os.path.join(args.out_dir, "planner_train.json"), "w") as f:
+            json.dump({"train_blocks": len(Xtr), "rows_per_block": args.rows, "epochs": args.epochs,
+                       "train_s": train_s, "history": history}, f, indent=1)
+        return
     ev = evaluate(ctx, net, PD.regime_stream(args.eval_blocks, args.rows, args.regime, args.seed + 1000),
                   args.exhaustive_every, log_every=50)
     res = {"train_blocks": len(Xtr), "rows_per_block": args.rows, "epochs": args.epochs, "train_s": train_s,

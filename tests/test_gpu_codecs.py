@@ -195,3 +195,29 @@ def test_rle_negative_run_length_is_corrupt(ctx):
         g.Encoding(ctx, g.RLE).Decode(bad, g.D_INT)
     # the context stays usable afterwards
     assert g.Encoding(ctx, g.RLE).Decode(outs, g.D_INT).to_bytes() == x.tobytes()
+
+
+MANY_TILES = 4096 * 4096 + 12_345  # more than one 4096-value chunk of tile offsets
+
+
+@pytest.mark.parametrize("enc,dtype,family", [(g.PATCH, g.D_INT, "rand"), (g.DICT, g.D_INT, "runs"),
+                                              (g.RLE, g.D_INT, "runs"), (g.CONST, g.D_INT, "runs"),
+                                              (g.DELTA, g.D_INT, "walk"), (g.GFC, g.D_DOUBLE, "walk"),
+                                              (g.AFL, g.D_INT, "rand")],
+                         ids=["patch", "dict", "rle", "const", "delta", "gfc", "afl"])
+def test_codec_parity_many_tiles(ctx, enc, dtype, family):
+    """Variable-length outputs whose tile offsets span several scan chunks."""
+    print(f"[codec many tiles] {g.ENCODING_NAMES[enc]} dtype={dtype} n={MANY_TILES}", flush=True)
+    x = make(dtype, MANY_TILES, family, seed=77)
+    check(ctx, enc, dtype, x)
+
+
+def test_delta_afl_tree_many_tiles(ctx):
+    """The fused delta -> FL encode and decode over more than 4096 tiles."""
+    x = make(g.D_INT, MANY_TILES, "walk", seed=78)
+    tree = g.CompressionTree(ctx, [g.DELTA, g.AFL, g.NONE], g.D_INT)
+    blob = tree.Compress(ctx.upload(x))
+    ref = O.tree_from_path([g.DELTA, g.AFL, g.NONE], O.D_INT)
+    ref.fix()
+    assert blob.to_bytes() == O.compress(ref, x)
+    assert np.array_equal(tree.Decompress(blob).to_numpy(np.int32), x)

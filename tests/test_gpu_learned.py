@@ -130,3 +130,30 @@ def test_plan_follows_the_smallest_prediction(ctx):
     # a margin nothing reaches sends the block to the exhaustive planner
     path2, fallback2, _ = net.Plan(buf, g.D_INT, min_margin=1e9)
     assert fallback2 and path2 == O.plan_exhaustive(x, O.D_INT)[0].preorder()
+
+
+def test_trained_weights_stay_near_exhaustive_and_beat_untrained(ctx):
+    """The committed weights, on a stream whose generator family changes every
+    10 blocks, choose trees whose total compressed size is within 2% of the
+    exhaustive planner's; a network of the same shape without training does
+    clearly worse on the same blocks (what the training gains)."""
+    from gputimeseriescompressionoptimizer_amd import planner_data as PD
+    from gputimeseriescompressionoptimizer_amd.train_planner import WEIGHTS
+
+    net = g.LearnedPlanner(ctx)
+    net.Load(WEIGHTS)
+    untrained = g.LearnedPlanner(ctx, seed=123)
+
+    def compressed(path, buf):
+        return g.CompressionTree(ctx, path, g.D_INT).Compress(buf).nbytes
+
+    tot = {"learned": 0, "untrained": 0, "exhaustive": 0}
+    for _, fam, x in PD.regime_stream(90, 65536, 10, seed=99):
+        buf = ctx.upload(x)
+        tot["learned"] += compressed(net.Plan(buf, g.D_INT)[0], buf)
+        tot["untrained"] += compressed(untrained.Plan(buf, g.D_INT)[0], buf)
+        tree, _, _ = g.CompressionOptimizer(ctx).PlanExhaustive(buf, g.D_INT)
+        tot["exhaustive"] += tree.Compress(buf).nbytes
+    print(tot, flush=True)
+    assert tot["learned"] <= 1.02 * tot["exhaustive"]
+    assert tot["untrained"] > 1.05 * tot["learned"]
