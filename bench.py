@@ -262,6 +262,8 @@ def main():
     ap.add_argument("--batch-workers", type=int, default=4)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-float", action="store_true")
+    ap.add_argument("--compress-only", action="store_true",
+                    help="warmup and timed compress steps only (for hardware-counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (columns would shard "
@@ -308,6 +310,10 @@ def main():
     def comp_step():
         holder["blobs"] = compress_all()
 
+    if args.compress_only:
+        t_comp, _ = timed(comp_step, False)
+        print(json.dumps({"compress_only": True, "ms_per_step": round(t_comp * 1e3, 3), "steps": args.steps}))
+        return
     t_comp, _ = timed(comp_step, False)           # value: no events in the way
     t_comp_prof, ktab = timed(comp_step, True)    # kernel table: HIP events around every launch
     blobs = holder["blobs"]
