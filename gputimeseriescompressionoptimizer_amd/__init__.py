@@ -115,6 +115,8 @@ def load_library(path: str = LIB_PATH):
         "ddj_ts_reader_next": (i32, [vp, P(vp), sz, P(sz)]),
         "ddj_ts_reader_close": (None, [vp]),
         "ddj_batch_compress": (i32, [vp, vp, u32, u32, P(_Buf)]),
+        "ddj_batch_compress_config": (i32, [vp, vp, u32, vp, P(_Buf)]),
+        "ddj_batch_autotune": (i32, [vp, vp, u32, vp, P(ctypes.c_double)]),
         "ddj_planner_candidates": (i32, [P(i32), sz, P(sz)]),
         "ddj_planner_features": (i32, [vp, i32, vp, sz, P(ctypes.c_float)]),
         "ddj_planner_labels": (i32, [vp, i32, vp, sz, P(ctypes.c_float)]),
@@ -641,13 +643,40 @@ class _Column(ctypes.Structure):
                 ("preorder", ctypes.POINTER(ctypes.c_int)), ("path_len", ctypes.c_size_t)]
 
 
-def batch_compress(ctx: Context, columns: Sequence[Tuple[DeviceBuffer, int, Sequence[int]]],
-                   workers: int = 4) -> List[DeviceBuffer]:
-    """Compresses (buffer, dtype, tree path) columns together (ddj_batch_compress)."""
+class _BatchConfig(ctypes.Structure):
+    _fields_ = [("workers", ctypes.c_uint32), ("range_rows_log2", ctypes.c_uint32), ("wg_per_cu", ctypes.c_uint32)]
+
+
+def _columns(columns):
     keep = [(ctypes.c_int * len(p))(*p) for _, _, p in columns]
-    arr = (_Column * len(columns))()
+    arr = (_Column * max(len(columns), 1))()
     for i, ((buf, dt, path), pa) in enumerate(zip(columns, keep)):
         arr[i] = _Column(dt, buf.dev or None, buf.nbytes, pa, len(path))
+    return arr, keep
+
+
+def batch_compress(ctx: Context, columns: Sequence[Tuple[DeviceBuffer, int, Sequence[int]]],
+                   workers: int = 4, config: Optional[dict] = None) -> List[DeviceBuffer]:
+    """Compresses (buffer, dtype, tree path) columns together
+    (ddj_batch_compress, or ddj_batch_compress_config with `config`:
+    {"workers", "range_rows_log2", "wg_per_cu"}, zero meaning default)."""
+    arr, keep = _columns(columns)
     outs = (_Buf * max(len(columns), 1))()
-    ctx.check(ctx._L.ddj_batch_compress(ctx._h, arr, len(columns), workers, outs), "batch compress")
+    if config is None:
+        ctx.check(ctx._L.ddj_batch_compress(ctx._h, arr, len(columns), workers, outs), "batch compress")
+    else:
+        cfg = _BatchConfig(config.get("workers", 0), config.get("range_rows_log2", 0), config.get("wg_per_cu", 0))
+        ctx.check(ctx._L.ddj_batch_compress_config(ctx._h, arr, len(columns), ctypes.byref(cfg), outs),
+                  "batch compress")
     return [ctx._wrap(outs[i]) for i in range(len(columns))]
+
+
+def batch_autotune(ctx: Context, columns: Sequence[Tuple[DeviceBuffer, int, Sequence[int]]]) -> Tuple[dict, float]:
+    """The fastest batch configuration for these columns (ddj_batch_autotune)
+    and its time in ms."""
+    arr, keep = _columns(columns)
+    cfg = _BatchConfig()
+    ms = ctypes.c_double()
+    ctx.check(ctx._L.ddj_batch_autotune(ctx._h, arr, len(columns), ctypes.byref(cfg), ctypes.byref(ms)),
+              "batch autotune")
+    return {"workers": cfg.workers, "range_rows_log2": cfg.range_rows_log2, "wg_per_cu": cfg.wg_per_cu}, ms.value

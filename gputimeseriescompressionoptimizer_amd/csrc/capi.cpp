@@ -1,5 +1,6 @@
 // The C ABI (include/ddj.h): status codes at the boundary, C++ inside.
 #include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -64,6 +65,7 @@ struct OwnedCtx {
 // the batch layout (pointers, sizes, types, trees) stays the same.
 struct BatchGraph {
     std::vector<std::tuple<const void*, size_t, int, int>> key;
+    uint32_t range_rows_log2 = 0, wg_per_cu = 0;
     std::unique_ptr<DevBuf> cols, ranges, tiles, arena;
     std::vector<BatchColumn> host_cols;
     hipGraph_t graph = nullptr;
@@ -681,17 +683,21 @@ int fl_leaf_kind(const ddj_column& col) {
 
 // Compresses the FL-leaf columns `idx` of `cols` through the captured graph
 // (captured on first use of a layout) into blobs owned by ctx.
-void batch_fl_graph(ddj_ctx* ctx, const ddj_column* cols, const std::vector<uint32_t>& idx, ddj_buf* blobs) {
+void batch_fl_graph(ddj_ctx* ctx, const ddj_column* cols, const std::vector<uint32_t>& idx,
+                    const ddj_batch_config& cfg, ddj_buf* blobs) {
     Ctx* c = &ctx->c;
     std::vector<std::tuple<const void*, size_t, int, int>> key;
     for (uint32_t i : idx) key.emplace_back(cols[i].d_in, cols[i].in_bytes, cols[i].dtype, fl_leaf_kind(cols[i]));
-    if (!ctx->bgraph || ctx->bgraph->key != key) {
+    if (!ctx->bgraph || ctx->bgraph->key != key || ctx->bgraph->range_rows_log2 != cfg.range_rows_log2 ||
+        ctx->bgraph->wg_per_cu != cfg.wg_per_cu) {
         ctx->bgraph.reset();
         std::unique_ptr<BatchGraph> g(new BatchGraph);
         g->key = key;
+        g->range_rows_log2 = cfg.range_rows_log2;
+        g->wg_per_cu = cfg.wg_per_cu;
         std::vector<BatchItem> ranges, tiles;
         uint64_t arena = 16;
-        constexpr uint64_t kRange = 1ull << 18;  // rows per min/max item
+        const uint64_t kRange = 1ull << cfg.range_rows_log2;  // rows per min/max item
         for (size_t k = 0; k < idx.size(); k++) {
             const ddj_column& col = cols[idx[k]];
             BatchColumn d{};
@@ -717,7 +723,8 @@ void batch_fl_graph(ddj_ctx* ctx, const ddj_column* cols, const std::vector<uint
         HIPC(hipStreamSynchronize(c->stream));
         HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         launch_batch_fl(c, g->cols->get<BatchColumn>(), (int)g->host_cols.size(), g->ranges->get<BatchItem>(),
-                        (int)ranges.size(), g->tiles->get<BatchItem>(), (int)tiles.size(), g->arena->bytes_ptr());
+                        (int)ranges.size(), g->tiles->get<BatchItem>(), (int)tiles.size(), g->arena->bytes_ptr(),
+                        cfg.wg_per_cu);
         HIPC(hipStreamEndCapture(c->stream, &g->graph));
         HIPC(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
         ctx->bgraph = std::move(g);
@@ -741,6 +748,92 @@ void batch_fl_graph(ddj_ctx* ctx, const ddj_column* cols, const std::vector<uint
     HIPC(hipStreamSynchronize(c->stream));
 }
 
+ddj_batch_config resolve(const ddj_batch_config* in) {
+    ddj_batch_config c = in ? *in : ddj_batch_config{0, 0, 0};
+    if (c.workers == 0) c.workers = 4;  // the hardware queues one process gets
+    if (c.range_rows_log2 == 0) c.range_rows_log2 = 18;
+    if (c.wg_per_cu == 0) c.wg_per_cu = 4;
+    if (c.workers > 16 || c.range_rows_log2 < 12 || c.range_rows_log2 > 24 || c.wg_per_cu > 16)
+        fail(DDJ_EINVAL, "batch configuration out of range");
+    return c;
+}
+
+void batch_run(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_cols, const ddj_batch_config& cfg, ddj_buf* blobs) {
+    if ((!cols || !blobs) && n_cols) fail(DDJ_EINVAL, "bad argument");
+    const uint32_t workers = cfg.workers;
+    while (ctx->pool.size() < workers) {
+        std::unique_ptr<OwnedCtx> w(new OwnedCtx);
+        const hipError_t e = init_ctx(w->c, ctx->c.device);
+        if (e != hipSuccess) fail(e == hipErrorOutOfMemory ? DDJ_ENOMEM : DDJ_EHIP, hipGetErrorString(e));
+        ctx->pool.push_back(std::move(w));
+    }
+    for (uint32_t i = 0; i < n_cols; i++) blobs[i] = ddj_buf{nullptr, 0};
+    HIPC(hipStreamSynchronize(ctx->c.stream));  // inputs written on the caller's stream are in place
+    // FL-leaf columns go through the captured graph on the caller's
+    // stream; the others through the worker contexts, concurrently
+    std::vector<uint32_t> graph_cols, other_cols;
+    for (uint32_t i = 0; i < n_cols; i++) (fl_leaf_kind(cols[i]) ? graph_cols : other_cols).push_back(i);
+    std::atomic<uint32_t> next{0};
+    std::mutex mu;  // the caller's allocator and the first error
+    ddj_status first_st = DDJ_OK;
+    std::string first_msg;
+    auto work = [&](Ctx* w) {
+        (void)hipSetDevice(w->device);
+        for (;;) {
+            const uint32_t q = next.fetch_add(1);
+            if (q >= other_cols.size()) return;
+            const uint32_t i = other_cols[q];
+            try {
+                NodeP t = tree_from_path(cols[i].preorder, cols[i].path_len, cols[i].dtype);
+                fix_tree(t);
+                Part blob = compress_tree(w, t, DView{static_cast<const uint8_t*>(cols[i].d_in), cols[i].in_bytes});
+                HIPC(hipStreamSynchronize(w->stream));
+                check_device_error(w);
+                // the blob's block moves from the worker's allocator to the caller's
+                const ddj_buf b = blob.owner->release();
+                size_t cls = 0;
+                if (!w->alloc->detach(b.dev, &cls)) fail(DDJ_EHIP, "blob not owned by the worker");
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    ctx->c.alloc->adopt(b.dev, cls);
+                }
+                blobs[i] = ddj_buf{b.dev, blob.n};
+            } catch (const Error& e) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (first_st == DDJ_OK) {
+                    first_st = e.status;
+                    first_msg = "column " + std::to_string(i) + ": " + e.msg;
+                }
+            } catch (const std::exception& e) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (first_st == DDJ_OK) {
+                    first_st = DDJ_EINVAL;
+                    first_msg = "column " + std::to_string(i) + ": " + e.what();
+                }
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (uint32_t k = 0; k < workers && k < other_cols.size(); k++) th.emplace_back(work, &ctx->pool[k]->c);
+    if (!graph_cols.empty()) {
+        try {
+            batch_fl_graph(ctx, cols, graph_cols, cfg, blobs);
+        } catch (const Error& e) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (first_st == DDJ_OK) {
+                first_st = e.status;
+                first_msg = "graph batch: " + e.msg;
+            }
+        }
+    }
+    for (auto& t : th) t.join();
+    if (first_st != DDJ_OK) {
+        for (uint32_t i = 0; i < n_cols; i++)
+            if (blobs[i].dev) ctx->c.alloc->free(blobs[i].dev), blobs[i] = ddj_buf{nullptr, 0};
+        fail(first_st, first_msg);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -748,78 +841,53 @@ extern "C" {
 ddj_status ddj_batch_compress(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_cols, uint32_t workers,
                               ddj_buf* blobs) {
     return guarded(ctx, [&] {
-        if ((!cols || !blobs) && n_cols) fail(DDJ_EINVAL, "bad argument");
-        if (workers == 0) workers = 4;  // the hardware queues one process gets
-        if (workers > 16) workers = 16;
-        while (ctx->pool.size() < workers) {
-            std::unique_ptr<OwnedCtx> w(new OwnedCtx);
-            const hipError_t e = init_ctx(w->c, ctx->c.device);
-            if (e != hipSuccess) fail(e == hipErrorOutOfMemory ? DDJ_ENOMEM : DDJ_EHIP, hipGetErrorString(e));
-            ctx->pool.push_back(std::move(w));
-        }
-        for (uint32_t i = 0; i < n_cols; i++) blobs[i] = ddj_buf{nullptr, 0};
-        HIPC(hipStreamSynchronize(ctx->c.stream));  // inputs written on the caller's stream are in place
-        // FL-leaf columns go through the captured graph on the caller's
-        // stream; the others through the worker contexts, concurrently
-        std::vector<uint32_t> graph_cols, other_cols;
-        for (uint32_t i = 0; i < n_cols; i++) (fl_leaf_kind(cols[i]) ? graph_cols : other_cols).push_back(i);
-        std::atomic<uint32_t> next{0};
-        std::mutex mu;  // the caller's allocator and the first error
-        ddj_status first_st = DDJ_OK;
-        std::string first_msg;
-        auto work = [&](Ctx* w) {
-            (void)hipSetDevice(w->device);
-            for (;;) {
-                const uint32_t q = next.fetch_add(1);
-                if (q >= other_cols.size()) return;
-                const uint32_t i = other_cols[q];
-                try {
-                    NodeP t = tree_from_path(cols[i].preorder, cols[i].path_len, cols[i].dtype);
-                    fix_tree(t);
-                    Part blob = compress_tree(w, t, DView{static_cast<const uint8_t*>(cols[i].d_in), cols[i].in_bytes});
-                    void* dst = nullptr;
-                    {
-                        std::lock_guard<std::mutex> lk(mu);
-                        dst = ctx->c.alloc->alloc(blob.n);
-                    }
-                    d2d(w, dst, blob.p, blob.n);
-                    HIPC(hipStreamSynchronize(w->stream));
-                    check_device_error(w);
-                    blobs[i] = ddj_buf{dst, blob.n};
-                } catch (const Error& e) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    if (first_st == DDJ_OK) {
-                        first_st = e.status;
-                        first_msg = "column " + std::to_string(i) + ": " + e.msg;
-                    }
-                } catch (const std::exception& e) {
-                    std::lock_guard<std::mutex> lk(mu);
-                    if (first_st == DDJ_OK) {
-                        first_st = DDJ_EINVAL;
-                        first_msg = "column " + std::to_string(i) + ": " + e.what();
-                    }
-                }
-            }
+        ddj_batch_config c{workers > 16 ? 16 : workers, 0, 0};
+        batch_run(ctx, cols, n_cols, resolve(&c), blobs);
+    });
+}
+
+ddj_status ddj_batch_compress_config(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_cols,
+                                     const ddj_batch_config* config, ddj_buf* blobs) {
+    return guarded(ctx, [&] { batch_run(ctx, cols, n_cols, resolve(config), blobs); });
+}
+
+ddj_status ddj_batch_autotune(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_cols, ddj_batch_config* best,
+                              double* best_ms) {
+    return guarded(ctx, [&] {
+        if (!best) fail(DDJ_EINVAL, "null output");
+        std::vector<ddj_buf> blobs(std::max<uint32_t>(n_cols, 1));
+        auto once = [&](const ddj_batch_config& cfg) {
+            batch_run(ctx, cols, n_cols, cfg, blobs.data());
+            HIPC(hipStreamSynchronize(ctx->c.stream));
+            for (uint32_t i = 0; i < n_cols; i++) ddj_buf_free(ctx, &blobs[i]);
         };
-        std::vector<std::thread> th;
-        for (uint32_t k = 0; k < workers && k < other_cols.size(); k++) th.emplace_back(work, &ctx->pool[k]->c);
-        if (!graph_cols.empty()) {
-            try {
-                batch_fl_graph(ctx, cols, graph_cols, blobs);
-            } catch (const Error& e) {
-                std::lock_guard<std::mutex> lk(mu);
-                if (first_st == DDJ_OK) {
-                    first_st = e.status;
-                    first_msg = "graph batch: " + e.msg;
+        auto timed = [&](const ddj_batch_config& cfg) {
+            once(cfg);  // graph capture, worker contexts, allocator warm-up
+            const auto t0 = std::chrono::steady_clock::now();
+            once(cfg);
+            return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        };
+        bool has_graph = false;
+        for (uint32_t i = 0; i < n_cols; i++) has_graph = has_graph || fl_leaf_kind(cols[i]) != 0;
+        ddj_batch_config cur = resolve(nullptr);
+        double cur_ms = 1e300;
+        for (uint32_t w : {2u, 4u, 8u, 16u}) {
+            const ddj_batch_config cfg{w, cur.range_rows_log2, cur.wg_per_cu};
+            const double ms = timed(cfg);
+            if (ms < cur_ms) cur_ms = ms, cur = cfg;
+        }
+        if (has_graph) {
+            const ddj_batch_config base = cur;
+            for (uint32_t r : {16u, 18u, 20u})
+                for (uint32_t g : {2u, 4u, 8u}) {
+                    const ddj_batch_config cfg{base.workers, r, g};
+                    if (cfg.range_rows_log2 == base.range_rows_log2 && cfg.wg_per_cu == base.wg_per_cu) continue;
+                    const double ms = timed(cfg);
+                    if (ms < cur_ms) cur_ms = ms, cur = cfg;
                 }
-            }
         }
-        for (auto& t : th) t.join();
-        if (first_st != DDJ_OK) {
-            for (uint32_t i = 0; i < n_cols; i++)
-                if (blobs[i].dev) ctx->c.alloc->free(blobs[i].dev), blobs[i] = ddj_buf{nullptr, 0};
-            fail(first_st, first_msg);
-        }
+        *best = cur;
+        if (best_ms) *best_ms = cur_ms;
     });
 }
 

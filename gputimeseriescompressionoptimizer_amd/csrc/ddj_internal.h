@@ -33,20 +33,27 @@ inline bool dtype_ok(int dtype) { return dtype >= DDJ_D_INT && dtype <= DDJ_D_CH
 // Stream-ordered caching allocator: every kernel of a context runs on one
 // stream, so a block freed by the host can be handed to the next request
 // without synchronising.  Requests round up to a size class; blocks go back
-// to a per-class free list.
+// to a per-class free list.  Thread-safe: a batched call's worker threads
+// hand blocks over while the calling thread allocates.
 class Allocator {
 public:
     explicit Allocator(int device) : device_(device) {}
     ~Allocator();
     void* alloc(size_t bytes);
     void free(void* p);
-    // Hands a live block to the caller (released later with hipFree).
-    bool detach(void* p);
+    // Hands a live block to the caller (released later with hipFree);
+    // *cls receives its size class.
+    bool detach(void* p, size_t* cls = nullptr);
+    // Takes over a block detached from another allocator of the same device
+    // (its stream work on the block must be complete).
+    void adopt(void* p, size_t cls);
     void trim();
     size_t in_use() const { return in_use_; }
 
 private:
     static size_t round_size(size_t bytes);
+    void trim_locked();
+    std::mutex mu_;
     int device_;
     std::map<size_t, std::vector<void*>> free_;
     std::map<void*, size_t> live_;

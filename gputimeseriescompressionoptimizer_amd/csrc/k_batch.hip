@@ -179,8 +179,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_pack(const BatchColumn* cols, 
 }  // namespace
 
 void launch_batch_fl(Ctx* c, BatchColumn* d_cols, int ncol, const BatchItem* d_ranges, int nranges,
-                     const BatchItem* d_tiles, int ntiles, uint8_t* arena) {
-    const unsigned persistent = (unsigned)c->cu_count * 4;
+                     const BatchItem* d_tiles, int ntiles, uint8_t* arena, unsigned wg_per_cu) {
+    const unsigned persistent = (unsigned)c->cu_count * wg_per_cu;
     hipLaunchKernelGGL(k_batch_init, dim3((ncol + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, d_cols, ncol);
     hipLaunchKernelGGL(k_batch_minmax, dim3(std::min<unsigned>(persistent, (unsigned)nranges)), dim3(kBlock), 0,
                        c->stream, d_cols, d_ranges, nranges);

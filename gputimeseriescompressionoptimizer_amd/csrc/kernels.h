@@ -156,7 +156,7 @@ constexpr uint64_t kFlTileRows = 4096;  // values per FL pack tile
 // The four launches (init, min/max, layout, pack); no host round trip, so
 // the sequence can be captured into a graph.
 void launch_batch_fl(Ctx* c, BatchColumn* d_cols, int ncol, const BatchItem* d_ranges, int nranges,
-                     const BatchItem* d_tiles, int ntiles, uint8_t* arena);
+                     const BatchItem* d_tiles, int ntiles, uint8_t* arena, unsigned wg_per_cu);
 
 // Zero-fill of device memory by a kernel (graph-capture safe).
 void launch_fill_zero(Ctx* c, void* p, size_t bytes);

@@ -44,6 +44,7 @@ size_t Allocator::round_size(size_t bytes) {
 }
 
 void* Allocator::alloc(size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu_);
     size_t s = round_size(bytes);
     auto it = free_.find(s);
     if (it != free_.end() && !it->second.empty()) {
@@ -57,7 +58,7 @@ void* Allocator::alloc(size_t bytes) {
     hipError_t e = hipMalloc(&p, s);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        trim();  // give cached blocks back and retry once
+        trim_locked();  // give cached blocks back and retry once
         e = hipMalloc(&p, s);
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -71,6 +72,7 @@ void* Allocator::alloc(size_t bytes) {
 
 void Allocator::free(void* p) {
     if (!p) return;
+    std::lock_guard<std::mutex> lk(mu_);
     auto it = live_.find(p);
     if (it == live_.end()) return;
     free_[it->second].push_back(p);
@@ -78,15 +80,28 @@ void Allocator::free(void* p) {
     live_.erase(it);
 }
 
-bool Allocator::detach(void* p) {
+bool Allocator::detach(void* p, size_t* cls) {
+    std::lock_guard<std::mutex> lk(mu_);
     auto it = live_.find(p);
     if (it == live_.end()) return false;
+    if (cls) *cls = it->second;
     in_use_ -= it->second;
     live_.erase(it);
     return true;
 }
 
+void Allocator::adopt(void* p, size_t cls) {
+    std::lock_guard<std::mutex> lk(mu_);
+    live_[p] = cls;
+    in_use_ += cls;
+}
+
 void Allocator::trim() {
+    std::lock_guard<std::mutex> lk(mu_);
+    trim_locked();
+}
+
+void Allocator::trim_locked() {
     (void)hipDeviceSynchronize();
     for (auto& kv : free_)
         for (void* p : kv.second) (void)hipFree(p);
@@ -94,7 +109,7 @@ void Allocator::trim() {
 }
 
 Allocator::~Allocator() {
-    trim();
+    trim_locked();
     for (auto& kv : live_) (void)hipFree(kv.first);
 }
 

@@ -177,6 +177,25 @@ typedef struct {
 } ddj_column;
 ddj_status ddj_batch_compress(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_cols, uint32_t workers /* 0: 4 */,
                               ddj_buf* blobs);
+/* The knobs of a batched call.  Columns whose tree is afl -> none or
+ * delta -> afl -> none on int32/int64 go through one captured graph of
+ * persistent kernels (min/max over work items of 2^range_rows_log2 rows,
+ * then FL packing over 4096-value tiles, wg_per_cu workgroups per CU);
+ * the others through `workers` worker contexts.  Zero fields take the
+ * defaults (4 workers, 2^18 rows, 4 workgroups per CU). */
+typedef struct {
+    uint32_t workers;          /* 1..16 */
+    uint32_t range_rows_log2;  /* 12..24 */
+    uint32_t wg_per_cu;        /* 1..16 */
+} ddj_batch_config;
+ddj_status ddj_batch_compress_config(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_cols,
+                                     const ddj_batch_config* config, ddj_buf* blobs);
+/* Autotuner: compresses the batch under candidate configurations (worker
+ * counts first, then the graph lane's chunk length and workgroups per CU
+ * with the best worker count), timing each after an untimed run, and
+ * returns the fastest in *best with its time.  The columns are read only. */
+ddj_status ddj_batch_autotune(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_cols, ddj_batch_config* best,
+                              double* best_ms);
 
 /* -------------------------------------------------------- learned planner */
 /* No counterpart in the reference.  A 32-128-128-48 network (ReLU, BF16

@@ -65,3 +65,35 @@ def test_batch_graph_replays_on_new_contents(ctx):
         t = O.tree_from_path(path, dt)
         t.fix()
         assert blob.to_bytes() == O.compress(t, y), (dt, path, y.size)
+
+
+@pytest.mark.parametrize("config", [{"workers": 1, "range_rows_log2": 12, "wg_per_cu": 1},
+                                    {"workers": 16, "range_rows_log2": 24, "wg_per_cu": 16},
+                                    {"workers": 3, "range_rows_log2": 16, "wg_per_cu": 2}])
+def test_batch_configurations_give_the_same_blobs(ctx, config):
+    """Every knob setting of the batched call (worker count, min/max chunk
+    length, workgroups per CU) yields the single-column blobs."""
+    cols = columns(7)
+    bufs = [(ctx.upload(x), dt, path) for x, dt, path in cols]
+    want = [g.CompressionTree(ctx, path, dt).Compress(buf).to_bytes() for buf, dt, path in bufs]
+    got = [b.to_bytes() for b in g.batch_compress(ctx, bufs, config=config)]
+    assert got == want
+
+
+def test_batch_autotune_picks_a_valid_configuration(ctx):
+    cols = columns(8)
+    bufs = [(ctx.upload(x), dt, path) for x, dt, path in cols]
+    cfg, ms = g.batch_autotune(ctx, bufs)
+    assert 1 <= cfg["workers"] <= 16 and 12 <= cfg["range_rows_log2"] <= 24 and 1 <= cfg["wg_per_cu"] <= 16
+    assert ms > 0
+    want = [g.CompressionTree(ctx, path, dt).Compress(buf).to_bytes() for buf, dt, path in bufs]
+    assert [b.to_bytes() for b in g.batch_compress(ctx, bufs, config=cfg)] == want
+
+
+def test_batch_configuration_out_of_range_is_rejected(ctx):
+    cols = columns(9)[:2]
+    bufs = [(ctx.upload(x), dt, path) for x, dt, path in cols]
+    with pytest.raises(g.DdjError):
+        g.batch_compress(ctx, bufs, config={"workers": 17})
+    with pytest.raises(g.DdjError):
+        g.batch_compress(ctx, bufs, config={"range_rows_log2": 30})
