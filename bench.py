@@ -327,6 +327,18 @@ def main():
     t_comp_prof, ktab = timed(comp_step, True)    # kernel table: HIP events around every launch
     blobs = holder["blobs"]
 
+    # the same job with the four columns overlapped (ddj_batch_compress,
+    # worker streams, autotuned): blobs must be identical
+    bcols = [(buf, g.D_INT, path) for _, buf, path, _ in dev]
+    bcfg, _ = g.batch_autotune(ctx, bcols)
+    same_concurrent = all(a.to_bytes() == b.to_bytes() for a, b in zip(g.batch_compress(ctx, bcols, config=bcfg), blobs))
+
+    def conc_step():
+        holder["cblobs"] = g.batch_compress(ctx, bcols, config=bcfg)
+
+    t_conc, _ = timed(conc_step, False)
+    holder.pop("cblobs", None)
+
     # decompression of the same blobs, verified once
     outs = decompress_all(blobs)
     for (_, _, _, x), o in zip(dev, outs):
@@ -383,6 +395,8 @@ def main():
         "ratios": ratios,
         "roofline": roofline,
         "profiled_ms_per_step": {"compress": round(t_comp_prof * 1e3, 3), "decompress": round(t_dec_prof * 1e3, 3)},
+        "concurrent_columns": {"compress_gbps": round(in_bytes / t_conc / 1e9, 3), "ms_per_step": round(t_conc * 1e3, 3),
+                               "config": bcfg, "blobs_identical": same_concurrent},
         "kernels": kernels,
         "decompress_kernels": dkernels,
     }

@@ -126,11 +126,11 @@ template <typename F> ddj_status guarded(ddj_ctx* ctx, F&& f) {
     }
 }
 
-ddj_buf to_buf(Ctx* c, const Part& hp) {
+ddj_buf to_buf(Ctx* c, const Part& p) {
     // hand out a buffer the caller owns: reuse the block when the part owns
-    // all of it, otherwise copy
-    if (hp.n == 0) return ddj_buf{nullptr, 0};
-    const Part p = on_device(c, hp);
+    // all of it, otherwise copy (host-resident metadata is uploaded)
+    if (p.n == 0) return ddj_buf{nullptr, 0};
+    if (p.on_host()) return make_buf(c, p.host->data(), p.n).release();
     if (p.owner && p.owner.use_count() == 1 && p.owner->bytes_ptr() == p.p) {
         DevBuf& b = *p.owner;
         b.set_size(p.n);

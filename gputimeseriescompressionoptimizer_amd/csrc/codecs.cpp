@@ -32,11 +32,6 @@ Part host_part(const void* h, size_t n) {
     return p;
 }
 
-Part on_device(Ctx* c, const Part& p) {
-    if (!p.on_host()) return p;
-    return own(make_buf(c, p.host->data(), p.n));
-}
-
 Part borrow(DView v, std::shared_ptr<DevBuf> keep) {
     Part p;
     p.owner = std::move(keep);

@@ -25,8 +25,6 @@ struct Part {
 Part own(DevBuf&& b);
 Part borrow(DView v, std::shared_ptr<DevBuf> keep = nullptr);
 Part host_part(const void* h, size_t n);
-// The part's bytes in device memory (host parts are uploaded).
-Part on_device(Ctx* c, const Part& p);
 
 int n_results(int enc);
 // GetReturnTypes (each encoding's .hpp)
