@@ -171,17 +171,23 @@ Part scale_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
 // -------------------------------------------------------------------- afl
 // AflEncoding::Encode<T> (compression/afl/afl_encoding.cu:15-56): metadata
 // {minBit, rest, 0, 0}; data = ceil(n*b/W) words of T.
-std::vector<Part> afl_encode_int(Ctx* c, int dtype, DView in) {
+std::vector<Part> afl_encode_int(Ctx* c, int dtype, DView in, const MinMaxResult* known = nullptr) {
     const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
     if (n == 0) return {empty(), empty()};
     Part x = aligned_view(c, in, es);
-    DevBuf mmb(c, sizeof(MinMaxResult));
-    launch_minmax(c, dtype, x.p, n, mmb.get<MinMaxResult>());
+    DevBuf mmb;
+    const MinMaxResult* mm = known;
+    if (!mm) {
+        mmb = DevBuf(c, sizeof(MinMaxResult));
+        launch_minmax(c, dtype, x.p, n, mmb.get<MinMaxResult>());
+        kernel_bytes(c, "k_minmax", (double)(n * es));
+        mm = mmb.get<MinMaxResult>();
+    }
     const uint64_t W = 8 * es;
     DevBuf data(c, (size_t)((W * n + W - 1) / W) * es);  // worst case b = W
-    launch_fl_pack(c, dtype, x.p, n, mmb.get<MinMaxResult>(), 0, data.get());
+    launch_fl_pack(c, dtype, x.p, n, mm, 0, data.get());
     MinMaxResult h;
-    d2h(c, &h, mmb.get(), sizeof h);
+    d2h(c, &h, mm, sizeof h);
     int b = dispatch_int((int)es, [&](auto tag) {
         using T = decltype(tag);
         T mn, mx;
@@ -190,7 +196,6 @@ std::vector<Part> afl_encode_int(Ctx* c, int dtype, DView in) {
     });
     const uint64_t bytes = (b * (uint64_t)n + W - 1) / W * es;
     data.set_size((size_t)bytes);
-    kernel_bytes(c, "k_minmax", (double)(n * es));
     kernel_bytes(c, "k_fl_pack", (double)(n * es + bytes));
     uint8_t meta[4] = {(uint8_t)b, (uint8_t)((bytes * 8 - (uint64_t)n * b) & 0xff), 0, 0};
     return {host_part(meta, 4), own(std::move(data))};
@@ -354,14 +359,22 @@ std::vector<Part> patch_encode(Ctx* c, int dtype, DView in, bool outside) {
         }
     });
     DevBuf st(c, 1 + (n + 7) / 8), yes(c, n * es), no(c, n * es), cnt(c, 8);
-    launch_split(c, dtype, x.p, n, p, st.bytes_ptr(), yes.get(), no.get(), nullptr, cnt.get<uint64_t>());
+    // integer outputs carry their min/max (from the split's counting pass)
+    const bool integral = dtype != DDJ_D_FLOAT && dtype != DDJ_D_DOUBLE;
+    auto mm_yes = integral ? std::make_shared<DevBuf>(c, sizeof(MinMaxResult)) : nullptr;
+    auto mm_no = integral ? std::make_shared<DevBuf>(c, sizeof(MinMaxResult)) : nullptr;
+    launch_split(c, dtype, x.p, n, p, st.bytes_ptr(), yes.get(), no.get(), nullptr, cnt.get<uint64_t>(),
+                 integral ? mm_yes->get<MinMaxResult>() : nullptr, integral ? mm_no->get<MinMaxResult>() : nullptr);
     uint64_t m = 0;
     d2h(c, &m, cnt.get(), 8);
     yes.set_size(m * es);
     no.set_size((n - m) * es);
     kernel_bytes(c, "k_minmax", (double)(n * es));
     kernel_bytes(c, "k_split", (double)(2 * n * es + 1 + (n + 7) / 8));
-    return {own(std::move(st)), own(std::move(yes)), own(std::move(no))};
+    Part py = own(std::move(yes)), pn = own(std::move(no));
+    py.minmax = mm_yes;
+    pn.minmax = mm_no;
+    return {own(std::move(st)), py, pn};
 }
 Part patch_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     const size_t es = dtype_size(dtype);
@@ -370,7 +383,7 @@ Part patch_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     Part y = aligned_view(c, in[1], es), z = aligned_view(c, in[2], es);
     if (y.n / es + z.n / es < n) fail(DDJ_ECORRUPT, "patch outputs shorter than the stencil");
     DevBuf out(c, n * es);
-    launch_merge(c, dtype, in[0].p + 1, n, y.p, z.p ? z.p : y.p, 0, out.get());
+    launch_merge(c, dtype, in[0].p + 1, n, y.p, y.n / es, z.p, z.n / es, false, 0, out.get());
     return own(std::move(out));
 }
 
@@ -402,7 +415,7 @@ Part const_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     const size_t n = stencil_rows(c, DView{in[0].p + es, in[0].n - es});
     Part y = aligned_view(c, in[1], es);
     DevBuf out(c, n * es);
-    launch_merge(c, dtype, in[0].p + es + 1, n, y.p, nullptr, cv, out.get());
+    launch_merge(c, dtype, in[0].p + es + 1, n, y.p, y.n / es, nullptr, 0, true, cv, out.get());
     return own(std::move(out));
 }
 
@@ -421,27 +434,27 @@ std::vector<Part> dict_encode(Ctx* c, int dtype, DView in) {
     p.kind = PRED_IN_SET;
     p.set_n = kDictK;
     for (int i = 0; i < kDictK; i++) p.set_bits[i] = keys[i];
-    DevBuf st(c, 1 + (n + 7) / 8), codes(c, n), others(c, n * es), cnt(c, 8);
-    launch_split(c, dtype, x.p, n, p, st.bytes_ptr(), nullptr, others.get(), codes.get<uint8_t>(),
+    // freq = [size_t K][size_t m][K keys][ceil(m / 16) words of 2-bit codes]
+    // (ALT_BITLEN(K - 1) = 2); the split packs the codes in place, into words
+    // sized for every row being frequent
+    const size_t head = 16 + kDictK * es;
+    DevBuf st(c, 1 + (n + 7) / 8), f(c, head + 4 * ((n + 15) / 16)), others(c, n * es), cnt(c, 8);
+    launch_fill_zero(c, f.bytes_ptr() + head, 4 * ((n + 15) / 16));
+    launch_split(c, dtype, x.p, n, p, st.bytes_ptr(), nullptr, others.get(), f.bytes_ptr() + head,
                  cnt.get<uint64_t>());
     uint64_t m = 0;
     d2h(c, &m, cnt.get(), 8);
     others.set_size((n - m) * es);
     kernel_bytes(c, "k_hist_count", (double)(n * es));
-    kernel_bytes(c, "k_split", (double)(n * es + 1 + (n + 7) / 8 + m + (n - m) * es));
-    kernel_bytes(c, "k_pack_codes", (double)(m + (m + 15) / 16 * 4));
+    kernel_bytes(c, "k_split", (double)(n * es + 1 + (n + 7) / 8 + (m + 15) / 16 * 4 + (n - m) * es));
     Part freq = empty();
     if (m > 0) {
-        const int bits = 2;  // ALT_BITLEN(K - 1)
-        const size_t words = (m + 15) / 16;
-        const size_t head = 16 + kDictK * es;
-        DevBuf f(c, head + 4 * words);
         std::vector<uint8_t> h(head);
         size_t sizes[2] = {(size_t)kDictK, (size_t)m};
         memcpy(h.data(), sizes, 16);
         for (int i = 0; i < kDictK; i++) memcpy(h.data() + 16 + i * es, &keys[i], es);
         h2d(c, f.get(), h.data(), head);
-        launch_pack_codes(c, codes.get<uint8_t>(), m, bits, reinterpret_cast<uint32_t*>(f.bytes_ptr() + head));
+        f.set_size(head + 4 * ((m + 15) / 16));
         freq = own(std::move(f));
     }
     return {own(std::move(st)), freq, own(std::move(others))};
@@ -451,7 +464,8 @@ Part dict_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     const size_t es = dtype_size(dtype);
     if (in[1].n == 0 && in[2].n == 0) return empty();
     const size_t n = stencil_rows(c, in[0]);
-    Part freq = empty();
+    Part z = aligned_view(c, in[2], es);
+    DevBuf out(c, n * es);
     if (in[1].n > 0) {
         if (in[1].n < 16) fail(DDJ_ECORRUPT, "dict header too short");
         size_t sizes[2];
@@ -464,14 +478,14 @@ Part dict_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
         const int P = 32 / bits;
         const size_t off = 16 + K * es;
         if ((m + P - 1) / P * 4 > in[1].n - off) fail(DDJ_ECORRUPT, "dict codes shorter than their count");
-        DevBuf f(c, m * es);
-        launch_unpack_codes(c, dtype, in[1].p + off, m, bits, in[1].p + 16, f.get());
-        freq = own(std::move(f));
+        if (m + z.n / es < n) fail(DDJ_ECORRUPT, "dict outputs shorter than the stencil");
+        // _decompressUniqueKernel (unique_encoding.cu:183-204) folded into the merge
+        launch_merge_codes(c, dtype, in[0].p + 1, n, in[1].p + off, m, bits, in[1].p + 16, (uint32_t)K, z.p,
+                           z.n / es, out.get());
+    } else {
+        if (z.n / es < n) fail(DDJ_ECORRUPT, "dict outputs shorter than the stencil");
+        launch_merge(c, dtype, in[0].p + 1, n, nullptr, 0, z.p, z.n / es, false, 0, out.get());
     }
-    Part z = aligned_view(c, in[2], es);
-    if (freq.n / es + z.n / es < n) fail(DDJ_ECORRUPT, "dict outputs shorter than the stencil");
-    DevBuf out(c, n * es);
-    launch_merge(c, dtype, in[0].p + 1, n, freq.p ? freq.p : z.p, z.p ? z.p : freq.p, 0, out.get());
     return own(std::move(out));
 }
 
@@ -648,7 +662,7 @@ void return_types(int enc, int dtype, int out[2]) {
     if (enc == DDJ_ENC_RLE || enc == DDJ_ENC_FLOAT_TO_INT) out[0] = DDJ_D_INT;  // rle_encoding.hpp:35, float_encoding.hpp:31
 }
 
-std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in) {
+std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResult* minmax) {
     if (!dtype_ok(dtype)) fail(DDJ_EINVAL, "unknown data type");
     switch (enc) {
     case DDJ_ENC_NONE: return none_encode(c, in);
@@ -657,7 +671,7 @@ std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in) {
     case DDJ_ENC_AFL:
         if (dtype == DDJ_D_DOUBLE) fail(DDJ_EUNSUPPORTED, "AFL on double returns no outputs (afl_encoding.cu:268-271)");
         if (dtype == DDJ_D_FLOAT) return afl_encode_float(c, in);
-        return afl_encode_int(c, dtype, in);
+        return afl_encode_int(c, dtype, in, minmax);
     case DDJ_ENC_RLE: return rle_encode(c, dtype, in);
     case DDJ_ENC_DICT: return dict_encode(c, dtype, in);
     case DDJ_ENC_CONST: return const_encode(c, dtype, in);

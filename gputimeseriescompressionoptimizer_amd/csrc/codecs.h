@@ -19,6 +19,9 @@ struct Part {
     const uint8_t* p = nullptr;
     size_t n = 0;
     std::shared_ptr<const std::vector<uint8_t>> host;  // set: the bytes live here, p is null
+    // Optional: a MinMaxResult (device) of the part's values, computed by the
+    // encoder that produced them, so that an FL child skips its own pass.
+    std::shared_ptr<DevBuf> minmax;
     bool on_host() const { return host != nullptr; }
     DView view() const { return DView{p, n}; }
 };
@@ -32,7 +35,8 @@ void return_types(int enc, int dtype, int out[2]);
 bool enc_ok(int enc);
 
 // Encoding::Encode: out[0] = metadata, out[1..n_results(enc)] = outputs.
-std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in);
+// `minmax` (optional, device): the input's min/max keys, when known.
+std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResult* minmax = nullptr);
 // Encoding::Decode: in[0] = metadata, in[1..] = children outputs.
 Part decode(Ctx* c, int enc, int dtype, const std::vector<DView>& in);
 // GetMetadataSize / GetCompressedSize.

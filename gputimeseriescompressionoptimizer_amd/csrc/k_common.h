@@ -100,6 +100,21 @@ __device__ __forceinline__ uint64_t wave_or(uint64_t v) {
     return v;
 }
 
+// Minimum (MAX: maximum) of a 32-bit value over the 64 lanes of a wave:
+// DPP steps inside every row of 16 lanes (swap neighbours, swap pairs, mirror
+// half-rows, mirror rows), then the four rows' results by readlane.  The
+// result is the same in every lane.
+template <bool MAX> __device__ __forceinline__ int32_t wave_extreme_i32(int32_t v) {
+    auto op = [](int32_t a, int32_t b) { return MAX ? (a > b ? a : b) : (a < b ? a : b); };
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+    v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
+    const int32_t r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int32_t r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return op(op(r0, r1), op(r2, r3));
+}
+
 // Inclusive prefix sum over the 64 lanes of a wave.
 template <typename T> __device__ __forceinline__ T wave_inclusive_scan(T v) {
     const int lane = threadIdx.x & (kWave - 1);

@@ -264,13 +264,55 @@ __global__ __launch_bounds__(kBlock) void k_delta_decode(const uint8_t* first_by
 // holds value t of every row (all loads and stores coalesced and in flight
 // together), each row is scanned across the workgroup (wave scans, then the
 // waves' totals in LDS), and the rows are chained by their totals.
+// With `keys`, the same workgroup first folds per-tile {min, max, min, max}
+// order keys into *mm_a and *mm_b.
 __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const uint64_t* __restrict__ vals, size_t tiles,
-                                                             uint64_t* __restrict__ prefix) {
+                                                             uint64_t* __restrict__ prefix,
+                                                             const uint64_t* __restrict__ keys, MinMaxResult* mm_a,
+                                                             MinMaxResult* mm_b) {
     constexpr int R = kScanRows, NW = kScanThreads / kWave;
     __shared__ uint64_t wsum[R][NW];  // per row: the waves' totals, then their exclusive offsets
     __shared__ uint64_t rowtot[R];
     __shared__ uint64_t rowoff[R + 1];
+    __shared__ unsigned long long mm[4];
     const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
+    if (keys) {  // uniform
+        if (t < 4) mm[t] = (t & 1) ? 0ull : ~0ull;
+        __syncthreads();
+        uint64_t r[4] = {~0ull, 0, ~0ull, 0};
+        constexpr int KR = 8;  // tiles per thread and step, loads all in flight
+        for (size_t t0 = 0; t0 < tiles; t0 += (size_t)KR * kScanThreads) {
+            uint64_t v[KR][4];
+#pragma unroll
+            for (int k = 0; k < KR; k++) {
+                const size_t i = t0 + (size_t)k * kScanThreads + t;
+#pragma unroll
+                for (int j = 0; j < 4; j++) v[k][j] = i < tiles ? keys[4 * i + j] : ((j & 1) ? 0ull : ~0ull);
+            }
+#pragma unroll
+            for (int k = 0; k < KR; k++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    r[j] = (j & 1) ? (v[k][j] > r[j] ? v[k][j] : r[j]) : (v[k][j] < r[j] ? v[k][j] : r[j]);
+        }
+        r[0] = wave_min(r[0]);
+        r[1] = wave_max(r[1]);
+        r[2] = wave_min(r[2]);
+        r[3] = wave_max(r[3]);
+        if (lane == 0) {
+            atomicMin(&mm[0], (unsigned long long)r[0]);
+            atomicMax(&mm[1], (unsigned long long)r[1]);
+            atomicMin(&mm[2], (unsigned long long)r[2]);
+            atomicMax(&mm[3], (unsigned long long)r[3]);
+        }
+        __syncthreads();
+        if (t == 0) {
+            mm_a->min_key = mm[0];
+            mm_a->max_key = mm[1];
+            mm_b->min_key = mm[2];
+            mm_b->max_key = mm[3];
+        }
+    }
     uint64_t carry = 0;
     for (size_t c0 = 0; c0 < tiles; c0 += (size_t)R * kScanThreads) {  // uniform
         uint64_t v[R], inc[R];
@@ -387,9 +429,10 @@ TileScan::TileScan(Ctx* c, size_t n_tiles) : mem(c, 8 * (2 * n_tiles + 1)), tile
     prefix = vals + n_tiles;
 }
 
-void TileScan::scan(Ctx* c) {
-    kernel_bytes(c, "k_scan_tiles", 8.0 * (double)(2 * tiles + 1));
-    { KernelTimer kt_(c, "k_scan_tiles"); hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, c->stream, vals, tiles, prefix); }
+void TileScan::scan(Ctx* c, const uint64_t* keys, MinMaxResult* mm_a, MinMaxResult* mm_b) {
+    if (keys && (!mm_a || !mm_b)) fail(DDJ_EINVAL, "tile key fold without outputs");
+    kernel_bytes(c, "k_scan_tiles", 8.0 * (double)(2 * tiles + 1) + (keys ? 32.0 * (double)tiles : 0.0));
+    { KernelTimer kt_(c, "k_scan_tiles"); hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, c->stream, vals, tiles, prefix, keys, mm_a, mm_b); }
     HIPC(hipGetLastError());
 }
 

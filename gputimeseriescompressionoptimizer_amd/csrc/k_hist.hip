@@ -162,6 +162,26 @@ __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, 
     __syncthreads();
     bool negzero = false;
     unsigned long long special = 0;
+    // Adds `add` rows of `key` to the LDS table (or, when its probe run is
+    // full, to the global table).
+    auto lds_add = [&](uint64_t key, uint32_t add) {
+        const uint32_t h = lds_slot(key);
+        for (int p = 0; p < 16; p++) {
+            const uint32_t s = (h + p) & (kLdsSlots - 1);
+            unsigned long long k = skey[s];
+            if (k == kEmpty) k = atomicCAS(&skey[s], kEmpty, (unsigned long long)key);
+            if (k == kEmpty || k == key) {
+                atomicAdd(&scnt[s], add);
+                return;
+            }
+        }
+        global_insert(keys, counts, cap_mask, key, (unsigned long long)add, ctl);
+    };
+    // Two keys per thread counted in registers (the first two distinct keys
+    // the thread meets): on skewed columns most rows hit them and skip the
+    // LDS atomics, whose same-address conflicts are the cost.
+    uint64_t rk0 = kEmpty, rk1 = kEmpty;
+    uint32_t rc0 = 0, rc1 = 0;
     auto count = [&](T v) {
         if constexpr (std::is_floating_point<T>::value) {
             if (v != v) {
@@ -175,17 +195,19 @@ __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, 
             special++;
             return;
         }
-        const uint32_t h = lds_slot(key);
-        for (int p = 0; p < 16; p++) {
-            const uint32_t s = (h + p) & (kLdsSlots - 1);
-            unsigned long long k = skey[s];
-            if (k == kEmpty) k = atomicCAS(&skey[s], kEmpty, (unsigned long long)key);
-            if (k == kEmpty || k == key) {
-                atomicAdd(&scnt[s], 1u);
-                return;
-            }
+        if (key == rk0) {
+            rc0++;
+        } else if (key == rk1) {
+            rc1++;
+        } else if (rk0 == kEmpty) {
+            rk0 = key;
+            rc0 = 1;
+        } else if (rk1 == kEmpty) {
+            rk1 = key;
+            rc1 = 1;
+        } else {
+            lds_add(key, 1u);
         }
-        global_insert(keys, counts, cap_mask, key, 1ull, ctl);
     };
     constexpr int kRows = 4096;
     constexpr int V = 16 / sizeof(T);
@@ -211,6 +233,8 @@ __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, 
             for (int i = threadIdx.x; i < cnt; i += kBlock) count(x[base + i]);
         }
     }
+    if (rk0 != kEmpty) lds_add(rk0, rc0);
+    if (rk1 != kEmpty) lds_add(rk1, rc1);
     __syncthreads();
     for (int s = threadIdx.x; s < kLdsSlots; s += kBlock)
         if (skey[s] != kEmpty) global_insert(keys, counts, cap_mask, skey[s], scnt[s], ctl);

@@ -10,6 +10,8 @@
 // [16t, 16t+16) of the tile, i.e. exactly two stencil bytes.  Inputs are
 // staged through LDS with striped (coalesced) loads, outputs are compacted in
 // LDS and written back striped.
+#include <limits>
+
 #include "dispatch.h"
 #include "k_common.h"
 #include "kernels.h"
@@ -59,13 +61,35 @@ template <typename T> T bits_to(uint64_t b) {
 }
 
 // Rows of each tile that satisfy the predicate (the split's counting pass).
-template <typename T>
+// MM (integer types): also the order keys of the minimum and maximum of the
+// rows going to each side, keys[4 t .. 4 t + 3] = {yes min, yes max, no min,
+// no max}, so that an FL encoder of either output needs no pass of its own.
+template <typename T, bool MM>
 __global__ __launch_bounds__(kBlock) void k_split_count(const T* __restrict__ x, size_t n, Pred<T> p,
-                                                        uint64_t* __restrict__ vals) {
+                                                        uint64_t* __restrict__ vals, uint64_t* __restrict__ keys) {
     __shared__ uint64_t red[kBlock / kWave];
+    __shared__ unsigned long long mm[4];
     const size_t base = (size_t)blockIdx.x * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     uint32_t c = 0, code;
+    // yes side: ylo / yhi, no side: nlo / nhi
+    T ylo = std::numeric_limits<T>::max(), yhi = std::numeric_limits<T>::min();
+    T nlo = ylo, nhi = yhi;
+    if constexpr (MM) {
+        if (threadIdx.x < 4) mm[threadIdx.x] = (threadIdx.x & 1) ? 0ull : ~0ull;
+        __syncthreads();  // before any thread folds into mm
+    }
+    auto take = [&](T v) {
+        const bool f = p(v, code);
+        c += f ? 1u : 0u;
+        if constexpr (MM) {
+            const T a = f ? v : ylo, b = f ? v : yhi, d = f ? nlo : v, e = f ? nhi : v;
+            ylo = a < ylo ? a : ylo;
+            yhi = b > yhi ? b : yhi;
+            nlo = d < nlo ? d : nlo;
+            nhi = e > nhi ? e : nhi;
+        }
+    };
     if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
         constexpr int V = 16 / sizeof(T);
         const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
@@ -75,28 +99,53 @@ __global__ __launch_bounds__(kBlock) void k_split_count(const T* __restrict__ x,
             T v[V];
             __builtin_memcpy(v, &w, 16);
 #pragma unroll
-            for (int k = 0; k < V; k++) c += p(v[k], code) ? 1u : 0u;
+            for (int k = 0; k < V; k++) take(v[k]);
         }
     } else {
-        for (int i = threadIdx.x; i < cnt; i += kBlock) c += p(x[base + i], code) ? 1u : 0u;
+        for (int i = threadIdx.x; i < cnt; i += kBlock) take(x[base + i]);
+    }
+    if constexpr (MM) {  // wave extremes, then one LDS atomic per wave and key
+        uint64_t k4[4];
+        if constexpr (sizeof(T) <= 4) {
+            k4[0] = order_key((T)wave_extreme_i32<false>((int32_t)ylo));
+            k4[1] = order_key((T)wave_extreme_i32<true>((int32_t)yhi));
+            k4[2] = order_key((T)wave_extreme_i32<false>((int32_t)nlo));
+            k4[3] = order_key((T)wave_extreme_i32<true>((int32_t)nhi));
+        } else {
+            k4[0] = wave_min(order_key(ylo));
+            k4[1] = wave_max(order_key(yhi));
+            k4[2] = wave_min(order_key(nlo));
+            k4[3] = wave_max(order_key(nhi));
+        }
+        if ((threadIdx.x & (kWave - 1)) == 0) {
+            atomicMin(&mm[0], (unsigned long long)k4[0]);
+            atomicMax(&mm[1], (unsigned long long)k4[1]);
+            atomicMin(&mm[2], (unsigned long long)k4[2]);
+            atomicMax(&mm[3], (unsigned long long)k4[3]);
+        }
     }
     uint64_t total;
-    block_exclusive_scan<uint64_t>((uint64_t)c, &total, red);
+    block_exclusive_scan<uint64_t>((uint64_t)c, &total, red);  // a barrier: mm is complete
     if (threadIdx.x == 0) vals[blockIdx.x] = total;
+    if (MM && threadIdx.x < 4) keys[4 * (size_t)blockIdx.x + threadIdx.x] = mm[threadIdx.x];
 }
+
 
 template <typename T, bool CODES, bool WRITE_NO>
 __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_t n, Pred<T> p,
                                                   uint8_t* __restrict__ stencil, T* __restrict__ yes,
                                                   T* __restrict__ no, uint8_t* __restrict__ codes,
                                                   const uint64_t* __restrict__ prefix, uint64_t* d_count) {
+    constexpr int kCodeWords = kTile / 16 + 2;  // 2-bit code words a tile's codes can touch
     __shared__ T buf[kTilePad];
-    __shared__ uint8_t cbuf[CODES ? kTile : 1];
+    __shared__ uint32_t cwl[CODES ? kCodeWords : 1];  // those words, aligned to the output's word grid
     __shared__ uint64_t red[kBlock / kWave];
     const uint32_t tile = blockIdx.x;
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     const T padv = x[n - 1];  // stencil padding repeats the last row (reference stencil.cu:26-27)
+    if (CODES)
+        for (int q = threadIdx.x; q < kCodeWords; q += kBlock) cwl[q] = 0;
     if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
         constexpr int V = 16 / sizeof(T);
         const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
@@ -131,16 +180,22 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)c, &total, red);
     const uint64_t Y0 = prefix[tile];
+    // codes j in [Y0, Y0 + total) go to field j % 16 of 2-bit word j / 16
+    // (_compressUniqueKernel, unique_encoding.cu:48-76): tile-local code i
+    // lands in cwl[(lead + i) / 16]
+    const int lead = (int)(Y0 % 16);
     int yr = (int)excl, nr = threadIdx.x * kItems - (int)excl;
     const int nbase = CODES ? 0 : (int)total;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
         if (threadIdx.x * kItems + k >= cnt) break;
         if ((mask >> k) & 1) {
-            if (CODES)
-                cbuf[yr] = (uint8_t)((cpack >> (4 * k)) & 15);
-            else
+            if (CODES) {
+                const int j = lead + yr;
+                atomicOr(&cwl[j >> 4], (uint32_t)((cpack >> (4 * k)) & 3) << (2 * (j & 15)));
+            } else {
                 buf[P(yr)] = v[k];
+            }
             yr++;
         } else {
             buf[P(nbase + nr)] = v[k];
@@ -148,13 +203,21 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
         }
     }
     __syncthreads();
-    if (CODES || yes)
-        for (int i = threadIdx.x; i < (int)total; i += kBlock) {
-            if (CODES)
-                codes[Y0 + i] = cbuf[i];
+    if (CODES) {
+        // words shared with a neighbouring tile are OR-ed into the zeroed output
+        uint32_t* cw = reinterpret_cast<uint32_t*>(codes);
+        const uint64_t w_first = Y0 / 16, w_end = (Y0 + total + 15) / 16;
+        const int nt = (int)total;
+        for (int q = threadIdx.x; q < (int)(w_end - w_first); q += kBlock) {
+            const int i0 = 16 * q - lead;  // tile-local index of the word's field 0
+            if (i0 >= 0 && i0 + 16 <= nt)
+                cw[w_first + q] = cwl[q];
             else
-                yes[Y0 + i] = buf[P(i)];
+                atomicOr(&cw[w_first + q], cwl[q]);
         }
+    } else if (yes) {
+        for (int i = threadIdx.x; i < (int)total; i += kBlock) yes[Y0 + i] = buf[P(i)];
+    }
     if (WRITE_NO) {
         const size_t N0 = base - Y0;
         for (int i = threadIdx.x; i < cnt - (int)total; i += kBlock) no[N0 + i] = buf[P(nbase + i)];
@@ -188,15 +251,54 @@ __global__ __launch_bounds__(kBlock) void k_stencil_count(const uint8_t* __restr
     if (lane == 0) vals[tile] = c;
 }
 
-template <typename T, bool CONST_NO>
+// The "yes" rows of a merge given as dictionary codes: row j is
+// dict[code j], code j being the `bits`-wide field j % (32 / bits) of the
+// little-endian word j / (32 / bits) (_decompressUniqueKernel,
+// unique_encoding.cu:183-204).  Codes at or beyond K (corrupt input) take
+// the last key.
+struct DictCodes {
+    const uint8_t* words;  // any byte address
+    const uint8_t* dict;
+    uint32_t bits, K;
+    int pshift;  // log2(32 / bits) when bits is a power of two, else -1
+};
+constexpr uint32_t kLdsDict = 64;  // dictionaries up to this size are staged in LDS
+
+// Code j: word j / per read as two aligned 32-bit loads joined by a funnel
+// shift (an aligned word holding a byte of the stream never crosses a page).
+template <typename T>
+__device__ __forceinline__ T dict_value(const DictCodes& d, const T* sdict, uint64_t j) {
+    const uint64_t per = 32 / d.bits;
+    const uint64_t w = d.pshift >= 0 ? (j >> d.pshift) : j / per;
+    const uint32_t f = (uint32_t)(j - w * per);
+    const uintptr_t a = reinterpret_cast<uintptr_t>(d.words + 4 * w);
+    const uint32_t* wa = reinterpret_cast<const uint32_t*>(a & ~(uintptr_t)3);
+    const uint32_t sh = (uint32_t)(a & 3) * 8;
+    const uint32_t word = sh ? __builtin_amdgcn_alignbit(wa[1], wa[0], sh) : wa[0];
+    uint32_t code = d.bits >= 32 ? word : (word >> (d.bits * f)) & ((1u << d.bits) - 1u);
+    code = code < d.K ? code : d.K - 1;
+    if (d.K <= kLdsDict) return sdict[code];
+    T v;
+    __builtin_memcpy(&v, d.dict + sizeof(T) * code, sizeof(T));
+    return v;
+}
+
+// Merge: out[i] = bit i ? yes[k++] : no[j++], with `no` the constant
+// `no_const` (CONST_NO) and `yes` from dictionary codes (CODES).  Rows past
+// n_yes / n_no (a stencil with more set or clear bits than the outputs hold:
+// corrupt input) read as zero and raise the error flag.
+template <typename T, bool CONST_NO, bool CODES>
 __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bits, size_t n, const T* __restrict__ yes,
-                                                  const T* __restrict__ no, T no_const, T* __restrict__ out,
-                                                  const uint64_t* __restrict__ prefix) {
+                                                  size_t n_yes, DictCodes dc, const T* __restrict__ no, size_t n_no,
+                                                  T no_const, T* __restrict__ out, const uint64_t* __restrict__ prefix,
+                                                  uint32_t* error) {
     __shared__ T buf[kTilePad];
     __shared__ uint64_t red[kBlock / kWave];
+    __shared__ T sdict[CODES ? kLdsDict : 1];
     const uint32_t tile = blockIdx.x;
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
+    if (CODES && dc.K <= kLdsDict && threadIdx.x < dc.K) __builtin_memcpy(&sdict[threadIdx.x], dc.dict + sizeof(T) * threadIdx.x, sizeof(T));
     const size_t g0 = base + (size_t)threadIdx.x * kItems;
     uint32_t mask = 0;
     if (g0 < n) mask = bits[g0 / 8];
@@ -207,15 +309,30 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
     else if (mine < kItems)
         mask &= (1u << mine) - 1u;
     uint64_t total;
-    const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &total, red);
+    const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &total, red);  // also publishes sdict
     const uint64_t Y0 = prefix[tile];
     const size_t N0 = base - Y0;
+    bool bad = false;
     for (int i = threadIdx.x; i < cnt; i += kBlock) {
-        if (i < (int)total)
-            buf[P(i)] = yes[Y0 + i];
-        else
-            buf[P(i)] = CONST_NO ? no_const : no[N0 + (i - (int)total)];
+        T v = T(0);
+        if (i < (int)total) {
+            const uint64_t j = Y0 + i;
+            if (j < n_yes)
+                v = CODES ? dict_value<T>(dc, sdict, j) : yes[j];
+            else
+                bad = true;
+        } else if (CONST_NO) {
+            v = no_const;
+        } else {
+            const uint64_t j = N0 + (i - (int)total);
+            if (j < n_no)
+                v = no[j];
+            else
+                bad = true;
+        }
+        buf[P(i)] = v;
     }
+    if (bad) __hip_atomic_store(error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     T v[kItems];
     int yr = (int)excl, nr = (int)total + threadIdx.x * kItems - (int)excl;
@@ -231,40 +348,6 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
     for (int i = threadIdx.x; i < cnt; i += kBlock) out[base + i] = buf[P(i)];
 }
 
-// _compressUniqueKernel word layout (reference compression/unique/unique_encoding.cu:48-76):
-// code j of the m coded rows sits at bits [(j % P) * b, ...) of word j / P.
-__global__ __launch_bounds__(kBlock) void k_pack_codes(const uint8_t* __restrict__ codes, size_t m, int b,
-                                                       uint32_t* __restrict__ words) {
-    const int P = 32 / b;
-    const size_t nw = (m + P - 1) / P;
-    const size_t stride = (size_t)gridDim.x * kBlock;
-    for (size_t w = (size_t)blockIdx.x * kBlock + threadIdx.x; w < nw; w += stride) {
-        uint32_t r = 0;
-        for (int i = 0; i < P; i++) {
-            size_t j = w * P + i;
-            if (j < m) r |= (uint32_t)codes[j] << (b * i);
-        }
-        words[w] = r;
-    }
-}
-
-// _decompressUniqueKernel (unique_encoding.cu:183-204); words and the
-// dictionary may sit at any byte offset.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_unpack_codes(const uint8_t* __restrict__ words, size_t m, int b,
-                                                         const uint8_t* __restrict__ dict, T* __restrict__ out) {
-    const int P = 32 / b;
-    const uint32_t mask = b >= 32 ? ~0u : ((1u << b) - 1u);
-    const size_t stride = (size_t)gridDim.x * kBlock;
-    for (size_t j = (size_t)blockIdx.x * kBlock + threadIdx.x; j < m; j += stride) {
-        uint32_t w;
-        __builtin_memcpy(&w, words + 4 * (j / P), 4);
-        uint32_t code = (w >> (b * (j % P))) & mask;
-        T v;
-        __builtin_memcpy(&v, dict + sizeof(T) * code, sizeof(T));
-        out[j] = v;
-    }
-}
 
 // RleEncoding (reference compression/rle/rle_encode.cu:18-62): a run starts
 // at row 0 and wherever !(x[i] == x[i-1]); the run's value is its first row.
@@ -527,7 +610,7 @@ inline unsigned grid_for(Ctx* c, size_t items) {
 }  // namespace
 
 void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa, uint8_t* stencil, void* yes,
-                  void* no, uint8_t* codes, uint64_t* d_counts) {
+                  void* no, uint8_t* codes, uint64_t* d_counts, MinMaxResult* mm_yes, MinMaxResult* mm_no) {
     if (n == 0) return;
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
@@ -543,9 +626,21 @@ void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa
         auto ys = static_cast<T*>(yes);
         auto ns = static_cast<T*>(no);
         kernel_bytes(c, "k_split_count", (double)n * sizeof(T));
-        { KernelTimer kt_(c, "k_split_count"); hipLaunchKernelGGL(k_split_count<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p, ts.vals); }
+        bool counted = false;
+        DevBuf keys;
+        if constexpr (std::is_integral<T>::value) {
+            if (mm_yes && mm_no) {
+                keys = DevBuf(c, 32 * tiles);
+                { KernelTimer kt_(c, "k_split_count"); hipLaunchKernelGGL((k_split_count<T, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p, ts.vals, keys.get<uint64_t>()); }
+                counted = true;
+            }
+        }
+        if (!counted) {
+            if (mm_yes || mm_no) fail(DDJ_EINVAL, "split min/max outputs need an integer type and both outputs");
+            { KernelTimer kt_(c, "k_split_count"); hipLaunchKernelGGL((k_split_count<T, false>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p, ts.vals, nullptr); }
+        }
         HIPC(hipGetLastError());
-        ts.scan(c);
+        ts.scan(c, counted ? keys.get<uint64_t>() : nullptr, mm_yes, mm_no);
         const uint64_t* pre = ts.prefix;
         if (codes) {
             { KernelTimer kt_(c, "k_split"); hipLaunchKernelGGL((k_split<T, true, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p,
@@ -561,48 +656,71 @@ void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa
     });
 }
 
-void launch_merge(Ctx* c, int dtype, const uint8_t* bits, size_t n, const uint8_t* yes, const uint8_t* no,
-                  uint64_t no_const, void* out) {
-    if (n == 0) return;
-    dispatch_native(dtype, [&](auto tag) {
-        using T = decltype(tag);
-        const size_t tiles = (n + kTile - 1) / kTile;
-        TileScan ts(c, tiles);
-        const unsigned cg = (unsigned)((tiles + kBlock / kWave - 1) / (kBlock / kWave));
-        kernel_bytes(c, "k_stencil_count", (double)((n + 7) / 8));
-        kernel_bytes(c, "k_merge", (double)((n + 7) / 8) + 2.0 * (double)n * sizeof(T));
-        { KernelTimer kt_(c, "k_stencil_count"); hipLaunchKernelGGL(k_stencil_count, dim3(cg), dim3(kBlock), 0, c->stream, bits, n, tiles, ts.vals); }
-        HIPC(hipGetLastError());
-        ts.scan(c);
-        const uint64_t* pre = ts.prefix;
-        auto ys = reinterpret_cast<const T*>(yes);
-        auto ns = reinterpret_cast<const T*>(no);
-        if (no)
-            { KernelTimer kt_(c, "k_merge"); hipLaunchKernelGGL((k_merge<T, false>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, bits, n, ys,
-                               ns, T(0), static_cast<T*>(out), pre); }
-        else
-            { KernelTimer kt_(c, "k_merge"); hipLaunchKernelGGL((k_merge<T, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, bits, n, ys,
-                               ns, bits_to<T>(no_const), static_cast<T*>(out), pre); }
-        HIPC(hipGetLastError());
-    });
-}
+namespace {
 
-void launch_pack_codes(Ctx* c, const uint8_t* codes, size_t m, int bits, uint32_t* words) {
-    if (!m) return;
-    const size_t nw = (m + (32 / bits) - 1) / (32 / bits);
-    { KernelTimer kt_(c, "k_pack_codes"); hipLaunchKernelGGL(k_pack_codes, dim3(grid_for(c, nw)), dim3(kBlock), 0, c->stream, codes, m, bits, words); }
+// Stencil counts, their scan and the merge; `dc.words` set: yes rows from codes.
+template <typename T>
+void merge_impl(Ctx* c, const uint8_t* bits, size_t n, const T* yes, size_t n_yes, const DictCodes& dc, const T* no,
+                size_t n_no, bool const_no, T no_const, T* out) {
+    const size_t tiles = (n + kTile - 1) / kTile;
+    TileScan ts(c, tiles);
+    arm_error_flag(c);
+    const unsigned cg = (unsigned)((tiles + kBlock / kWave - 1) / (kBlock / kWave));
+    kernel_bytes(c, "k_stencil_count", (double)((n + 7) / 8));
+    { KernelTimer kt_(c, "k_stencil_count"); hipLaunchKernelGGL(k_stencil_count, dim3(cg), dim3(kBlock), 0, c->stream, bits, n, tiles, ts.vals); }
+    HIPC(hipGetLastError());
+    ts.scan(c);
+    const uint64_t* pre = ts.prefix;
+    uint32_t* err = &c->control->error_flag;
+    const dim3 grid((unsigned)tiles);
+    KernelTimer kt_(c, "k_merge");
+    if (dc.words) {
+        kernel_bytes(c, "k_merge", (double)((n + 7) / 8) + (double)n_yes * dc.bits / 8 + (double)n_no * sizeof(T) +
+                                       (double)n * sizeof(T));
+        hipLaunchKernelGGL((k_merge<T, false, true>), grid, dim3(kBlock), 0, c->stream, bits, n, yes, n_yes, dc, no, n_no,
+                           T(0), out, pre, err);
+    } else if (const_no) {
+        kernel_bytes(c, "k_merge", (double)((n + 7) / 8) + (double)n_yes * sizeof(T) + (double)n * sizeof(T));
+        hipLaunchKernelGGL((k_merge<T, true, false>), grid, dim3(kBlock), 0, c->stream, bits, n, yes, n_yes, dc, no, n_no,
+                           no_const, out, pre, err);
+    } else {
+        kernel_bytes(c, "k_merge", (double)((n + 7) / 8) + 2.0 * (double)n * sizeof(T));
+        hipLaunchKernelGGL((k_merge<T, false, false>), grid, dim3(kBlock), 0, c->stream, bits, n, yes, n_yes, dc, no, n_no,
+                           T(0), out, pre, err);
+    }
     HIPC(hipGetLastError());
 }
 
-void launch_unpack_codes(Ctx* c, int dtype, const uint8_t* words, size_t m, int bits, const uint8_t* dict, void* out) {
-    if (!m) return;
+}  // namespace
+
+void launch_merge(Ctx* c, int dtype, const uint8_t* bits, size_t n, const uint8_t* yes, size_t n_yes,
+                  const uint8_t* no, size_t n_no, bool const_no, uint64_t no_const, void* out) {
+    if (n == 0) return;
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
-        { KernelTimer kt_(c, "k_unpack_codes"); hipLaunchKernelGGL(k_unpack_codes<T>, dim3(grid_for(c, m)), dim3(kBlock), 0, c->stream, words, m, bits, dict,
-                           static_cast<T*>(out)); }
-        HIPC(hipGetLastError());
+        merge_impl<T>(c, bits, n, reinterpret_cast<const T*>(yes), n_yes, DictCodes{}, reinterpret_cast<const T*>(no),
+                      n_no, const_no, bits_to<T>(no_const), static_cast<T*>(out));
     });
 }
+
+void launch_merge_codes(Ctx* c, int dtype, const uint8_t* bits, size_t n, const uint8_t* words, size_t m, int code_bits,
+                        const uint8_t* dict, uint32_t K, const uint8_t* no, size_t n_no, void* out) {
+    if (n == 0) return;
+    if (code_bits < 1 || code_bits > 32 || K == 0) fail(DDJ_ECORRUPT, "dictionary code width out of range");
+    DictCodes dc{words, dict, (uint32_t)code_bits, K, -1};
+    if ((code_bits & (code_bits - 1)) == 0) {
+        int l = 0;
+        while ((1 << l) < code_bits) l++;
+        dc.pshift = 5 - l;
+    }
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        merge_impl<T>(c, bits, n, nullptr, m, dc, reinterpret_cast<const T*>(no), n_no, false, T(0),
+                      static_cast<T*>(out));
+    });
+}
+
+
 
 void launch_rle_encode(Ctx* c, int dtype, const void* x, size_t n, int32_t* lengths, void* values, uint64_t* d_runs) {
     if (n == 0) return;

@@ -10,13 +10,16 @@ namespace ddj {
 // pass: a counting kernel fills vals[0..tiles) (one value per tile), scan()
 // launches k_scan_tiles, which writes prefix[t] = vals[0] + ... + vals[t-1]
 // for t <= tiles (prefix[tiles] is the total).  Sums wrap at 64 bits.
+struct MinMaxResult;
 struct TileScan {
     DevBuf mem;
     uint64_t* vals = nullptr;
     uint64_t* prefix = nullptr;
     size_t tiles = 0;
     TileScan(Ctx* c, size_t tiles);
-    void scan(Ctx* c);
+    // keys (optional): per-tile {min, max, min, max} order keys of two
+    // outputs, folded in the same launch into *mm_a and *mm_b.
+    void scan(Ctx* c, const uint64_t* keys = nullptr, MinMaxResult* mm_a = nullptr, MinMaxResult* mm_b = nullptr);
 };
 
 // Kernels that can raise the device error flag (encoded input failing a
@@ -86,14 +89,23 @@ struct PredArgs {
     uint64_t set_bits[4];     // PRED_IN_SET keys
     int set_n;
 };
+// mm_yes / mm_no (integer types, optional): the order keys of each output's
+// minimum and maximum, computed by the counting pass.
+// codes (dictionary split): the 2-bit codes packed 16 to a little-endian
+// 32-bit word, ceil(n / 16) words zeroed beforehand.
 void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& p, uint8_t* stencil /* 1+ceil(n/8) */,
-                  void* yes, void* no, uint8_t* codes /* dict codes or null */, uint64_t* d_counts);
-// Merge: out[i] = bit i ? yes[k++] : no[j++] (no == null: `no_const`).
+                  void* yes, void* no, uint8_t* codes /* packed dict codes or null */, uint64_t* d_counts,
+                  MinMaxResult* mm_yes = nullptr, MinMaxResult* mm_no = nullptr);
+// Merge: out[i] = bit i ? yes[k++] : no[j++] (const_no: `no_const` for every
+// clear bit).  n_yes / n_no rows are available; a stencil asking for more
+// raises the device error flag (corrupt input).
 void launch_merge(Ctx* c, int dtype, const uint8_t* stencil_bits /* packed bits, no header */, size_t n,
-                  const uint8_t* yes, const uint8_t* no, uint64_t no_const, void* out);
-// dict codes: pack bytes (2-bit codes) into uint32 words of 16 / unpack.
-void launch_pack_codes(Ctx* c, const uint8_t* codes, size_t m, int bits, uint32_t* words);
-void launch_unpack_codes(Ctx* c, int dtype, const uint8_t* words, size_t m, int bits, const uint8_t* dict, void* out);
+                  const uint8_t* yes, size_t n_yes, const uint8_t* no, size_t n_no, bool const_no, uint64_t no_const,
+                  void* out);
+// Merge whose set-bit rows are dict[code j] for the m `code_bits`-wide codes
+// in `words` (the dictionary decoder without materialising those rows).
+void launch_merge_codes(Ctx* c, int dtype, const uint8_t* stencil_bits, size_t n, const uint8_t* words, size_t m,
+                        int code_bits, const uint8_t* dict, uint32_t K, const uint8_t* no, size_t n_no, void* out);
 
 // RLE.
 void launch_rle_encode(Ctx* c, int dtype, const void* x, size_t n, int32_t* lengths, void* values, uint64_t* d_runs);

@@ -125,7 +125,10 @@ public:
         if (node->enc == DDJ_ENC_NONE) return 8 + 8 + data.n;  // header, size_t length, the data
         const auto ek = std::make_tuple(data.p, data.n, node->enc, node->dtype);
         auto it = enc_.find(ek);
-        if (it == enc_.end()) it = enc_.emplace(ek, encode(c_, node->enc, node->dtype, data.view())).first;
+        if (it == enc_.end())
+            it = enc_.emplace(ek, encode(c_, node->enc, node->dtype, data.view(),
+                                         data.minmax ? data.minmax->get<MinMaxResult>() : nullptr))
+                     .first;
         const std::vector<Part>& out = it->second;
         size_t total = 8 + out[0].n;
         if (node->ch.empty()) return total + (out.size() > 1 ? out[1].n : 0);

@@ -61,10 +61,10 @@ std::vector<Candidate> full_statistics_update(Ctx* c, const Part& data, int et, 
         std::vector<Part> compr;
         if (cont == DDJ_ENC_NONE) {
             size_t len = data.n;
-            compr.push_back(own(make_buf(c, &len, sizeof len)));
+            compr.push_back(host_part(&len, sizeof len));
             compr.push_back(data);
         } else {
-            compr = encode(c, cont, dt, data.view());
+            compr = encode(c, cont, dt, data.view(), data.minmax ? data.minmax->get<MinMaxResult>() : nullptr);
         }
         size_t psize = 8;
         for (auto& p : compr) psize += p.n;

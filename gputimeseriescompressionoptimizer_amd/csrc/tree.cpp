@@ -82,7 +82,7 @@ size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& 
         res = {f[0], deltas};
         child_pre = {f[1], f[2]};
     } else {
-        res = encode(c, node.enc, node.dtype, data.view());
+        res = encode(c, node.enc, node.dtype, data.view(), data.minmax ? data.minmax->get<MinMaxResult>() : nullptr);
     }
     Piece h;
     h.is_header = true;
