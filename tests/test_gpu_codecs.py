@@ -197,6 +197,23 @@ def test_rle_negative_run_length_is_corrupt(ctx):
     assert g.Encoding(ctx, g.RLE).Decode(outs, g.D_INT).to_bytes() == x.tobytes()
 
 
+@pytest.mark.parametrize("enc", [g.PATCH, g.DICT])
+def test_stencil_asking_for_more_rows_is_corrupt(ctx, enc):
+    """A stencil whose set bits outnumber the values of its output (here every
+    row claims one) is rejected as corrupt; the merge reads no further than
+    the outputs hold."""
+    x = make(g.D_INT, 10_000, "runs" if enc == g.DICT else "rand", seed=5)
+    outs = g.Encoding(ctx, enc).Encode(ctx.upload(x), g.D_INT)
+    meta = bytearray(outs[0].to_bytes())
+    for i in range(1, len(meta)):
+        meta[i] = 0xFF
+    bad = [ctx.upload(np.frombuffer(bytes(meta), np.uint8))] + list(outs[1:])
+    with pytest.raises(g.DdjError) as e:
+        g.Encoding(ctx, enc).Decode(bad, g.D_INT)
+    assert e.value.status == 3  # DDJ_ECORRUPT
+    assert g.Encoding(ctx, enc).Decode(outs, g.D_INT).to_bytes() == x.tobytes()
+
+
 MANY_TILES = 4096 * 4096 + 12_345  # more than one 4096-value chunk of tile offsets
 
 
