@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     __syncthreads();
     T v[kItems];
     uint32_t mask = 0;
-    uint64_t cpack = 0;
+    uint32_t cchunk = 0;  // CODES: the thread's codes, 2 bits each, in row order
     int c = 0;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
@@ -170,8 +170,10 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
         uint32_t code;
         bool f = p(v[k], code);
         mask |= (uint32_t)f << k;
-        cpack |= (uint64_t)code << (4 * k);
-        if (threadIdx.x * kItems + k < cnt) c += f;
+        if (threadIdx.x * kItems + k < cnt && f) {
+            if (CODES) cchunk |= (code & 3u) << (2 * c);
+            c++;
+        }
     }
     const size_t g0 = base + (size_t)threadIdx.x * kItems;
     if (g0 < n) stencil[1 + g0 / 8] = (uint8_t)(mask & 0xff);
@@ -184,18 +186,18 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     // (_compressUniqueKernel, unique_encoding.cu:48-76): tile-local code i
     // lands in cwl[(lead + i) / 16]
     const int lead = (int)(Y0 % 16);
+    if (CODES && c > 0) {  // the chunk covers code bits [2 (lead + excl), +2c): at most two words
+        const int bit = 2 * (lead + (int)excl), sh = bit & 31;
+        atomicOr(&cwl[bit >> 5], cchunk << sh);
+        if (sh && sh + 2 * c > 32) atomicOr(&cwl[(bit >> 5) + 1], cchunk >> (32 - sh));
+    }
     int yr = (int)excl, nr = threadIdx.x * kItems - (int)excl;
     const int nbase = CODES ? 0 : (int)total;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
         if (threadIdx.x * kItems + k >= cnt) break;
         if ((mask >> k) & 1) {
-            if (CODES) {
-                const int j = lead + yr;
-                atomicOr(&cwl[j >> 4], (uint32_t)((cpack >> (4 * k)) & 3) << (2 * (j & 15)));
-            } else {
-                buf[P(yr)] = v[k];
-            }
+            if (!CODES) buf[P(yr)] = v[k];
             yr++;
         } else {
             buf[P(nbase + nr)] = v[k];
@@ -570,17 +572,37 @@ __global__ __launch_bounds__(kBlock) void k_rle_expand(const uint64_t* __restric
         for (int i = threadIdx.x; i <= (int)nr; i += kBlock) soff[i] = offsets[r0 + i];
         for (int i = threadIdx.x; i < (int)nr; i += kBlock) __builtin_memcpy(&sval[i], values + sizeof(T) * (r0 + i), sizeof(T));
         __syncthreads();
-        for (int k = threadIdx.x; k < cnt; k += kBlock) {
-            const uint64_t pos = o0 + k;
-            int lo = 0, hi = (int)nr;  // soff[lo] <= pos < soff[hi]
+        // thread t writes rows [8t, 8t + 8) of the tile: one binary search for
+        // the first, then the runs advance monotonically
+        constexpr int R = kExpTile / kBlock;
+        const int k0 = threadIdx.x * R;
+        if (k0 < cnt) {
+            const uint64_t pos0 = o0 + k0;
+            int lo = 0, hi = (int)nr;  // soff[lo] <= pos0 < soff[hi]
             while (hi - lo > 1) {
                 const int mid = (lo + hi) / 2;
-                if (soff[mid] <= pos)
+                if (soff[mid] <= pos0)
                     lo = mid;
                 else
                     hi = mid;
             }
-            out[pos] = sval[lo];
+            T v[R];
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                while (lo + 1 < (int)nr && soff[lo + 1] <= pos0 + k) lo++;
+                v[k] = sval[lo];
+            }
+            if (k0 + R <= cnt && (R * sizeof(T)) % 16 == 0) {
+                u32x4* d4 = reinterpret_cast<u32x4*>(out + pos0);  // 16-byte aligned: tiles of 2048 rows
+#pragma unroll
+                for (int q = 0; q < (int)(R * sizeof(T) / 16); q++) {
+                    u32x4 w;
+                    __builtin_memcpy(&w, reinterpret_cast<const uint8_t*>(v) + 16 * q, 16);
+                    d4[q] = w;
+                }
+            } else {
+                for (int k = 0; k < R && k0 + k < cnt; k++) out[pos0 + k] = v[k];
+            }
         }
     } else {
         for (int k = threadIdx.x; k < cnt; k += kBlock) {
