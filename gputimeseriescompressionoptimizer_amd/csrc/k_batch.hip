@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_layout(BatchColumn* cols, int 
 
 __global__ __launch_bounds__(kBlock) void k_batch_pack(const BatchColumn* cols, const BatchItem* tiles, int ntiles,
                                                        uint8_t* arena) {
-    __shared__ __attribute__((aligned(16))) int64_t vals[kFlTile + 2];
+    __shared__ __attribute__((aligned(16))) int64_t vals[kFlLdsVals];
     for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {  // uniform per workgroup
         const BatchItem item = tiles[it];
         const BatchColumn c = cols[item.col];

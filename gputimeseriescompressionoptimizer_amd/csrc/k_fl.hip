@@ -30,7 +30,7 @@ namespace {
 template <typename T, bool DELTA = false>
 __global__ __launch_bounds__(kBlock) void k_fl_pack(const T* __restrict__ x, size_t n, const MinMaxResult* mm,
                                                     int fixed_bits, uint32_t* __restrict__ out, size_t out_words) {
-    __shared__ __attribute__((aligned(16))) T vals[kFlTile + 16 / sizeof(T)];
+    __shared__ __attribute__((aligned(16))) T vals[kFlLdsVals];
     int b = fixed_bits;
     if (mm) b = bitlen_of<T>(from_order_key<T>(mm->min_key), from_order_key<T>(mm->max_key));
     fl_pack_tile<T, DELTA>(x, n, b, out, out_words, blockIdx.x, vals);

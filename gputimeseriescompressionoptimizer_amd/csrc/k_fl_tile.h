@@ -6,60 +6,113 @@
 namespace ddj {
 
 constexpr int kFlTile = 4096;
+constexpr int kFlGroups = kFlTile / 32;  // 32 values per packing thread
+// LDS the packer needs, in values of T: the tile with one pad slot per 32
+// values (so that the packing threads' reads fall in distinct banks), the
+// row after the tile, and slack.  The packed words reuse the same bytes.
+constexpr int kFlLdsVals = kFlTile + kFlGroups + 4;
+static_assert(kBlock >= kFlGroups, "one packing thread per 32-value group");
+
+__device__ __forceinline__ int fl_slot(int v) { return v + (v >> 5); }
 
 // Packs tile `tile` (kFlTile values) of an FL stream of n values with b
-// bits into out (32-bit words, out_words of them).  vals: LDS for
-// kFlTile + 16 / sizeof(T) values.  DELTA: value i is x[i+1] - x[i]
-// (wrapping), read straight from x (n + 1 rows).
+// bits (b <= 8 * sizeof(T)) into out (32-bit words, out_words of them,
+// 4-byte aligned).  vals: LDS for kFlLdsVals values, 16-byte aligned.
+// DELTA: value i is x[i+1] - x[i] (wrapping), read straight from x (n + 1
+// rows).
+//
+// The tile is staged in LDS with coalesced 16-byte loads; thread g < 128
+// then takes values [32g, 32g + 32) into registers and packs them into
+// exactly b words (32 * b bits), LSB first, which land at words [g*b,
+// g*b + b) of the tile; the tile's 128 * b words are written out with
+// 16-byte stores.
 template <typename T, bool DELTA>
 __device__ __forceinline__ void fl_pack_tile(const T* __restrict__ x, size_t n, int b, uint32_t* __restrict__ out,
                                              size_t out_words, size_t tile, T* vals) {
+    using U = typename UnsignedOf<T>::type;
     const size_t v0 = tile * kFlTile;
     const int nv = (int)((n - v0) < (size_t)kFlTile ? (n - v0) : (size_t)kFlTile);
     const T* src = x + v0;
     if (nv == kFlTile && (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
+        constexpr int V = 16 / (int)sizeof(T);  // values per 16-byte load, all in one 32-value group
+        constexpr int N4 = kFlTile / V;
+        static_assert(N4 % kBlock == 0, "whole loads per thread");
         const u32x4* s4 = reinterpret_cast<const u32x4*>(src);
-        u32x4* d4 = reinterpret_cast<u32x4*>(vals);
-        constexpr int N4 = kFlTile * (int)sizeof(T) / 16;
+        u32x4 q[N4 / kBlock];
 #pragma unroll
-        for (int i = threadIdx.x; i < N4; i += kBlock) d4[i] = __builtin_nontemporal_load(s4 + i);
-        if (DELTA && threadIdx.x == 0) vals[kFlTile] = src[kFlTile];  // x has n + 1 rows
+        for (int r = 0; r < N4 / kBlock; r++) q[r] = __builtin_nontemporal_load(s4 + threadIdx.x + r * kBlock);
+#pragma unroll
+        for (int r = 0; r < N4 / kBlock; r++) {
+            T t[V];
+            __builtin_memcpy(t, &q[r], 16);
+            const int p = fl_slot((threadIdx.x + r * kBlock) * V);
+#pragma unroll
+            for (int k = 0; k < V; k++) vals[p + k] = t[k];
+        }
+        if (DELTA && threadIdx.x == 0) vals[fl_slot(kFlTile)] = src[kFlTile];  // x has n + 1 rows
     } else {
-        for (int i = threadIdx.x; i < kFlTile + (DELTA ? 1 : 0); i += kBlock) vals[i] = i < nv + (DELTA ? 1 : 0) ? src[i] : T(0);
+        for (int i = threadIdx.x; i < kFlTile + (DELTA ? 1 : 0); i += kBlock)
+            vals[fl_slot(i)] = i < nv + (DELTA ? 1 : 0) ? src[i] : T(0);
     }
     __syncthreads();
-    if (DELTA) {  // in place: vals[i] = vals[i+1] - vals[i]
-        using U = typename UnsignedOf<T>::type;
-        T d[kFlTile / kBlock];
+    const int g = threadIdx.x;
+    U v[33];
+    if (g < kFlGroups) {
 #pragma unroll
-        for (int k = 0; k < kFlTile / kBlock; k++) {
-            const int i = threadIdx.x + k * kBlock;
-            d[k] = i < nv ? (T)(U)((U)vals[i + 1] - (U)vals[i]) : T(0);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int k = 0; k < kFlTile / kBlock; k++) vals[threadIdx.x + k * kBlock] = d[k];
-        __syncthreads();
+        for (int k = 0; k < 32; k++) v[k] = (U)vals[33 * g + k];
+        if (DELTA) v[32] = (U)vals[33 * g + 33];  // fl_slot(32 * (g + 1))
     }
-    const int tile_words = kFlTile / 32 * b;  // 128 * b
-    const size_t w0 = tile * (size_t)tile_words;
-    const uint64_t mask = lowmask64(b);
-    // q / b as (q * ceil(2^32 / b)) >> 32: exact for every b <= 64 and
-    // q < 2^18 + 64 (the error term stays below 1/b)
-    const uint64_t rb = ((1ull << 32) + (uint64_t)b - 1) / (uint64_t)b;
-    for (int j = threadIdx.x; j < tile_words; j += kBlock) {
-        if (w0 + j >= out_words) break;
-        const int bit0 = j * 32;
-        const int va = (int)(((uint64_t)bit0 * rb) >> 32);
-        int vb = (int)(((uint64_t)(bit0 + 31) * rb) >> 32);
-        vb = vb < kFlTile - 1 ? vb : kFlTile - 1;
-        uint64_t word = 0;
-        for (int v = va; v <= vb; v++) {
-            const uint64_t val = ubits(vals[v]) & mask;
-            const int off = v * b - bit0;
-            word |= off >= 0 ? (val << off) : (val >> (-off));
+    __syncthreads();  // the packed words overwrite vals
+    uint32_t* words = reinterpret_cast<uint32_t*>(vals);
+    if (g < kFlGroups) {
+        const uint64_t mask = lowmask64(b);
+        uint32_t* w = words + g * b;
+        uint64_t acc = 0;  // pending bits, LSB first
+        int nb = 0, k = 0;
+        // every thread has the same b, so the branches are uniform
+        auto push = [&](uint64_t bits, int len) {  // len <= 32
+            acc |= bits << nb;
+            nb += len;
+            if (nb >= 32) {
+                w[k++] = (uint32_t)acc;
+                acc >>= 32;
+                nb -= 32;
+            }
+        };
+        const int lim = nv - 32 * g;  // values of the group inside the stream
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+            // past the stream's end the deltas, like the values, are zero
+            const U d = DELTA ? (i < lim ? (U)(v[i + 1] - v[i]) : U(0)) : v[i];
+            const uint64_t val = (uint64_t)d & mask;
+            if (sizeof(T) <= 4) {
+                push(val, b);
+            } else if (b <= 32) {
+                push(val, b);
+            } else {
+                push(val & 0xffffffffull, 32);
+                push(val >> 32, b - 32);
+            }
         }
-        out[w0 + j] = (uint32_t)word;
+    }
+    __syncthreads();
+    const int tile_words = kFlGroups * b;
+    const size_t w0 = tile * (size_t)tile_words;
+    uint32_t* o = out + w0;
+    if (w0 + tile_words <= out_words) {
+        // scalar words up to the first 16-byte boundary, then 16-byte stores
+        const int head = (int)((4 - ((reinterpret_cast<uintptr_t>(o) >> 2) & 3)) & 3);
+        const int lead = head < tile_words ? head : tile_words;
+        if ((int)threadIdx.x < lead) o[threadIdx.x] = words[threadIdx.x];
+        const int nq = (tile_words - lead) / 4;
+        u32x4* o4 = reinterpret_cast<u32x4*>(o + lead);
+        for (int i = threadIdx.x; i < nq; i += kBlock) {
+            const uint32_t* s = words + lead + 4 * i;
+            __builtin_nontemporal_store(u32x4{s[0], s[1], s[2], s[3]}, o4 + i);
+        }
+        for (int j = lead + 4 * nq + threadIdx.x; j < tile_words; j += kBlock) o[j] = words[j];
+    } else {
+        for (int j = threadIdx.x; j < tile_words && w0 + j < out_words; j += kBlock) o[j] = words[j];
     }
 }
 

@@ -4,6 +4,8 @@
 // One launch each: every workgroup reduces a grid-stride share with 16-byte
 // loads, writes a partial, and the last workgroup to finish (ticket on a
 // control counter that it resets) folds the partials.
+#include <limits>
+
 #include "dispatch.h"
 #include "k_common.h"
 #include "kernels.h"
@@ -59,21 +61,36 @@ __device__ __forceinline__ uint32_t rd32(const uint32_t* p) {
 // means: NaN rows never win, so a NaN in row 0 is both the minimum and the
 // maximum; -0.0 and +0.0 tie, and the first zero in row order gives the sign.
 // Rows feed add(); `zf` keeps (row << 1 | sign) of the first zero.
-template <typename T> struct FoldKeys {
-    uint64_t mn = ~0ull, mx = 0, zf = ~0ull;
+template <typename T, bool FLOAT = std::is_floating_point<T>::value> struct FoldKeys {
+    uint64_t zf = ~0ull;
+    uint64_t kmn = ~0ull, kmx = 0;
     __device__ __forceinline__ void add(T v, size_t row) {
-        if constexpr (std::is_floating_point<T>::value) {
-            if (v != v) return;
-            if (v == T(0)) {
-                const uint64_t z = ((uint64_t)row << 1) | (__builtin_signbit(v) ? 1ull : 0ull);
-                zf = z < zf ? z : zf;
-                v = T(0);
-            }
+        if (v != v) return;
+        if (v == T(0)) {
+            const uint64_t z = ((uint64_t)row << 1) | (__builtin_signbit(v) ? 1ull : 0ull);
+            zf = z < zf ? z : zf;
+            v = T(0);
         }
         const uint64_t key = order_key(v);
-        mn = key < mn ? key : mn;
-        mx = key > mx ? key : mx;
+        kmn = key < kmn ? key : kmn;
+        kmx = key > kmx ? key : kmx;
     }
+    __device__ __forceinline__ uint64_t min_key() const { return kmn; }
+    __device__ __forceinline__ uint64_t max_key() const { return kmx; }
+};
+// Integers: native compares per row (order keys are monotone in the value),
+// keys only for the result; with no rows the keys are ~0 / 0 as above.
+template <typename T> struct FoldKeys<T, false> {
+    uint64_t zf = ~0ull;
+    T mn = std::numeric_limits<T>::max(), mx = std::numeric_limits<T>::min();
+    bool any = false;
+    __device__ __forceinline__ void add(T v, size_t) {
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+        any = true;
+    }
+    __device__ __forceinline__ uint64_t min_key() const { return any ? order_key(mn) : ~0ull; }
+    __device__ __forceinline__ uint64_t max_key() const { return any ? order_key(mx) : 0ull; }
 };
 
 // Final keys from the reduced ones and row 0.
@@ -125,8 +142,8 @@ __global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size
         done = nv * V;
     }
     for (size_t i = done + tid; i < n; i += stride) f.add(x[i], i);
-    uint64_t mn = block_reduce_min(f.mn, sm);
-    uint64_t mx = block_reduce_max(f.mx, sm);
+    uint64_t mn = block_reduce_min(f.min_key(), sm);
+    uint64_t mx = block_reduce_max(f.max_key(), sm);
     uint64_t zf = block_reduce_min(f.zf, sm);
     if (threadIdx.x == 0) {
         st_agent(partials + 3 * blockIdx.x, mn);
@@ -163,12 +180,13 @@ __global__ __launch_bounds__(kBlock) void k_minmax_delta(const T* __restrict__ x
     __shared__ uint64_t sm[kBlock / kWave];
     __shared__ bool last;
     using U = typename UnsignedOf<T>::type;
-    uint64_t mn = ~0ull, mx = 0;
+    T dmn = std::numeric_limits<T>::max(), dmx = std::numeric_limits<T>::min();
+    bool any = false;
     auto add = [&](T a, T b) {  // d = b - a, wrapping in T
         const T d = (T)(U)((U)b - (U)a);
-        const uint64_t key = order_key(d);
-        mn = key < mn ? key : mn;
-        mx = key > mx ? key : mx;
+        dmn = d < dmn ? d : dmn;
+        dmx = d > dmx ? d : dmx;
+        any = true;
     };
     const size_t m = n - 1;
     const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x, stride = (size_t)gridDim.x * kBlock;
@@ -178,24 +196,37 @@ __global__ __launch_bounds__(kBlock) void k_minmax_delta(const T* __restrict__ x
         const u32x4* xv = reinterpret_cast<const u32x4*>(x);
         const size_t nv = m / V;  // vectors whose V deltas are all in range (x[i*V+V] exists)
         constexpr int Uf = 4;
+        const int lane = threadIdx.x & (kWave - 1);
         for (size_t i0 = tid; i0 < nv; i0 += Uf * stride) {
             u32x4 v[Uf];
             T nx[Uf];
+            // The row after vector i starts vector i + 1, which the next lane
+            // holds (lanes take consecutive vectors); the last lane of the
+            // wave, and the thread with the last vector, load it with the
+            // vectors so that the loop waits on memory once.
 #pragma unroll
             for (int u = 0; u < Uf; u++) {
                 const size_t i = i0 + u * stride;
                 if (i < nv) {
                     v[u] = __builtin_nontemporal_load(xv + i);
-                    nx[u] = x[(i + 1) * V];
+                    if (lane == kWave - 1 || i + 1 >= nv) nx[u] = x[(i + 1) * V];  // (i + 1) * V <= m < n
                 }
             }
 #pragma unroll
             for (int u = 0; u < Uf; u++) {
                 const size_t i = i0 + u * stride;
-                if (i >= nv) break;
+                // first 8 bytes of the next lane's vector (all lanes take part)
+                const uint32_t nb0 = (uint32_t)__shfl_down((int)v[u].x, 1, kWave);
+                const uint32_t nb1 = (uint32_t)__shfl_down((int)v[u].y, 1, kWave);
+                if (i >= nv) continue;
                 T vals[V + 1];
                 __builtin_memcpy(vals, &v[u], 16);
-                vals[V] = nx[u];
+                if (lane == kWave - 1 || i + 1 >= nv) {
+                    vals[V] = nx[u];
+                } else {
+                    const uint64_t nb = ((uint64_t)nb1 << 32) | nb0;
+                    __builtin_memcpy(&vals[V], &nb, sizeof(T));
+                }
 #pragma unroll
                 for (int k = 0; k < V; k++) add(vals[k], vals[k + 1]);
             }
@@ -203,8 +234,8 @@ __global__ __launch_bounds__(kBlock) void k_minmax_delta(const T* __restrict__ x
         done = nv * V;
     }
     for (size_t i = done + tid; i < m; i += stride) add(x[i], x[i + 1]);
-    mn = block_reduce_min(mn, sm);
-    mx = block_reduce_max(mx, sm);
+    uint64_t mn = block_reduce_min(any ? order_key(dmn) : ~0ull, sm);
+    uint64_t mx = block_reduce_max(any ? order_key(dmx) : 0ull, sm);
     if (threadIdx.x == 0) {
         st_agent(partials + 2 * blockIdx.x, mn);
         st_agent(partials + 2 * blockIdx.x + 1, mx);
@@ -287,11 +318,20 @@ struct StatsAcc {
 template <typename T>
 __device__ __forceinline__ void stats_elem(StatsAcc& a, T v, T next, bool has_next, bool has_next2, size_t row,
                                            uint32_t* sm_hist, uint32_t* sm_hll) {
-    FoldKeys<T> fk{a.mn, a.mx, a.zf};
-    fk.add(v, row);
-    a.mn = fk.mn;
-    a.mx = fk.mx;
-    a.zf = fk.zf;
+    if constexpr (std::is_floating_point<T>::value) {
+        FoldKeys<T> fk;
+        fk.kmn = a.mn;
+        fk.kmx = a.mx;
+        fk.zf = a.zf;
+        fk.add(v, row);
+        a.mn = fk.kmn;
+        a.mx = fk.kmx;
+        a.zf = fk.zf;
+    } else {
+        const uint64_t key = order_key(v);
+        a.mn = key < a.mn ? key : a.mn;
+        a.mx = key > a.mx ? key : a.mx;
+    }
     if constexpr (std::is_floating_point<T>::value) {
         a.sum_f += (double)v;
         uint32_t p = float_precision((float)v);
@@ -444,7 +484,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const T* __restrict__ x, size_
 int reduce_grid(Ctx* c, size_t n, int elem) {
     const size_t per_block = (size_t)kBlock * (16 / elem) * 4;
     const size_t g = (n + per_block - 1) / per_block;
-    const size_t cap = std::min<size_t>(kMaxReduceBlocks, (size_t)c->cu_count * 4);
+    const size_t cap = std::min<size_t>(kMaxReduceBlocks, (size_t)c->cu_count * 8);
     return (int)std::max<size_t>(1, std::min(g, cap));
 }
 
