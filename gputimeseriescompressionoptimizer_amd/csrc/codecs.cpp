@@ -171,7 +171,23 @@ Part scale_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
 // -------------------------------------------------------------------- afl
 // AflEncoding::Encode<T> (compression/afl/afl_encoding.cu:15-56): metadata
 // {minBit, rest, 0, 0}; data = ceil(n*b/W) words of T.
-std::vector<Part> afl_encode_int(Ctx* c, int dtype, DView in, const MinMaxResult* known = nullptr) {
+int bits_of(int dtype, const MinMaxResult& h) {
+    return dispatch_int((int)dtype_size(dtype), [&](auto tag) {
+        using T = decltype(tag);
+        T mn, mx;
+        decode_minmax(dtype, h, &mn, &mx);
+        return bitlen_of<T>(mn, mx);
+    });
+}
+
+// The FL data of n values at b bits: ceil(n*b/W) words of T.
+uint64_t fl_bytes(size_t es, int b, size_t n) {
+    const uint64_t W = 8 * es;
+    return (b * (uint64_t)n + W - 1) / W * es;
+}
+
+std::vector<Part> afl_encode_int(Ctx* c, int dtype, DView in, const MinMaxResult* known = nullptr,
+                                 bool defer = false) {
     const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
     if (n == 0) return {empty(), empty()};
     Part x = aligned_view(c, in, es);
@@ -184,21 +200,29 @@ std::vector<Part> afl_encode_int(Ctx* c, int dtype, DView in, const MinMaxResult
         mm = mmb.get<MinMaxResult>();
     }
     const uint64_t W = 8 * es;
-    DevBuf data(c, (size_t)((W * n + W - 1) / W) * es);  // worst case b = W
-    launch_fl_pack(c, dtype, x.p, n, mm, 0, data.get());
     MinMaxResult h;
-    d2h(c, &h, mm, sizeof h);
-    int b = dispatch_int((int)es, [&](auto tag) {
-        using T = decltype(tag);
-        T mn, mx;
-        decode_minmax(dtype, h, &mn, &mx);
-        return bitlen_of<T>(mn, mx);
-    });
-    const uint64_t bytes = (b * (uint64_t)n + W - 1) / W * es;
-    data.set_size((size_t)bytes);
-    kernel_bytes(c, "k_fl_pack", (double)(n * es + bytes));
+    Part data;
+    if (defer) {  // the width first; the pack runs when the blob is assembled
+        d2h(c, &h, mm, sizeof h);
+        auto job = std::make_shared<FlJob>();
+        job->dtype = dtype;
+        job->bits = bits_of(dtype, h);
+        job->n = n;
+        job->x = x;
+        data.n = (size_t)fl_bytes(es, job->bits, n);
+        data.fl = std::move(job);
+    } else {
+        DevBuf buf(c, (size_t)((W * n + W - 1) / W) * es);  // worst case b = W
+        launch_fl_pack(c, dtype, x.p, n, mm, 0, buf.get());
+        d2h(c, &h, mm, sizeof h);
+        buf.set_size((size_t)fl_bytes(es, bits_of(dtype, h), n));
+        kernel_bytes(c, "k_fl_pack", (double)(n * es + buf.size()));
+        data = own(std::move(buf));
+    }
+    const int b = bits_of(dtype, h);
+    const uint64_t bytes = data.n;
     uint8_t meta[4] = {(uint8_t)b, (uint8_t)((bytes * 8 - (uint64_t)n * b) & 0xff), 0, 0};
-    return {host_part(meta, 4), own(std::move(data))};
+    return {host_part(meta, 4), data};
 }
 
 Part fl_decode_raw(Ctx* c, int dtype, const uint8_t* data, size_t bytes, int b, int rest) {
@@ -608,30 +632,50 @@ Part delta_afl_decode(Ctx* c, int dtype, DView dmeta, DView ameta, DView data) {
     return own(std::move(out));
 }
 
-std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in) {
+std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in, bool defer) {
     const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
     if (n < 2) fail(DDJ_EINVAL, "fused delta/FL needs two rows");
     Part x = aligned_view(c, in, es);
     const size_t m = n - 1;
     DevBuf mmb(c, sizeof(MinMaxResult));
     launch_minmax_delta(c, (int)es, x.p, n, mmb.get<MinMaxResult>());
-    const uint64_t W = 8 * es;
-    DevBuf data(c, (size_t)((W * m + W - 1) / W) * es);  // worst case b = W
-    launch_fl_pack_delta(c, (int)es, x.p, m, mmb.get<MinMaxResult>(), data.get());
-    MinMaxResult h;
-    d2h(c, &h, mmb.get(), sizeof h);
-    const int b = dispatch_int((int)es, [&](auto tag) {
-        using T = decltype(tag);
-        T mn, mx;
-        decode_minmax(dtype, h, &mn, &mx);
-        return bitlen_of<T>(mn, mx);
-    });
-    const uint64_t bytes = (b * (uint64_t)m + W - 1) / W * es;
-    data.set_size((size_t)bytes);
     kernel_bytes(c, "k_minmax", (double)(n * es));
-    kernel_bytes(c, "k_fl_pack", (double)(n * es + bytes));
+    const uint64_t W = 8 * es;
+    MinMaxResult h;
+    Part data;
+    if (defer) {  // the width first; the pack runs when the blob is assembled
+        d2h(c, &h, mmb.get(), sizeof h);
+        auto job = std::make_shared<FlJob>();
+        job->dtype = dtype;
+        job->delta = true;
+        job->bits = bits_of(dtype, h);
+        job->n = m;
+        job->x = x;
+        data.n = (size_t)fl_bytes(es, job->bits, m);
+        data.fl = std::move(job);
+    } else {
+        DevBuf buf(c, (size_t)((W * m + W - 1) / W) * es);  // worst case b = W
+        launch_fl_pack_delta(c, (int)es, x.p, m, mmb.get<MinMaxResult>(), buf.get());
+        d2h(c, &h, mmb.get(), sizeof h);
+        buf.set_size((size_t)fl_bytes(es, bits_of(dtype, h), m));
+        kernel_bytes(c, "k_fl_pack", (double)(n * es + buf.size()));
+        data = own(std::move(buf));
+    }
+    const int b = bits_of(dtype, h);
+    const uint64_t bytes = data.n;
     uint8_t meta[4] = {(uint8_t)b, (uint8_t)((bytes * 8 - (uint64_t)m * b) & 0xff), 0, 0};
-    return {borrow(DView{x.p, es}, x.owner), host_part(meta, 4), own(std::move(data))};
+    return {borrow(DView{x.p, es}, x.owner), host_part(meta, 4), data};
+}
+
+void run_fl_job(Ctx* c, const FlJob& j, void* out, size_t bytes) {
+    const size_t es = dtype_size(j.dtype);
+    if (j.n == 0 || bytes == 0) return;
+    if (bytes != fl_bytes(es, j.bits, j.n)) fail(DDJ_EHIP, "FL job size mismatch");
+    if (j.delta)
+        launch_fl_pack_delta(c, (int)es, j.x.p, j.n, nullptr, out, j.bits, bytes);
+    else
+        launch_fl_pack(c, j.dtype, j.x.p, j.n, nullptr, j.bits, out, bytes);
+    kernel_bytes(c, "k_fl_pack", (double)((j.n + (j.delta ? 1 : 0)) * es + fl_bytes(es, j.bits, j.n)));
 }
 
 bool enc_ok(int enc) {
@@ -662,8 +706,9 @@ void return_types(int enc, int dtype, int out[2]) {
     if (enc == DDJ_ENC_RLE || enc == DDJ_ENC_FLOAT_TO_INT) out[0] = DDJ_D_INT;  // rle_encoding.hpp:35, float_encoding.hpp:31
 }
 
-std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResult* minmax) {
+std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResult* minmax, bool defer_fl) {
     if (!dtype_ok(dtype)) fail(DDJ_EINVAL, "unknown data type");
+    if (in.n && !in.p) fail(DDJ_EINVAL, "encoder input has no device bytes");
     switch (enc) {
     case DDJ_ENC_NONE: return none_encode(c, in);
     case DDJ_ENC_DELTA: return delta_encode(c, int_view(dtype), in);
@@ -671,7 +716,7 @@ std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResul
     case DDJ_ENC_AFL:
         if (dtype == DDJ_D_DOUBLE) fail(DDJ_EUNSUPPORTED, "AFL on double returns no outputs (afl_encoding.cu:268-271)");
         if (dtype == DDJ_D_FLOAT) return afl_encode_float(c, in);
-        return afl_encode_int(c, dtype, in, minmax);
+        return afl_encode_int(c, dtype, in, minmax, defer_fl);
     case DDJ_ENC_RLE: return rle_encode(c, dtype, in);
     case DDJ_ENC_DICT: return dict_encode(c, dtype, in);
     case DDJ_ENC_CONST: return const_encode(c, dtype, in);

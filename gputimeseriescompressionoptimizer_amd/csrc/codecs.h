@@ -10,10 +10,14 @@
 
 namespace ddj {
 
+struct FlJob;
+
 // A device byte range that keeps its storage alive.
 // Device bytes, or a few host bytes (metadata computed on the host, which
 // blob assembly uploads together with the node headers instead of one copy
 // per node).  Only encoder metadata (out[0]) is ever host-resident.
+// A part with `fl` set holds no bytes yet: its FL pack runs when the blob
+// is assembled, straight into the blob (p is null, n is the exact size).
 struct Part {
     std::shared_ptr<DevBuf> owner;  // may be null for caller-owned memory
     const uint8_t* p = nullptr;
@@ -22,9 +26,22 @@ struct Part {
     // Optional: a MinMaxResult (device) of the part's values, computed by the
     // encoder that produced them, so that an FL child skips its own pass.
     std::shared_ptr<DevBuf> minmax;
+    std::shared_ptr<const FlJob> fl;
     bool on_host() const { return host != nullptr; }
     DView view() const { return DView{p, n}; }
 };
+// An FL pack whose bit width is known (its min/max pass and read-back ran)
+// and whose output goes straight into a blob being assembled.
+struct FlJob {
+    int dtype = 0;
+    bool delta = false;  // value i is x[i+1] - x[i]; x holds n + 1 rows
+    int bits = 0;
+    size_t n = 0;  // values packed
+    Part x;        // the rows, kept alive until the pack runs
+};
+// Runs the pack into the `bytes` stream bytes at out (any byte address).
+void run_fl_job(Ctx* c, const FlJob& j, void* out, size_t bytes);
+
 Part own(DevBuf&& b);
 Part borrow(DView v, std::shared_ptr<DevBuf> keep = nullptr);
 Part host_part(const void* h, size_t n);
@@ -36,7 +53,10 @@ bool enc_ok(int enc);
 
 // Encoding::Encode: out[0] = metadata, out[1..n_results(enc)] = outputs.
 // `minmax` (optional, device): the input's min/max keys, when known.
-std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResult* minmax = nullptr);
+// `defer_fl`: an integer AFL encoder returns its data as an FlJob part (for
+// a blob being assembled, when the data goes to the blob unchanged).
+std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResult* minmax = nullptr,
+                         bool defer_fl = false);
 // Encoding::Decode: in[0] = metadata, in[1..] = children outputs.
 Part decode(Ctx* c, int enc, int dtype, const std::vector<DView>& in);
 // GetMetadataSize / GetCompressedSize.
@@ -47,7 +67,7 @@ ddj_stats_t stats(Ctx* c, int dtype, DView in);
 // DELTA followed by AFL on an integer column of >= 2 rows, without writing
 // the deltas: {delta metadata, AFL metadata, AFL data}, byte-identical to
 // delta_encode followed by the AFL encoder on its output.
-std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in);
+std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in, bool defer_fl = false);
 // The inverse: AFL data (non-empty) decoded and prefix-summed in one pass.
 Part delta_afl_decode(Ctx* c, int dtype, DView delta_meta, DView afl_meta, DView afl_data);
 

@@ -154,23 +154,23 @@ __global__ __launch_bounds__(kBlock) void k_batch_pack(const BatchColumn* cols, 
     for (int it = blockIdx.x; it < ntiles; it += gridDim.x) {  // uniform per workgroup
         const BatchItem item = tiles[it];
         const BatchColumn c = cols[item.col];
-        uint32_t* out = reinterpret_cast<uint32_t*>(arena + c.data_off);
-        const size_t out_words = (c.data_bytes + 3) / 4;
+        uint8_t* out = arena + c.data_off;
+        const size_t out_bytes = c.data_bytes;
         const uint64_t m = c.delta ? c.n - 1 : c.n;
         const int b = (int)c.bits;
         if (c.es == 8) {
             const int64_t* x = reinterpret_cast<const int64_t*>(c.x);
             if (c.delta)
-                fl_pack_tile<int64_t, true>(x, m, b, out, out_words, item.start, vals);
+                fl_pack_tile<int64_t, true>(x, m, b, out, out_bytes, item.start, vals);
             else
-                fl_pack_tile<int64_t, false>(x, m, b, out, out_words, item.start, vals);
+                fl_pack_tile<int64_t, false>(x, m, b, out, out_bytes, item.start, vals);
         } else {
             const int32_t* x = reinterpret_cast<const int32_t*>(c.x);
             int32_t* v32 = reinterpret_cast<int32_t*>(vals);
             if (c.delta)
-                fl_pack_tile<int32_t, true>(x, m, b, out, out_words, item.start, v32);
+                fl_pack_tile<int32_t, true>(x, m, b, out, out_bytes, item.start, v32);
             else
-                fl_pack_tile<int32_t, false>(x, m, b, out, out_words, item.start, v32);
+                fl_pack_tile<int32_t, false>(x, m, b, out, out_bytes, item.start, v32);
         }
         __syncthreads();  // vals is reused by the next item
     }

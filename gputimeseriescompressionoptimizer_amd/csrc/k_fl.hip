@@ -29,11 +29,11 @@ namespace {
 
 template <typename T, bool DELTA = false>
 __global__ __launch_bounds__(kBlock) void k_fl_pack(const T* __restrict__ x, size_t n, const MinMaxResult* mm,
-                                                    int fixed_bits, uint32_t* __restrict__ out, size_t out_words) {
+                                                    int fixed_bits, uint8_t* __restrict__ out, size_t out_bytes) {
     __shared__ __attribute__((aligned(16))) T vals[kFlLdsVals];
     int b = fixed_bits;
     if (mm) b = bitlen_of<T>(from_order_key<T>(mm->min_key), from_order_key<T>(mm->max_key));
-    fl_pack_tile<T, DELTA>(x, n, b, out, out_words, blockIdx.x, vals);
+    fl_pack_tile<T, DELTA>(x, n, b, out, out_bytes, blockIdx.x, vals);
 }
 
 // Staged words of one tile: at most kFlTile * 64 bits, plus the bytes before
@@ -175,32 +175,36 @@ __global__ __launch_bounds__(kBlock) void k_fl_unpack_delta_scan(const uint8_t* 
 
 }  // namespace
 
-void launch_fl_pack(Ctx* c, int dtype, const void* x, size_t n, const MinMaxResult* mm, int bits, void* out) {
+void launch_fl_pack(Ctx* c, int dtype, const void* x, size_t n, const MinMaxResult* mm, int bits, void* out,
+                    size_t out_bytes) {
     if (n == 0) return;
     const int esize = (int)dtype_size(dtype);
     dispatch_int(esize, [&](auto tag) {
         using T = decltype(tag);
-        const int maxb = mm ? 8 * (int)sizeof(T) : bits;
-        const uint64_t W = 8 * sizeof(T);
-        const size_t out_bytes = (size_t)((maxb * (uint64_t)n + W - 1) / W) * sizeof(T);
-        const size_t out_words = (out_bytes + 3) / 4;
+        if (out_bytes == 0) {  // the widest b, rounded up to whole words
+            const int maxb = mm ? 8 * (int)sizeof(T) : bits;
+            const uint64_t W = 8 * sizeof(T);
+            out_bytes = ((size_t)((maxb * (uint64_t)n + W - 1) / W) * sizeof(T) + 3) & ~size_t(3);
+        }
         const size_t tiles = (n + kFlTile - 1) / kFlTile;
         { KernelTimer kt_(c, "k_fl_pack"); hipLaunchKernelGGL(k_fl_pack<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, static_cast<const T*>(x),
-                           n, mm, bits, static_cast<uint32_t*>(out), out_words); }
+                           n, mm, bits, static_cast<uint8_t*>(out), out_bytes); }
         HIPC(hipGetLastError());
     });
 }
 
-void launch_fl_pack_delta(Ctx* c, int elem_size, const void* x, size_t m, const MinMaxResult* mm, void* out) {
+void launch_fl_pack_delta(Ctx* c, int elem_size, const void* x, size_t m, const MinMaxResult* mm, void* out, int bits,
+                          size_t out_bytes) {
     if (m == 0) return;
     dispatch_int(elem_size, [&](auto tag) {
         using T = decltype(tag);
-        const uint64_t W = 8 * sizeof(T);
-        const size_t out_bytes = (size_t)((W * m + W - 1) / W) * sizeof(T);
-        const size_t out_words = (out_bytes + 3) / 4;
+        if (out_bytes == 0) {  // the widest b, rounded up to whole words
+            const uint64_t W = 8 * sizeof(T);
+            out_bytes = ((size_t)((W * m + W - 1) / W) * sizeof(T) + 3) & ~size_t(3);
+        }
         const size_t tiles = (m + kFlTile - 1) / kFlTile;
         { KernelTimer kt_(c, "k_fl_pack"); hipLaunchKernelGGL((k_fl_pack<T, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                           static_cast<const T*>(x), m, mm, 0, static_cast<uint32_t*>(out), out_words); }
+                           static_cast<const T*>(x), m, mm, bits, static_cast<uint8_t*>(out), out_bytes); }
         HIPC(hipGetLastError());
     });
 }

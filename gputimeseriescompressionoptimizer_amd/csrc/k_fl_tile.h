@@ -16,19 +16,21 @@ static_assert(kBlock >= kFlGroups, "one packing thread per 32-value group");
 __device__ __forceinline__ int fl_slot(int v) { return v + (v >> 5); }
 
 // Packs tile `tile` (kFlTile values) of an FL stream of n values with b
-// bits (b <= 8 * sizeof(T)) into out (32-bit words, out_words of them,
-// 4-byte aligned).  vals: LDS for kFlLdsVals values, 16-byte aligned.
-// DELTA: value i is x[i+1] - x[i] (wrapping), read straight from x (n + 1
-// rows).
+// bits (b <= 8 * sizeof(T)) into the stream's bytes at out (any byte
+// address), writing no byte at or past out_bytes.  vals: LDS for kFlLdsVals
+// values, 16-byte aligned.  DELTA: value i is x[i+1] - x[i] (wrapping), read
+// straight from x (n + 1 rows).
 //
 // The tile is staged in LDS with coalesced 16-byte loads; thread g < 128
 // then takes values [32g, 32g + 32) into registers and packs them into
 // exactly b words (32 * b bits), LSB first, which land at words [g*b,
-// g*b + b) of the tile; the tile's 128 * b words are written out with
-// 16-byte stores.
+// g*b + b) of the tile; the tile's 128 * b words go out as 16-byte stores,
+// realigned to the destination's byte offset, with single bytes at either
+// end (so that a stream can start and end inside a word it shares with its
+// neighbours in a blob).
 template <typename T, bool DELTA>
-__device__ __forceinline__ void fl_pack_tile(const T* __restrict__ x, size_t n, int b, uint32_t* __restrict__ out,
-                                             size_t out_words, size_t tile, T* vals) {
+__device__ __forceinline__ void fl_pack_tile(const T* __restrict__ x, size_t n, int b, uint8_t* __restrict__ out,
+                                             size_t out_bytes, size_t tile, T* vals) {
     using U = typename UnsignedOf<T>::type;
     const size_t v0 = tile * kFlTile;
     const int nv = (int)((n - v0) < (size_t)kFlTile ? (n - v0) : (size_t)kFlTile);
@@ -96,24 +98,28 @@ __device__ __forceinline__ void fl_pack_tile(const T* __restrict__ x, size_t n, 
         }
     }
     __syncthreads();
-    const int tile_words = kFlGroups * b;
-    const size_t w0 = tile * (size_t)tile_words;
-    uint32_t* o = out + w0;
-    if (w0 + tile_words <= out_words) {
-        // scalar words up to the first 16-byte boundary, then 16-byte stores
-        const int head = (int)((4 - ((reinterpret_cast<uintptr_t>(o) >> 2) & 3)) & 3);
-        const int lead = head < tile_words ? head : tile_words;
-        if ((int)threadIdx.x < lead) o[threadIdx.x] = words[threadIdx.x];
-        const int nq = (tile_words - lead) / 4;
-        u32x4* o4 = reinterpret_cast<u32x4*>(o + lead);
-        for (int i = threadIdx.x; i < nq; i += kBlock) {
-            const uint32_t* s = words + lead + 4 * i;
-            __builtin_nontemporal_store(u32x4{s[0], s[1], s[2], s[3]}, o4 + i);
-        }
-        for (int j = lead + 4 * nq + threadIdx.x; j < tile_words; j += kBlock) o[j] = words[j];
-    } else {
-        for (int j = threadIdx.x; j < tile_words && w0 + j < out_words; j += kBlock) o[j] = words[j];
+    const int tile_bytes = 4 * kFlGroups * b;
+    const size_t B0 = tile * (size_t)tile_bytes;
+    if (B0 >= out_bytes) return;
+    const int L = (int)((out_bytes - B0) < (size_t)tile_bytes ? (out_bytes - B0) : (size_t)tile_bytes);
+    uint8_t* d = out + B0;
+    const uint8_t* wb = reinterpret_cast<const uint8_t*>(words);
+    int head = (int)((16 - (reinterpret_cast<uintptr_t>(d) & 15)) & 15);  // bytes before a 16-byte boundary
+    head = head < L ? head : L;
+    if ((int)threadIdx.x < head) d[threadIdx.x] = wb[threadIdx.x];
+    const int r = head & 3;  // the same for every word of the body
+    auto src_word = [&](int s) -> uint32_t {  // tile bytes [s, s + 4), s + 4 <= L
+        const int q = s >> 2;
+        if (r == 0) return words[q];
+        return (uint32_t)((((uint64_t)words[q + 1] << 32) | words[q]) >> (8 * r));
+    };
+    const int nq = (L - head) / 16;
+    u32x4* d4 = reinterpret_cast<u32x4*>(d + head);
+    for (int i = threadIdx.x; i < nq; i += kBlock) {
+        const int s = head + 16 * i;
+        __builtin_nontemporal_store(u32x4{src_word(s), src_word(s + 4), src_word(s + 8), src_word(s + 12)}, d4 + i);
     }
+    for (int j = head + 16 * nq + threadIdx.x; j < L; j += kBlock) d[j] = wb[j];
 }
 
 }  // namespace ddj

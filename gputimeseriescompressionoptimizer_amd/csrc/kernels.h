@@ -37,7 +37,10 @@ void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_r
 // min/max of the wrapping differences x[i+1] - x[i] of n integer rows.
 void launch_minmax_delta(Ctx* c, int elem_size, const void* x, size_t n, MinMaxResult* d_result);
 // FL pack of the m = n - 1 differences of n integer rows (bits from mm).
-void launch_fl_pack_delta(Ctx* c, int elem_size, const void* x, size_t m, const MinMaxResult* mm, void* out);
+// With mm null the width is `bits`; no byte at or past out + out_bytes is
+// written (0: room for the widest b, rounded up to whole words).
+void launch_fl_pack_delta(Ctx* c, int elem_size, const void* x, size_t m, const MinMaxResult* mm, void* out,
+                          int bits = 0, size_t out_bytes = 0);
 // Converts order keys back to the typed min/max (host).
 void decode_minmax(int dtype, const MinMaxResult& r, void* mn, void* mx);
 
@@ -57,9 +60,13 @@ struct DevStats {
 };
 void launch_stats(Ctx* c, int dtype, const void* x, size_t n, DevStats* d_out);
 
-// FL pack: bits from `mm` (device min/max) when bits == 0, otherwise fixed.
-// Writes ceil(n*b/W)*sizeof(T) bytes rounded up to 4 into out (32-bit words).
-void launch_fl_pack(Ctx* c, int dtype, const void* x, size_t n, const MinMaxResult* mm, int bits, void* out);
+// FL pack: bits from `mm` (device min/max) when mm is set, otherwise `bits`.
+// Writes the ceil(n*b/W)*sizeof(T) stream bytes to out (any byte address),
+// and zeros after them up to the end of the last 32-bit word, but no byte at
+// or past out + out_bytes (0: room for the widest b, rounded up to whole
+// words).
+void launch_fl_pack(Ctx* c, int dtype, const void* x, size_t n, const MinMaxResult* mm, int bits, void* out,
+                    size_t out_bytes = 0);
 // FL unpack of m b-bit deltas fused with the delta decode: out[0] = first
 // (device, one value), out[i+1] = out[i] + d[i] (wrapping).
 void launch_fl_unpack_delta(Ctx* c, int elem_size, const uint8_t* first, const uint8_t* in, size_t in_bytes, size_t m,

@@ -46,13 +46,18 @@ struct Piece {
 
 bool is_int_type(int dt) { return dt == DDJ_D_INT || dt == DDJ_D_TIME || dt == DDJ_D_SHORT || dt == DDJ_D_CHAR; }
 
+// The node's single output goes to the blob unchanged: no child, or one
+// none leaf.
+bool output_is_leaf(const TNode& node) {
+    return node.ch.empty() || (node.ch.size() == 1 && node.ch[0]->enc == DDJ_ENC_NONE && node.ch[0]->ch.empty());
+}
+
 // DELTA -> AFL (-> none) on an integer column: the deltas feed the FL pass
 // straight from the column (delta_afl_encode) and are never written.
 bool fusable_delta_afl(const TNode& node, const Part& data) {
     if (node.enc != DDJ_ENC_DELTA || !is_int_type(node.dtype) || node.ch.size() != 1) return false;
     const TNode& a = *node.ch[0];
-    if (a.enc != DDJ_ENC_AFL || a.dtype != node.dtype) return false;
-    if (!(a.ch.empty() || (a.ch.size() == 1 && a.ch[0]->enc == DDJ_ENC_NONE && a.ch[0]->ch.empty()))) return false;
+    if (a.enc != DDJ_ENC_AFL || a.dtype != node.dtype || !output_is_leaf(a)) return false;
     return data.n / dtype_size(node.dtype) >= 2;
 }
 
@@ -75,14 +80,18 @@ size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& 
         res.push_back(host_part(&len, sizeof len));
         res.push_back(data);
     } else if (fusable_delta_afl(node, data)) {
-        auto f = delta_afl_encode(c, node.dtype, data.view());
+        auto f = delta_afl_encode(c, node.dtype, data.view(), true);
         const size_t es = dtype_size(node.dtype);
         Part deltas;  // never materialised: only its size is read
         deltas.n = (data.n / es - 1) * es;
         res = {f[0], deltas};
         child_pre = {f[1], f[2]};
     } else {
-        res = encode(c, node.enc, node.dtype, data.view(), data.minmax ? data.minmax->get<MinMaxResult>() : nullptr);
+        // an integer AFL whose data goes to the blob unchanged packs it there
+        // when the blob is assembled
+        const bool defer = node.enc == DDJ_ENC_AFL && is_int_type(node.dtype) && output_is_leaf(node);
+        res = encode(c, node.enc, node.dtype, data.view(), data.minmax ? data.minmax->get<MinMaxResult>() : nullptr,
+                     defer);
     }
     Piece h;
     h.is_header = true;
@@ -190,9 +199,12 @@ std::string tree_string(const NodeP& n) {
 Part compress_tree(Ctx* c, const NodeP& root, DView data) {
     std::vector<Piece> pieces;
     size_t total = compress_node(c, *root, borrow(data), pieces);
-    // Concatenate (util/copy/cuda_array_copy.cu:14-33) as one gather kernel.
-    // The copy table, the node headers and the host-resident metadata travel
-    // in one upload: their ranges point into the table buffer itself.
+    // Concatenate (util/copy/cuda_array_copy.cu:14-33): deferred FL packs
+    // write straight into their blob ranges, everything else moves in one
+    // gather kernel.  The copy table, the node headers and the host-resident
+    // metadata travel in one upload: their ranges point into the table
+    // buffer itself.
+    DevBuf blob(c, total);
     size_t n_ranges = 0, inline_bytes = 0;
     for (auto& p : pieces) {
         if (p.keep_alive) continue;
@@ -200,7 +212,7 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
             inline_bytes += 8;
         else if (p.part.on_host())
             inline_bytes += p.part.n;
-        if (p.is_header || p.part.n) n_ranges++;
+        if (p.is_header || (p.part.n && !p.part.fl)) n_ranges++;
     }
     DevBuf rdev(c, sizeof(CopyRange) * n_ranges + inline_bytes);
     const uint8_t* inl_dev = rdev.bytes_ptr() + sizeof(CopyRange) * n_ranges;
@@ -209,6 +221,11 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
     uint8_t* inl = table.data() + sizeof(CopyRange) * n_ranges;
     size_t off = 0, io = 0, ri = 0;
     uint64_t max_bytes = 0, copied = 0;
+    struct JobAt {
+        const FlJob* job;
+        uint64_t off, bytes;
+    };
+    std::vector<JobAt> jobs;
     for (auto& p : pieces) {
         if (p.keep_alive) continue;
         if (p.is_header) {
@@ -216,24 +233,31 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
             ranges[ri++] = CopyRange{inl_dev + io, off, 8};
             io += 8;
             off += 8;
+            copied += 8;
         } else if (p.part.n) {
-            if (p.part.on_host()) {
-                memcpy(inl + io, p.part.host->data(), p.part.n);
-                ranges[ri++] = CopyRange{inl_dev + io, off, p.part.n};
-                io += p.part.n;
+            const size_t len = p.part.n;
+            if (p.part.fl) {  // packed in place below
+                jobs.push_back(JobAt{p.part.fl.get(), off, len});
+            } else if (p.part.on_host()) {
+                memcpy(inl + io, p.part.host->data(), len);
+                ranges[ri++] = CopyRange{inl_dev + io, off, len};
+                io += len;
             } else {
-                ranges[ri++] = CopyRange{p.part.p, off, p.part.n};
+                ranges[ri++] = CopyRange{p.part.p, off, len};
             }
-            max_bytes = std::max<uint64_t>(max_bytes, p.part.n);
-            off += p.part.n;
+            if (!p.part.fl) {
+                max_bytes = std::max<uint64_t>(max_bytes, len);
+                copied += len;
+            }
+            off += len;
         }
-        copied += p.is_header ? 8 : p.part.n;
     }
-    if (off != total) fail(DDJ_EHIP, "blob size mismatch");
+    if (off != total || ri != n_ranges) fail(DDJ_EHIP, "blob size mismatch");
     h2d(c, rdev.get(), table.data(), table.size());
-    DevBuf blob(c, total);
     kernel_bytes(c, "k_gather_copy", 2.0 * (double)copied);
     launch_gather_copy(c, rdev.get<CopyRange>(), (int)n_ranges, blob.bytes_ptr(), std::max<uint64_t>(max_bytes, 8));
+    // the FL streams go straight to their blob ranges (any byte offset)
+    for (const JobAt& j : jobs) run_fl_job(c, *j.job, blob.bytes_ptr() + j.off, j.bytes);
     return own(std::move(blob));
 }
 
