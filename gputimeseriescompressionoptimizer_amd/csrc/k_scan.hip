@@ -11,6 +11,7 @@
 // staged through LDS with striped (coalesced) loads, outputs are compacted in
 // LDS and written back striped.
 #include <limits>
+#include <type_traits>
 
 #include "dispatch.h"
 #include "k_common.h"
@@ -27,14 +28,16 @@ constexpr int kTile = kBlock * kItems;
 constexpr int kTilePad = kTile + kTile / 16;
 __device__ __forceinline__ int P(int i) { return i + (i >> 4); }
 
+// The split predicate.  Kernels take its kind as a template argument K (the
+// switch folds away, so the per-row test is one or a few compares).
 template <typename T> struct Pred {
     int kind;
     T a, b;
     T set[4];
     int set_n;
-    __device__ __forceinline__ bool operator()(T v, uint32_t& code) const {
+    template <int K> __device__ __forceinline__ bool eval(T v, uint32_t& code) const {
         code = 0;
-        switch (kind) {
+        switch (K) {
         case PRED_LOWER: return v < a;                // LowerOperator (util/stencil/stencil_operators.hpp)
         case PRED_INSIDE: return !(v > b || v < a);   // OutsideOperator: true inside [low, high]
         case PRED_NOT_EQUAL: return v != a;           // _constEncodeStencilKernel (const_encoding.cu:19-25)
@@ -64,7 +67,7 @@ template <typename T> T bits_to(uint64_t b) {
 // MM (integer types): also the order keys of the minimum and maximum of the
 // rows going to each side, keys[4 t .. 4 t + 3] = {yes min, yes max, no min,
 // no max}, so that an FL encoder of either output needs no pass of its own.
-template <typename T, bool MM>
+template <typename T, bool MM, int K>
 __global__ __launch_bounds__(kBlock) void k_split_count(const T* __restrict__ x, size_t n, Pred<T> p,
                                                         uint64_t* __restrict__ vals, uint64_t* __restrict__ keys) {
     __shared__ uint64_t red[kBlock / kWave];
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_split_count(const T* __restrict__ x,
         __syncthreads();  // before any thread folds into mm
     }
     auto take = [&](T v) {
-        const bool f = p(v, code);
+        const bool f = p.template eval<K>(v, code);
         c += f ? 1u : 0u;
         if constexpr (MM) {
             const T a = f ? v : ylo, b = f ? v : yhi, d = f ? nlo : v, e = f ? nhi : v;
@@ -131,7 +134,7 @@ __global__ __launch_bounds__(kBlock) void k_split_count(const T* __restrict__ x,
 }
 
 
-template <typename T, bool CODES, bool WRITE_NO>
+template <typename T, bool CODES, bool WRITE_NO, int K>
 __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_t n, Pred<T> p,
                                                   uint8_t* __restrict__ stencil, T* __restrict__ yes,
                                                   T* __restrict__ no, uint8_t* __restrict__ codes,
@@ -168,7 +171,7 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     for (int k = 0; k < kItems; k++) {
         v[k] = buf[P(threadIdx.x * kItems + k)];
         uint32_t code;
-        bool f = p(v[k], code);
+        bool f = p.template eval<K>(v[k], code);
         mask |= (uint32_t)f << k;
         if (threadIdx.x * kItems + k < cnt && f) {
             if (CODES) cchunk |= (code & 3u) << (2 * c);
@@ -268,6 +271,14 @@ constexpr uint32_t kLdsDict = 64;  // dictionaries up to this size are staged in
 
 // Code j: word j / per read as two aligned 32-bit loads joined by a funnel
 // shift (an aligned word holding a byte of the stream never crosses a page).
+template <typename T> __device__ __forceinline__ T dict_key(const DictCodes& d, const T* sdict, uint32_t word, uint32_t f) {
+    uint32_t code = d.bits >= 32 ? word : (word >> (d.bits * f)) & ((1u << d.bits) - 1u);
+    code = code < d.K ? code : d.K - 1;
+    if (d.K <= kLdsDict) return sdict[code];
+    T v;
+    __builtin_memcpy(&v, d.dict + sizeof(T) * code, sizeof(T));
+    return v;
+}
 template <typename T>
 __device__ __forceinline__ T dict_value(const DictCodes& d, const T* sdict, uint64_t j) {
     const uint64_t per = 32 / d.bits;
@@ -277,18 +288,55 @@ __device__ __forceinline__ T dict_value(const DictCodes& d, const T* sdict, uint
     const uint32_t* wa = reinterpret_cast<const uint32_t*>(a & ~(uintptr_t)3);
     const uint32_t sh = (uint32_t)(a & 3) * 8;
     const uint32_t word = sh ? __builtin_amdgcn_alignbit(wa[1], wa[0], sh) : wa[0];
-    uint32_t code = d.bits >= 32 ? word : (word >> (d.bits * f)) & ((1u << d.bits) - 1u);
-    code = code < d.K ? code : d.K - 1;
-    if (d.K <= kLdsDict) return sdict[code];
-    T v;
-    __builtin_memcpy(&v, d.dict + sizeof(T) * code, sizeof(T));
-    return v;
+    return dict_key<T>(d, sdict, word, f);
 }
+
+// Rows [s0, s0 + len) of src (element-aligned) into LDS rows dst0 + e, with
+// aligned 16-byte loads (an aligned 16-byte chunk holding a valid row never
+// crosses a page); rows at or past `avail` read as zero.  Returns whether
+// the range ran past avail (corrupt input).
+template <typename T>
+__device__ __forceinline__ bool stage_rows(const T* __restrict__ src, uint64_t s0, int len, uint64_t avail, T* buf,
+                                           int dst0) {
+    if (len <= 0) return false;
+    const int ok = s0 >= avail ? 0 : (int)((avail - s0) < (uint64_t)len ? (avail - s0) : (uint64_t)len);
+    for (int e = ok + threadIdx.x; e < len; e += kBlock) buf[P(dst0 + e)] = T(0);
+    if (ok > 0) {
+        constexpr int V = 16 / sizeof(T);
+        const uintptr_t a = reinterpret_cast<uintptr_t>(src + s0);
+        const int sh = (int)((a & 15) / sizeof(T));  // rows before s0 in its chunk
+        const u32x4* v4 = reinterpret_cast<const u32x4*>(a & ~(uintptr_t)15);
+        const int nv = (sh + ok + V - 1) / V;
+        for (int k = threadIdx.x; k < nv; k += kBlock) {
+            const u32x4 w = __builtin_nontemporal_load(v4 + k);
+            T ev[V];
+            __builtin_memcpy(ev, &w, 16);
+            const int e0 = V * k - sh;
+#pragma unroll
+            for (int j = 0; j < V; j++) {
+                const int e = e0 + j;
+                if (e >= 0 && e < ok) buf[P(dst0 + e)] = ev[j];
+            }
+        }
+    }
+    return ok < len;
+}
+
+// Code words a tile's codes can touch when codes are at most 8 bits wide,
+// plus a chunk of realignment slack.
+constexpr int kCodeLdsWords = kTile / 4 + 2 + 8;
 
 // Merge: out[i] = bit i ? yes[k++] : no[j++], with `no` the constant
 // `no_const` (CONST_NO) and `yes` from dictionary codes (CODES).  Rows past
 // n_yes / n_no (a stencil with more set or clear bits than the outputs hold:
 // corrupt input) read as zero and raise the error flag.
+//
+// The tile's yes count is prefix[tile + 1] - prefix[tile], so its yes rows
+// [Y0, Y0 + total) and no rows [N0, N0 + cnt - total) are contiguous ranges
+// known at the start: they are staged in LDS (yes first, then no) with
+// 16-byte loads, the stencil scan gives each row its source slot, and the
+// tile is written back striped.  Codes of at most 8 bits are staged the
+// same way (as the bytes of their words) and looked up per row.
 template <typename T, bool CONST_NO, bool CODES>
 __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bits, size_t n, const T* __restrict__ yes,
                                                   size_t n_yes, DictCodes dc, const T* __restrict__ no, size_t n_no,
@@ -297,10 +345,39 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
     __shared__ T buf[kTilePad];
     __shared__ uint64_t red[kBlock / kWave];
     __shared__ T sdict[CODES ? kLdsDict : 1];
+    __shared__ __attribute__((aligned(16))) uint32_t cw[CODES ? kCodeLdsWords : 4];
     const uint32_t tile = blockIdx.x;
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
-    if (CODES && dc.K <= kLdsDict && threadIdx.x < dc.K) __builtin_memcpy(&sdict[threadIdx.x], dc.dict + sizeof(T) * threadIdx.x, sizeof(T));
+    const uint64_t Y0 = prefix[tile];
+    const int total = (int)(prefix[tile + 1] - Y0);  // 0 <= total <= cnt: counted from the same stencil
+    const size_t N0 = base - Y0;
+    bool bad = false;
+    // CODES: bytes of code words [w_lo, w_hi] staged from chunk boundary `al`
+    uint64_t w_lo = 0;
+    int off0 = 0;
+    bool staged = false;
+    if constexpr (CODES) {
+        if (dc.K <= kLdsDict && threadIdx.x < dc.K)
+            __builtin_memcpy(&sdict[threadIdx.x], dc.dict + sizeof(T) * threadIdx.x, sizeof(T));
+        const uint64_t j_end = Y0 + (uint64_t)total < n_yes ? Y0 + (uint64_t)total : n_yes;  // codes that exist
+        if (Y0 + (uint64_t)total > n_yes) bad = true;
+        if (dc.bits <= 8 && j_end > Y0) {
+            staged = true;
+            const uint64_t per = 32 / dc.bits;
+            w_lo = Y0 / per;
+            const uint64_t w_hi = (j_end - 1) / per;
+            const uintptr_t a = reinterpret_cast<uintptr_t>(dc.words + 4 * w_lo);
+            off0 = (int)(a & 15);
+            const u32x4* c4 = reinterpret_cast<const u32x4*>(a & ~(uintptr_t)15);
+            const int nchunks = (off0 + 4 * (int)(w_hi - w_lo + 1) + 15) / 16;  // <= kCodeLdsWords / 4
+            u32x4* l4 = reinterpret_cast<u32x4*>(cw);
+            for (int k = threadIdx.x; k < nchunks; k += kBlock) l4[k] = __builtin_nontemporal_load(c4 + k);
+        }
+    } else {
+        bad |= stage_rows(yes, Y0, total, n_yes, buf, 0);
+    }
+    if (!CONST_NO) bad |= stage_rows(no, N0, cnt - total, n_no, buf, total);
     const size_t g0 = base + (size_t)threadIdx.x * kItems;
     uint32_t mask = 0;
     if (g0 < n) mask = bits[g0 / 8];
@@ -310,37 +387,39 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
         mask = 0;
     else if (mine < kItems)
         mask &= (1u << mine) - 1u;
-    uint64_t total;
-    const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &total, red);  // also publishes sdict
-    const uint64_t Y0 = prefix[tile];
-    const size_t N0 = base - Y0;
-    bool bad = false;
-    for (int i = threadIdx.x; i < cnt; i += kBlock) {
-        T v = T(0);
-        if (i < (int)total) {
-            const uint64_t j = Y0 + i;
-            if (j < n_yes)
-                v = CODES ? dict_value<T>(dc, sdict, j) : yes[j];
-            else
-                bad = true;
-        } else if (CONST_NO) {
-            v = no_const;
-        } else {
-            const uint64_t j = N0 + (i - (int)total);
-            if (j < n_no)
-                v = no[j];
-            else
-                bad = true;
-        }
-        buf[P(i)] = v;
-    }
+    uint64_t tsum;
+    const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &tsum, red);  // a barrier: LDS staged
     if (bad) __hip_atomic_store(error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
     T v[kItems];
-    int yr = (int)excl, nr = (int)total + threadIdx.x * kItems - (int)excl;
+    int yr = (int)excl, nr = total + threadIdx.x * kItems - (int)excl;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
-        if (threadIdx.x * kItems + k < cnt) v[k] = ((mask >> k) & 1) ? buf[P(yr++)] : buf[P(nr++)];
+        if ((mask >> k) & 1) {
+            if constexpr (CODES) {
+                const uint64_t j = Y0 + (uint64_t)yr;
+                T val = T(0);
+                if (j < n_yes) {
+                    if (staged) {
+                        const uint64_t per = 32 / dc.bits;
+                        const uint64_t w = dc.pshift >= 0 ? (j >> dc.pshift) : j / per;
+                        const uint32_t f = (uint32_t)(j - w * per);
+                        const int boff = off0 + 4 * (int)(w - w_lo);
+                        const int q = boff >> 2, r = boff & 3;
+                        const uint32_t word = r ? __builtin_amdgcn_alignbyte(cw[q + 1], cw[q], (uint32_t)r) : cw[q];
+                        val = dict_key<T>(dc, sdict, word, f);
+                    } else {
+                        val = dict_value<T>(dc, sdict, j);
+                    }
+                }
+                v[k] = val;
+            } else {
+                v[k] = buf[P(yr < kTile ? yr : kTile - 1)];
+            }
+            yr++;
+        } else {
+            v[k] = CONST_NO ? no_const : buf[P(nr < kTile ? nr : kTile - 1)];
+            nr++;
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -631,6 +710,19 @@ inline unsigned grid_for(Ctx* c, size_t items) {
 
 }  // namespace
 
+namespace {
+// Calls f(std::integral_constant<int, kind>) for a predicate kind.
+template <typename F> void with_kind(int kind, F&& f) {
+    switch (kind) {
+    case PRED_LOWER: f(std::integral_constant<int, PRED_LOWER>{}); break;
+    case PRED_INSIDE: f(std::integral_constant<int, PRED_INSIDE>{}); break;
+    case PRED_NOT_EQUAL: f(std::integral_constant<int, PRED_NOT_EQUAL>{}); break;
+    case PRED_IN_SET: f(std::integral_constant<int, PRED_IN_SET>{}); break;
+    default: fail(DDJ_EINVAL, "unknown split predicate");
+    }
+}
+}  // namespace
+
 void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa, uint8_t* stencil, void* yes,
                   void* no, uint8_t* codes, uint64_t* d_counts, MinMaxResult* mm_yes, MinMaxResult* mm_no) {
     if (n == 0) return;
@@ -648,32 +740,45 @@ void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa
         auto ys = static_cast<T*>(yes);
         auto ns = static_cast<T*>(no);
         kernel_bytes(c, "k_split_count", (double)n * sizeof(T));
-        bool counted = false;
+        const bool mm = mm_yes && mm_no;
+        if ((mm_yes || mm_no) && !(mm && std::is_integral<T>::value))
+            fail(DDJ_EINVAL, "split min/max outputs need an integer type and both outputs");
+        if (codes && p.kind != PRED_IN_SET) fail(DDJ_EINVAL, "split codes need a dictionary predicate");
         DevBuf keys;
-        if constexpr (std::is_integral<T>::value) {
-            if (mm_yes && mm_no) {
-                keys = DevBuf(c, 32 * tiles);
-                { KernelTimer kt_(c, "k_split_count"); hipLaunchKernelGGL((k_split_count<T, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p, ts.vals, keys.get<uint64_t>()); }
-                counted = true;
+        if (mm) keys = DevBuf(c, 32 * tiles);
+        const dim3 grid((unsigned)tiles), block(kBlock);
+        with_kind(p.kind, [&](auto kind) {
+            constexpr int K = decltype(kind)::value;
+            KernelTimer kt_(c, "k_split_count");
+            if constexpr (std::is_integral<T>::value) {
+                if (mm) {
+                    hipLaunchKernelGGL((k_split_count<T, true, K>), grid, block, 0, c->stream, xs, n, p, ts.vals,
+                                       keys.get<uint64_t>());
+                    return;
+                }
             }
-        }
-        if (!counted) {
-            if (mm_yes || mm_no) fail(DDJ_EINVAL, "split min/max outputs need an integer type and both outputs");
-            { KernelTimer kt_(c, "k_split_count"); hipLaunchKernelGGL((k_split_count<T, false>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p, ts.vals, nullptr); }
-        }
+            hipLaunchKernelGGL((k_split_count<T, false, K>), grid, block, 0, c->stream, xs, n, p, ts.vals, nullptr);
+        });
         HIPC(hipGetLastError());
-        ts.scan(c, counted ? keys.get<uint64_t>() : nullptr, mm_yes, mm_no);
+        ts.scan(c, mm ? keys.get<uint64_t>() : nullptr, mm_yes, mm_no);
         const uint64_t* pre = ts.prefix;
-        if (codes) {
-            { KernelTimer kt_(c, "k_split"); hipLaunchKernelGGL((k_split<T, true, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p,
-                               stencil, ys, ns, codes, pre, d_counts); }
-        } else if (no) {
-            { KernelTimer kt_(c, "k_split"); hipLaunchKernelGGL((k_split<T, false, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n, p,
-                               stencil, ys, ns, codes, pre, d_counts); }
-        } else {
-            { KernelTimer kt_(c, "k_split"); hipLaunchKernelGGL((k_split<T, false, false>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, xs, n,
-                               p, stencil, ys, ns, codes, pre, d_counts); }
-        }
+        with_kind(p.kind, [&](auto kind) {
+            constexpr int K = decltype(kind)::value;
+            KernelTimer kt_(c, "k_split");
+            if constexpr (K == PRED_IN_SET) {
+                if (codes) {
+                    hipLaunchKernelGGL((k_split<T, true, true, K>), grid, block, 0, c->stream, xs, n, p, stencil, ys, ns,
+                                       codes, pre, d_counts);
+                    return;
+                }
+            }
+            if (no)
+                hipLaunchKernelGGL((k_split<T, false, true, K>), grid, block, 0, c->stream, xs, n, p, stencil, ys, ns,
+                                   codes, pre, d_counts);
+            else
+                hipLaunchKernelGGL((k_split<T, false, false, K>), grid, block, 0, c->stream, xs, n, p, stencil, ys, ns,
+                                   codes, pre, d_counts);
+        });
         HIPC(hipGetLastError());
     });
 }
