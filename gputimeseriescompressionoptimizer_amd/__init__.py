@@ -240,7 +240,7 @@ class Context:
                                               ctypes.byref(b)), "kernel_time")
         return ms.value, n.value, b.value
 
-    KERNELS = ["k_adam", "k_add_scalar", "k_copy_pairs", "k_delta_decode", "k_delta_encode", "k_delta_sums",
+    KERNELS = ["k_adam", "k_add_scalar", "k_code_edges", "k_copy_pairs", "k_delta_decode", "k_delta_encode", "k_delta_sums",
                "k_fill_u64", "k_fill_zero", "k_fl_delta_sums", "k_fl_pack", "k_fl_unpack", "k_fl_unpack_delta_scan",
                "k_float_compose", "k_float_split", "k_float_to_int", "k_gather_copy", "k_gfc_chunk_bytes",
                "k_gfc_decode", "k_gfc_encode", "k_gfc_sizes", "k_hist_count", "k_hist_topk", "k_int_to_float",

@@ -460,10 +460,9 @@ std::vector<Part> dict_encode(Ctx* c, int dtype, DView in) {
     for (int i = 0; i < kDictK; i++) p.set_bits[i] = keys[i];
     // freq = [size_t K][size_t m][K keys][ceil(m / 16) words of 2-bit codes]
     // (ALT_BITLEN(K - 1) = 2); the split packs the codes in place, into words
-    // sized for every row being frequent
+    // sized for every row being frequent (it writes every word of the m codes)
     const size_t head = 16 + kDictK * es;
     DevBuf st(c, 1 + (n + 7) / 8), f(c, head + 4 * ((n + 15) / 16)), others(c, n * es), cnt(c, 8);
-    launch_fill_zero(c, f.bytes_ptr() + head, 4 * ((n + 15) / 16));
     launch_split(c, dtype, x.p, n, p, st.bytes_ptr(), nullptr, others.get(), f.bytes_ptr() + head,
                  cnt.get<uint64_t>());
     uint64_t m = 0;

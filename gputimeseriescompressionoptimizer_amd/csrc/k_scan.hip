@@ -138,7 +138,8 @@ template <typename T, bool CODES, bool WRITE_NO, int K>
 __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_t n, Pred<T> p,
                                                   uint8_t* __restrict__ stencil, T* __restrict__ yes,
                                                   T* __restrict__ no, uint8_t* __restrict__ codes,
-                                                  const uint64_t* __restrict__ prefix, uint64_t* d_count) {
+                                                  uint32_t* __restrict__ edges, const uint64_t* __restrict__ prefix,
+                                                  uint64_t* d_count) {
     constexpr int kCodeWords = kTile / 16 + 2;  // 2-bit code words a tile's codes can touch
     __shared__ T buf[kTilePad];
     __shared__ uint32_t cwl[CODES ? kCodeWords : 1];  // those words, aligned to the output's word grid
@@ -209,16 +210,18 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     }
     __syncthreads();
     if (CODES) {
-        // words shared with a neighbouring tile are OR-ed into the zeroed output
+        // words shared with a neighbouring tile (the first and the last, when
+        // partly filled) go to edges[2 tile], edges[2 tile + 1]: k_code_edges
+        // assembles them (device-scope atomics on the words cost more)
         uint32_t* cw = reinterpret_cast<uint32_t*>(codes);
         const uint64_t w_first = Y0 / 16, w_end = (Y0 + total + 15) / 16;
-        const int nt = (int)total;
-        for (int q = threadIdx.x; q < (int)(w_end - w_first); q += kBlock) {
+        const int nt = (int)total, nw = (int)(w_end - w_first);
+        for (int q = threadIdx.x; q < nw; q += kBlock) {
             const int i0 = 16 * q - lead;  // tile-local index of the word's field 0
             if (i0 >= 0 && i0 + 16 <= nt)
                 cw[w_first + q] = cwl[q];
             else
-                atomicOr(&cw[w_first + q], cwl[q]);
+                edges[2 * (size_t)tile + (q == 0 ? 0 : 1)] = cwl[q];
         }
     } else if (yes) {
         for (int i = threadIdx.x; i < (int)total; i += kBlock) yes[Y0 + i] = buf[P(i)];
@@ -228,6 +231,28 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
         for (int i = threadIdx.x; i < cnt - (int)total; i += kBlock) no[N0 + i] = buf[P(nbase + i)];
     }
     if (threadIdx.x == 0 && base + cnt == n) d_count[0] = Y0 + total;
+}
+
+// Code words that several tiles fill: tile t's codes are [prefix[t],
+// prefix[t+1]); a word whose field 0 lies in tile t and which t leaves
+// partly empty is assembled by t's thread from t's contribution and the
+// first-word contributions of the following tiles that reach into it.
+// Every other word was stored whole by k_split.
+__global__ __launch_bounds__(kBlock) void k_code_edges(const uint64_t* __restrict__ prefix, size_t tiles,
+                                                       const uint32_t* __restrict__ edges, uint32_t* __restrict__ cw) {
+    const size_t t = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (t >= tiles) return;
+    const uint64_t y0 = prefix[t], y1 = prefix[t + 1];
+    if (y1 == y0) return;
+    const uint64_t wf = y0 / 16, wl = (y1 - 1) / 16, c0 = 16 * wl;
+    if (c0 < y0 || y1 >= c0 + 16) return;  // field 0 in an earlier tile, or the word is whole
+    uint32_t w = edges[2 * t + (wl == wf ? 0 : 1)];
+    for (size_t u = t + 1; u < tiles; u++) {
+        const uint64_t u0 = prefix[u];
+        if (u0 >= c0 + 16) break;
+        if (prefix[u + 1] > u0) w |= edges[2 * u];  // its first word is this one
+    }
+    cw[wl] = w;
 }
 
 // Splitter::Merge (reference util/splitter/splitter.cu:140-184) driven by a
@@ -762,23 +787,32 @@ void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa
         HIPC(hipGetLastError());
         ts.scan(c, mm ? keys.get<uint64_t>() : nullptr, mm_yes, mm_no);
         const uint64_t* pre = ts.prefix;
+        DevBuf edges;
+        if (codes) edges = DevBuf(c, 8 * tiles);
+        uint32_t* ed = codes ? edges.get<uint32_t>() : nullptr;
         with_kind(p.kind, [&](auto kind) {
             constexpr int K = decltype(kind)::value;
             KernelTimer kt_(c, "k_split");
             if constexpr (K == PRED_IN_SET) {
                 if (codes) {
                     hipLaunchKernelGGL((k_split<T, true, true, K>), grid, block, 0, c->stream, xs, n, p, stencil, ys, ns,
-                                       codes, pre, d_counts);
+                                       codes, ed, pre, d_counts);
                     return;
                 }
             }
             if (no)
                 hipLaunchKernelGGL((k_split<T, false, true, K>), grid, block, 0, c->stream, xs, n, p, stencil, ys, ns,
-                                   codes, pre, d_counts);
+                                   codes, ed, pre, d_counts);
             else
                 hipLaunchKernelGGL((k_split<T, false, false, K>), grid, block, 0, c->stream, xs, n, p, stencil, ys, ns,
-                                   codes, pre, d_counts);
+                                   codes, ed, pre, d_counts);
         });
+        if (codes) {
+            HIPC(hipGetLastError());
+            KernelTimer kt_(c, "k_code_edges");
+            hipLaunchKernelGGL(k_code_edges, dim3((unsigned)((tiles + kBlock - 1) / kBlock)), block, 0, c->stream, pre,
+                               tiles, ed, reinterpret_cast<uint32_t*>(codes));
+        }
         HIPC(hipGetLastError());
     });
 }

@@ -98,8 +98,9 @@ struct PredArgs {
 };
 // mm_yes / mm_no (integer types, optional): the order keys of each output's
 // minimum and maximum, computed by the counting pass.
-// codes (dictionary split): the 2-bit codes packed 16 to a little-endian
-// 32-bit word, ceil(n / 16) words zeroed beforehand.
+// codes (dictionary split, 4-byte aligned): the 2-bit codes packed 16 to a
+// little-endian 32-bit word; every word holding a code is written (room for
+// ceil(n / 16) words).
 void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& p, uint8_t* stencil /* 1+ceil(n/8) */,
                   void* yes, void* no, uint8_t* codes /* packed dict codes or null */, uint64_t* d_counts,
                   MinMaxResult* mm_yes = nullptr, MinMaxResult* mm_no = nullptr);

@@ -229,6 +229,19 @@ def test_codec_parity_many_tiles(ctx, enc, dtype, family):
     check(ctx, enc, dtype, x)
 
 
+@pytest.mark.parametrize("every", [700, 1500, 5000, 20000])
+def test_dict_codes_shared_between_tiles(ctx, every):
+    """Dictionary rows so sparse that a 16-code word spans several 4096-row
+    tiles (and some tiles hold none): the words those tiles share are
+    assembled from each tile's part."""
+    r = np.random.default_rng(every)
+    n = 3_000_000 + every
+    x = r.integers(0, 1 << 30, n, dtype=np.int64).astype(np.int32)
+    keys = np.array([-5, -6, -7, -8], np.int32)
+    x[::every] = keys[np.arange(len(x[::every])) % 4]
+    check(ctx, g.DICT, g.D_INT, x)
+
+
 def test_patch_afl_tree_many_tiles(ctx):
     """Both FL streams of a patch tree start at odd blob offsets (the stencil
     metadata has an odd length) and span many tiles: they are packed in
