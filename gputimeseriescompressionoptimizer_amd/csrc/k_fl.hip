@@ -18,6 +18,8 @@
 // 16-byte chunks that hold it (an aligned chunk containing a valid byte never
 // crosses a page, so the stream may start at any byte address) and every
 // thread extracts values with a 64/96-bit funnel shift.
+#include <type_traits>
+
 #include "dispatch.h"
 #include "k_common.h"
 #include "kernels.h"
@@ -36,9 +38,12 @@ __global__ __launch_bounds__(kBlock) void k_fl_pack(const T* __restrict__ x, siz
     fl_pack_tile<T, DELTA>(x, n, b, out, out_bytes, blockIdx.x, vals);
 }
 
-// Staged words of one tile: at most kFlTile * 64 bits, plus the bytes before
-// the stream start in its first 16-byte chunk and one zero chunk after it.
-constexpr int kMaxWords = kFlTile * 64 / 32 + 16;
+// Staged words of one tile at b <= MAXB bits: at most kFlTile * MAXB bits,
+// plus the bytes before the stream start in its first 16-byte chunk and one
+// zero chunk after it.  The unpack kernels come in MAXB = 16, 32, 64, so that
+// narrow streams stage in less LDS and more workgroups fit on a CU.
+template <int MAXB> constexpr int fl_stage_words() { return kFlTile * MAXB / 32 + 16; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 // Bytes [lo, hi) of a 32-bit word kept, the others cleared.
 __device__ __forceinline__ uint32_t byte_mask(int lo, int hi) {
@@ -66,7 +71,7 @@ __device__ __forceinline__ int stage_fl_words(const uint8_t* __restrict__ in, si
     const int a = (int)(reinterpret_cast<uintptr_t>(base) & 15);
     const u32x4* wp4 = reinterpret_cast<const u32x4*>(base - a);
     const int n4_valid = (a + avail + 15) / 16;
-    const int n4_fill = n4_valid + 1;  // <= kMaxWords / 4
+    const int n4_fill = n4_valid + 1;  // <= fl_stage_words<MAXB>() / 4 for b <= MAXB
     u32x4* w4 = reinterpret_cast<u32x4*>(w);
     constexpr int U = 4;
     for (int i0 = threadIdx.x; i0 < n4_fill; i0 += U * kBlock) {
@@ -102,15 +107,16 @@ __device__ __forceinline__ uint64_t fl_value(const uint32_t* w, int k, int b, in
     return val & mask;
 }
 
-template <typename T>
+template <typename T, int MAXB>
 __global__ __launch_bounds__(kBlock) void k_fl_unpack(const uint8_t* __restrict__ in, size_t in_bytes, size_t n,
                                                       int b, T* __restrict__ out) {
-    __shared__ __attribute__((aligned(16))) uint32_t w[kMaxWords];
+    __shared__ __attribute__((aligned(16))) uint32_t w[fl_stage_words<MAXB>()];
     const size_t v0 = (size_t)blockIdx.x * kFlTile;
     const int nv = (int)((n - v0) < (size_t)kFlTile ? (n - v0) : (size_t)kFlTile);
     const int a = stage_fl_words(in, in_bytes, v0, nv, b, w);
     __syncthreads();
     const uint64_t mask = lowmask64(b);
+    // consecutive lanes take consecutive values: their LDS reads spread over the banks
     for (int k = threadIdx.x; k < nv; k += kBlock) out[v0 + k] = (T)fl_value(w, k, b, a, mask);
 }
 
@@ -121,9 +127,10 @@ __global__ __launch_bounds__(kBlock) void k_fl_unpack(const uint8_t* __restrict_
 // tile again and prefix-sums it (wrapping in T) on top of the first row and
 // its prefix.  The packed stream is b/8 bytes a row, so reading it twice
 // costs far less than writing and re-reading the deltas.  out has m + 1 rows.
+template <int MAXB>
 __global__ __launch_bounds__(kBlock) void k_fl_delta_sums(const uint8_t* __restrict__ in, size_t in_bytes, size_t m,
                                                           int b, uint64_t* __restrict__ vals) {
-    __shared__ __attribute__((aligned(16))) uint32_t w[kMaxWords];
+    __shared__ __attribute__((aligned(16))) uint32_t w[fl_stage_words<MAXB>()];
     __shared__ uint64_t red[kBlock / kWave];
     const size_t v0 = (size_t)blockIdx.x * kFlTile;
     const int nv = (int)((m - v0) < (size_t)kFlTile ? (m - v0) : (size_t)kFlTile);
@@ -137,11 +144,12 @@ __global__ __launch_bounds__(kBlock) void k_fl_delta_sums(const uint8_t* __restr
     if (threadIdx.x == 0) vals[blockIdx.x] = total;
 }
 
-template <typename T>
+template <typename T, int MAXB>
 __global__ __launch_bounds__(kBlock) void k_fl_unpack_delta_scan(const uint8_t* first_bytes, const uint8_t* __restrict__ in,
                                                                  size_t in_bytes, size_t m, int b, T* __restrict__ out,
                                                                  const uint64_t* __restrict__ prefix) {
-    __shared__ __attribute__((aligned(16))) uint32_t w[kMaxWords];  // the staged words, then the tile's results
+    // the staged words, then the tile's results
+    __shared__ __attribute__((aligned(16))) uint32_t w[cmax(fl_stage_words<MAXB>(), kFlTile * (int)sizeof(T) / 4)];
     __shared__ uint64_t red[kBlock / kWave];
     constexpr int kPer = kFlTile / kBlock;  // 16 consecutive values per thread
     T first;
@@ -173,6 +181,19 @@ __global__ __launch_bounds__(kBlock) void k_fl_unpack_delta_scan(const uint8_t* 
     if (tile == 0 && threadIdx.x == 0) out[0] = first;
 }
 
+}  // namespace
+
+namespace {
+// Calls f(std::integral_constant<int, MAXB>) with the narrowest staging class
+// (16, 32 or 64 bits) that holds b.
+template <typename F> void with_maxb(int b, F&& f) {
+    if (b <= 16)
+        f(std::integral_constant<int, 16>{});
+    else if (b <= 32)
+        f(std::integral_constant<int, 32>{});
+    else
+        f(std::integral_constant<int, 64>{});
+}
 }  // namespace
 
 void launch_fl_pack(Ctx* c, int dtype, const void* x, size_t n, const MinMaxResult* mm, int bits, void* out,
@@ -221,12 +242,15 @@ void launch_fl_unpack_delta(Ctx* c, int elem_size, const uint8_t* first, const u
         const size_t tiles = (m + kFlTile - 1) / kFlTile;
         TileScan ts(c, tiles);
         kernel_bytes(c, "k_fl_delta_sums", (double)((m * (uint64_t)bits + 7) / 8));
-        { KernelTimer kt_(c, "k_fl_delta_sums"); hipLaunchKernelGGL(k_fl_delta_sums, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                           in, in_bytes, m, bits, ts.vals); }
-        HIPC(hipGetLastError());
-        ts.scan(c);
-        { KernelTimer kt_(c, "k_fl_unpack_delta_scan"); hipLaunchKernelGGL(k_fl_unpack_delta_scan<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                           first, in, in_bytes, m, bits, static_cast<T*>(out), (const uint64_t*)ts.prefix); }
+        with_maxb(bits, [&](auto mb) {
+            constexpr int MAXB = decltype(mb)::value;
+            { KernelTimer kt_(c, "k_fl_delta_sums"); hipLaunchKernelGGL(k_fl_delta_sums<MAXB>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                               in, in_bytes, m, bits, ts.vals); }
+            HIPC(hipGetLastError());
+            ts.scan(c);
+            { KernelTimer kt_(c, "k_fl_unpack_delta_scan"); hipLaunchKernelGGL((k_fl_unpack_delta_scan<T, MAXB>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                               first, in, in_bytes, m, bits, static_cast<T*>(out), (const uint64_t*)ts.prefix); }
+        });
         HIPC(hipGetLastError());
     });
 }
@@ -238,8 +262,12 @@ void launch_fl_unpack(Ctx* c, int dtype, const uint8_t* in, size_t in_bytes, siz
         using T = decltype(tag);
         if (bits <= 0 || bits > 8 * (int)sizeof(T)) fail(DDJ_ECORRUPT, "FL bit width out of range");
         const size_t tiles = (n + kFlTile - 1) / kFlTile;
-        { KernelTimer kt_(c, "k_fl_unpack"); hipLaunchKernelGGL(k_fl_unpack<T>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, in, in_bytes, n, bits,
-                           static_cast<T*>(out)); }
+        with_maxb(bits, [&](auto mb) {
+            constexpr int MAXB = decltype(mb)::value;
+            KernelTimer kt_(c, "k_fl_unpack");
+            hipLaunchKernelGGL((k_fl_unpack<T, MAXB>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, in, in_bytes, n,
+                               bits, static_cast<T*>(out));
+        });
         HIPC(hipGetLastError());
     });
 }

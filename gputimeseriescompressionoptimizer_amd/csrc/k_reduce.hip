@@ -1,9 +1,10 @@
 // Single-pass reductions: min/max (FL bit width, scale, patch thresholds)
 // and the fused statistics pass the planner reads.
 //
-// One launch each: every workgroup reduces a grid-stride share with 16-byte
-// loads, writes a partial, and the last workgroup to finish (ticket on a
-// control counter that it resets) folds the partials.
+// Every workgroup reduces a grid-stride share with 16-byte loads and writes a
+// partial.  Min/max partials are folded by a one-workgroup launch
+// (k_minmax_fold); the statistics pass folds its own in the last workgroup
+// to finish (ticket on a control counter that it resets).
 #include <limits>
 
 #include "dispatch.h"
@@ -110,10 +111,8 @@ template <typename T> __device__ __forceinline__ void fold_fixup(const T* x, uin
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size_t n, uint64_t* __restrict__ partials,
-                                                   uint32_t* counter, MinMaxResult* out) {
+__global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size_t n, uint64_t* __restrict__ partials) {
     __shared__ uint64_t sm[kBlock / kWave];
-    __shared__ bool last;
     FoldKeys<T> f;
     const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x, stride = (size_t)gridDim.x * kBlock;
     constexpr int V = 16 / sizeof(T);
@@ -150,35 +149,14 @@ __global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size
         st_agent(partials + 3 * blockIdx.x + 1, mx);
         st_agent(partials + 3 * blockIdx.x + 2, zf);
     }
-    if (!last_block_arrives(counter, &last)) return;
-    mn = ~0ull;
-    mx = 0;
-    zf = ~0ull;
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
-        const uint64_t a = rd(partials + 3 * b), z = rd(partials + 3 * b + 1), q = rd(partials + 3 * b + 2);
-        mn = a < mn ? a : mn;
-        mx = z > mx ? z : mx;
-        zf = q < zf ? q : zf;
-    }
-    mn = block_reduce_min(mn, sm);
-    mx = block_reduce_max(mx, sm);
-    zf = block_reduce_min(zf, sm);
-    if (threadIdx.x == 0) {
-        fold_fixup(x, mn, mx, zf);
-        out->min_key = mn;
-        out->max_key = mx;
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
 // Min/max of the wrapping differences d[i] = x[i+1] - x[i], i < n - 1, read
 // straight from x (the delta encoder's output is never written when an FL
 // node consumes it).  Integer types only.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_minmax_delta(const T* __restrict__ x, size_t n, uint64_t* __restrict__ partials,
-                                                         uint32_t* counter, MinMaxResult* out) {
+__global__ __launch_bounds__(kBlock) void k_minmax_delta(const T* __restrict__ x, size_t n, uint64_t* __restrict__ partials) {
     __shared__ uint64_t sm[kBlock / kWave];
-    __shared__ bool last;
     using U = typename UnsignedOf<T>::type;
     T dmn = std::numeric_limits<T>::max(), dmx = std::numeric_limits<T>::min();
     bool any = false;
@@ -240,20 +218,33 @@ __global__ __launch_bounds__(kBlock) void k_minmax_delta(const T* __restrict__ x
         st_agent(partials + 2 * blockIdx.x, mn);
         st_agent(partials + 2 * blockIdx.x + 1, mx);
     }
-    if (!last_block_arrives(counter, &last)) return;
-    mn = ~0ull;
-    mx = 0;
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += kBlock) {
-        const uint64_t a = rd(partials + 2 * b), z = rd(partials + 2 * b + 1);
+}
+
+// Folds the per-workgroup partials of k_minmax (STRIDE 3: min, max, first
+// zero) or k_minmax_delta (STRIDE 2) into the result, in a launch of its own:
+// a "last workgroup" ticket would take one device-scope atomic per
+// workgroup on a single address, which costs more than this launch.
+template <typename T, int STRIDE>
+__global__ __launch_bounds__(kBlock) void k_minmax_fold(const uint64_t* __restrict__ partials, int blocks, const T* x,
+                                                        MinMaxResult* out) {
+    __shared__ uint64_t sm[kBlock / kWave];
+    uint64_t mn = ~0ull, mx = 0, zf = ~0ull;
+    for (int b = threadIdx.x; b < blocks; b += kBlock) {
+        const uint64_t a = partials[STRIDE * b], z = partials[STRIDE * b + 1];
         mn = a < mn ? a : mn;
         mx = z > mx ? z : mx;
+        if (STRIDE == 3) {
+            const uint64_t q = partials[STRIDE * b + 2];
+            zf = q < zf ? q : zf;
+        }
     }
     mn = block_reduce_min(mn, sm);
     mx = block_reduce_max(mx, sm);
+    if (STRIDE == 3) zf = block_reduce_min(zf, sm);
     if (threadIdx.x == 0) {
+        if (STRIDE == 3) fold_fixup(x, mn, mx, zf);
         out->min_key = mn;
         out->max_key = mx;
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -484,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const T* __restrict__ x, size_
 int reduce_grid(Ctx* c, size_t n, int elem) {
     const size_t per_block = (size_t)kBlock * (16 / elem) * 4;
     const size_t g = (n + per_block - 1) / per_block;
-    const size_t cap = std::min<size_t>(kMaxReduceBlocks, (size_t)c->cu_count * 8);
+    const size_t cap = std::min<size_t>(kMaxReduceBlocks, (size_t)c->cu_count * 4);
     return (int)std::max<size_t>(1, std::min(g, cap));
 }
 
@@ -497,7 +488,10 @@ void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_r
         const int grid = reduce_grid(c, n, sizeof(T));
         DevBuf partials(c, 24 * (size_t)grid);
         { KernelTimer kt_(c, "k_minmax"); hipLaunchKernelGGL(k_minmax<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
-                           partials.get<uint64_t>(), &c->control->counters[0], d_result); }
+                           partials.get<uint64_t>()); }
+        HIPC(hipGetLastError());
+        { KernelTimer kt_(c, "k_minmax_fold"); hipLaunchKernelGGL((k_minmax_fold<T, 3>), dim3(1), dim3(kBlock), 0, c->stream,
+                           partials.get<uint64_t>(), grid, static_cast<const T*>(x), d_result); }
         HIPC(hipGetLastError());
     });
 }
@@ -509,7 +503,10 @@ void launch_minmax_delta(Ctx* c, int elem_size, const void* x, size_t n, MinMaxR
         const int grid = reduce_grid(c, n, sizeof(T));
         DevBuf partials(c, 16 * (size_t)grid);
         { KernelTimer kt_(c, "k_minmax"); hipLaunchKernelGGL(k_minmax_delta<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
-                           partials.get<uint64_t>(), &c->control->counters[0], d_result); }
+                           partials.get<uint64_t>()); }
+        HIPC(hipGetLastError());
+        { KernelTimer kt_(c, "k_minmax_fold"); hipLaunchKernelGGL((k_minmax_fold<T, 2>), dim3(1), dim3(kBlock), 0, c->stream,
+                           partials.get<uint64_t>(), grid, static_cast<const T*>(x), d_result); }
         HIPC(hipGetLastError());
     });
 }

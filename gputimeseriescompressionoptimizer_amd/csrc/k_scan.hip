@@ -148,6 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     const T padv = x[n - 1];  // stencil padding repeats the last row (reference stencil.cu:26-27)
+    const uint64_t Y0 = prefix[tile];  // loaded with the tile
     if (CODES)
         for (int q = threadIdx.x; q < kCodeWords; q += kBlock) cwl[q] = 0;
     if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
@@ -185,7 +186,6 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     if (tile == 0 && threadIdx.x == 0) stencil[0] = (uint8_t)(n % 8);
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)c, &total, red);
-    const uint64_t Y0 = prefix[tile];
     // codes j in [Y0, Y0 + total) go to field j % 16 of 2-bit word j / 16
     // (_compressUniqueKernel, unique_encoding.cu:48-76): tile-local code i
     // lands in cwl[(lead + i) / 16]
@@ -482,17 +482,25 @@ __global__ __launch_bounds__(kBlock) void k_rle_count(const T* __restrict__ x, s
     uint32_t c = 0;
     if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
         constexpr int V = 16 / sizeof(T);
+        constexpr int Q = kTile / V / kBlock;
         const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
+        const bool lane0 = (threadIdx.x & (kWave - 1)) == 0;
+        u32x4 w[Q];
+        T before[Q];  // lane 0: the row before its chunk, loaded with the chunks
 #pragma unroll
-        for (int q = 0; q < kTile / V / kBlock; q++) {
-            const int i = threadIdx.x + kBlock * q;
-            const u32x4 w = __builtin_nontemporal_load(s4 + i);
+        for (int q = 0; q < Q; q++) {
+            const size_t g = base + (size_t)V * (threadIdx.x + kBlock * q);
+            w[q] = __builtin_nontemporal_load(s4 + threadIdx.x + kBlock * q);
+            if (lane0 && g > 0) before[q] = x[g - 1];
+        }
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const size_t g = base + (size_t)V * (threadIdx.x + kBlock * q);
             T v[V];
-            __builtin_memcpy(v, &w, 16);
-            const size_t g = base + (size_t)V * i;
+            __builtin_memcpy(v, &w[q], 16);
             // the row before this chunk ends the previous lane's chunk
             T prev = shfl_up_value(v[V - 1]);
-            if ((threadIdx.x & (kWave - 1)) == 0) prev = g > 0 ? x[g - 1] : v[0];
+            if (lane0) prev = g > 0 ? before[q] : v[0];
             c += (g == 0 || !(v[0] == prev)) ? 1u : 0u;
 #pragma unroll
             for (int k = 1; k < V; k++) c += !(v[k] == v[k - 1]) ? 1u : 0u;
@@ -517,6 +525,7 @@ __global__ __launch_bounds__(kBlock) void k_rle_encode(const T* __restrict__ x, 
     const uint32_t tile = blockIdx.x;
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
+    const uint64_t s_prefix = prefix[tile];  // loaded with the tile
     if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
         // buf[i + 1] = x[base + i]; buf[0] = the row before the tile
         constexpr int V = 16 / sizeof(T);
@@ -547,7 +556,6 @@ __global__ __launch_bounds__(kBlock) void k_rle_encode(const T* __restrict__ x, 
     }
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)__popc(mask), &total, red);
-    const uint64_t s_prefix = prefix[tile];
     uint64_t r = s_prefix + excl;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
@@ -639,46 +647,50 @@ __global__ __launch_bounds__(kBlock) void k_scan_lengths(const uint8_t* __restri
 // Run expansion: a workgroup owns 4096 output rows, finds the first run
 // overlapping them by binary search, stages the overlapping runs in LDS and
 // gives every row its run by a binary search in LDS.
-constexpr int kExpTile = 2048;
+// Expansion tiles: TILE output rows per workgroup; LDS holds up to kExpRuns
+// runs of a tile.
+constexpr int kExpRuns = 2048;
 
-// tile_run[t] = the run holding row t * kExpTile (offsets are non-decreasing:
+// tile_run[t] = the run holding row t * TILE (offsets are non-decreasing:
 // lengths were checked non-negative by the scan); tile_run[tiles] = R.
+template <int TILE>
 __global__ __launch_bounds__(kBlock) void k_rle_tile_runs(const uint64_t* __restrict__ offsets, size_t R, size_t tiles,
                                                           uint32_t* __restrict__ tile_run) {
     const size_t stride = (size_t)gridDim.x * kBlock;
     for (size_t r = (size_t)blockIdx.x * kBlock + threadIdx.x; r < R; r += stride) {
         const uint64_t s = offsets[r], e = offsets[r + 1];
-        const uint64_t t0 = (s + kExpTile - 1) / kExpTile, t1 = (e + kExpTile - 1) / kExpTile;  // tiles starting in [s, e)
+        const uint64_t t0 = (s + TILE - 1) / TILE, t1 = (e + TILE - 1) / TILE;  // tiles starting in [s, e)
         for (uint64_t t = t0; t < t1 && t < tiles; t++) tile_run[t] = (uint32_t)r;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) tile_run[tiles] = (uint32_t)R;
 }
 
-// Run expansion: a workgroup owns kExpTile output rows; its runs are
+// Run expansion: a workgroup owns TILE output rows; its runs are
 // tile_run[b] .. tile_run[b + 1] (the run holding the next tile's first row).
 // They are staged in LDS and every row finds its run by binary search there;
-// a tile overlapping more runs than LDS holds (only possible with empty runs)
-// searches the offsets in global memory instead.
-template <typename T>
+// a tile overlapping more runs than LDS holds (empty runs, or short runs in
+// a tile of more than kExpRuns rows) searches the offsets in global memory
+// instead.
+template <typename T, int TILE>
 __global__ __launch_bounds__(kBlock) void k_rle_expand(const uint64_t* __restrict__ offsets, size_t R,
                                                        const uint8_t* __restrict__ values, size_t total,
                                                        const uint32_t* __restrict__ tile_run, T* __restrict__ out) {
-    __shared__ uint64_t soff[kExpTile + 2];
-    __shared__ T sval[kExpTile + 1];
-    const size_t o0 = (size_t)blockIdx.x * kExpTile;
-    const int cnt = (int)((total - o0) < (size_t)kExpTile ? (total - o0) : (size_t)kExpTile);
+    __shared__ uint64_t soff[kExpRuns + 2];
+    __shared__ T sval[kExpRuns + 1];
+    const size_t o0 = (size_t)blockIdx.x * TILE;
+    const int cnt = (int)((total - o0) < (size_t)TILE ? (total - o0) : (size_t)TILE);
     const size_t r0 = tile_run[blockIdx.x];
     size_t r1 = tile_run[blockIdx.x + 1];  // last run needed (inclusive)
     r1 = r1 < R ? r1 : R - 1;
     r1 = r1 < r0 ? r0 : r1;
     const size_t nr = r1 - r0 + 1;
-    if (nr <= (size_t)kExpTile + 1) {
+    if (nr <= (size_t)kExpRuns + 1) {
         for (int i = threadIdx.x; i <= (int)nr; i += kBlock) soff[i] = offsets[r0 + i];
         for (int i = threadIdx.x; i < (int)nr; i += kBlock) __builtin_memcpy(&sval[i], values + sizeof(T) * (r0 + i), sizeof(T));
         __syncthreads();
-        // thread t writes rows [8t, 8t + 8) of the tile: one binary search for
-        // the first, then the runs advance monotonically
-        constexpr int R = kExpTile / kBlock;
+        // thread t writes rows [R t, R t + R) of the tile: one binary search
+        // for the first, then the runs advance monotonically
+        constexpr int R = TILE / kBlock;
         const int k0 = threadIdx.x * R;
         if (k0 < cnt) {
             const uint64_t pos0 = o0 + k0;
@@ -697,7 +709,7 @@ __global__ __launch_bounds__(kBlock) void k_rle_expand(const uint64_t* __restric
                 v[k] = sval[lo];
             }
             if (k0 + R <= cnt && (R * sizeof(T)) % 16 == 0) {
-                u32x4* d4 = reinterpret_cast<u32x4*>(out + pos0);  // 16-byte aligned: tiles of 2048 rows
+                u32x4* d4 = reinterpret_cast<u32x4*>(out + pos0);  // 16-byte aligned: R * sizeof(T) % 16 == 0
 #pragma unroll
                 for (int q = 0; q < (int)(R * sizeof(T) / 16); q++) {
                     u32x4 w;
@@ -930,14 +942,20 @@ DevBuf rle_decode_dev(Ctx* c, int dtype, const uint8_t* lengths, const uint8_t* 
         d2h(c, &total, offsets.get<uint64_t>() + runs, 8);
         DevBuf out(c, total * sizeof(T));
         if (total == 0) return out;
-        const size_t etiles = (total + kExpTile - 1) / kExpTile;
-        DevBuf tile_run(c, 4 * (etiles + 1));
-        { KernelTimer kt_(c, "k_rle_tile_runs"); hipLaunchKernelGGL(k_rle_tile_runs, dim3(grid_for(c, runs)), dim3(kBlock), 0, c->stream,
-                           offsets.get<uint64_t>(), runs, etiles, tile_run.get<uint32_t>()); }
         kernel_bytes(c, "k_rle_expand", (double)total * sizeof(T) + (double)runs * (8 + sizeof(T)));
-        { KernelTimer kt_(c, "k_rle_expand"); hipLaunchKernelGGL(k_rle_expand<T>, dim3((unsigned)etiles), dim3(kBlock), 0, c->stream,
-                           offsets.get<uint64_t>(), runs, values, (size_t)total, tile_run.get<uint32_t>(), out.get<T>()); }
-        HIPC(hipGetLastError());
+        auto expand = [&](auto tile_c) {
+            constexpr int TILE = decltype(tile_c)::value;
+            const size_t etiles = (total + TILE - 1) / TILE;
+            DevBuf tile_run(c, 4 * (etiles + 1));
+            { KernelTimer kt_(c, "k_rle_tile_runs"); hipLaunchKernelGGL(k_rle_tile_runs<TILE>, dim3(grid_for(c, runs)), dim3(kBlock), 0, c->stream,
+                               offsets.get<uint64_t>(), runs, etiles, tile_run.get<uint32_t>()); }
+            { KernelTimer kt_(c, "k_rle_expand"); hipLaunchKernelGGL((k_rle_expand<T, TILE>), dim3((unsigned)etiles), dim3(kBlock), 0, c->stream,
+                               offsets.get<uint64_t>(), runs, values, (size_t)total, tile_run.get<uint32_t>(), out.get<T>()); }
+            HIPC(hipGetLastError());
+        };
+        // 2048-row tiles: bigger ones (fewer, longer-lived workgroups, each
+        // row walking further) measured slower even on long runs
+        expand(std::integral_constant<int, 2048>{});
         return out;
     });
 }

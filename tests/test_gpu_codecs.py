@@ -242,6 +242,16 @@ def test_dict_codes_shared_between_tiles(ctx, every):
     check(ctx, g.DICT, g.D_INT, x)
 
 
+def test_rle_long_runs_with_a_stretch_of_short_ones(ctx):
+    """Long runs around a stretch of single-row runs: expansion tiles see
+    from one run to as many runs as rows."""
+    r = np.random.default_rng(3)
+    long_runs = np.repeat(r.integers(0, 1000, 8000), r.integers(50, 150, 8000)).astype(np.int32)
+    short = np.arange(5000, dtype=np.int32) + 2000
+    x = np.concatenate([long_runs[:400_000], short, long_runs[400_000:]])
+    check(ctx, g.RLE, g.D_INT, x)
+
+
 def test_patch_afl_tree_many_tiles(ctx):
     """Both FL streams of a patch tree start at odd blob offsets (the stencil
     metadata has an odd length) and span many tiles: they are packed in
