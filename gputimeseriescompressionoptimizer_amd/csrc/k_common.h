@@ -115,15 +115,35 @@ template <bool MAX> __device__ __forceinline__ int32_t wave_extreme_i32(int32_t 
     return op(op(r0, r1), op(r2, r3));
 }
 
-// Inclusive prefix sum over the 64 lanes of a wave.
-template <typename T> __device__ __forceinline__ T wave_inclusive_scan(T v) {
-    const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        T w = __shfl_up(v, o, kWave);
-        if (lane >= o) v += w;
+// v shifted across lanes by a DPP control (lanes the control or the row /
+// bank masks leave without a source read 0), for 32- or 64-bit integers.
+template <int CTRL, int ROWM, int BANKM, typename T> __device__ __forceinline__ T dpp_shift(T v) {
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = (uint64_t)v;
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)u, CTRL, ROWM, BANKM, false);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(u >> 32), CTRL, ROWM, BANKM, false);
+        return (T)(((uint64_t)hi << 32) | lo);
+    } else {
+        return (T)(uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, CTRL, ROWM, BANKM, false);
     }
-    return v;
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave (32- or 64-bit
+// integers), by DPP: inside each row of 16 lanes, the sum of the last four
+// lanes (row_shr 1, 2, 3 of the input), then of the last eight (row_shr 4
+// into banks 1-3) and sixteen (row_shr 8 into banks 2-3); across rows,
+// lane 15 into row 1 and lane 47 into row 3 (row_bcast 15), then lane 31
+// into rows 2 and 3 (row_bcast 31).
+template <typename T> __device__ __forceinline__ T wave_inclusive_scan(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    T x = v + dpp_shift<0x111, 0xF, 0xF>(v);  // row_shr:1
+    x += dpp_shift<0x112, 0xF, 0xF>(v);       // row_shr:2
+    x += dpp_shift<0x113, 0xF, 0xF>(v);       // row_shr:3
+    x += dpp_shift<0x114, 0xF, 0xE>(x);       // row_shr:4, banks 1-3
+    x += dpp_shift<0x118, 0xF, 0xC>(x);       // row_shr:8, banks 2-3
+    x += dpp_shift<0x142, 0xA, 0xF>(x);       // row_bcast:15, rows 1 and 3
+    x += dpp_shift<0x143, 0xC, 0xF>(x);       // row_bcast:31, rows 2 and 3
+    return x;
 }
 
 // Block-wide exclusive scan of one value per thread (NT threads).
