@@ -254,6 +254,22 @@ def learned_planner_section(ctx, blocks: int, rows: int, regime: int, exhaustive
     return ev
 
 
+def copy_calibration():
+    import numpy as np
+    import gputimeseriescompressionoptimizer_amd as g
+
+    with g.Context(0) as ctx:
+        x = np.arange(1 << 26, dtype=np.int32)
+        buf = ctx.upload(x)
+        tree = g.CompressionTree(ctx, [g.NONE], g.D_INT)
+        for _ in range(5):
+            blob = tree.Compress(buf)
+            back = tree.Decompress(blob)
+            ctx.sync()
+        assert blob.nbytes == x.nbytes + 16 and back.nbytes == x.nbytes
+        print(json.dumps({"copy_calibration": {"column_bytes": x.nbytes, "blob_bytes": blob.nbytes, "passes": 5}}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,11 +289,21 @@ def main():
     ap.add_argument("--no-float", action="store_true")
     ap.add_argument("--compress-only", action="store_true",
                     help="warmup and timed compress steps only (for hardware-counter runs)")
+    ap.add_argument("--no-concurrent", action="store_true",
+                    help="skip the four columns through the batch API (for kernel traces of the steps alone)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run plain copies of known size (for scaling hardware counters): a [none] "
+                         "tree over a 64M-row int32 column, 5 times; its blob assembly is one "
+                         "k_gather_copy reading the 256 MiB column (16-byte aligned), and decompressing "
+                         "the blob copies its 256 MiB of data into a fresh (aligned) column")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (columns would shard "
               "across GPUs with a host-side scan of blob sizes)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     import numpy as np
     import gputimeseriescompressionoptimizer_amd as g
@@ -329,15 +355,20 @@ def main():
 
     # the same job with the four columns overlapped (ddj_batch_compress,
     # worker streams, autotuned): blobs must be identical
-    bcols = [(buf, g.D_INT, path) for _, buf, path, _ in dev]
-    bcfg, _ = g.batch_autotune(ctx, bcols)
-    same_concurrent = all(a.to_bytes() == b.to_bytes() for a, b in zip(g.batch_compress(ctx, bcols, config=bcfg), blobs))
+    concurrent = None
+    if not args.no_concurrent:
+        bcols = [(buf, g.D_INT, path) for _, buf, path, _ in dev]
+        bcfg, _ = g.batch_autotune(ctx, bcols)
+        same_concurrent = all(a.to_bytes() == b.to_bytes()
+                              for a, b in zip(g.batch_compress(ctx, bcols, config=bcfg), blobs))
 
-    def conc_step():
-        holder["cblobs"] = g.batch_compress(ctx, bcols, config=bcfg)
+        def conc_step():
+            holder["cblobs"] = g.batch_compress(ctx, bcols, config=bcfg)
 
-    t_conc, _ = timed(conc_step, False)
-    holder.pop("cblobs", None)
+        t_conc, _ = timed(conc_step, False)
+        holder.pop("cblobs", None)
+        concurrent = {"compress_gbps": round(in_bytes / t_conc / 1e9, 3), "ms_per_step": round(t_conc * 1e3, 3),
+                      "config": bcfg, "blobs_identical": same_concurrent}
 
     # decompression of the same blobs, verified once
     outs = decompress_all(blobs)
@@ -395,8 +426,7 @@ def main():
         "ratios": ratios,
         "roofline": roofline,
         "profiled_ms_per_step": {"compress": round(t_comp_prof * 1e3, 3), "decompress": round(t_dec_prof * 1e3, 3)},
-        "concurrent_columns": {"compress_gbps": round(in_bytes / t_conc / 1e9, 3), "ms_per_step": round(t_conc * 1e3, 3),
-                               "config": bcfg, "blobs_identical": same_concurrent},
+        "concurrent_columns": concurrent,
         "kernels": kernels,
         "decompress_kernels": dkernels,
     }

@@ -107,7 +107,8 @@ struct Small4 {
 // Four smallest keys of the workgroup (each thread holds a Small4) into
 // out_keys[0..*out_n).  Wave rounds take the smallest head, then thread 0
 // merges the waves' lists.
-__device__ void block_small4(Small4& s, uint64_t* sm_keys /* kBlock/kWave * kMaxK */, int* sm_cnt /* kBlock/kWave */,
+template <int NT>
+__device__ void block_small4(Small4& s, uint64_t* sm_keys /* NT/kWave * kMaxK */, int* sm_cnt /* NT/kWave */,
                              uint64_t* out_keys, uint32_t* out_n) {
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     int got = 0;
@@ -125,7 +126,7 @@ __device__ void block_small4(Small4& s, uint64_t* sm_keys /* kBlock/kWave * kMax
     __syncthreads();
     if (threadIdx.x == 0) {
         Small4 b;
-        for (int w = 0; w < kBlock / kWave; w++)
+        for (int w = 0; w < NT / kWave; w++)
             for (int r = 0; r < sm_cnt[w]; r++) b.insert(sm_keys[w * kMaxK + r]);
 #pragma unroll
         for (int r = 0; r < kMaxK; r++)
@@ -142,20 +143,24 @@ __device__ __forceinline__ uint32_t lds_slot(uint64_t key) {
 }
 static_assert(kLdsSlots == 1 << 11, "lds_slot yields 11 bits");
 
-// Tiles of 4096 rows, each thread taking 16 rows with 16-byte loads issued
-// together; workgroups stride over the tiles.
+// Workgroups of 1024 threads, one per CU, stride over tiles of 16384 rows,
+// each thread taking 16 rows with 16-byte loads issued together.  Few, big
+// workgroups keep the final flush short: every workgroup adds each of its
+// distinct keys to the global table with a device-scope atomic, and those
+// serialize per key.
+constexpr int kHistThreads = 1024;
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, size_t n, unsigned long long* keys,
+__global__ __launch_bounds__(kHistThreads) void k_hist_count(const T* __restrict__ x, size_t n, unsigned long long* keys,
                                                        unsigned long long* counts, uint64_t cap_mask, HistCtl* ctl,
                                                        uint64_t* nan_keys /* [grid * 4] */,
                                                        uint32_t* nan_n /* [grid] */) {
     __shared__ unsigned long long skey[kLdsSlots];
     __shared__ uint32_t scnt[kLdsSlots];
-    __shared__ uint64_t s_nk[kBlock / kWave * kMaxK];
-    __shared__ int s_nc[kBlock / kWave];
+    __shared__ uint64_t s_nk[kHistThreads / kWave * kMaxK];
+    __shared__ int s_nc[kHistThreads / kWave];
     Small4 nan4;
     unsigned long long nans = 0;
-    for (int i = threadIdx.x; i < kLdsSlots; i += kBlock) {
+    for (int i = threadIdx.x; i < kLdsSlots; i += kHistThreads) {
         skey[i] = kEmpty;
         scnt[i] = 0;
     }
@@ -209,9 +214,9 @@ __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, 
             lds_add(key, 1u);
         }
     };
-    constexpr int kRows = 4096;
+    constexpr int kRows = 16 * kHistThreads;
     constexpr int V = 16 / sizeof(T);
-    constexpr int Q = kRows / V / kBlock;  // 16-byte loads per thread and tile
+    constexpr int Q = kRows / V / kHistThreads;  // 16-byte loads per thread and tile
     const size_t tiles = (n + kRows - 1) / kRows;
     const bool aligned = (reinterpret_cast<uintptr_t>(x) & 15) == 0;
     for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {  // uniform per workgroup
@@ -220,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, 
             const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
             u32x4 w[Q];
 #pragma unroll
-            for (int q = 0; q < Q; q++) w[q] = __builtin_nontemporal_load(s4 + threadIdx.x + kBlock * q);
+            for (int q = 0; q < Q; q++) w[q] = __builtin_nontemporal_load(s4 + threadIdx.x + kHistThreads * q);
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 T v[V];
@@ -230,13 +235,13 @@ __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, 
             }
         } else {
             const int cnt = (int)((n - base) < (size_t)kRows ? (n - base) : (size_t)kRows);
-            for (int i = threadIdx.x; i < cnt; i += kBlock) count(x[base + i]);
+            for (int i = threadIdx.x; i < cnt; i += kHistThreads) count(x[base + i]);
         }
     }
     if (rk0 != kEmpty) lds_add(rk0, rc0);
     if (rk1 != kEmpty) lds_add(rk1, rc1);
     __syncthreads();
-    for (int s = threadIdx.x; s < kLdsSlots; s += kBlock)
+    for (int s = threadIdx.x; s < kLdsSlots; s += kHistThreads)
         if (skey[s] != kEmpty) global_insert(keys, counts, cap_mask, skey[s], scnt[s], ctl);
     const uint64_t nz = wave_or((uint64_t)negzero);
     const unsigned long long sp = wave_sum(special);
@@ -247,7 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_hist_count(const T* __restrict__ x, 
         if (nn) atomicAdd(&ctl->nans, nn);
     }
     if constexpr (std::is_floating_point<T>::value)
-        block_small4(nan4, s_nk, s_nc, nan_keys + (size_t)blockIdx.x * kMaxK, nan_n + blockIdx.x);
+        block_small4<kHistThreads>(nan4, s_nk, s_nc, nan_keys + (size_t)blockIdx.x * kMaxK, nan_n + blockIdx.x);
 }
 
 struct Cand {
@@ -383,10 +388,10 @@ void topk_most_frequent(Ctx* c, int dtype, const void* x, size_t n, int K, uint6
                                (unsigned long long)kEmpty); }
             launch_fill_zero(c, counts.get(), 8 * cap);
             launch_fill_zero(c, ctl.get(), sizeof(HistCtl));
-            size_t g = (n + 4095) / 4096;  // one workgroup per 4096-row tile, at most 4 per CU
-            g = std::max<size_t>(1, std::min<size_t>(g, (size_t)c->cu_count * 4));
+            size_t g = (n + 16 * kHistThreads - 1) / (16 * kHistThreads);  // at most one workgroup per tile and per CU
+            g = std::max<size_t>(1, std::min<size_t>(g, (size_t)c->cu_count));
             DevBuf nan_keys(c, 8 * kMaxK * g), nan_n(c, 4 * g);
-            { KernelTimer kt_(c, "k_hist_count"); hipLaunchKernelGGL(k_hist_count<T>, dim3((unsigned)g), dim3(kBlock), 0, c->stream,
+            { KernelTimer kt_(c, "k_hist_count"); hipLaunchKernelGGL(k_hist_count<T>, dim3((unsigned)g), dim3(kHistThreads), 0, c->stream,
                                static_cast<const T*>(x), n, keys.get<unsigned long long>(),
                                counts.get<unsigned long long>(), cap - 1, ctl.get<HistCtl>(),
                                nan_keys.get<uint64_t>(), nan_n.get<uint32_t>()); }
