@@ -404,7 +404,7 @@ Part patch_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     const size_t es = dtype_size(dtype);
     if (in[1].n == 0 && in[2].n == 0) return empty();
     const size_t n = stencil_rows(c, in[0]);
-    Part y = aligned_view(c, in[1], es), z = aligned_view(c, in[2], es);
+    const DView y = in[1], z = in[2];  // the merge reads rows at any byte address
     if (y.n / es + z.n / es < n) fail(DDJ_ECORRUPT, "patch outputs shorter than the stencil");
     DevBuf out(c, n * es);
     launch_merge(c, dtype, in[0].p + 1, n, y.p, y.n / es, z.p, z.n / es, false, 0, out.get());
@@ -437,7 +437,7 @@ Part const_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     uint64_t cv = 0;
     read_small(c, &cv, in[0].p, es);
     const size_t n = stencil_rows(c, DView{in[0].p + es, in[0].n - es});
-    Part y = aligned_view(c, in[1], es);
+    const DView y = in[1];  // the merge reads rows at any byte address
     DevBuf out(c, n * es);
     launch_merge(c, dtype, in[0].p + es + 1, n, y.p, y.n / es, nullptr, 0, true, cv, out.get());
     return own(std::move(out));
@@ -487,7 +487,7 @@ Part dict_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     const size_t es = dtype_size(dtype);
     if (in[1].n == 0 && in[2].n == 0) return empty();
     const size_t n = stencil_rows(c, in[0]);
-    Part z = aligned_view(c, in[2], es);
+    const DView z = in[2];  // the merge reads rows at any byte address
     DevBuf out(c, n * es);
     if (in[1].n > 0) {
         if (in[1].n < 16) fail(DDJ_ECORRUPT, "dict header too short");

@@ -316,31 +316,70 @@ __device__ __forceinline__ T dict_value(const DictCodes& d, const T* sdict, uint
     return dict_key<T>(d, sdict, word, f);
 }
 
-// Rows [s0, s0 + len) of src (element-aligned) into LDS rows dst0 + e, with
-// aligned 16-byte loads (an aligned 16-byte chunk holding a valid row never
-// crosses a page); rows at or past `avail` read as zero.  Returns whether
-// the range ran past avail (corrupt input).
+// Rows [s0, s0 + len) of the T stream at src (any byte address) into LDS
+// rows dst0 + e, with aligned 16-byte loads (an aligned 16-byte chunk
+// holding a valid byte never crosses a page); rows at or past `avail` read
+// as zero.  Returns whether the range ran past avail (corrupt input).
+// A stream whose address is not a multiple of sizeof(T) (an output stored
+// unchanged in a blob, after metadata of odd length) is realigned on the
+// way: every row then starts the same r = address % sizeof(T) bytes past the
+// chunk grid's multiples of sizeof(T), and a row starting in chunk k is cut
+// from chunks k and k + 1 by byte funnel shifts.
 template <typename T>
-__device__ __forceinline__ bool stage_rows(const T* __restrict__ src, uint64_t s0, int len, uint64_t avail, T* buf,
-                                           int dst0) {
+__device__ __forceinline__ bool stage_rows(const uint8_t* __restrict__ src, uint64_t s0, int len, uint64_t avail,
+                                           T* buf, int dst0) {
+    constexpr int S = (int)sizeof(T), V = 16 / S;
     if (len <= 0) return false;
     const int ok = s0 >= avail ? 0 : (int)((avail - s0) < (uint64_t)len ? (avail - s0) : (uint64_t)len);
     for (int e = ok + threadIdx.x; e < len; e += kBlock) buf[P(dst0 + e)] = T(0);
     if (ok > 0) {
-        constexpr int V = 16 / sizeof(T);
-        const uintptr_t a = reinterpret_cast<uintptr_t>(src + s0);
-        const int sh = (int)((a & 15) / sizeof(T));  // rows before s0 in its chunk
+        const uintptr_t a = reinterpret_cast<uintptr_t>(src + s0 * S);
+        const int sh = (int)(a & 15);  // bytes before the first row in its chunk
         const u32x4* v4 = reinterpret_cast<const u32x4*>(a & ~(uintptr_t)15);
-        const int nv = (sh + ok + V - 1) / V;
-        for (int k = threadIdx.x; k < nv; k += kBlock) {
-            const u32x4 w = __builtin_nontemporal_load(v4 + k);
-            T ev[V];
-            __builtin_memcpy(ev, &w, 16);
-            const int e0 = V * k - sh;
+        const int nv = (sh + ok * S + 15) / 16;
+        if (sh % S == 0) {  // rows on the chunk grid (uniform)
+            for (int k = threadIdx.x; k < nv; k += kBlock) {
+                const u32x4 w = __builtin_nontemporal_load(v4 + k);
+                T ev[V];
+                __builtin_memcpy(ev, &w, 16);
+                const int e0 = V * k - sh / S;
 #pragma unroll
-            for (int j = 0; j < V; j++) {
-                const int e = e0 + j;
-                if (e >= 0 && e < ok) buf[P(dst0 + e)] = ev[j];
+                for (int j = 0; j < V; j++) {
+                    const int e = e0 + j;
+                    if (e >= 0 && e < ok) buf[P(dst0 + e)] = ev[j];
+                }
+            }
+        } else if constexpr (S > 1) {
+            const int r = sh % S;
+            for (int k = threadIdx.x; k < nv; k += kBlock) {
+                const u32x4 c0 = __builtin_nontemporal_load(v4 + k);
+                const u32x4 c1 = k + 1 < nv ? v4[k + 1] : u32x4{0u, 0u, 0u, 0u};
+                const uint32_t w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+                const int e0 = (16 * k + r - sh) / S;  // the first row starting in chunk k
+#pragma unroll
+                for (int j = 0; j < V; j++) {
+                    const int e = e0 + j;
+                    // row e's bytes: [r + j S, r + j S + S) of the 32-byte window
+                    T v;
+                    if constexpr (S == 8) {
+                        const uint32_t rr = (uint32_t)(r & 3);  // r >> 2 is 0 or 1 for every thread
+                        uint32_t lo, hi;
+                        if ((r >> 2) == 0) {
+                            lo = __builtin_amdgcn_alignbyte(w[2 * j + 1], w[2 * j], rr);
+                            hi = __builtin_amdgcn_alignbyte(w[2 * j + 2], w[2 * j + 1], rr);
+                        } else {
+                            lo = __builtin_amdgcn_alignbyte(w[2 * j + 2], w[2 * j + 1], rr);
+                            hi = __builtin_amdgcn_alignbyte(w[(2 * j + 3) & 7], w[2 * j + 2], rr);
+                        }
+                        const uint64_t u = ((uint64_t)hi << 32) | lo;
+                        __builtin_memcpy(&v, &u, 8);
+                    } else {  // S = 2 or 4: the row lies in words q, q + 1
+                        const int off = r + j * S;
+                        const uint32_t u = __builtin_amdgcn_alignbyte(w[(off >> 2) + 1], w[off >> 2], (uint32_t)(off & 3));
+                        __builtin_memcpy(&v, &u, S);
+                    }
+                    if (e >= 0 && e < ok) buf[P(dst0 + e)] = v;
+                }
             }
         }
     }
@@ -363,8 +402,8 @@ constexpr int kCodeLdsWords = kTile / 4 + 2 + 8;
 // tile is written back striped.  Codes of at most 8 bits are staged the
 // same way (as the bytes of their words) and looked up per row.
 template <typename T, bool CONST_NO, bool CODES>
-__global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bits, size_t n, const T* __restrict__ yes,
-                                                  size_t n_yes, DictCodes dc, const T* __restrict__ no, size_t n_no,
+__global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bits, size_t n, const uint8_t* __restrict__ yes,
+                                                  size_t n_yes, DictCodes dc, const uint8_t* __restrict__ no, size_t n_no,
                                                   T no_const, T* __restrict__ out, const uint64_t* __restrict__ prefix,
                                                   uint32_t* error) {
     __shared__ T buf[kTilePad];
@@ -400,9 +439,9 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
             for (int k = threadIdx.x; k < nchunks; k += kBlock) l4[k] = __builtin_nontemporal_load(c4 + k);
         }
     } else {
-        bad |= stage_rows(yes, Y0, total, n_yes, buf, 0);
+        bad |= stage_rows<T>(yes, Y0, total, n_yes, buf, 0);
     }
-    if (!CONST_NO) bad |= stage_rows(no, N0, cnt - total, n_no, buf, total);
+    if (!CONST_NO) bad |= stage_rows<T>(no, N0, cnt - total, n_no, buf, total);
     const size_t g0 = base + (size_t)threadIdx.x * kItems;
     uint32_t mask = 0;
     if (g0 < n) mask = bits[g0 / 8];
@@ -833,8 +872,8 @@ namespace {
 
 // Stencil counts, their scan and the merge; `dc.words` set: yes rows from codes.
 template <typename T>
-void merge_impl(Ctx* c, const uint8_t* bits, size_t n, const T* yes, size_t n_yes, const DictCodes& dc, const T* no,
-                size_t n_no, bool const_no, T no_const, T* out) {
+void merge_impl(Ctx* c, const uint8_t* bits, size_t n, const uint8_t* yes, size_t n_yes, const DictCodes& dc,
+                const uint8_t* no, size_t n_no, bool const_no, T no_const, T* out) {
     const size_t tiles = (n + kTile - 1) / kTile;
     TileScan ts(c, tiles);
     arm_error_flag(c);
@@ -871,7 +910,7 @@ void launch_merge(Ctx* c, int dtype, const uint8_t* bits, size_t n, const uint8_
     if (n == 0) return;
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
-        merge_impl<T>(c, bits, n, reinterpret_cast<const T*>(yes), n_yes, DictCodes{}, reinterpret_cast<const T*>(no),
+        merge_impl<T>(c, bits, n, yes, n_yes, DictCodes{}, no,
                       n_no, const_no, bits_to<T>(no_const), static_cast<T*>(out));
     });
 }
@@ -888,7 +927,7 @@ void launch_merge_codes(Ctx* c, int dtype, const uint8_t* bits, size_t n, const 
     }
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
-        merge_impl<T>(c, bits, n, nullptr, m, dc, reinterpret_cast<const T*>(no), n_no, false, T(0),
+        merge_impl<T>(c, bits, n, nullptr, m, dc, no, n_no, false, T(0),
                       static_cast<T*>(out));
     });
 }

@@ -105,8 +105,9 @@ void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& p,
                   void* yes, void* no, uint8_t* codes /* packed dict codes or null */, uint64_t* d_counts,
                   MinMaxResult* mm_yes = nullptr, MinMaxResult* mm_no = nullptr);
 // Merge: out[i] = bit i ? yes[k++] : no[j++] (const_no: `no_const` for every
-// clear bit).  n_yes / n_no rows are available; a stencil asking for more
-// raises the device error flag (corrupt input).
+// clear bit).  yes / no may sit at any byte address (outputs stored
+// unchanged in a blob); n_yes / n_no rows are available; a stencil asking
+// for more raises the device error flag (corrupt input).
 void launch_merge(Ctx* c, int dtype, const uint8_t* stencil_bits /* packed bits, no header */, size_t n,
                   const uint8_t* yes, size_t n_yes, const uint8_t* no, size_t n_no, bool const_no, uint64_t no_const,
                   void* out);

@@ -116,6 +116,22 @@ def test_decompress_misaligned_blob(ctx):
             assert np.array_equal(out.to_numpy(np.int32), x), (path, off)
 
 
+@pytest.mark.parametrize("dtype", [g.D_CHAR, g.D_SHORT, g.D_INT, g.D_TIME, g.D_FLOAT, g.D_DOUBLE])
+def test_merge_reads_rows_at_any_byte_offset(ctx, dtype):
+    """Stencil outputs stored unchanged ([x, none, none] trees) sit after
+    metadata of odd length; the merge stages them from any byte address,
+    for every row width and every blob misalignment."""
+    for n in (3001, 70_003):
+        x = column(dtype, n, seed=n)
+        for path in ([g.PATCH, g.NONE, g.NONE], [g.DICT, g.NONE, g.NONE], [g.CONST, g.NONE]):
+            tree = g.CompressionTree(ctx, path, dtype)
+            blob = tree.Compress(ctx.upload(x)).to_bytes()
+            for off in (0, 1, 2, 3, 6):
+                big = ctx.upload(b"\xab" * off + blob + b"\xcd" * 7)
+                out = tree.Decompress(big.view(off, len(blob)))
+                assert out.to_bytes() == x.tobytes(), (path, dtype, n, off)
+
+
 def test_corrupt_blob_is_an_error(ctx):
     x = D.timestamps(5000, seed=6)
     blob = g.CompressionTree(ctx, [g.DELTA, g.AFL, g.NONE], g.D_INT).Compress(ctx.upload(x)).to_bytes()
