@@ -187,10 +187,32 @@ uint64_t fl_bytes(size_t es, int b, size_t n) {
 }
 
 std::vector<Part> afl_encode_int(Ctx* c, int dtype, DView in, const MinMaxResult* known = nullptr,
-                                 bool defer = false) {
+                                 bool defer = false, const MinMaxResult* known_host = nullptr) {
     const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
     if (n == 0) return {empty(), empty()};
     Part x = aligned_view(c, in, es);
+    if (known_host) {  // the width is known on the host: no min/max pass, no read-back
+        const int b = bits_of(dtype, *known_host);
+        Part data;
+        if (defer) {
+            auto job = std::make_shared<FlJob>();
+            job->dtype = dtype;
+            job->bits = b;
+            job->n = n;
+            job->x = x;
+            data.n = (size_t)fl_bytes(es, b, n);
+            data.fl = std::move(job);
+        } else {
+            DevBuf buf(c, ((size_t)fl_bytes(es, b, n) + 3) & ~size_t(3));
+            launch_fl_pack(c, dtype, x.p, n, nullptr, b, buf.get(), buf.size());
+            buf.set_size((size_t)fl_bytes(es, b, n));
+            kernel_bytes(c, "k_fl_pack", (double)(n * es + buf.size()));
+            data = own(std::move(buf));
+        }
+        const uint64_t bytes = data.n;
+        uint8_t meta[4] = {(uint8_t)b, (uint8_t)((bytes * 8 - (uint64_t)n * b) & 0xff), 0, 0};
+        return {host_part(meta, 4), data};
+    }
     DevBuf mmb;
     const MinMaxResult* mm = known;
     if (!mm) {
@@ -325,14 +347,22 @@ std::vector<Part> rle_encode(Ctx* c, int dtype, DView in) {
     if (n == 0) return {host_part(&R, 4), empty(), empty()};
     Part x = aligned_view(c, in, es);
     DevBuf lens(c, 4 * n), vals(c, es * n), runs(c, 8);
-    launch_rle_encode(c, dtype, x.p, n, lens.get<int32_t>(), vals.get(), runs.get<uint64_t>());
-    uint64_t r = 0;
-    d2h(c, &r, runs.get(), 8);
+    // the run count comes back with both outputs' min/max (integer values):
+    // FL encoders of the outputs then need no pass or read-back of their own
+    const bool integral = dtype != DDJ_D_FLOAT && dtype != DDJ_D_DOUBLE;
+    auto hl = std::make_shared<MinMaxResult>(), hv = std::make_shared<MinMaxResult>();
+    const uint64_t r = launch_rle_encode(c, dtype, x.p, n, lens.get<int32_t>(), vals.get(), runs.get<uint64_t>(),
+                                         integral ? hv.get() : nullptr, hl.get());
     R = (int32_t)r;
     kernel_bytes(c, "k_rle_encode", (double)(n * es + r * (4 + es)));
     lens.set_size(4 * r);
     vals.set_size(es * r);
-    return {host_part(&R, 4), own(std::move(lens)), own(std::move(vals))};
+    Part pl = own(std::move(lens)), pv = own(std::move(vals));
+    if (r > 0) {
+        pl.minmax_host = hl;
+        if (integral) pv.minmax_host = hv;
+    }
+    return {host_part(&R, 4), pl, pv};
 }
 // rle_decode.cu:19-73
 Part rle_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
@@ -389,8 +419,18 @@ std::vector<Part> patch_encode(Ctx* c, int dtype, DView in, bool outside) {
     auto mm_no = integral ? std::make_shared<DevBuf>(c, sizeof(MinMaxResult)) : nullptr;
     launch_split(c, dtype, x.p, n, p, st.bytes_ptr(), yes.get(), no.get(), nullptr, cnt.get<uint64_t>(),
                  integral ? mm_yes->get<MinMaxResult>() : nullptr, integral ? mm_no->get<MinMaxResult>() : nullptr);
+    // the count and both outputs' min/max in one read-back: the FL encoders
+    // of the outputs then need no pass and no read-back of their own
     uint64_t m = 0;
-    d2h(c, &m, cnt.get(), 8);
+    auto hy = std::make_shared<MinMaxResult>(), hn = std::make_shared<MinMaxResult>();
+    if (integral) {
+        const ReadBack rb[3] = {{&m, cnt.get(), 8},
+                                {hy.get(), mm_yes->get(), sizeof(MinMaxResult)},
+                                {hn.get(), mm_no->get(), sizeof(MinMaxResult)}};
+        d2h_many(c, rb, 3);
+    } else {
+        d2h(c, &m, cnt.get(), 8);
+    }
     yes.set_size(m * es);
     no.set_size((n - m) * es);
     kernel_bytes(c, "k_minmax", (double)(n * es));
@@ -398,6 +438,10 @@ std::vector<Part> patch_encode(Ctx* c, int dtype, DView in, bool outside) {
     Part py = own(std::move(yes)), pn = own(std::move(no));
     py.minmax = mm_yes;
     pn.minmax = mm_no;
+    if (integral) {
+        py.minmax_host = hy;
+        pn.minmax_host = hn;
+    }
     return {own(std::move(st)), py, pn};
 }
 Part patch_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
@@ -705,7 +749,8 @@ void return_types(int enc, int dtype, int out[2]) {
     if (enc == DDJ_ENC_RLE || enc == DDJ_ENC_FLOAT_TO_INT) out[0] = DDJ_D_INT;  // rle_encoding.hpp:35, float_encoding.hpp:31
 }
 
-std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResult* minmax, bool defer_fl) {
+std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResult* minmax, bool defer_fl,
+                         const MinMaxResult* minmax_host) {
     if (!dtype_ok(dtype)) fail(DDJ_EINVAL, "unknown data type");
     if (in.n && !in.p) fail(DDJ_EINVAL, "encoder input has no device bytes");
     switch (enc) {
@@ -715,7 +760,7 @@ std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResul
     case DDJ_ENC_AFL:
         if (dtype == DDJ_D_DOUBLE) fail(DDJ_EUNSUPPORTED, "AFL on double returns no outputs (afl_encoding.cu:268-271)");
         if (dtype == DDJ_D_FLOAT) return afl_encode_float(c, in);
-        return afl_encode_int(c, dtype, in, minmax, defer_fl);
+        return afl_encode_int(c, dtype, in, minmax, defer_fl, minmax_host);
     case DDJ_ENC_RLE: return rle_encode(c, dtype, in);
     case DDJ_ENC_DICT: return dict_encode(c, dtype, in);
     case DDJ_ENC_CONST: return const_encode(c, dtype, in);

@@ -26,6 +26,7 @@ struct Part {
     // Optional: a MinMaxResult (device) of the part's values, computed by the
     // encoder that produced them, so that an FL child skips its own pass.
     std::shared_ptr<DevBuf> minmax;
+    std::shared_ptr<const MinMaxResult> minmax_host;  // the same keys on the host, when read back already
     std::shared_ptr<const FlJob> fl;
     bool on_host() const { return host != nullptr; }
     DView view() const { return DView{p, n}; }
@@ -56,7 +57,7 @@ bool enc_ok(int enc);
 // `defer_fl`: an integer AFL encoder returns its data as an FlJob part (for
 // a blob being assembled, when the data goes to the blob unchanged).
 std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResult* minmax = nullptr,
-                         bool defer_fl = false);
+                         bool defer_fl = false, const MinMaxResult* minmax_host = nullptr);
 // Encoding::Decode: in[0] = metadata, in[1..] = children outputs.
 Part decode(Ctx* c, int enc, int dtype, const std::vector<DView>& in);
 // GetMetadataSize / GetCompressedSize.

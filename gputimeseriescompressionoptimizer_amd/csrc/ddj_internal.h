@@ -190,6 +190,13 @@ struct DView {
 inline DView view(const DevBuf& b) { return DView{b.bytes_ptr(), b.size()}; }
 
 void d2h(Ctx* c, void* dst, const void* src, size_t bytes);  // synchronises the stream
+// Several small read-backs with one stream synchronisation.
+struct ReadBack {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+void d2h_many(Ctx* c, const ReadBack* rb, size_t count);
 void h2d(Ctx* c, void* dst, const void* src, size_t bytes);  // stream-ordered
 void d2d(Ctx* c, void* dst, const void* src, size_t bytes);  // stream-ordered
 DevBuf make_buf(Ctx* c, const void* host, size_t bytes);     // stream-ordered upload

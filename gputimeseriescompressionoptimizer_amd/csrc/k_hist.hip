@@ -396,15 +396,8 @@ void topk_most_frequent(Ctx* c, int dtype, const void* x, size_t n, int K, uint6
                                counts.get<unsigned long long>(), cap - 1, ctl.get<HistCtl>(),
                                nan_keys.get<uint64_t>(), nan_n.get<uint32_t>()); }
             HIPC(hipGetLastError());
-            HistCtl h;
-            d2h(c, &h, ctl.get(), sizeof h);
-            if (h.overflow || h.distinct * 10 > cap * 7) {
-                if (attempt == 1) fail(DDJ_ENOMEM, "histogram table overflow");
-                uint64_t want = 1;
-                while (want < 2 * n + 2) want <<= 1;
-                cap = std::max<uint64_t>(want, cap << 2);
-                continue;
-            }
+            // the top-K runs before the table's state is read back, so that one
+            // read-back brings both (its result is dropped if the table filled)
             const unsigned tg = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((cap + kBlock - 1) / kBlock, 128));
             DevBuf partials(c, sizeof(Cand) * K * tg), out(c, sizeof(Cand) * K);
             { KernelTimer kt_(c, "k_hist_topk"); hipLaunchKernelGGL(k_hist_topk<T>, dim3(tg), dim3(kBlock), 0, c->stream, keys.get<unsigned long long>(),
@@ -412,8 +405,17 @@ void topk_most_frequent(Ctx* c, int dtype, const void* x, size_t n, int K, uint6
                                ctl.get<HistCtl>(), nan_keys.get<uint64_t>(), nan_n.get<uint32_t>(), (uint32_t)g,
                                out.get<Cand>()); }
             HIPC(hipGetLastError());
+            HistCtl h;
             Cand res[kMaxK];
-            d2h(c, res, out.get(), sizeof(Cand) * K);
+            const ReadBack rb[2] = {{&h, ctl.get(), sizeof h}, {res, out.get(), sizeof(Cand) * K}};
+            d2h_many(c, rb, 2);
+            if (h.overflow || h.distinct * 10 > cap * 7) {
+                if (attempt == 1) fail(DDJ_ENOMEM, "histogram table overflow");
+                uint64_t want = 1;
+                while (want < 2 * n + 2) want <<= 1;
+                cap = std::max<uint64_t>(want, cap << 2);
+                continue;
+            }
             int nd = (int)(h.distinct + (h.special ? 1 : 0) + h.nans);
             if (n_distinct_host) *n_distinct_host = nd;
             // fewer than K distinct values: repeat the last real key

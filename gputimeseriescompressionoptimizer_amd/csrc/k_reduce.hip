@@ -111,8 +111,10 @@ template <typename T> __device__ __forceinline__ void fold_fixup(const T* x, uin
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size_t n, uint64_t* __restrict__ partials) {
+__global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size_t n, uint64_t* __restrict__ partials,
+                                                   const uint64_t* d_rows) {
     __shared__ uint64_t sm[kBlock / kWave];
+    if (d_rows) n = *d_rows < n ? *d_rows : n;  // rows counted on the device (uniform)
     FoldKeys<T> f;
     const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x, stride = (size_t)gridDim.x * kBlock;
     constexpr int V = 16 / sizeof(T);
@@ -481,14 +483,14 @@ int reduce_grid(Ctx* c, size_t n, int elem) {
 
 }  // namespace
 
-void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_result) {
+void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_result, const uint64_t* d_rows) {
     if (n == 0) fail(DDJ_EINVAL, "minmax of an empty array");
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
         const int grid = reduce_grid(c, n, sizeof(T));
         DevBuf partials(c, 24 * (size_t)grid);
         { KernelTimer kt_(c, "k_minmax"); hipLaunchKernelGGL(k_minmax<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
-                           partials.get<uint64_t>()); }
+                           partials.get<uint64_t>(), d_rows); }
         HIPC(hipGetLastError());
         { KernelTimer kt_(c, "k_minmax_fold"); hipLaunchKernelGGL((k_minmax_fold<T, 3>), dim3(1), dim3(kBlock), 0, c->stream,
                            partials.get<uint64_t>(), grid, static_cast<const T*>(x), d_result); }

@@ -608,8 +608,9 @@ __global__ __launch_bounds__(kBlock) void k_rle_encode(const T* __restrict__ x, 
     if (threadIdx.x == 0 && base + cnt == n) d_runs[0] = s_prefix + total;
 }
 
-__global__ __launch_bounds__(kBlock) void k_rle_lengths(const uint32_t* __restrict__ starts, size_t runs, size_t n,
-                                                        int32_t* __restrict__ lengths) {
+__global__ __launch_bounds__(kBlock) void k_rle_lengths(const uint32_t* __restrict__ starts, const uint64_t* d_runs,
+                                                        size_t n, int32_t* __restrict__ lengths) {
+    const size_t runs = *d_runs;
     const size_t stride = (size_t)gridDim.x * kBlock;
     for (size_t r = (size_t)blockIdx.x * kBlock + threadIdx.x; r < runs; r += stride) {
         const uint64_t e = r + 1 < runs ? starts[r + 1] : n;
@@ -934,10 +935,11 @@ void launch_merge_codes(Ctx* c, int dtype, const uint8_t* bits, size_t n, const 
 
 
 
-void launch_rle_encode(Ctx* c, int dtype, const void* x, size_t n, int32_t* lengths, void* values, uint64_t* d_runs) {
-    if (n == 0) return;
+uint64_t launch_rle_encode(Ctx* c, int dtype, const void* x, size_t n, int32_t* lengths, void* values, uint64_t* d_runs,
+                           MinMaxResult* h_values, MinMaxResult* h_lengths) {
+    if (n == 0) return 0;
     if (n >= (1ull << 31)) fail(DDJ_EUNSUPPORTED, "RLE run lengths are int32 (reference rle_encoding.hpp:35)");
-    dispatch_native(dtype, [&](auto tag) {
+    return dispatch_native(dtype, [&](auto tag) -> uint64_t {
         using T = decltype(tag);
         const size_t tiles = (n + kTile - 1) / kTile;
         TileScan ts(c, tiles);
@@ -951,11 +953,22 @@ void launch_rle_encode(Ctx* c, int dtype, const void* x, size_t n, int32_t* leng
                            static_cast<const T*>(x), n, static_cast<T*>(values), starts.get<uint32_t>(),
                            (const uint64_t*)ts.prefix, d_runs); }
         HIPC(hipGetLastError());
-        uint64_t runs = 0;
-        d2h(c, &runs, d_runs, 8);
-        { KernelTimer kt_(c, "k_rle_lengths"); hipLaunchKernelGGL(k_rle_lengths, dim3(grid_for(c, runs)), dim3(kBlock), 0, c->stream, starts.get<uint32_t>(),
-                           runs, n, lengths); }
+        // lengths (and the outputs' min/max) over the run count the device
+        // holds, so that one read-back brings everything the next nodes need
+        { KernelTimer kt_(c, "k_rle_lengths"); hipLaunchKernelGGL(k_rle_lengths, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream,
+                           starts.get<uint32_t>(), (const uint64_t*)d_runs, n, lengths); }
         HIPC(hipGetLastError());
+        DevBuf mm(c, 2 * sizeof(MinMaxResult));
+        MinMaxResult* dmm = mm.get<MinMaxResult>();
+        if (h_values && std::is_integral<T>::value) launch_minmax(c, dtype, values, n, dmm, d_runs);
+        if (h_lengths) launch_minmax(c, DDJ_D_INT, lengths, n, dmm + 1, d_runs);
+        uint64_t runs = 0;
+        ReadBack rb[3] = {{&runs, d_runs, 8}, {nullptr, dmm, 0}, {nullptr, dmm + 1, 0}};
+        size_t k = 1;
+        if (h_values && std::is_integral<T>::value) rb[k++] = ReadBack{h_values, dmm, sizeof(MinMaxResult)};
+        if (h_lengths) rb[k++] = ReadBack{h_lengths, dmm + 1, sizeof(MinMaxResult)};
+        d2h_many(c, rb, k);
+        return runs;
     });
 }
 

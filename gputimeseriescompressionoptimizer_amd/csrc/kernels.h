@@ -33,7 +33,8 @@ struct MinMaxResult {
 };
 
 // min/max of n values of `dtype` into d_result (device).  One launch.
-void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_result);
+void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_result,
+                   const uint64_t* d_rows = nullptr /* device row count (<= n) read by the kernel, or null */);
 // min/max of the wrapping differences x[i+1] - x[i] of n integer rows.
 void launch_minmax_delta(Ctx* c, int elem_size, const void* x, size_t n, MinMaxResult* d_result);
 // FL pack of the m = n - 1 differences of n integer rows (bits from mm).
@@ -117,7 +118,11 @@ void launch_merge_codes(Ctx* c, int dtype, const uint8_t* stencil_bits, size_t n
                         int code_bits, const uint8_t* dict, uint32_t K, const uint8_t* no, size_t n_no, void* out);
 
 // RLE.
-void launch_rle_encode(Ctx* c, int dtype, const void* x, size_t n, int32_t* lengths, void* values, uint64_t* d_runs);
+// Returns the run count (one read-back, which also brings the min/max of the
+// values and of the lengths into *h_values / *h_lengths when those are set;
+// values: integer types only).
+uint64_t launch_rle_encode(Ctx* c, int dtype, const void* x, size_t n, int32_t* lengths, void* values, uint64_t* d_runs,
+                           MinMaxResult* h_values = nullptr, MinMaxResult* h_lengths = nullptr);
 DevBuf rle_decode_dev(Ctx* c, int dtype, const uint8_t* lengths, const uint8_t* values, size_t runs);
 
 // Top-K most frequent keys (count desc, key asc), K <= 4.

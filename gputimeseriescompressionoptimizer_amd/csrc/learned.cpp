@@ -127,7 +127,8 @@ public:
         auto it = enc_.find(ek);
         if (it == enc_.end())
             it = enc_.emplace(ek, encode(c_, node->enc, node->dtype, data.view(),
-                                         data.minmax ? data.minmax->get<MinMaxResult>() : nullptr))
+                                         data.minmax ? data.minmax->get<MinMaxResult>() : nullptr, false,
+                                         data.minmax_host.get()))
                      .first;
         const std::vector<Part>& out = it->second;
         size_t total = 8 + out[0].n;

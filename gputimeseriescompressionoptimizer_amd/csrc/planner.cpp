@@ -64,7 +64,8 @@ std::vector<Candidate> full_statistics_update(Ctx* c, const Part& data, int et, 
             compr.push_back(host_part(&len, sizeof len));
             compr.push_back(data);
         } else {
-            compr = encode(c, cont, dt, data.view(), data.minmax ? data.minmax->get<MinMaxResult>() : nullptr);
+            compr = encode(c, cont, dt, data.view(), data.minmax ? data.minmax->get<MinMaxResult>() : nullptr, false,
+                           data.minmax_host.get());
         }
         size_t psize = 8;
         for (auto& p : compr) psize += p.n;

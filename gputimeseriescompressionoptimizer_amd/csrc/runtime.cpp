@@ -122,6 +122,28 @@ void d2h(Ctx* c, void* dst, const void* src, size_t bytes) {
     c->staging_off = 0;
 }
 
+void d2h_many(Ctx* c, const ReadBack* rb, size_t count) {
+    size_t total = 0;
+    for (size_t i = 0; i < count; i++) total += (rb[i].bytes + 63) & ~(size_t)63;
+    if (!c->readback || total > Ctx::kReadbackSize) {
+        for (size_t i = 0; i < count; i++) d2h(c, rb[i].dst, rb[i].src, rb[i].bytes);
+        return;
+    }
+    size_t off = 0;
+    for (size_t i = 0; i < count; i++) {
+        if (rb[i].bytes)
+            HIPC(hipMemcpyAsync(c->readback + off, rb[i].src, rb[i].bytes, hipMemcpyDeviceToHost, c->stream));
+        off += (rb[i].bytes + 63) & ~(size_t)63;
+    }
+    HIPC(hipStreamSynchronize(c->stream));
+    off = 0;
+    for (size_t i = 0; i < count; i++) {
+        if (rb[i].bytes) memcpy(rb[i].dst, c->readback + off, rb[i].bytes);
+        off += (rb[i].bytes + 63) & ~(size_t)63;
+    }
+    c->staging_off = 0;
+}
+
 void read_small(Ctx* c, void* dst, const void* src, size_t bytes) {
     const uint8_t* s = static_cast<const uint8_t*>(src);
     for (const auto& sh : c->shadows) {

@@ -91,7 +91,7 @@ size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& 
         // when the blob is assembled
         const bool defer = node.enc == DDJ_ENC_AFL && is_int_type(node.dtype) && output_is_leaf(node);
         res = encode(c, node.enc, node.dtype, data.view(), data.minmax ? data.minmax->get<MinMaxResult>() : nullptr,
-                     defer);
+                     defer, data.minmax_host.get());
     }
     Piece h;
     h.is_header = true;
