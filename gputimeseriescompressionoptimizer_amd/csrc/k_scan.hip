@@ -990,8 +990,20 @@ DevBuf rle_decode_dev(Ctx* c, int dtype, const uint8_t* lengths, const uint8_t* 
         { KernelTimer kt_(c, "k_scan_lengths"); hipLaunchKernelGGL(k_scan_lengths, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, lengths, runs,
                            offsets.get<uint64_t>(), (const uint64_t*)ts.prefix); }
         HIPC(hipGetLastError());
+        // the output size and the error flag in one read-back: a negative
+        // length fails here; otherwise nothing launched so far raised the
+        // flag and the expansion cannot, so the call ends without reading it
         uint64_t total = 0;
-        d2h(c, &total, offsets.get<uint64_t>() + runs, 8);
+        uint32_t err = 0;
+        const ReadBack rb[2] = {{&total, offsets.get<uint64_t>() + runs, 8}, {&err, &c->control->error_flag, 4}};
+        d2h_many(c, rb, 2);
+        if (err) {
+            const uint32_t zero = 0;
+            h2d(c, &c->control->error_flag, &zero, 4);
+            c->error_flag_armed = false;
+            fail(DDJ_ECORRUPT, "encoded input failed a device-side check");
+        }
+        c->error_flag_armed = false;
         DevBuf out(c, total * sizeof(T));
         if (total == 0) return out;
         kernel_bytes(c, "k_rle_expand", (double)total * sizeof(T) + (double)runs * (8 + sizeof(T)));
