@@ -492,7 +492,7 @@ Part const_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
 // (dict_encoding.hpp:30) and UniqueEncoding::CompressUnique
 // (unique_encoding.cu:88-131): out1 = [size_t K][size_t m][K x T][codes].
 constexpr int kDictK = 4;
-std::vector<Part> dict_encode(Ctx* c, int dtype, DView in) {
+std::vector<Part> dict_encode(Ctx* c, int dtype, DView in, bool defer = false) {
     const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
     if (n == 0) return {empty(), empty(), empty()};
     Part x = aligned_view(c, in, es);
@@ -506,6 +506,36 @@ std::vector<Part> dict_encode(Ctx* c, int dtype, DView in) {
     // (ALT_BITLEN(K - 1) = 2); the split packs the codes in place, into words
     // sized for every row being frequent (it writes every word of the m codes)
     const size_t head = 16 + kDictK * es;
+    if (defer) {
+        // counting pass now, the writing pass at blob assembly: the other
+        // rows then go straight to their blob range
+        auto job = std::make_shared<SplitJob>();
+        job->plan = launch_split_count(c, dtype, x.p, n, p);
+        uint64_t m = 0;
+        d2h(c, &m, job->plan.ts->prefix + job->plan.ts->tiles, 8);
+        DevBuf st(c, 1 + (n + 7) / 8), f(c, head + 4 * ((m + 15) / 16));
+        kernel_bytes(c, "k_hist_count", (double)(n * es));
+        kernel_bytes(c, "k_split", (double)(n * es + 1 + (n + 7) / 8 + (m + 15) / 16 * 4 + (n - m) * es));
+        Part freq = empty();
+        if (m > 0) {
+            std::vector<uint8_t> h(head);
+            size_t sizes[2] = {(size_t)kDictK, (size_t)m};
+            memcpy(h.data(), sizes, 16);
+            for (int i = 0; i < kDictK; i++) memcpy(h.data() + 16 + i * es, &keys[i], es);
+            h2d(c, f.get(), h.data(), head);
+            job->codes = f.bytes_ptr() + head;
+            freq = own(std::move(f));
+        }
+        Part stp = own(std::move(st));
+        job->x = x;
+        job->stencil = stp;
+        job->freq = freq;
+        job->count = std::make_shared<DevBuf>(c, 8);
+        Part others;  // with no rows (every row frequent) the job still runs: it writes the stencil and codes
+        others.n = (size_t)(n - m) * es;
+        others.split = job;
+        return {stp, freq, others};
+    }
     DevBuf st(c, 1 + (n + 7) / 8), f(c, head + 4 * ((n + 15) / 16)), others(c, n * es), cnt(c, 8);
     launch_split(c, dtype, x.p, n, p, st.bytes_ptr(), nullptr, others.get(), f.bytes_ptr() + head,
                  cnt.get<uint64_t>());
@@ -710,6 +740,11 @@ std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in, bool defer) {
     return {borrow(DView{x.p, es}, x.owner), host_part(meta, 4), data};
 }
 
+void run_split_job(Ctx* c, const SplitJob& j, void* no_rows) {
+    launch_split_write(c, j.plan, const_cast<uint8_t*>(j.stencil.p), nullptr, static_cast<uint8_t*>(no_rows), j.codes,
+                       j.count->get<uint64_t>());
+}
+
 void run_fl_job(Ctx* c, const FlJob& j, void* out, size_t bytes) {
     const size_t es = dtype_size(j.dtype);
     if (j.n == 0 || bytes == 0) return;
@@ -750,7 +785,7 @@ void return_types(int enc, int dtype, int out[2]) {
 }
 
 std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResult* minmax, bool defer_fl,
-                         const MinMaxResult* minmax_host) {
+                         const MinMaxResult* minmax_host, bool defer_split) {
     if (!dtype_ok(dtype)) fail(DDJ_EINVAL, "unknown data type");
     if (in.n && !in.p) fail(DDJ_EINVAL, "encoder input has no device bytes");
     switch (enc) {
@@ -762,7 +797,7 @@ std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResul
         if (dtype == DDJ_D_FLOAT) return afl_encode_float(c, in);
         return afl_encode_int(c, dtype, in, minmax, defer_fl, minmax_host);
     case DDJ_ENC_RLE: return rle_encode(c, dtype, in);
-    case DDJ_ENC_DICT: return dict_encode(c, dtype, in);
+    case DDJ_ENC_DICT: return dict_encode(c, dtype, in, defer_split);
     case DDJ_ENC_CONST: return const_encode(c, dtype, in);
     case DDJ_ENC_PATCH: return patch_encode(c, dtype, in, false);
     case DDJ_ENC_OUTSIDE_PATCH: return patch_encode(c, dtype, in, true);

@@ -11,6 +11,7 @@
 namespace ddj {
 
 struct FlJob;
+struct SplitJob;
 
 // A device byte range that keeps its storage alive.
 // Device bytes, or a few host bytes (metadata computed on the host, which
@@ -28,6 +29,10 @@ struct Part {
     std::shared_ptr<DevBuf> minmax;
     std::shared_ptr<const MinMaxResult> minmax_host;  // the same keys on the host, when read back already
     std::shared_ptr<const FlJob> fl;
+    // set: rows a deferred split writes straight into the blob (p is null,
+    // n is the exact size); the split also fills the other outputs of its
+    // encoder, so blob assembly runs it before gathering those
+    std::shared_ptr<const SplitJob> split;
     bool on_host() const { return host != nullptr; }
     DView view() const { return DView{p, n}; }
 };
@@ -43,6 +48,17 @@ struct FlJob {
 // Runs the pack into the `bytes` stream bytes at out (any byte address).
 void run_fl_job(Ctx* c, const FlJob& j, void* out, size_t bytes);
 
+// A dictionary split whose counting pass ran (the yes count is known): its
+// writing pass fills the stencil and code buffers (parts of its encoder's
+// output) and writes the other rows at the blob offset assembly gives it.
+struct SplitJob {
+    SplitPlan plan;
+    Part x, stencil, freq;  // kept alive until the split runs
+    uint8_t* codes = nullptr;
+    std::shared_ptr<DevBuf> count;
+};
+void run_split_job(Ctx* c, const SplitJob& j, void* no_rows);
+
 Part own(DevBuf&& b);
 Part borrow(DView v, std::shared_ptr<DevBuf> keep = nullptr);
 Part host_part(const void* h, size_t n);
@@ -57,7 +73,7 @@ bool enc_ok(int enc);
 // `defer_fl`: an integer AFL encoder returns its data as an FlJob part (for
 // a blob being assembled, when the data goes to the blob unchanged).
 std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResult* minmax = nullptr,
-                         bool defer_fl = false, const MinMaxResult* minmax_host = nullptr);
+                         bool defer_fl = false, const MinMaxResult* minmax_host = nullptr, bool defer_split = false);
 // Encoding::Decode: in[0] = metadata, in[1..] = children outputs.
 Part decode(Ctx* c, int enc, int dtype, const std::vector<DView>& in);
 // GetMetadataSize / GetCompressedSize.

@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_split_count(const T* __restrict__ x,
 template <typename T, bool CODES, bool WRITE_NO, int K>
 __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_t n, Pred<T> p,
                                                   uint8_t* __restrict__ stencil, T* __restrict__ yes,
-                                                  T* __restrict__ no, uint8_t* __restrict__ codes,
+                                                  uint8_t* __restrict__ no_bytes, uint8_t* __restrict__ codes,
                                                   uint32_t* __restrict__ edges, const uint64_t* __restrict__ prefix,
                                                   uint64_t* d_count) {
     constexpr int kCodeWords = kTile / 16 + 2;  // 2-bit code words a tile's codes can touch
@@ -228,7 +228,45 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     }
     if (WRITE_NO) {
         const size_t N0 = base - Y0;
-        for (int i = threadIdx.x; i < cnt - (int)total; i += kBlock) no[N0 + i] = buf[P(nbase + i)];
+        const int nn = cnt - (int)total;
+        if ((reinterpret_cast<uintptr_t>(no_bytes) % sizeof(T)) == 0) {
+            T* no = reinterpret_cast<T*>(no_bytes);
+            for (int i = threadIdx.x; i < nn; i += kBlock) no[N0 + i] = buf[P(nbase + i)];
+        } else {
+            // rows stored unchanged into a blob at any byte address: the
+            // tile's bytes go out as 16-byte stores realigned to that address,
+            // with single bytes at either end
+            auto row_bits = [&](int e) -> uint64_t { return e < nn ? ubits(buf[P(nbase + e)]) : 0ull; };  // bit pattern
+            auto word = [&](int w) -> uint32_t {  // 32-bit word w of the tile's row bytes
+                if constexpr (sizeof(T) == 8) {
+                    const uint64_t v = row_bits(w >> 1);
+                    return (w & 1) ? (uint32_t)(v >> 32) : (uint32_t)v;
+                } else if constexpr (sizeof(T) == 4) {
+                    return (uint32_t)row_bits(w);
+                } else if constexpr (sizeof(T) == 2) {
+                    return (uint32_t)row_bits(2 * w) | ((uint32_t)row_bits(2 * w + 1) << 16);
+                } else {
+                    return (uint32_t)row_bits(4 * w) | ((uint32_t)row_bits(4 * w + 1) << 8) |
+                           ((uint32_t)row_bits(4 * w + 2) << 16) | ((uint32_t)row_bits(4 * w + 3) << 24);
+                }
+            };
+            uint8_t* d = no_bytes + N0 * sizeof(T);
+            const int L = nn * (int)sizeof(T);
+            int head = (int)((16 - (reinterpret_cast<uintptr_t>(d) & 15)) & 15);
+            head = head < L ? head : L;
+            if ((int)threadIdx.x < head) d[threadIdx.x] = (uint8_t)(word(threadIdx.x >> 2) >> (8 * (threadIdx.x & 3)));
+            const uint32_t r = (uint32_t)(head & 3);
+            const int nq = (L - head) / 16;
+            u32x4* d4 = reinterpret_cast<u32x4*>(d + head);
+            for (int i = threadIdx.x; i < nq; i += kBlock) {
+                const int q = (head + 16 * i) >> 2;  // first word of the chunk's bytes
+                const uint32_t w0 = word(q), w1 = word(q + 1), w2 = word(q + 2), w3 = word(q + 3), w4 = word(q + 4);
+                __builtin_nontemporal_store(u32x4{__builtin_amdgcn_alignbyte(w1, w0, r), __builtin_amdgcn_alignbyte(w2, w1, r),
+                                                  __builtin_amdgcn_alignbyte(w3, w2, r), __builtin_amdgcn_alignbyte(w4, w3, r)},
+                                            d4 + i);
+            }
+            for (int j = head + 16 * nq + threadIdx.x; j < L; j += kBlock) d[j] = (uint8_t)(word(j >> 2) >> (8 * (j & 3)));
+        }
     }
     if (threadIdx.x == 0 && base + cnt == n) d_count[0] = Y0 + total;
 }
@@ -800,27 +838,37 @@ template <typename F> void with_kind(int kind, F&& f) {
 }
 }  // namespace
 
-void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa, uint8_t* stencil, void* yes,
-                  void* no, uint8_t* codes, uint64_t* d_counts, MinMaxResult* mm_yes, MinMaxResult* mm_no) {
-    if (n == 0) return;
+namespace {
+template <typename T> Pred<T> make_pred(const PredArgs& pa) {
+    Pred<T> p;
+    p.kind = pa.kind;
+    p.a = bits_to<T>(pa.a_bits);
+    p.b = bits_to<T>(pa.b_bits);
+    for (int j = 0; j < 4; j++) p.set[j] = bits_to<T>(pa.set_bits[j]);
+    p.set_n = pa.set_n;
+    return p;
+}
+}  // namespace
+
+SplitPlan launch_split_count(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa, MinMaxResult* mm_yes,
+                             MinMaxResult* mm_no) {
+    SplitPlan plan;
+    plan.dtype = dtype;
+    plan.x = x;
+    plan.n = n;
+    plan.p = pa;
+    if (n == 0) return plan;
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
-        Pred<T> p;
-        p.kind = pa.kind;
-        p.a = bits_to<T>(pa.a_bits);
-        p.b = bits_to<T>(pa.b_bits);
-        for (int j = 0; j < 4; j++) p.set[j] = bits_to<T>(pa.set_bits[j]);
-        p.set_n = pa.set_n;
+        const Pred<T> p = make_pred<T>(pa);
         const size_t tiles = (n + kTile - 1) / kTile;
-        TileScan ts(c, tiles);
+        plan.ts = std::make_shared<TileScan>(c, tiles);
+        TileScan& ts = *plan.ts;
         auto xs = static_cast<const T*>(x);
-        auto ys = static_cast<T*>(yes);
-        auto ns = static_cast<T*>(no);
         kernel_bytes(c, "k_split_count", (double)n * sizeof(T));
         const bool mm = mm_yes && mm_no;
         if ((mm_yes || mm_no) && !(mm && std::is_integral<T>::value))
             fail(DDJ_EINVAL, "split min/max outputs need an integer type and both outputs");
-        if (codes && p.kind != PRED_IN_SET) fail(DDJ_EINVAL, "split codes need a dictionary predicate");
         DevBuf keys;
         if (mm) keys = DevBuf(c, 32 * tiles);
         const dim3 grid((unsigned)tiles), block(kBlock);
@@ -838,7 +886,22 @@ void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa
         });
         HIPC(hipGetLastError());
         ts.scan(c, mm ? keys.get<uint64_t>() : nullptr, mm_yes, mm_no);
-        const uint64_t* pre = ts.prefix;
+    });
+    return plan;
+}
+
+void launch_split_write(Ctx* c, const SplitPlan& plan, uint8_t* stencil, void* yes, uint8_t* no, uint8_t* codes,
+                        uint64_t* d_counts) {
+    if (plan.n == 0) return;
+    dispatch_native(plan.dtype, [&](auto tag) {
+        using T = decltype(tag);
+        const Pred<T> p = make_pred<T>(plan.p);
+        if (codes && p.kind != PRED_IN_SET) fail(DDJ_EINVAL, "split codes need a dictionary predicate");
+        const size_t n = plan.n, tiles = plan.ts->tiles;
+        auto xs = static_cast<const T*>(plan.x);
+        auto ys = static_cast<T*>(yes);
+        const uint64_t* pre = plan.ts->prefix;
+        const dim3 grid((unsigned)tiles), block(kBlock);
         DevBuf edges;
         if (codes) edges = DevBuf(c, 8 * tiles);
         uint32_t* ed = codes ? edges.get<uint32_t>() : nullptr;
@@ -847,16 +910,16 @@ void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa
             KernelTimer kt_(c, "k_split");
             if constexpr (K == PRED_IN_SET) {
                 if (codes) {
-                    hipLaunchKernelGGL((k_split<T, true, true, K>), grid, block, 0, c->stream, xs, n, p, stencil, ys, ns,
+                    hipLaunchKernelGGL((k_split<T, true, true, K>), grid, block, 0, c->stream, xs, n, p, stencil, ys, no,
                                        codes, ed, pre, d_counts);
                     return;
                 }
             }
             if (no)
-                hipLaunchKernelGGL((k_split<T, false, true, K>), grid, block, 0, c->stream, xs, n, p, stencil, ys, ns,
+                hipLaunchKernelGGL((k_split<T, false, true, K>), grid, block, 0, c->stream, xs, n, p, stencil, ys, no,
                                    codes, ed, pre, d_counts);
             else
-                hipLaunchKernelGGL((k_split<T, false, false, K>), grid, block, 0, c->stream, xs, n, p, stencil, ys, ns,
+                hipLaunchKernelGGL((k_split<T, false, false, K>), grid, block, 0, c->stream, xs, n, p, stencil, ys, no,
                                    codes, ed, pre, d_counts);
         });
         if (codes) {
@@ -867,6 +930,13 @@ void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa
         }
         HIPC(hipGetLastError());
     });
+}
+
+void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa, uint8_t* stencil, void* yes,
+                  void* no, uint8_t* codes, uint64_t* d_counts, MinMaxResult* mm_yes, MinMaxResult* mm_no) {
+    if (n == 0) return;
+    const SplitPlan plan = launch_split_count(c, dtype, x, n, pa, mm_yes, mm_no);
+    launch_split_write(c, plan, stencil, yes, static_cast<uint8_t*>(no), codes, d_counts);
 }
 
 namespace {

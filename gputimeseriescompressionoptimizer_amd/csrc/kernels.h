@@ -2,6 +2,8 @@
 // ctx->stream and returns without synchronising unless stated otherwise.
 #pragma once
 
+#include <memory>
+
 #include "ddj_internal.h"
 
 namespace ddj {
@@ -102,9 +104,23 @@ struct PredArgs {
 // codes (dictionary split, 4-byte aligned): the 2-bit codes packed 16 to a
 // little-endian 32-bit word; every word holding a code is written (room for
 // ceil(n / 16) words).
+// `no` may sit at any byte address (rows stored unchanged into a blob).
 void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& p, uint8_t* stencil /* 1+ceil(n/8) */,
                   void* yes, void* no, uint8_t* codes /* packed dict codes or null */, uint64_t* d_counts,
                   MinMaxResult* mm_yes = nullptr, MinMaxResult* mm_no = nullptr);
+// The same in two steps: the counting pass and scan (the yes count is then
+// prefix[tiles] of the plan's scan, on the device), and the writing pass.
+struct SplitPlan {
+    int dtype = 0;
+    const void* x = nullptr;
+    size_t n = 0;
+    PredArgs p{};
+    std::shared_ptr<TileScan> ts;
+};
+SplitPlan launch_split_count(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& p,
+                             MinMaxResult* mm_yes = nullptr, MinMaxResult* mm_no = nullptr);
+void launch_split_write(Ctx* c, const SplitPlan& plan, uint8_t* stencil, void* yes, uint8_t* no, uint8_t* codes,
+                        uint64_t* d_counts);
 // Merge: out[i] = bit i ? yes[k++] : no[j++] (const_no: `no_const` for every
 // clear bit).  yes / no may sit at any byte address (outputs stored
 // unchanged in a blob); n_yes / n_no rows are available; a stencil asking

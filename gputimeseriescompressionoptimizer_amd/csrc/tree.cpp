@@ -88,10 +88,14 @@ size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& 
         child_pre = {f[1], f[2]};
     } else {
         // an integer AFL whose data goes to the blob unchanged packs it there
-        // when the blob is assembled
+        // when the blob is assembled; so does a dictionary split whose two
+        // outputs are stored unchanged (its writing pass runs then)
         const bool defer = node.enc == DDJ_ENC_AFL && is_int_type(node.dtype) && output_is_leaf(node);
+        auto none_leaf = [](const NodeP& ch) { return ch->enc == DDJ_ENC_NONE && ch->ch.empty(); };
+        const bool defer_split =
+            node.enc == DDJ_ENC_DICT && node.ch.size() == 2 && none_leaf(node.ch[0]) && none_leaf(node.ch[1]);
         res = encode(c, node.enc, node.dtype, data.view(), data.minmax ? data.minmax->get<MinMaxResult>() : nullptr,
-                     defer, data.minmax_host.get());
+                     defer, data.minmax_host.get(), defer_split);
     }
     Piece h;
     h.is_header = true;
@@ -212,7 +216,7 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
             inline_bytes += 8;
         else if (p.part.on_host())
             inline_bytes += p.part.n;
-        if (p.is_header || (p.part.n && !p.part.fl)) n_ranges++;
+        if (p.is_header || (p.part.n && !p.part.fl && !p.part.split)) n_ranges++;
     }
     DevBuf rdev(c, sizeof(CopyRange) * n_ranges + inline_bytes);
     const uint8_t* inl_dev = rdev.bytes_ptr() + sizeof(CopyRange) * n_ranges;
@@ -226,8 +230,14 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
         uint64_t off, bytes;
     };
     std::vector<JobAt> jobs;
+    std::vector<std::pair<const SplitJob*, uint64_t>> splits;
     for (auto& p : pieces) {
         if (p.keep_alive) continue;
+        if (!p.is_header && p.part.split) {  // rows written in place by the split (also when there are none)
+            splits.emplace_back(p.part.split.get(), off);
+            off += p.part.n;
+            continue;
+        }
         if (p.is_header) {
             memcpy(inl + io, p.hdr, 8);
             ranges[ri++] = CopyRange{inl_dev + io, off, 8};
@@ -253,6 +263,8 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
         }
     }
     if (off != total || ri != n_ranges) fail(DDJ_EHIP, "blob size mismatch");
+    // deferred splits first: they also fill outputs that the gather copies
+    for (const auto& sj : splits) run_split_job(c, *sj.first, blob.bytes_ptr() + sj.second);
     h2d(c, rdev.get(), table.data(), table.size());
     kernel_bytes(c, "k_gather_copy", 2.0 * (double)copied);
     launch_gather_copy(c, rdev.get<CopyRange>(), (int)n_ranges, blob.bytes_ptr(), std::max<uint64_t>(max_bytes, 8));

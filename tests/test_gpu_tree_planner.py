@@ -132,6 +132,18 @@ def test_merge_reads_rows_at_any_byte_offset(ctx, dtype):
                 assert out.to_bytes() == x.tobytes(), (path, dtype, n, off)
 
 
+@pytest.mark.parametrize("dtype", [g.D_CHAR, g.D_SHORT, g.D_INT, g.D_TIME, g.D_FLOAT, g.D_DOUBLE])
+def test_dict_rows_written_in_place(ctx, dtype):
+    """[dict, none, none]: the split's writing pass runs at blob assembly and
+    writes the other rows straight into the blob (any byte offset); blobs
+    equal the oracle's, also with no other rows and with no frequent rows."""
+    for n in (1, 5000, 70_001):
+        check_tree(ctx, [g.DICT, g.NONE, g.NONE], dtype, column(dtype, n, seed=n + 1))
+    npdt = g.NP_DTYPE[dtype]
+    check_tree(ctx, [g.DICT, g.NONE, g.NONE], dtype, np.array([3, 1, 3, 2, 1, 4] * 2000, dtype=npdt))
+    check_tree(ctx, [g.DICT, g.NONE, g.NONE], dtype, np.arange(9000).astype(npdt))
+
+
 def test_corrupt_blob_is_an_error(ctx):
     x = D.timestamps(5000, seed=6)
     blob = g.CompressionTree(ctx, [g.DELTA, g.AFL, g.NONE], g.D_INT).Compress(ctx.upload(x)).to_bytes()
