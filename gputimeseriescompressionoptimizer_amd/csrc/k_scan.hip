@@ -725,9 +725,8 @@ __global__ __launch_bounds__(kBlock) void k_scan_lengths(const uint8_t* __restri
 // Run expansion: a workgroup owns 4096 output rows, finds the first run
 // overlapping them by binary search, stages the overlapping runs in LDS and
 // gives every row its run by a binary search in LDS.
-// Expansion tiles: TILE output rows per workgroup; LDS holds up to kExpRuns
-// runs of a tile.
-constexpr int kExpRuns = 2048;
+// Expansion tiles: TILE output rows per workgroup; LDS holds up to TILE
+// runs of a tile (all of them, unless some runs are empty).
 
 // tile_run[t] = the run holding row t * TILE (offsets are non-decreasing:
 // lengths were checked non-negative by the scan); tile_run[tiles] = R.
@@ -746,15 +745,14 @@ __global__ __launch_bounds__(kBlock) void k_rle_tile_runs(const uint64_t* __rest
 // Run expansion: a workgroup owns TILE output rows; its runs are
 // tile_run[b] .. tile_run[b + 1] (the run holding the next tile's first row).
 // They are staged in LDS and every row finds its run by binary search there;
-// a tile overlapping more runs than LDS holds (empty runs, or short runs in
-// a tile of more than kExpRuns rows) searches the offsets in global memory
-// instead.
+// a tile overlapping more runs than LDS holds (only possible with empty
+// runs) searches the offsets in global memory instead.
 template <typename T, int TILE>
 __global__ __launch_bounds__(kBlock) void k_rle_expand(const uint64_t* __restrict__ offsets, size_t R,
                                                        const uint8_t* __restrict__ values, size_t total,
                                                        const uint32_t* __restrict__ tile_run, T* __restrict__ out) {
-    __shared__ uint64_t soff[kExpRuns + 2];
-    __shared__ T sval[kExpRuns + 1];
+    __shared__ uint64_t soff[TILE + 2];
+    __shared__ T sval[TILE + 1];
     const size_t o0 = (size_t)blockIdx.x * TILE;
     const int cnt = (int)((total - o0) < (size_t)TILE ? (total - o0) : (size_t)TILE);
     const size_t r0 = tile_run[blockIdx.x];
@@ -762,7 +760,7 @@ __global__ __launch_bounds__(kBlock) void k_rle_expand(const uint64_t* __restric
     r1 = r1 < R ? r1 : R - 1;
     r1 = r1 < r0 ? r0 : r1;
     const size_t nr = r1 - r0 + 1;
-    if (nr <= (size_t)kExpRuns + 1) {
+    if (nr <= (size_t)TILE + 1) {
         for (int i = threadIdx.x; i <= (int)nr; i += kBlock) soff[i] = offsets[r0 + i];
         for (int i = threadIdx.x; i < (int)nr; i += kBlock) __builtin_memcpy(&sval[i], values + sizeof(T) * (r0 + i), sizeof(T));
         __syncthreads();
@@ -1087,8 +1085,8 @@ DevBuf rle_decode_dev(Ctx* c, int dtype, const uint8_t* lengths, const uint8_t* 
                                offsets.get<uint64_t>(), runs, values, (size_t)total, tile_run.get<uint32_t>(), out.get<T>()); }
             HIPC(hipGetLastError());
         };
-        // 2048-row tiles: bigger ones (fewer, longer-lived workgroups, each
-        // row walking further) measured slower even on long runs
+        // 2048-row tiles: 1024-row tiles (more, shorter-lived workgroups) and
+        // 8192-row tiles (fewer, each row walking further) measured slower
         expand(std::integral_constant<int, 2048>{});
         return out;
     });
