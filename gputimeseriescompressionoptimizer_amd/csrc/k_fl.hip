@@ -87,10 +87,12 @@ __device__ __forceinline__ int stage_fl_words(const uint8_t* __restrict__ in, si
             if (i < n4_fill) {
                 const int b0 = 16 * i;  // first byte of the chunk, relative to wp4
                 u32x4 m = v[u];
-                m.x &= byte_mask(a - b0, a + avail - b0);
-                m.y &= byte_mask(a - b0 - 4, a + avail - b0 - 4);
-                m.z &= byte_mask(a - b0 - 8, a + avail - b0 - 8);
-                m.w &= byte_mask(a - b0 - 12, a + avail - b0 - 12);
+                if (b0 < a || b0 + 16 > a + avail) {  // only the chunks at either end hold bytes outside the stream
+                    m.x &= byte_mask(a - b0, a + avail - b0);
+                    m.y &= byte_mask(a - b0 - 4, a + avail - b0 - 4);
+                    m.z &= byte_mask(a - b0 - 8, a + avail - b0 - 8);
+                    m.w &= byte_mask(a - b0 - 12, a + avail - b0 - 12);
+                }
                 w4[i] = m;
             }
         }
