@@ -144,6 +144,22 @@ def test_dict_rows_written_in_place(ctx, dtype):
     check_tree(ctx, [g.DICT, g.NONE, g.NONE], dtype, np.arange(9000).astype(npdt))
 
 
+@pytest.mark.parametrize("dtype", [g.D_CHAR, g.D_SHORT, g.D_INT, g.D_TIME])
+def test_fl_leaves_packed_in_place(ctx, dtype):
+    """Integer AFL leaves are packed straight into the blob at whatever byte
+    offset the pieces before them leave (odd stencil lengths, 1- and 2-byte
+    metadata); blobs equal the oracle's for every width."""
+    npdt = g.NP_DTYPE[dtype]
+    r = np.random.default_rng(dtype)
+    for n in (1, 33, 4097, 70_001):
+        cols = [r.integers(-50, 50, n).astype(npdt), np.cumsum(r.integers(0, 3, n)).astype(npdt),
+                r.integers(0, 1 << (4 * np.dtype(npdt).itemsize), n).astype(npdt)]
+        for x in cols:
+            for path in ([g.AFL, g.NONE], [g.DELTA, g.AFL, g.NONE], [g.PATCH, g.AFL, g.NONE, g.AFL, g.NONE],
+                         [g.RLE, g.AFL, g.NONE, g.AFL, g.NONE]):
+                check_tree(ctx, path, dtype, x)
+
+
 def test_corrupt_blob_is_an_error(ctx):
     x = D.timestamps(5000, seed=6)
     blob = g.CompressionTree(ctx, [g.DELTA, g.AFL, g.NONE], g.D_INT).Compress(ctx.upload(x)).to_bytes()
