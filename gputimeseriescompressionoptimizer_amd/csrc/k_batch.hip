@@ -15,6 +15,8 @@
 //                   packing straight into each blob's data region
 //
 // Blobs come out byte-identical to CompressionTree::Compress on each column.
+#include <cstdint>
+
 #include "dispatch.h"
 #include "k_common.h"
 #include "k_fl_tile.h"
@@ -31,21 +33,56 @@ __device__ __forceinline__ int64_t value_of(uint64_t k, uint32_t es) {
     return es == 8 ? (int64_t)(k ^ (1ull << 63)) : (int64_t)k - (1ll << 31);
 }
 
-// Row i of column c as a signed value; with delta, the wrapping difference
-// x[i+1] - x[i] in the column's type.
-__device__ __forceinline__ int64_t row_value(const BatchColumn& c, uint64_t i) {
-    if (c.es == 8) {
-        const int64_t* x = reinterpret_cast<const int64_t*>(c.x);
-        return c.delta ? (int64_t)((uint64_t)x[i + 1] - (uint64_t)x[i]) : x[i];
-    }
-    const int32_t* x = reinterpret_cast<const int32_t*>(c.x);
-    return c.delta ? (int64_t)(int32_t)((uint32_t)x[i + 1] - (uint32_t)x[i]) : (int64_t)x[i];
-}
-
 __global__ __launch_bounds__(kBlock) void k_batch_init(BatchColumn* cols, int ncol) {
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < ncol; i += gridDim.x * kBlock) {
         cols[i].min_key = ~0ull;
         cols[i].max_key = 0;
+    }
+}
+
+// Min/max (native compares) of the value stream rows [start, end) of one
+// column: x itself, or with delta the differences x[i+1] - x[i].  Lanes take
+// consecutive 16-byte vectors; with delta, the row after a vector is the
+// next lane's first (the wave's last lane, and the range's last vector,
+// load it).  start is a multiple of 16 / sizeof(T) (ranges are 2^k rows).
+template <typename T>
+__device__ __forceinline__ void range_minmax(const T* __restrict__ x, bool delta, uint64_t start, uint64_t end, T& mn,
+                                             T& mx) {
+    using U = typename UnsignedOf<T>::type;
+    constexpr int V = 16 / (int)sizeof(T);
+    const uint64_t v0 = start / V, v1 = end / V;
+    const int lane = threadIdx.x & (kWave - 1);
+    const u32x4* x4 = reinterpret_cast<const u32x4*>(x);
+    for (uint64_t b = v0; b < v1; b += kBlock) {  // uniform trip count: every lane takes part in the shuffles
+        const uint64_t i = b + threadIdx.x;
+        const bool valid = i < v1;
+        u32x4 w = valid ? __builtin_nontemporal_load(x4 + i) : u32x4{0u, 0u, 0u, 0u};
+        T nx = T(0);
+        if (delta && valid && (lane == kWave - 1 || i + 1 >= v1)) nx = x[(i + 1) * V];  // row (i+1)V <= end exists
+        const uint32_t nb0 = (uint32_t)__shfl_down((int)w.x, 1, kWave);
+        const uint32_t nb1 = (uint32_t)__shfl_down((int)w.y, 1, kWave);
+        if (!valid) continue;
+        T v[V + 1];
+        __builtin_memcpy(v, &w, 16);
+        if (delta) {
+            if (lane == kWave - 1 || i + 1 >= v1) {
+                v[V] = nx;
+            } else {
+                const uint64_t nb = ((uint64_t)nb1 << 32) | nb0;
+                __builtin_memcpy(&v[V], &nb, sizeof(T));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < V; k++) {
+            const T d = delta ? (T)(U)((U)v[k + 1] - (U)v[k]) : v[k];
+            mn = d < mn ? d : mn;
+            mx = d > mx ? d : mx;
+        }
+    }
+    for (uint64_t r = v1 * V + threadIdx.x; r < end; r += kBlock) {
+        const T d = delta ? (T)(U)((U)x[r + 1] - (U)x[r]) : x[r];
+        mn = d < mn ? d : mn;
+        mx = d > mx ? d : mx;
     }
 }
 
@@ -55,10 +92,20 @@ __global__ __launch_bounds__(kBlock) void k_batch_minmax(BatchColumn* cols, cons
         const BatchItem item = items[it];
         const BatchColumn c = cols[item.col];
         uint64_t mn = ~0ull, mx = 0;
-        for (uint64_t i = item.start + threadIdx.x; i < item.end; i += kBlock) {
-            const uint64_t k = key_of(row_value(c, i), c.es);
-            mn = k < mn ? k : mn;
-            mx = k > mx ? k : mx;
+        if (c.es == 8) {
+            int64_t a = INT64_MAX, z = INT64_MIN;
+            range_minmax<int64_t>(reinterpret_cast<const int64_t*>(c.x), c.delta, item.start, item.end, a, z);
+            if (a <= z) {
+                mn = key_of(a, 8);
+                mx = key_of(z, 8);
+            }
+        } else {
+            int32_t a = INT32_MAX, z = INT32_MIN;
+            range_minmax<int32_t>(reinterpret_cast<const int32_t*>(c.x), c.delta, item.start, item.end, a, z);
+            if (a <= z) {
+                mn = key_of(a, 4);
+                mx = key_of(z, 4);
+            }
         }
         mn = wave_min(mn);
         mx = wave_max(mx);
