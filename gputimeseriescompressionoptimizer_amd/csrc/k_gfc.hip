@@ -218,8 +218,30 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     if (threadIdx.x == 0) s_bad = 0;
     __syncthreads();
     const uint64_t P = prefix[chunk];
-    for (uint32_t i = threadIdx.x; i < (uint32_t)chunk_bytes; i += kBlock)
-        buf[i] = (P + i < payload_bytes) ? payload[P + i] : 0;
+    {
+        // the chunk's bytes as 32-bit words: aligned word loads joined by a
+        // byte funnel shift (an aligned word holding a payload byte never
+        // crosses a page); bytes past the payload read as zero
+        const uintptr_t a0 = reinterpret_cast<uintptr_t>(payload + P);
+        const uint32_t sh = (uint32_t)(a0 & 3);
+        const uint32_t* gw = reinterpret_cast<const uint32_t*>(a0 & ~(uintptr_t)3);
+        const uintptr_t pend = reinterpret_cast<uintptr_t>(payload) + payload_bytes;
+        auto gword = [&](uint32_t q) -> uint32_t {
+            return reinterpret_cast<uintptr_t>(gw + q) < pend ? gw[q] : 0u;
+        };
+        uint32_t* bw = reinterpret_cast<uint32_t*>(buf);
+        const uint32_t nwords = ((uint32_t)chunk_bytes + 3) / 4;
+        for (uint32_t k = threadIdx.x; k < nwords; k += kBlock) {
+            const uint32_t lo = gword(k), hi = sh ? gword(k + 1) : 0u;
+            uint32_t w = sh ? __builtin_amdgcn_alignbyte(hi, lo, sh) : lo;
+            const uint64_t b0 = P + 4ull * k;  // payload offset of the word's first byte
+            if (b0 + 4 > payload_bytes) {
+                const uint64_t keep = payload_bytes > b0 ? payload_bytes - b0 : 0;  // 0..3 valid bytes
+                w &= keep >= 4 ? ~0u : ((1u << (8 * keep)) - 1u);
+            }
+            bw[k] = w;
+        }
+    }
     __syncthreads();
     // header walk: one wave, lanes 0..31 read one nibble each
     if (threadIdx.x < kWave) {
@@ -233,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
                 const int code = nib & 7;
                 nb = dbl ? (code >= 2 ? code + 1 : code) : code;
             }
-            const int tot = wave_sum(nb);
+            const int tot = __builtin_amdgcn_readlane(wave_inclusive_scan(nb), 31);  // lanes 0..31
             if (lane == 0) sub_off[s] = off;
             off += 16 + tot;
         }
