@@ -114,9 +114,21 @@ def float_section(ctx, rows: int, steps: int):
                 del back
             ctx.sync()
             td = (time.perf_counter() - t0) / steps
+            # one more round trip with HIP events around every launch: the
+            # largest kernels' device time, in microseconds
+            ctx.profile(True)
+            ctx.profile_reset()
+            blob = tree.Compress(buf)
+            back = tree.Decompress(blob)
+            del back
+            ctx.sync()
+            ktab = ctx.kernel_table()
+            ctx.profile(False)
+            top = sorted(ktab.items(), key=lambda kv: -kv[1][0])[:6]
             out[f"{name} {tname}"] = {"compress_gbps": round(x.nbytes / tc / 1e9, 2),
                                       "decompress_gbps": round(x.nbytes / td / 1e9, 2),
-                                      "ratio": round(x.nbytes / blob.nbytes, 3), "bit_exact": exact}
+                                      "ratio": round(x.nbytes / blob.nbytes, 3), "bit_exact": exact,
+                                      "kernels_us": {k: round(v[0] * 1e3, 1) for k, v in top}}
             del blob
         del buf
     return out

@@ -56,18 +56,39 @@ template <typename U> struct GfcRows {
     uint32_t nbsum = 0;  // residual bytes of the thread's rows
 };
 
+// Whether thread t's 8 rows of the chunk at c0 are all inside the column
+// and 16-byte aligned at p, so that they move as whole 16-byte vectors.
+template <typename U>
+__device__ __forceinline__ bool gfc_rows_whole(const U* p, long long n, long long c0, int rows) {
+    const int r0 = threadIdx.x * kGfcRpt;
+    return r0 + kGfcRpt <= rows && c0 + r0 + kGfcRpt <= n && (reinterpret_cast<uintptr_t>(p + c0 + r0) & 15) == 0;
+}
+
 template <typename U>
 __device__ __forceinline__ void gfc_rows(const U* __restrict__ x, long long n, long long c0, int rows, U last_c,
                                          GfcRows<U>& g) {
     constexpr bool dbl = sizeof(U) == 8;
+    constexpr int V = 16 / (int)sizeof(U);
+    const int r0 = threadIdx.x * kGfcRpt;
+    U v8[kGfcRpt];
+    if (gfc_rows_whole(x, n, c0, rows)) {
+        const u32x4* s4 = reinterpret_cast<const u32x4*>(x + c0 + r0);
+#pragma unroll
+        for (int q = 0; q < kGfcRpt / V; q++) {
+            const u32x4 t = s4[q];
+            __builtin_memcpy(&v8[q * V], &t, 16);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kGfcRpt; k++) v8[k] = (r0 + k < rows && c0 + r0 + k < n) ? x[c0 + r0 + k] : U(0);
+    }
 #pragma unroll
     for (int k = 0; k < kGfcRpt; k++) {
-        const int r = threadIdx.x * kGfcRpt + k;
+        const int r = r0 + k;
         g.nbs[k] = 0;
         g.d[k] = 0;
         if (r < rows) {
-            const long long gi = c0 + r;
-            const U v = gi < n ? x[gi] : U(0);
+            const U v = v8[k];
             const U last = (r < 32) ? U(0) : last_c;
             U diff = (U)(v - last);
             const uint32_t sign = (uint32_t)(diff >> (8 * sizeof(U) - 1)) & 1u;
@@ -158,23 +179,73 @@ __global__ __launch_bounds__(kBlock) void k_gfc_encode(const U* __restrict__ x, 
     const int sub = threadIdx.x / 4;
     __shared__ uint32_t sub_excl[kGfcMaxPer / 32];
     if ((threadIdx.x & 3) == 0 && threadIdx.x * RPT < rows) sub_excl[sub] = (uint32_t)excl;
+    // the chunk is assembled by OR-ing each thread's bytes into zeroed words
+    // (neighbouring threads share the words at the ends of their bytes)
+    uint32_t* ow = reinterpret_cast<uint32_t*>(out);
+    constexpr uint32_t kOutWords = sizeof(out) / 4;
+    const uint32_t nwords = ((uint32_t)chunk_bytes + 7) / 4;  // one spare word for the funnel shift below
+    for (uint32_t k = threadIdx.x; k < nwords && k < kOutWords; k += kBlock) ow[k] = 0;
     __syncthreads();
+    auto or_word = [&](uint32_t at, uint32_t v) {
+        if (v) __hip_atomic_fetch_or(ow + at, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
     if (threadIdx.x * RPT < rows) {
         const uint32_t sub_off = 16u * sub + sub_excl[sub];
         const uint32_t h = sub_off + 4u * (threadIdx.x & 3);
-#pragma unroll
-        for (int j = 0; j < 4; j++) out[h + j] = (uint8_t)((nibs >> (8 * j)) & 0xff);
-        uint32_t pos = 16u * (sub + 1) + (uint32_t)excl;
+        const uint32_t hs = 8u * (h & 3);
+        or_word(h >> 2, nibs << hs);
+        if (hs) or_word((h >> 2) + 1, nibs >> (32 - hs));
+        // the rows' low bytes, LSB first, through a 64-bit accumulator that
+        // starts at the word holding the first byte
+        const uint32_t pos = 16u * (sub + 1) + (uint32_t)excl;
+        uint32_t wi = pos >> 2;
+        uint64_t acc = 0;
+        uint32_t nbits = 8u * (pos & 3);
+        auto push = [&](uint32_t v, uint32_t len) {  // len <= 32 bits, v < 2^len
+            acc |= (uint64_t)v << nbits;
+            nbits += len;
+            if (nbits >= 32) {
+                or_word(wi++, (uint32_t)acc);
+                acc >>= 32;
+                nbits -= 32;
+            }
+        };
 #pragma unroll
         for (int k = 0; k < RPT; k++) {
-            uint64_t dv = (uint64_t)d[k];
-            for (int b = 0; b < nbs[k]; b++) out[pos + b] = (uint8_t)(dv >> (8 * b));
-            pos += nbs[k];
+            const uint64_t dv = (uint64_t)d[k];
+            const uint32_t nb = (uint32_t)nbs[k];
+            const uint32_t lo = nb < 4 ? nb : 4u;
+            push((uint32_t)dv & (lo == 4 ? ~0u : (1u << (8 * lo)) - 1u), 8 * lo);
+            if (sizeof(U) == 8 && nb > 4) {
+                const uint32_t hb = nb - 4;
+                push((uint32_t)(dv >> 32) & (hb == 4 ? ~0u : (1u << (8 * hb)) - 1u), 8 * hb);
+            }
         }
+        if (nbits) or_word(wi, (uint32_t)acc);
     }
     __syncthreads();
+    // out at payload + O (any byte address): single bytes up to a 16-byte
+    // boundary, then 16-byte stores of funnel-shifted words, single bytes at
+    // the end; the chunks on either side own the other bytes of the edge
+    // vectors
     const uint64_t O = prefix[chunk];
-    for (uint32_t i = threadIdx.x; i < (uint32_t)chunk_bytes; i += kBlock) payload[O + i] = out[i];
+    uint8_t* dst = payload + O;
+    const int L = (int)chunk_bytes;
+    int head = (int)((16 - (reinterpret_cast<uintptr_t>(dst) & 15)) & 15);
+    head = head < L ? head : L;
+    if ((int)threadIdx.x < head) dst[threadIdx.x] = out[threadIdx.x];
+    const int r = head & 3;
+    auto src_word = [&](int s) -> uint32_t {  // chunk bytes [s, s + 4), s + 4 <= L
+        const int q = s >> 2;
+        return r ? __builtin_amdgcn_alignbyte(ow[q + 1], ow[q], r) : ow[q];
+    };
+    const int nq = (L - head) / 16;
+    u32x4* d4 = reinterpret_cast<u32x4*>(dst + head);
+    for (int i = threadIdx.x; i < nq; i += kBlock) {
+        const int s = head + 16 * i;
+        __builtin_nontemporal_store(u32x4{src_word(s), src_word(s + 4), src_word(s + 8), src_word(s + 12)}, d4 + i);
+    }
+    for (int j = head + 16 * nq + threadIdx.x; j < L; j += kBlock) dst[j] = out[j];
 }
 
 // Payload bytes of a chunk from the offsets table; out-of-range entries
@@ -207,7 +278,6 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     constexpr int kMaxBytes = kGfcMaxPer / 32 * (16 + 32 * (int)sizeof(U));
     __shared__ __attribute__((aligned(16))) uint8_t buf[kMaxBytes + 16];
     __shared__ uint32_t sub_off[kGfcMaxPer / 32 + 1];
-    __shared__ U vals[kGfcMaxPer];
     __shared__ U s_last;
     __shared__ int s_bad;
     const uint32_t chunk = blockIdx.x;
@@ -266,27 +336,43 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     }
     __syncthreads();
     const int sub = threadIdx.x / 4;
+    const uint32_t* bw = reinterpret_cast<const uint32_t*>(buf);
+    constexpr uint32_t kBufWords = sizeof(buf) / 4;
+    // the 4 bytes at LDS byte offset a (any alignment); words past the
+    // buffer (corrupt input only) read as zero
+    auto lds_word = [&](uint32_t a) -> uint32_t {
+        const uint32_t q = a >> 2, s = a & 3;
+        const uint32_t lo = q < kBufWords ? bw[q] : 0u;
+        if (!s) return lo;
+        const uint32_t hi = q + 1 < kBufWords ? bw[q + 1] : 0u;
+        return __builtin_amdgcn_alignbyte(hi, lo, s);
+    };
+    auto nb_of = [](uint32_t nib) -> uint32_t {
+        const uint32_t code = nib & 7;
+        return dbl ? (code >= 2 ? code + 1 : code) : code;
+    };
+    // this thread's rows are nibbles 8*(t%4) .. +7 of its subchunk's header:
+    // one header word; the rows of the threads before it in the subchunk
+    // come from their words by lane shuffles
+    const bool mine = threadIdx.x * RPT < rows;
+    const uint32_t so = mine ? sub_off[sub] : 0u;
+    const uint32_t hw = mine ? lds_word(so + 4u * (threadIdx.x & 3)) : 0u;
+    uint32_t own = 0;
+#pragma unroll
+    for (int k = 0; k < RPT; k++) own += nb_of(hw >> (4 * k));
+    const int j = threadIdx.x & 3;
+    const uint32_t up1 = __shfl_up(own, 1, 4), up2 = __shfl_up(own, 2, 4), up3 = __shfl_up(own, 3, 4);
+    const uint32_t pre = (j >= 1 ? up1 : 0u) + (j >= 2 ? up2 : 0u) + (j >= 3 ? up3 : 0u);
     U dec[RPT];
-    if (threadIdx.x * RPT < rows) {
-        const uint32_t so = sub_off[sub];
-        // nibbles of rows 0..31 of the subchunk; this thread's rows start at lane 8*(t%4)
-        int pre = 0;
-        const int first_lane = 8 * (threadIdx.x & 3);
-        for (int l = 0; l < first_lane; l++) {
-            const uint8_t byte = buf[so + (l >> 1)];
-            const int code = ((l & 1) ? (byte >> 4) : byte) & 7;
-            pre += dbl ? (code >= 2 ? code + 1 : code) : code;
-        }
+    if (mine) {
         uint32_t pos = so + 16 + pre;
 #pragma unroll
         for (int k = 0; k < RPT; k++) {
-            const int l = first_lane + k;
-            const uint8_t byte = buf[so + (l >> 1)];
-            const int nib = (l & 1) ? (byte >> 4) : (byte & 15);
-            const int code = nib & 7;
-            const int nb = dbl ? (code >= 2 ? code + 1 : code) : code;
-            uint64_t dv = 0;
-            for (int b = nb - 1; b >= 0; b--) dv = (dv << 8) | (pos + b < sizeof(buf) ? buf[pos + b] : 0);
+            const uint32_t nib = (hw >> (4 * k)) & 15;
+            const uint32_t nb = nb_of(nib);
+            uint64_t dv = lds_word(pos);
+            if (dbl && nb > 4) dv |= (uint64_t)lds_word(pos + 4) << 32;
+            dv &= nb >= 8 ? ~0ull : (1ull << (8 * nb)) - 1ull;
             pos += nb;
             U diff = (U)dv;
             if (nib & 8) diff = (U)(U(0) - diff);
@@ -296,14 +382,27 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     // row 31 of the chunk is the predictor of every later subchunk
     if (threadIdx.x == 3 && rows > 0) s_last = dec[RPT - 1];
     __syncthreads();
-    if (threadIdx.x * RPT < rows) {
+    if (mine) {
         const U last = sub == 0 ? U(0) : s_last;
+        U v8[RPT];
 #pragma unroll
-        for (int k = 0; k < RPT; k++) vals[threadIdx.x * RPT + k] = (U)(last + dec[k]);
+        for (int k = 0; k < RPT; k++) v8[k] = (U)(last + dec[k]);
+        const long long g0 = c0 + threadIdx.x * RPT;
+        if (gfc_rows_whole(out, n, c0, rows)) {
+            constexpr int V = 16 / (int)sizeof(U);
+            u32x4* d4 = reinterpret_cast<u32x4*>(out + g0);
+#pragma unroll
+            for (int q = 0; q < RPT / V; q++) {
+                u32x4 t;
+                __builtin_memcpy(&t, &v8[q * V], 16);
+                d4[q] = t;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < RPT; k++)
+                if (g0 + k < n) out[g0 + k] = v8[k];
+        }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < rows; i += kBlock)
-        if (c0 + i < n) out[c0 + i] = vals[i];
     if (threadIdx.x == 0 && s_bad) __hip_atomic_store(error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
