@@ -276,7 +276,11 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     constexpr int RPT = kGfcMaxPer / kBlock;
     constexpr bool dbl = sizeof(U) == 8;
     constexpr int kMaxBytes = kGfcMaxPer / 32 * (16 + 32 * (int)sizeof(U));
-    __shared__ __attribute__((aligned(16))) uint8_t buf[kMaxBytes + 16];
+    // a chunk at any 16-byte phase, in whole 16-byte vectors, plus one
+    // spare word for the funnel shift of the last row
+    constexpr int kVec = (kMaxBytes + 15 + 15) / 16;
+    constexpr int kVR = (kVec + kBlock - 1) / kBlock;
+    __shared__ __attribute__((aligned(16))) uint8_t buf[16 * kVec + 16];
     __shared__ uint32_t sub_off[kGfcMaxPer / 32 + 1];
     __shared__ U s_last;
     __shared__ int s_bad;
@@ -288,29 +292,29 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     if (threadIdx.x == 0) s_bad = 0;
     __syncthreads();
     const uint64_t P = prefix[chunk];
+    // the aligned 16-byte vectors that hold the chunk's bytes, all loads in
+    // flight together; chunk byte i lands at buf[base + i].  A vector that
+    // starts before the payload's end never crosses a page; its bytes past
+    // the payload are never decoded (a chunk that claims them is an error).
+    const uintptr_t a0 = reinterpret_cast<uintptr_t>(payload + P);
+    const uint32_t base = (uint32_t)(a0 & 15);
     {
-        // the chunk's bytes as 32-bit words: aligned word loads joined by a
-        // byte funnel shift (an aligned word holding a payload byte never
-        // crosses a page); bytes past the payload read as zero
-        const uintptr_t a0 = reinterpret_cast<uintptr_t>(payload + P);
-        const uint32_t sh = (uint32_t)(a0 & 3);
-        const uint32_t* gw = reinterpret_cast<const uint32_t*>(a0 & ~(uintptr_t)3);
+        const u32x4* gv = reinterpret_cast<const u32x4*>(a0 - base);
         const uintptr_t pend = reinterpret_cast<uintptr_t>(payload) + payload_bytes;
-        auto gword = [&](uint32_t q) -> uint32_t {
-            return reinterpret_cast<uintptr_t>(gw + q) < pend ? gw[q] : 0u;
-        };
-        uint32_t* bw = reinterpret_cast<uint32_t*>(buf);
-        const uint32_t nwords = ((uint32_t)chunk_bytes + 3) / 4;
-        for (uint32_t k = threadIdx.x; k < nwords; k += kBlock) {
-            const uint32_t lo = gword(k), hi = sh ? gword(k + 1) : 0u;
-            uint32_t w = sh ? __builtin_amdgcn_alignbyte(hi, lo, sh) : lo;
-            const uint64_t b0 = P + 4ull * k;  // payload offset of the word's first byte
-            if (b0 + 4 > payload_bytes) {
-                const uint64_t keep = payload_bytes > b0 ? payload_bytes - b0 : 0;  // 0..3 valid bytes
-                w &= keep >= 4 ? ~0u : ((1u << (8 * keep)) - 1u);
-            }
-            bw[k] = w;
+        const uint32_t nvec = (base + (uint32_t)chunk_bytes + 15) / 16;
+        u32x4 t[kVR];
+#pragma unroll
+        for (int r = 0; r < kVR; r++) {
+            const uint32_t k = threadIdx.x + r * kBlock;
+            t[r] = (k < nvec && reinterpret_cast<uintptr_t>(gv + k) < pend) ? gv[k] : u32x4{0u, 0u, 0u, 0u};
         }
+        u32x4* b4 = reinterpret_cast<u32x4*>(buf);
+#pragma unroll
+        for (int r = 0; r < kVR; r++) {
+            const uint32_t k = threadIdx.x + r * kBlock;
+            if (k < nvec) b4[k] = t[r];
+        }
+        if (threadIdx.x == 0 && P + chunk_bytes > payload_bytes) s_bad = 1;
     }
     __syncthreads();
     // header walk: one wave, lanes 0..31 read one nibble each
@@ -320,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
         for (int s = 0; s < nsub; s++) {
             int nb = 0;
             if (lane < 32 && off + 16 <= chunk_bytes) {
-                const uint8_t byte = buf[off + (lane >> 1)];
+                const uint8_t byte = buf[base + off + (lane >> 1)];
                 const int nib = (lane & 1) ? (byte >> 4) : (byte & 15);
                 const int code = nib & 7;
                 nb = dbl ? (code >= 2 ? code + 1 : code) : code;
@@ -338,9 +342,10 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     const int sub = threadIdx.x / 4;
     const uint32_t* bw = reinterpret_cast<const uint32_t*>(buf);
     constexpr uint32_t kBufWords = sizeof(buf) / 4;
-    // the 4 bytes at LDS byte offset a (any alignment); words past the
-    // buffer (corrupt input only) read as zero
+    // chunk bytes [a, a + 4); words past the buffer (corrupt input only)
+    // read as zero
     auto lds_word = [&](uint32_t a) -> uint32_t {
+        a += base;
         const uint32_t q = a >> 2, s = a & 3;
         const uint32_t lo = q < kBufWords ? bw[q] : 0u;
         if (!s) return lo;
