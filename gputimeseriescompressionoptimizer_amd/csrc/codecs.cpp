@@ -663,7 +663,7 @@ Part float_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
 // -------------------------------------------------------------------- gfc
 // GfcEncoding (compression/gfc/gfc_encoding.cpp:16-166): float/double; the
 // integer types go through AFL plus an empty third output.
-std::vector<Part> gfc_encode_any(Ctx* c, int dtype, DView in) {
+std::vector<Part> gfc_encode_any(Ctx* c, int dtype, DView in, bool defer) {
     if (dtype != DDJ_D_FLOAT && dtype != DDJ_D_DOUBLE) {
         auto r = afl_encode_int(c, dtype, in);
         r.push_back(empty());
@@ -673,9 +673,27 @@ std::vector<Part> gfc_encode_any(Ctx* c, int dtype, DView in) {
     if (n == 0) return {empty(), empty(), empty()};
     if (n > (size_t)std::numeric_limits<int32_t>::max() / 9) fail(DDJ_EUNSUPPORTED, "GFC offsets are int32");
     Part x = aligned_view(c, in, es);
-    DevBuf meta, offs, pay;
-    gfc_encode(c, dtype == DDJ_D_DOUBLE, x.p, n, meta, offs, pay);
-    return {own(std::move(meta)), own(std::move(offs)), own(std::move(pay))};
+    DevBuf meta, offs;
+    if (!defer) {
+        DevBuf pay;
+        gfc_encode(c, dtype == DDJ_D_DOUBLE, x.p, n, meta, offs, pay);
+        return {own(std::move(meta)), own(std::move(offs)), own(std::move(pay))};
+    }
+    // the payload's size is known after the size pass; its chunks are
+    // written straight into the blob
+    GfcPlan plan = gfc_plan(c, dtype == DDJ_D_DOUBLE, x.p, n, meta, offs);
+    Part data;
+    data.n = plan.payload_bytes;
+    auto job = std::make_shared<FlJob>();
+    job->dtype = dtype;
+    job->n = n;
+    job->x = x;
+    job->write = [plan](Ctx* c, uint8_t* out, size_t bytes) {
+        if (bytes != plan.payload_bytes) fail(DDJ_EHIP, "GFC payload size mismatch");
+        gfc_write(c, plan, out);
+    };
+    data.fl = std::move(job);
+    return {own(std::move(meta)), own(std::move(offs)), data};
 }
 Part gfc_decode_any(Ctx* c, int dtype, const std::vector<DView>& in) {
     if (dtype != DDJ_D_FLOAT && dtype != DDJ_D_DOUBLE) return afl_decode_int(c, dtype, in);
@@ -746,6 +764,10 @@ void run_split_job(Ctx* c, const SplitJob& j, void* no_rows) {
 }
 
 void run_fl_job(Ctx* c, const FlJob& j, void* out, size_t bytes) {
+    if (j.write) {
+        if (bytes) j.write(c, static_cast<uint8_t*>(out), bytes);
+        return;
+    }
     const size_t es = dtype_size(j.dtype);
     if (j.n == 0 || bytes == 0) return;
     if (bytes != fl_bytes(es, j.bits, j.n)) fail(DDJ_EHIP, "FL job size mismatch");
@@ -802,7 +824,7 @@ std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResul
     case DDJ_ENC_PATCH: return patch_encode(c, dtype, in, false);
     case DDJ_ENC_OUTSIDE_PATCH: return patch_encode(c, dtype, in, true);
     case DDJ_ENC_FLOAT_TO_INT: return float_encode(c, dtype, in);
-    case DDJ_ENC_GFC: return gfc_encode_any(c, dtype, in);
+    case DDJ_ENC_GFC: return gfc_encode_any(c, dtype, in, defer_fl);
     case DDJ_ENC_UNIQUE: fail(DDJ_EUNSUPPORTED, "standalone unique encoding is not on the planner's path");
     default: fail(DDJ_EINVAL, "unknown encoding " + std::to_string(enc));
     }

@@ -2,6 +2,7 @@
 // orchestrated on the host, computed by the HIP kernels.
 #pragma once
 
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -37,13 +38,16 @@ struct Part {
     DView view() const { return DView{p, n}; }
 };
 // An FL pack whose bit width is known (its min/max pass and read-back ran)
-// and whose output goes straight into a blob being assembled.
+// and whose output goes straight into a blob being assembled; or another
+// stream of known size written the same way (`write`).
 struct FlJob {
     int dtype = 0;
     bool delta = false;  // value i is x[i+1] - x[i]; x holds n + 1 rows
     int bits = 0;
     size_t n = 0;  // values packed
     Part x;        // the rows, kept alive until the pack runs
+    // set: writes the stream itself (a GFC payload) instead of an FL pack
+    std::function<void(Ctx*, uint8_t*, size_t)> write;
 };
 // Runs the pack into the `bytes` stream bytes at out (any byte address).
 void run_fl_job(Ctx* c, const FlJob& j, void* out, size_t bytes);
@@ -70,8 +74,9 @@ bool enc_ok(int enc);
 
 // Encoding::Encode: out[0] = metadata, out[1..n_results(enc)] = outputs.
 // `minmax` (optional, device): the input's min/max keys, when known.
-// `defer_fl`: an integer AFL encoder returns its data as an FlJob part (for
-// a blob being assembled, when the data goes to the blob unchanged).
+// `defer_fl`: an integer AFL encoder returns its data as an FlJob part, a
+// float GFC encoder its payload (for a blob being assembled, when that
+// output goes to the blob unchanged).
 std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResult* minmax = nullptr,
                          bool defer_fl = false, const MinMaxResult* minmax_host = nullptr, bool defer_split = false);
 // Encoding::Decode: in[0] = metadata, in[1..] = children outputs.

@@ -413,35 +413,54 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
 
 }  // namespace
 
-void gfc_encode(Ctx* c, bool dbl, const void* x, size_t n, DevBuf& meta, DevBuf& offsets, DevBuf& payload) {
+GfcPlan gfc_plan(Ctx* c, bool dbl, const void* x, size_t n, DevBuf& meta, DevBuf& offsets) {
     const GfcGeom g = gfc_geom(dbl, (long long)n);
-    const size_t tiles = (size_t)g.Wc;
-    TileScan ts(c, tiles);
-    offsets = DevBuf(c, 4 * tiles);
-    const size_t worst = (size_t)(g.N / 32) * (16 + 32 * (dbl ? 8 : 4));
-    payload = DevBuf(c, worst);
+    GfcPlan p;
+    p.dbl = dbl;
+    p.x = x;
+    p.n = n;
+    p.N = g.N;
+    p.per = g.per;
+    p.chunks = (size_t)g.Wc;
+    p.scan = std::make_shared<TileScan>(c, p.chunks);
+    offsets = DevBuf(c, 4 * p.chunks);
     auto run = [&](auto tag) {
         using U = decltype(tag);
-        const U* xs = static_cast<const U*>(x);
-        kernel_bytes(c, "k_gfc_sizes", (double)n * sizeof(U) + 12.0 * (double)tiles);
-        { KernelTimer kt_(c, "k_gfc_sizes"); hipLaunchKernelGGL(k_gfc_sizes<U>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                           xs, g.n, g.N, g.per, (int)g.S, ts.vals, offsets.get<int32_t>()); }
-        HIPC(hipGetLastError());
-        ts.scan(c);
-        { KernelTimer kt_(c, "k_gfc_encode"); hipLaunchKernelGGL(k_gfc_encode<U>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                           xs, g.n, g.N, g.per, payload.get<uint8_t>(), (const uint64_t*)ts.prefix); }
+        kernel_bytes(c, "k_gfc_sizes", (double)n * sizeof(U) + 12.0 * (double)p.chunks);
+        { KernelTimer kt_(c, "k_gfc_sizes"); hipLaunchKernelGGL(k_gfc_sizes<U>, dim3((unsigned)p.chunks), dim3(kBlock), 0, c->stream,
+                           static_cast<const U*>(x), g.n, g.N, g.per, (int)g.S, p.scan->vals, offsets.get<int32_t>()); }
         HIPC(hipGetLastError());
     };
     if (dbl)
         run(uint64_t());
     else
         run(uint32_t());
+    p.scan->scan(c);
     int32_t h_meta[3] = {(int32_t)g.blocks, (int32_t)g.wpb, (int32_t)n};
     meta = make_buf(c, h_meta, 12);
-    uint64_t tot = 0;
-    d2h(c, &tot, ts.prefix + tiles, 8);
-    payload.set_size(tot);
-    kernel_bytes(c, "k_gfc_encode", (double)n * (dbl ? 8 : 4) + (double)tot);
+    d2h(c, &p.payload_bytes, p.scan->prefix + p.chunks, 8);
+    return p;
+}
+
+void gfc_write(Ctx* c, const GfcPlan& p, uint8_t* payload) {
+    auto run = [&](auto tag) {
+        using U = decltype(tag);
+        kernel_bytes(c, "k_gfc_encode", (double)p.n * sizeof(U) + (double)p.payload_bytes);
+        { KernelTimer kt_(c, "k_gfc_encode"); hipLaunchKernelGGL(k_gfc_encode<U>, dim3((unsigned)p.chunks), dim3(kBlock), 0, c->stream,
+                           static_cast<const U*>(p.x), (long long)p.n, p.N, p.per, payload, (const uint64_t*)p.scan->prefix); }
+        HIPC(hipGetLastError());
+    };
+    if (p.dbl)
+        run(uint64_t());
+    else
+        run(uint32_t());
+}
+
+void gfc_encode(Ctx* c, bool dbl, const void* x, size_t n, DevBuf& meta, DevBuf& offsets, DevBuf& payload) {
+    const GfcPlan p = gfc_plan(c, dbl, x, n, meta, offsets);
+    payload = DevBuf(c, std::max<size_t>(p.payload_bytes, 1));
+    payload.set_size(p.payload_bytes);
+    gfc_write(c, p, payload.get<uint8_t>());
 }
 
 void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const uint8_t* offsets, size_t off_bytes,

@@ -146,6 +146,21 @@ void topk_most_frequent(Ctx* c, int dtype, const void* x, size_t n, int K, uint6
                         int* n_distinct_host);
 
 // GFC (float: 32-bit words, double: 64-bit words).
+// A GFC encode whose size pass ran: every chunk's place in the payload and
+// the payload's size are known, the chunks are not written yet.
+struct GfcPlan {
+    bool dbl = false;
+    const void* x = nullptr;
+    size_t n = 0;
+    long long N = 0, per = 0;
+    size_t chunks = 0;
+    std::shared_ptr<TileScan> scan;  // prefix[chunk]: the chunk's payload offset
+    uint64_t payload_bytes = 0;
+};
+GfcPlan gfc_plan(Ctx* c, bool dbl, const void* x, size_t n, DevBuf& meta, DevBuf& offsets);
+// Writes the planned chunks at payload (payload_bytes bytes, any byte address).
+void gfc_write(Ctx* c, const GfcPlan& p, uint8_t* payload);
+// gfc_plan, then gfc_write into a payload buffer of its own.
 void gfc_encode(Ctx* c, bool dbl, const void* x, size_t n, DevBuf& meta, DevBuf& offsets, DevBuf& payload);
 void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const uint8_t* offsets, size_t off_bytes,
                 const uint8_t* payload, size_t payload_bytes, DevBuf& out);

@@ -88,10 +88,13 @@ size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& 
         child_pre = {f[1], f[2]};
     } else {
         // an integer AFL whose data goes to the blob unchanged packs it there
-        // when the blob is assembled; so does a dictionary split whose two
-        // outputs are stored unchanged (its writing pass runs then)
-        const bool defer = node.enc == DDJ_ENC_AFL && is_int_type(node.dtype) && output_is_leaf(node);
+        // when the blob is assembled; a dictionary split whose two outputs
+        // are stored unchanged runs its writing pass then
         auto none_leaf = [](const NodeP& ch) { return ch->enc == DDJ_ENC_NONE && ch->ch.empty(); };
+        // so does a float GFC payload stored unchanged
+        const bool gfc_leaf = node.enc == DDJ_ENC_GFC && !is_int_type(node.dtype) &&
+                              (node.ch.empty() || (node.ch.size() == 2 && none_leaf(node.ch[1])));
+        const bool defer = (node.enc == DDJ_ENC_AFL && is_int_type(node.dtype) && output_is_leaf(node)) || gfc_leaf;
         const bool defer_split =
             node.enc == DDJ_ENC_DICT && node.ch.size() == 2 && none_leaf(node.ch[0]) && none_leaf(node.ch[1]);
         res = encode(c, node.enc, node.dtype, data.view(), data.minmax ? data.minmax->get<MinMaxResult>() : nullptr,

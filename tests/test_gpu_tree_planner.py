@@ -247,3 +247,14 @@ def test_planner_nodes_match_oracle(ctx, name, gen, dtype):
     """Every intermediate buffer of the floating-point planner cases: same
     statistics, same encoder outputs."""
     _walk_planner_nodes(ctx, O.as_bytes(gen()), O.NONE, dtype, 0, [])
+
+
+@pytest.mark.parametrize("dtype", [g.D_FLOAT, g.D_DOUBLE])
+def test_gfc_payload_written_in_place(ctx, dtype):
+    """Float GFC payloads stored unchanged are encoded straight into the blob
+    at whatever byte offset the pieces before them leave (a patch stencil of
+    odd length before them); blobs equal the oracle's."""
+    for n in (1, 33, 4097, 70_001):
+        x = column(dtype, n, seed=n + 3)
+        check_tree(ctx, [g.GFC, g.NONE, g.NONE], dtype, x)
+        check_tree(ctx, [g.PATCH, g.GFC, g.NONE, g.NONE, g.GFC, g.NONE, g.NONE], dtype, x)
