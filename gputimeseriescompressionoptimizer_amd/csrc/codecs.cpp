@@ -595,13 +595,12 @@ std::vector<Part> float_encode(Ctx* c, int dtype, DView in) {
     int precision = 0;
     MinMaxResult mmr;
     if (dtype == DDJ_D_FLOAT || dtype == DDJ_D_DOUBLE) {
-        DevBuf s(c, sizeof(DevStats));
-        launch_stats(c, dtype, x.p, n, s.get<DevStats>());
-        DevStats h;
+        DevBuf s(c, sizeof(FloatScanResult));
+        launch_float_scan(c, dtype, x.p, n, s.get<FloatScanResult>());
+        FloatScanResult h;
         d2h(c, &h, s.get(), sizeof h);
         precision = (int)h.precision;
-        mmr.min_key = h.min_key;
-        mmr.max_key = h.max_key;
+        mmr = h.mm;
     } else {
         mmr = minmax(c, dtype, x.p, n).host;  // float(integer) always has precision 0
     }

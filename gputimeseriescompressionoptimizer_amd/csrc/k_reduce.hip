@@ -230,23 +230,30 @@ template <typename T, int STRIDE>
 __global__ __launch_bounds__(kBlock) void k_minmax_fold(const uint64_t* __restrict__ partials, int blocks, const T* x,
                                                         MinMaxResult* out) {
     __shared__ uint64_t sm[kBlock / kWave];
-    uint64_t mn = ~0ull, mx = 0, zf = ~0ull;
+    uint64_t mn = ~0ull, mx = 0, zf = ~0ull, pr = 0;
     for (int b = threadIdx.x; b < blocks; b += kBlock) {
         const uint64_t a = partials[STRIDE * b], z = partials[STRIDE * b + 1];
         mn = a < mn ? a : mn;
         mx = z > mx ? z : mx;
-        if (STRIDE == 3) {
+        if (STRIDE >= 3) {
             const uint64_t q = partials[STRIDE * b + 2];
             zf = q < zf ? q : zf;
+        }
+        if (STRIDE == 4) {
+            const uint64_t q = partials[STRIDE * b + 3];
+            pr = q > pr ? q : pr;
         }
     }
     mn = block_reduce_min(mn, sm);
     mx = block_reduce_max(mx, sm);
-    if (STRIDE == 3) zf = block_reduce_min(zf, sm);
+    if (STRIDE >= 3) zf = block_reduce_min(zf, sm);
+    if (STRIDE == 4) pr = block_reduce_max(pr, sm);
     if (threadIdx.x == 0) {
-        if (STRIDE == 3) fold_fixup(x, mn, mx, zf);
+        if (STRIDE >= 3) fold_fixup(x, mn, mx, zf);
         out->min_key = mn;
         out->max_key = mx;
+        // out is the first member of a FloatScanResult
+        if (STRIDE == 4) reinterpret_cast<FloatScanResult*>(out)->precision = pr;
     }
 }
 
@@ -263,6 +270,60 @@ __device__ __forceinline__ uint32_t float_precision(float number) {
         k++;
     }
     return k;
+}
+
+// Min/max keys and the float precision (the maximum of float_precision
+// over the rows, as the statistics pass computes it) in one pass: the two
+// inputs of the floatToInt encoder (float_to_int_encoding.cu).  Partials:
+// {min, max, first zero, precision} per workgroup.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_minmax_prec(const T* __restrict__ x, size_t n,
+                                                        uint64_t* __restrict__ partials) {
+    __shared__ uint64_t sm[kBlock / kWave];
+    FoldKeys<T> f;
+    uint32_t prec = 0;
+    auto add = [&](T v, size_t row) {
+        f.add(v, row);
+        const uint32_t p = float_precision((float)v);
+        prec = p > prec ? p : prec;
+    };
+    const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x, stride = (size_t)gridDim.x * kBlock;
+    constexpr int V = 16 / sizeof(T);
+    size_t done = 0;
+    if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+        const u32x4* xv = reinterpret_cast<const u32x4*>(x);
+        const size_t nv = n / V;
+        constexpr int U = 4;  // loads in flight per thread
+        for (size_t i0 = tid; i0 < nv; i0 += U * stride) {
+            u32x4 v[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const size_t i = i0 + u * stride;
+                if (i < nv) v[u] = __builtin_nontemporal_load(xv + i);
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const size_t i = i0 + u * stride;
+                if (i >= nv) break;
+                T vals[V];
+                __builtin_memcpy(vals, &v[u], 16);
+#pragma unroll
+                for (int k = 0; k < V; k++) add(vals[k], i * V + k);
+            }
+        }
+        done = nv * V;
+    }
+    for (size_t i = done + tid; i < n; i += stride) add(x[i], i);
+    const uint64_t mn = block_reduce_min(f.min_key(), sm);
+    const uint64_t mx = block_reduce_max(f.max_key(), sm);
+    const uint64_t zf = block_reduce_min(f.zf, sm);
+    const uint64_t pr = block_reduce_max((uint64_t)prec, sm);
+    if (threadIdx.x == 0) {
+        st_agent(partials + 4 * blockIdx.x, mn);
+        st_agent(partials + 4 * blockIdx.x + 1, mx);
+        st_agent(partials + 4 * blockIdx.x + 2, zf);
+        st_agent(partials + 4 * blockIdx.x + 3, pr);
+    }
 }
 
 template <typename T> __device__ __forceinline__ int32_t to_int32_sat(T v) {
@@ -495,6 +556,26 @@ void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_r
         { KernelTimer kt_(c, "k_minmax_fold"); hipLaunchKernelGGL((k_minmax_fold<T, 3>), dim3(1), dim3(kBlock), 0, c->stream,
                            partials.get<uint64_t>(), grid, static_cast<const T*>(x), d_result); }
         HIPC(hipGetLastError());
+    });
+}
+
+void launch_float_scan(Ctx* c, int dtype, const void* x, size_t n, FloatScanResult* d_result) {
+    if (n == 0) fail(DDJ_EINVAL, "float scan of an empty array");
+    dispatch_native(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        if constexpr (std::is_floating_point<T>::value) {
+            const int grid = reduce_grid(c, n, sizeof(T));
+            DevBuf partials(c, 32 * (size_t)grid);
+            kernel_bytes(c, "k_minmax_prec", (double)n * sizeof(T));
+            { KernelTimer kt_(c, "k_minmax_prec"); hipLaunchKernelGGL(k_minmax_prec<T>, dim3(grid), dim3(kBlock), 0, c->stream,
+                               static_cast<const T*>(x), n, partials.get<uint64_t>()); }
+            HIPC(hipGetLastError());
+            { KernelTimer kt_(c, "k_minmax_fold"); hipLaunchKernelGGL((k_minmax_fold<T, 4>), dim3(1), dim3(kBlock), 0, c->stream,
+                               partials.get<uint64_t>(), grid, static_cast<const T*>(x), &d_result->mm); }
+            HIPC(hipGetLastError());
+        } else {
+            fail(DDJ_EINVAL, "float scan of an integer column");
+        }
     });
 }
 

@@ -34,6 +34,14 @@ struct MinMaxResult {
     uint64_t max_key;  // order_key of the maximum
 };
 
+// The floatToInt encoder's inputs for a float / double column: min/max keys
+// and the float precision (maximum over the rows, as DevStats::precision).
+struct FloatScanResult {
+    MinMaxResult mm;
+    uint64_t precision;
+};
+void launch_float_scan(Ctx* c, int dtype, const void* x, size_t n, FloatScanResult* d_result);
+
 // min/max of n values of `dtype` into d_result (device).  One launch.
 void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_result,
                    const uint64_t* d_rows = nullptr /* device row count (<= n) read by the kernel, or null */);
