@@ -100,10 +100,26 @@ __global__ __launch_bounds__(kBlock) void k_float_to_int(const T* __restrict__ x
                                                          int32_t* __restrict__ out) {
     const double f = kPow10[p];
     const size_t stride = (size_t)gridDim.x * kBlock;
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        double prod = __dmul_rn((double)x[i], f);
-        out[i] = (int32_t)(long)__builtin_rint(prod);
+    auto conv = [&](T v) -> int32_t { return (int32_t)(long)__builtin_rint(__dmul_rn((double)v, f)); };
+    // 4 rows per thread and step: 16-byte loads and stores
+    // (4- and 8-byte T; narrower T takes the row loop below)
+    constexpr int L = sizeof(T) >= 4 ? (int)sizeof(T) / 4 : 1;  // 16-byte loads per 4 rows
+    const bool vec = sizeof(T) >= 4 && ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    const size_t n4 = vec ? n / 4 : 0;
+    for (size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x; g < n4; g += stride) {
+        u32x4 a[L];
+#pragma unroll
+        for (int l = 0; l < L; l++) a[l] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(x + 4 * g) + l);
+        T v[4] = {};
+        if constexpr (sizeof(T) >= 4) __builtin_memcpy(v, a, 16 * L);
+        int32_t r[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) r[k] = conv(v[k]);
+        u32x4 o;
+        __builtin_memcpy(&o, r, 16);
+        __builtin_nontemporal_store(o, reinterpret_cast<u32x4*>(out + 4 * g));
     }
+    for (size_t i = 4 * n4 + (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) out[i] = conv(x[i]);
 }
 
 // IntegerToFloatingPointOperator (transform_operators.hpp:121-131): (T)x / 10^p in double.
@@ -112,10 +128,25 @@ __global__ __launch_bounds__(kBlock) void k_int_to_float(const int32_t* __restri
                                                          T* __restrict__ out) {
     const double f = kPow10[p];
     const size_t stride = (size_t)gridDim.x * kBlock;
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const int32_t v = x[i];
-        out[i] = (T)__ddiv_rn((double)(T)v, f);
+    auto conv = [&](int32_t v) -> T { return (T)__ddiv_rn((double)(T)v, f); };
+    // (4-byte T: measured slower for double, whose row loop already stores
+    // 8 bytes per row)
+    constexpr int L = sizeof(T) >= 4 ? (int)sizeof(T) / 4 : 1;  // 16-byte stores per 4 rows
+    const bool vec = sizeof(T) == 4 && ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
+    const size_t n4 = vec ? n / 4 : 0;
+    for (size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x; g < n4; g += stride) {
+        const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(x + 4 * g));
+        int32_t v[4];
+        __builtin_memcpy(v, &a, 16);
+        T r[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) r[k] = conv(v[k]);
+        u32x4 o[L] = {};
+        if constexpr (sizeof(T) >= 4) __builtin_memcpy(o, r, 16 * L);
+#pragma unroll
+        for (int l = 0; l < L; l++) __builtin_nontemporal_store(o[l], reinterpret_cast<u32x4*>(out + 4 * g) + l);
     }
+    for (size_t i = 4 * n4 + (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) out[i] = conv(x[i]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_fill_zero(uint32_t* p, size_t words) {
@@ -408,7 +439,7 @@ void launch_float_to_int(Ctx* c, int dtype, const void* x, size_t n, int precisi
     if (!n) return;
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
-        { KernelTimer kt_(c, "k_float_to_int"); hipLaunchKernelGGL(k_float_to_int<T>, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream,
+        { KernelTimer kt_(c, "k_float_to_int"); hipLaunchKernelGGL(k_float_to_int<T>, dim3(grid_for(c, n, 4)), dim3(kBlock), 0, c->stream,
                            static_cast<const T*>(x), n, precision, out); }
         HIPC(hipGetLastError());
     });
@@ -418,7 +449,7 @@ void launch_int_to_float(Ctx* c, int dtype, const uint8_t* x, size_t n, int prec
     if (!n) return;
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
-        { KernelTimer kt_(c, "k_int_to_float"); hipLaunchKernelGGL(k_int_to_float<T>, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream,
+        { KernelTimer kt_(c, "k_int_to_float"); hipLaunchKernelGGL(k_int_to_float<T>, dim3(grid_for(c, n, 4)), dim3(kBlock), 0, c->stream,
                            reinterpret_cast<const int32_t*>(x), n, precision, static_cast<T*>(out)); }
         HIPC(hipGetLastError());
     });
