@@ -7,10 +7,11 @@ of the four blobs is timed separately with the same step count.  The
 steps behind `value` run without instrumentation; the same number of steps
 is then run again with HIP events around every launch on the library's
 stream, and the dominant kernel of that compress region is set against the
-8 TB/s HBM peak (its average launch time, its algorithmic bytes); the CPU
-oracle (the
-reference's codecs restated on the host, one thread) compresses a bounded
-prefix of the same columns as the CPU baseline.
+8 TB/s HBM peak (its average launch time, its algorithmic bytes).  CPU
+baselines: the oracle (the reference's codecs restated on the host)
+compresses a bounded prefix of the same columns on one and on all host
+threads, and the reference's only CPU path, its FL host loops, packs the
+1M-row plumbing column and the 64M timestamp deltas.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--rows N]
 """
@@ -59,23 +60,28 @@ def columns(rows: int):
     ]
 
 
-def cpu_baseline(cols, sample_rows: int, budget_s: float = 15.0):
-    """The oracle (reference codecs restated in C++, one thread) compressing the
-    first `sample_rows` rows of each column with the same trees."""
-    from oracle import oracle as O
+def cpu_threads() -> int:
+    """Host threads for the all-cores CPU baselines: the process's CPU
+    affinity, at most 16 (one GPU's share of its host)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
 
-    trees = [(O.tree_from_path(path, O.D_INT), x[:sample_rows]) for _, x, path in cols]
-    nbytes = sum(x.nbytes for _, x in trees)
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        for t, x in trees:
-            O.compress(t, x)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s:
-            break
-    return {"value": round(nbytes * reps / el / 1e9, 4), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"first {sample_rows} rows of each of the 4 columns, {reps} passes, {el:.1f} s"}
+
+def cpu_baseline(cols, sample_rows: int, budget_s: float = 10.0):
+    """The oracle (the reference's codecs restated in C++) compressing the
+    first `sample_rows` rows of each column with the same trees: on one
+    thread, and on all host threads (each column sample cut into one part
+    per thread, as the column API's batches are independent)."""
+    from oracle import cpu_baseline as CB
+
+    columns = [(path, x[:sample_rows]) for _, x, path in cols]
+    v1, reps1, el1 = CB.oracle_trees(columns, budget_s / 2, 1)
+    nt = cpu_threads()
+    vn, repsn, eln = CB.oracle_trees(columns, budget_s / 2, nt)
+    return {"value": round(vn / 1e9, 4), "unit": "GB/s", "cores": nt, "kind": "port",
+            "value_1_core": round(v1 / 1e9, 4),
+            "sample": f"first {sample_rows} rows of each of the 4 columns through the same trees: "
+                      f"{reps1} passes in {el1:.1f} s on 1 thread, {repsn} passes in {eln:.1f} s on {nt} threads",
+            "host": CB.host_cores()}
 
 
 def float_section(ctx, rows: int, steps: int):
@@ -211,41 +217,50 @@ def batched_section(ctx, n_cols: int, min_rows: int, max_rows: int, workers: int
 def fl_plumbing_section(ctx):
     """The reference's CPU-runnable case: 1,000,000 int32 uniform in [0,1024)
     FL-packed to 10 bits and unpacked, on the GPU and with the reference's
-    own host loops (run_afl_compress_cpu<int,1>, compiled from its sources
-    into oracle/_ref when they were available at build time), one thread."""
-    import ctypes
+    only CPU path (its FL host loops run_afl_compress_cpu<int,1> /
+    run_afl_decompress_cpu<int,1>, restated and built with g++ -O3 in
+    oracle/) on one host thread and on all host threads.  The same for the
+    FL stage of the 64M-row timestamp column (its deltas at their width)."""
     import numpy as np
     import gputimeseriescompressionoptimizer_amd as g
     from gputimeseriescompressionoptimizer_amd import datasets as D
+    from oracle import cpu_baseline as CB
 
-    x = D.uniform_int(1_000_000, 1024, seed=0)
-    buf = ctx.upload(x)
-    packed = ctx.fl_pack(buf, g.D_INT, 10, x.size)
-    exact = np.array_equal(ctx.fl_unpack(packed, g.D_INT, 10, x.size).to_numpy(np.int32), x)
-    reps = 200
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        packed = ctx.fl_pack(buf, g.D_INT, 10, x.size)
-    ctx.sync()
-    t_gpu = (time.perf_counter() - t0) / reps
-    out = {"rows": int(x.size), "bits": 10, "gpu_pack_gbps": round(x.nbytes / t_gpu / 1e9, 2), "round_trip_exact": exact}
-    ref = os.path.join(ROOT, "oracle", "_ref", "libddj_ref.so")
-    if os.path.exists(ref):
-        L = ctypes.CDLL(ref)
-        L.ref_fl_pack.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_ulong, ctypes.c_void_p,
-                                  ctypes.c_ulong]
-        words = (10 * x.size + 31) // 32
-        o = np.zeros(words, np.int32)
-        n_ref, t0 = 0, time.perf_counter()
-        while time.perf_counter() - t0 < 3.0:
-            L.ref_fl_pack(4, 10, x.ctypes.data, x.size, o.ctypes.data, words)
-            n_ref += 1
-        t_ref = (time.perf_counter() - t0) / n_ref
-        out["reference_cpu_pack_gbps"] = round(x.nbytes / t_ref / 1e9, 3)
-        out["reference_cpu_cores"] = 1
-        out["stream_matches_reference"] = o.tobytes() == packed.to_bytes()
-    return out
+    nt = cpu_threads()
+
+    def one(x, bits, reps_gpu, reps_cpu):
+        buf = ctx.upload(x)
+        packed = ctx.fl_pack(buf, g.D_INT, bits, x.size)
+        exact = np.array_equal(ctx.fl_unpack(packed, g.D_INT, bits, x.size).to_numpy(np.int32), x)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps_gpu):
+            packed = ctx.fl_pack(buf, g.D_INT, bits, x.size)
+        ctx.sync()
+        t_pack = (time.perf_counter() - t0) / reps_gpu
+        t0 = time.perf_counter()
+        for _ in range(reps_gpu):
+            back = ctx.fl_unpack(packed, g.D_INT, bits, x.size)
+            del back
+        ctx.sync()
+        t_unpack = (time.perf_counter() - t0) / reps_gpu
+        stream = packed.to_bytes()
+        out = {"rows": int(x.size), "bits": bits, "gpu_pack_gbps": round(x.nbytes / t_pack / 1e9, 2),
+               "gpu_unpack_gbps": round(x.nbytes / t_unpack / 1e9, 2), "round_trip_exact": exact}
+        for label, threads in (("cpu_1_thread", 1), ("cpu_all_threads", nt)):
+            ps, us, cpu_stream, cpu_out = CB.fl_loops(x, bits, threads, reps_cpu)
+            out[label] = {"threads": threads, "pack_gbps": round(x.nbytes / ps / 1e9, 3),
+                          "unpack_gbps": round(x.nbytes / us / 1e9, 3),
+                          "stream_identical_to_gpu": cpu_stream.tobytes() == stream,
+                          "round_trip_exact": bool(np.array_equal(cpu_out, x))}
+        return out
+
+    res = {"cfg0_uniform_1m": one(D.uniform_int(1_000_000, 1024, seed=0), 10, 200, 10)}
+    ts = D.timestamps(1 << 26, seed=11)
+    d = np.diff(ts.astype(np.int64)).astype(np.int32)
+    res["timestamp_deltas_64m"] = one(d, int(d.max()).bit_length(), 20, 5)
+    res["cpu"] = {"compiler": "g++ -O3", "isa": CB.isa_level(), **CB.host_cores()}
+    return res
 
 
 def learned_planner_section(ctx, blocks: int, rows: int, regime: int, exhaustive_every: int):
@@ -446,7 +461,7 @@ def main():
         print(f"[bench] {name}", file=sys.stderr, flush=True)
 
     step("fl plumbing")
-    res["fl_uniform_1m"] = fl_plumbing_section(ctx)
+    res["fl_plumbing"] = fl_plumbing_section(ctx)
     if not args.no_float:
         step("float columns")
         res["float_columns"] = float_section(ctx, args.float_rows, max(1, args.steps // 2))

@@ -99,6 +99,7 @@ def load_library(path: str = LIB_PATH):
         "ddj_fl_unpack": (i32, [vp, i32, i32, vp, sz, sz, vp]),
         "ddj_tree_compress": (i32, [vp, P(i32), sz, i32, vp, sz, P(_Buf), P(ctypes.c_double)]),
         "ddj_tree_decompress": (i32, [vp, vp, sz, P(_Buf), P(i32)]),
+        "ddj_tree_decompress_noalloc": (i32, [vp, vp, sz, vp, sz, P(sz), P(i32)]),
         "ddj_tree_predict_size": (i32, [vp, P(i32), sz, i32, vp, sz, P(sz)]),
         "ddj_plan": (i32, [vp, i32, i32, vp, sz, P(i32), sz, P(sz), P(ctypes.c_double), P(u64)]),
         "ddj_optimizer_create": (i32, [vp, P(vp)]),

@@ -2,6 +2,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstdio>
+#include <sys/stat.h>
 #include <cstring>
 #include <mutex>
 #include <thread>
@@ -38,6 +39,9 @@ static hipError_t init_ctx(Ctx& c, int device) {
         return e;
     }
     c.alloc.reset(new Allocator(device));
+    // the device address of the no-allocation decoder's scratch, taken once
+    // per device; a failure is reported by that entry point, not here
+    (void)noalloc_init_device(device);
     return hipSuccess;
 }
 
@@ -350,6 +354,14 @@ ddj_status ddj_tree_decompress(ddj_ctx* ctx, const void* d_blob, size_t blob_byt
     });
 }
 
+ddj_status ddj_tree_decompress_noalloc(ddj_ctx* ctx, const void* d_blob, size_t blob_bytes, void* d_out,
+                                      size_t out_cap, size_t* out_bytes, int* dtype) {
+    return guarded(ctx, [&] {
+        tree_decompress_noalloc(&ctx->c, static_cast<const uint8_t*>(d_blob), blob_bytes, d_out, out_cap, out_bytes,
+                                dtype);
+    });
+}
+
 ddj_status ddj_tree_predict_size(ddj_ctx* ctx, const int* preorder_path, size_t path_len, int dtype, const void* d_in,
                                  size_t in_bytes, size_t* size) {
     return guarded(ctx, [&] {
@@ -520,14 +532,35 @@ ddj_status ddj_ts_reader_next(ddj_ts_reader* r, void* const* host_columns, size_
         if (!host_columns || !rows) fail(DDJ_EINVAL, "bad argument");
         *rows = 0;
         Ctx* c = &r->ctx->c;
+        // Every column of the batch is read and decoded before any is copied
+        // out; on an error the file goes back to where the batch began and
+        // the caller's buffers are left as they were.
+        const off_t start = ftello(r->f);
+        struct stat st;
+        if (start < 0 || fstat(fileno(r->f), &st) != 0) fail(DDJ_EINVAL, "cannot read the column file");
+        const uint64_t file_bytes = (uint64_t)st.st_size;
+        struct Rewind {
+            FILE* f;
+            off_t at;
+            bool armed = true;
+            ~Rewind() {
+                if (armed) fseeko(f, at, SEEK_SET);
+            }
+        } rewind_on_error{r->f, start};
         std::vector<uint8_t> host;
+        std::vector<Part> cols;
         size_t batch = 0;
         for (size_t k = 0; k < r->dtypes.size(); k++) {
             size_t sz = 0;
             if (fread(&sz, sizeof sz, 1, r->f) != 1) {
-                if (k == 0 && feof(r->f)) return;  // end of file
+                if (k == 0 && feof(r->f)) {  // end of file
+                    rewind_on_error.armed = false;
+                    return;
+                }
                 fail(DDJ_ECORRUPT, "truncated column file");
             }
+            const off_t at = ftello(r->f);
+            if (at < 0 || sz > file_bytes - (uint64_t)at) fail(DDJ_ECORRUPT, "blob length beyond the end of the file");
             host.resize(sz);
             if (sz && fread(host.data(), 1, sz, r->f) != sz) fail(DDJ_ECORRUPT, "truncated blob");
             DevBuf d(c, sz);
@@ -539,8 +572,10 @@ ddj_status ddj_ts_reader_next(ddj_ts_reader* r, void* const* host_columns, size_
             if (k == 0) batch = nr;
             if (nr != batch) fail(DDJ_ECORRUPT, "columns of one batch differ in length");
             if (nr > cap_rows) fail(DDJ_EINVAL, "batch larger than the capacity given");
-            d2h(c, host_columns[k], out.p, nr * es);
+            cols.push_back(std::move(out));
         }
+        for (size_t k = 0; k < cols.size(); k++) d2h(c, host_columns[k], cols[k].p, cols[k].n);
+        rewind_on_error.armed = false;
         *rows = batch;
     });
 }

@@ -467,6 +467,14 @@ void TileScan::scan(Ctx* c, const uint64_t* keys, MinMaxResult* mm_a, MinMaxResu
     HIPC(hipGetLastError());
 }
 
+void launch_scan_tiles(Ctx* c, const uint64_t* vals, size_t tiles, uint64_t* prefix) {
+    kernel_bytes(c, "k_scan_tiles", 8.0 * (double)(2 * tiles + 1));
+    { KernelTimer kt_(c, "k_scan_tiles"); hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, c->stream, vals, tiles, prefix,
+                                                           (const uint64_t*)nullptr, (MinMaxResult*)nullptr,
+                                                           (MinMaxResult*)nullptr); }
+    HIPC(hipGetLastError());
+}
+
 void arm_error_flag(Ctx* c) { c->error_flag_armed = true; }
 
 void launch_fill_zero(Ctx* c, void* p, size_t bytes) {

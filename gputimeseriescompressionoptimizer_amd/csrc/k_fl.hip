@@ -232,8 +232,10 @@ void launch_fl_pack_delta(Ctx* c, int elem_size, const void* x, size_t m, const 
     });
 }
 
+size_t fl_unpack_delta_scratch_words(size_t m) { return 2 * ((m + kFlTile - 1) / kFlTile) + 1; }
+
 void launch_fl_unpack_delta(Ctx* c, int elem_size, const uint8_t* first, const uint8_t* in, size_t in_bytes, size_t m,
-                           int bits, void* out) {
+                           int bits, void* out, uint64_t* tile_scratch) {
     dispatch_int(elem_size, [&](auto tag) {
         using T = decltype(tag);
         if (bits <= 0 || bits > 8 * (int)sizeof(T)) fail(DDJ_ECORRUPT, "FL bit width out of range");
@@ -242,16 +244,22 @@ void launch_fl_unpack_delta(Ctx* c, int elem_size, const uint8_t* first, const u
             return;
         }
         const size_t tiles = (m + kFlTile - 1) / kFlTile;
-        TileScan ts(c, tiles);
+        std::unique_ptr<TileScan> own_scan;
+        uint64_t* vals = tile_scratch;
+        if (!vals) {
+            own_scan = std::make_unique<TileScan>(c, tiles);
+            vals = own_scan->vals;
+        }
+        uint64_t* prefix = vals + tiles;
         kernel_bytes(c, "k_fl_delta_sums", (double)((m * (uint64_t)bits + 7) / 8));
         with_maxb(bits, [&](auto mb) {
             constexpr int MAXB = decltype(mb)::value;
             { KernelTimer kt_(c, "k_fl_delta_sums"); hipLaunchKernelGGL(k_fl_delta_sums<MAXB>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                               in, in_bytes, m, bits, ts.vals); }
+                               in, in_bytes, m, bits, vals); }
             HIPC(hipGetLastError());
-            ts.scan(c);
+            launch_scan_tiles(c, vals, tiles, prefix);
             { KernelTimer kt_(c, "k_fl_unpack_delta_scan"); hipLaunchKernelGGL((k_fl_unpack_delta_scan<T, MAXB>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                               first, in, in_bytes, m, bits, static_cast<T*>(out), (const uint64_t*)ts.prefix); }
+                               first, in, in_bytes, m, bits, static_cast<T*>(out), (const uint64_t*)prefix); }
         });
         HIPC(hipGetLastError());
     });

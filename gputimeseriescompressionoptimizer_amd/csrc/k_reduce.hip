@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_minmax_delta(const T* __restrict__ x
 // workgroup on a single address, which costs more than this launch.
 template <typename T, int STRIDE>
 __global__ __launch_bounds__(kBlock) void k_minmax_fold(const uint64_t* __restrict__ partials, int blocks, const T* x,
-                                                        MinMaxResult* out) {
+                                                        MinMaxResult* out, uint64_t* precision_out) {
     __shared__ uint64_t sm[kBlock / kWave];
     uint64_t mn = ~0ull, mx = 0, zf = ~0ull, pr = 0;
     for (int b = threadIdx.x; b < blocks; b += kBlock) {
@@ -252,8 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_minmax_fold(const uint64_t* __restri
         if (STRIDE >= 3) fold_fixup(x, mn, mx, zf);
         out->min_key = mn;
         out->max_key = mx;
-        // out is the first member of a FloatScanResult
-        if (STRIDE == 4) reinterpret_cast<FloatScanResult*>(out)->precision = pr;
+        if (STRIDE == 4) *precision_out = pr;
     }
 }
 
@@ -282,10 +281,15 @@ __global__ __launch_bounds__(kBlock) void k_minmax_prec(const T* __restrict__ x,
     __shared__ uint64_t sm[kBlock / kWave];
     FoldKeys<T> f;
     uint32_t prec = 0;
+    // The precision is a maximum capped at 10: a lane that has seen 10
+    // (noisy or NaN data) only folds min/max from then on, and once every
+    // lane of the wave has, the precision loop is skipped by the whole wave.
     auto add = [&](T v, size_t row) {
         f.add(v, row);
-        const uint32_t p = float_precision((float)v);
-        prec = p > prec ? p : prec;
+        if (prec < 10) {
+            const uint32_t p = float_precision((float)v);
+            prec = p > prec ? p : prec;
+        }
     };
     const size_t tid = (size_t)blockIdx.x * kBlock + threadIdx.x, stride = (size_t)gridDim.x * kBlock;
     constexpr int V = 16 / sizeof(T);
@@ -554,7 +558,7 @@ void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_r
                            partials.get<uint64_t>(), d_rows); }
         HIPC(hipGetLastError());
         { KernelTimer kt_(c, "k_minmax_fold"); hipLaunchKernelGGL((k_minmax_fold<T, 3>), dim3(1), dim3(kBlock), 0, c->stream,
-                           partials.get<uint64_t>(), grid, static_cast<const T*>(x), d_result); }
+                           partials.get<uint64_t>(), grid, static_cast<const T*>(x), d_result, nullptr); }
         HIPC(hipGetLastError());
     });
 }
@@ -571,7 +575,8 @@ void launch_float_scan(Ctx* c, int dtype, const void* x, size_t n, FloatScanResu
                                static_cast<const T*>(x), n, partials.get<uint64_t>()); }
             HIPC(hipGetLastError());
             { KernelTimer kt_(c, "k_minmax_fold"); hipLaunchKernelGGL((k_minmax_fold<T, 4>), dim3(1), dim3(kBlock), 0, c->stream,
-                               partials.get<uint64_t>(), grid, static_cast<const T*>(x), &d_result->mm); }
+                               partials.get<uint64_t>(), grid, static_cast<const T*>(x), &d_result->mm,
+                               &d_result->precision); }
             HIPC(hipGetLastError());
         } else {
             fail(DDJ_EINVAL, "float scan of an integer column");
@@ -589,7 +594,7 @@ void launch_minmax_delta(Ctx* c, int elem_size, const void* x, size_t n, MinMaxR
                            partials.get<uint64_t>()); }
         HIPC(hipGetLastError());
         { KernelTimer kt_(c, "k_minmax_fold"); hipLaunchKernelGGL((k_minmax_fold<T, 2>), dim3(1), dim3(kBlock), 0, c->stream,
-                           partials.get<uint64_t>(), grid, static_cast<const T*>(x), d_result); }
+                           partials.get<uint64_t>(), grid, static_cast<const T*>(x), d_result, nullptr); }
         HIPC(hipGetLastError());
     });
 }

@@ -23,6 +23,8 @@ struct TileScan {
     // outputs, folded in the same launch into *mm_a and *mm_b.
     void scan(Ctx* c, const uint64_t* keys = nullptr, MinMaxResult* mm_a = nullptr, MinMaxResult* mm_b = nullptr);
 };
+// k_scan_tiles over caller memory: prefix[t] for t <= tiles.
+void launch_scan_tiles(Ctx* c, const uint64_t* vals, size_t tiles, uint64_t* prefix);
 
 // Kernels that can raise the device error flag (encoded input failing a
 // device-side check) arm it first; the call then reads and clears it.
@@ -79,9 +81,13 @@ void launch_stats(Ctx* c, int dtype, const void* x, size_t n, DevStats* d_out);
 void launch_fl_pack(Ctx* c, int dtype, const void* x, size_t n, const MinMaxResult* mm, int bits, void* out,
                     size_t out_bytes = 0);
 // FL unpack of m b-bit deltas fused with the delta decode: out[0] = first
-// (device, one value), out[i+1] = out[i] + d[i] (wrapping).
+// (device, one value), out[i+1] = out[i] + d[i] (wrapping).  The per-tile
+// sums and prefixes go to tile_scratch (fl_unpack_delta_scratch_words(m)
+// 64-bit words, every one written before it is read) or, when it is null,
+// to memory from the context's allocator.
+size_t fl_unpack_delta_scratch_words(size_t m);
 void launch_fl_unpack_delta(Ctx* c, int elem_size, const uint8_t* first, const uint8_t* in, size_t in_bytes, size_t m,
-                           int bits, void* out);
+                           int bits, void* out, uint64_t* tile_scratch = nullptr);
 // FL unpack of n values of b bits from an arbitrary byte address.
 void launch_fl_unpack(Ctx* c, int dtype, const uint8_t* in, size_t in_bytes, size_t n, int bits, void* out);
 

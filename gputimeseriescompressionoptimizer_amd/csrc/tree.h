@@ -51,4 +51,11 @@ struct ParsedNode {
 };
 void launch_parse_tree(Ctx* c, const uint8_t* blob, uint64_t bytes, ParsedNode* d_out, int max_nodes, int* d_status);
 
+// Decompression of none, afl -> none and delta -> afl -> none blobs (integer
+// columns) into caller memory with no device allocation (noalloc.hip).
+void tree_decompress_noalloc(Ctx* c, const uint8_t* blob, size_t blob_bytes, void* out, size_t out_cap,
+                             size_t* out_bytes, int* dtype_out);
+// Takes the device's address of that decoder's scratch (once per device).
+hipError_t noalloc_init_device(int device);
+
 }  // namespace ddj

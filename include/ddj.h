@@ -141,6 +141,19 @@ ddj_status ddj_tree_compress(ddj_ctx* ctx, const int* preorder, size_t path_len,
                              size_t in_bytes, ddj_buf* blob, double* node_ratios /* [path_len] or NULL */);
 /* CompressionTree::Decompress (tree/compression_tree.cpp:148-155) */
 ddj_status ddj_tree_decompress(ddj_ctx* ctx, const void* d_blob, size_t blob_bytes, ddj_buf* out, int* dtype);
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  Decompresses a blob whose tree is none, afl -> none or
+ * delta -> afl -> none over an integer column into the caller's d_out
+ * (out_cap bytes, aligned to the row type) without allocating: the delta
+ * decode's per-tile scratch lives in a fixed 1 MiB device array per GPU, so
+ * a delta column holds at most 268,431,361 rows (longer ones: DDJ_EINVAL).
+ * Other trees return DDJ_EUNSUPPORTED (use ddj_tree_decompress).  The
+ * kernels are enqueued on the context's stream.  All calls on one GPU share
+ * the scratch array: calls that overlap on one GPU (other contexts, other
+ * host threads) race and must be serialised by the caller. */
+ddj_status ddj_tree_decompress_noalloc(ddj_ctx* ctx, const void* d_blob, size_t blob_bytes, void* d_out,
+                                      size_t out_cap, size_t* out_bytes, int* dtype);
 /* CompressionTree::GetPredictedSizeAfterCompression (tree/compression_tree.cpp:157-160) */
 ddj_status ddj_tree_predict_size(ddj_ctx* ctx, const int* preorder, size_t path_len, int dtype, const void* d_in,
                                  size_t in_bytes, size_t* size);

@@ -71,3 +71,32 @@ def test_context_close_releases_dependents():
     assert opt._h is None and net._h is None and ts._h is None
     del opt, net, ts
     c.close()  # idempotent
+
+
+def test_column_reader_rejects_a_length_past_the_end(ctx, tmp_path):
+    """A blob length larger than the rest of the file is reported as corrupt
+    before anything is allocated; the batch's earlier columns are not copied
+    out and the file goes back to the start of the batch."""
+    import ctypes
+
+    rows = 3_000
+    cols = [D.timestamps(rows, seed=1), D.zipf_codes(rows, seed=2)]
+    path = str(tmp_path / "c.ddj")
+    g.ParallelTSCompressor(ctx, [g.D_INT, g.D_INT], 2_000).Compress(cols, path)
+    raw = bytearray(open(path, "rb").read())
+    (n0,) = struct.unpack_from("<Q", raw, 0)
+    struct.pack_into("<Q", raw, 8 + n0, 1 << 60)  # the second column of the first batch
+    open(path, "wb").write(bytes(raw))
+    L = ctx._L
+    arr = (ctypes.c_int * 2)(g.D_INT, g.D_INT)
+    h = ctypes.c_void_p()
+    assert L.ddj_ts_reader_open(ctx._h, path.encode(), 2, arr, ctypes.byref(h)) == 0
+    try:
+        bufs = [np.full(2_000, -7, np.int32) for _ in range(2)]
+        ptrs = (ctypes.c_void_p * 2)(*[b.ctypes.data for b in bufs])
+        got = ctypes.c_size_t(99)
+        for _ in range(2):  # the same error again: the batch was not half consumed
+            assert L.ddj_ts_reader_next(h, ptrs, 2_000, ctypes.byref(got)) == 3  # DDJ_ECORRUPT
+            assert all(np.all(b == -7) for b in bufs)
+    finally:
+        L.ddj_ts_reader_close(h)
