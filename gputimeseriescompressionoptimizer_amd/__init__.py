@@ -97,9 +97,14 @@ def load_library(path: str = LIB_PATH):
         "ddj_stats": (i32, [vp, i32, vp, sz, P(Stats)]),
         "ddj_fl_pack": (i32, [vp, i32, i32, vp, sz, vp]),
         "ddj_fl_unpack": (i32, [vp, i32, i32, vp, sz, sz, vp]),
+        "ddj_fl_fetch": (i32, [vp, i32, i32, i32, vp, sz, sz, vp, sz, vp]),
+        "ddj_fl_lanes_bytes": (sz, [i32, i32, sz]),
+        "ddj_fl_pack_lanes": (i32, [vp, i32, i32, vp, sz, vp]),
+        "ddj_fl_unpack_lanes": (i32, [vp, i32, i32, vp, sz, sz, vp]),
         "ddj_tree_compress": (i32, [vp, P(i32), sz, i32, vp, sz, P(_Buf), P(ctypes.c_double)]),
         "ddj_tree_decompress": (i32, [vp, vp, sz, P(_Buf), P(i32)]),
         "ddj_tree_decompress_noalloc": (i32, [vp, vp, sz, vp, sz, P(sz), P(i32)]),
+        "ddj_tree_decompress_rows": (i32, [vp, vp, sz, sz, sz, P(_Buf), P(i32)]),
         "ddj_tree_predict_size": (i32, [vp, P(i32), sz, i32, vp, sz, P(sz)]),
         "ddj_plan": (i32, [vp, i32, i32, vp, sz, P(i32), sz, P(sz), P(ctypes.c_double), P(u64)]),
         "ddj_optimizer_create": (i32, [vp, P(vp)]),
@@ -115,6 +120,17 @@ def load_library(path: str = LIB_PATH):
         "ddj_ts_reader_open": (i32, [vp, ctypes.c_char_p, u32, P(i32), P(vp)]),
         "ddj_ts_reader_next": (i32, [vp, P(vp), sz, P(sz)]),
         "ddj_ts_reader_close": (None, [vp]),
+        "ddj_io_last_error": (ctypes.c_char_p, []),
+        "ddj_file_def_read": (i32, [ctypes.c_char_p, P(vp)]),
+        "ddj_file_def_create": (i32, [u32, P(ctypes.c_char_p), P(i32), P(i32), P(vp)]),
+        "ddj_file_def_write": (i32, [vp, ctypes.c_char_p]),
+        "ddj_file_def_columns": (u32, [vp]),
+        "ddj_file_def_column": (i32, [vp, u32, P(ctypes.c_char_p), P(i32), P(i32)]),
+        "ddj_file_def_free": (None, [vp]),
+        "ddj_series_open": (i32, [vp, ctypes.c_char_p, i32, i32, i32, P(vp)]),
+        "ddj_series_read": (i32, [vp, sz, P(vp), P(sz)]),
+        "ddj_series_write": (i32, [vp, P(vp), sz]),
+        "ddj_series_close": (None, [vp]),
         "ddj_batch_compress": (i32, [vp, vp, u32, u32, P(_Buf)]),
         "ddj_batch_compress_config": (i32, [vp, vp, u32, vp, P(_Buf)]),
         "ddj_batch_autotune": (i32, [vp, vp, u32, vp, P(ctypes.c_double)]),
@@ -241,13 +257,16 @@ class Context:
                                               ctypes.byref(b)), "kernel_time")
         return ms.value, n.value, b.value
 
-    KERNELS = ["k_adam", "k_add_scalar", "k_code_edges", "k_copy_pairs", "k_delta_decode", "k_delta_encode", "k_delta_sums",
-               "k_fill_u64", "k_fill_zero", "k_fl_delta_sums", "k_fl_pack", "k_fl_unpack", "k_fl_unpack_delta_scan",
-               "k_float_compose", "k_float_split", "k_float_to_int", "k_gather_copy", "k_gfc_chunk_bytes",
-               "k_gfc_decode", "k_gfc_encode", "k_gfc_sizes", "k_hist_count", "k_hist_topk", "k_int_to_float",
-               "k_length_sums", "k_merge", "k_minmax", "k_minmax_fold", "k_minmax_prec", "k_mlp_backward", "k_mlp_forward",
-               "k_parse_tree", "k_rle_count", "k_rle_encode", "k_rle_expand", "k_rle_lengths", "k_rle_tile_runs",
-               "k_scan_lengths", "k_scan_tiles", "k_split", "k_split_count", "k_stats", "k_stencil_count"]
+    KERNELS = ["k_adam", "k_add_scalar", "k_code_edges", "k_copy_pairs", "k_delta_decode", "k_delta_encode",
+               "k_delta_sums", "k_fill_u64", "k_fill_zero", "k_fl_delta_sums", "k_fl_fetch", "k_fl_pack",
+               "k_fl_pack_lanes", "k_fl_unpack", "k_fl_unpack_delta_scan", "k_fl_unpack_lanes", "k_float_compose",
+               "k_float_split", "k_float_to_int", "k_gather_copy", "k_gfc_chunk_bytes", "k_gfc_decode",
+               "k_gfc_encode", "k_gfc_sizes", "k_hist_count", "k_hist_topk", "k_int_to_float", "k_length_sums",
+               "k_merge", "k_minmax", "k_minmax_fold", "k_minmax_prec", "k_mlp_backward", "k_mlp_forward",
+               "k_order_keys", "k_parse_tree", "k_radix_count", "k_radix_scatter", "k_rle_count", "k_rle_encode",
+               "k_rle_expand", "k_rle_lengths", "k_rle_tile_runs", "k_scan_block_apply", "k_scan_block_sums",
+               "k_scan_lengths", "k_scan_tiles", "k_split", "k_split_count", "k_stats", "k_stencil_count",
+               "k_unique_count", "k_unique_decode", "k_unique_encode", "k_unique_write"]
 
     def kernel_table(self):
         """{kernel: (ms, launches, algorithmic bytes)} for every kernel that ran."""
@@ -307,6 +326,30 @@ class Context:
         out = self.alloc(n * np.dtype(NP_DTYPE[dtype]).itemsize)
         self.check(self._L.ddj_fl_unpack(self._h, dtype, bits, buf.dev or None, buf.nbytes, n, out.dev or None),
                    "fl_unpack")
+        return out
+
+
+    def fl_fetch(self, stream: DeviceBuffer, dtype: int, bits: int, n: int, rows: DeviceBuffer,
+                 lanes: int = 1) -> DeviceBuffer:
+        """Values at the uint64 row indices `rows` of an n-value FL stream
+        (afl_decompress_value_gpu, compression/afl/afl_gpu.cu:113-121)."""
+        count = rows.nbytes // 8
+        out = self.alloc(count * np.dtype(NP_DTYPE[dtype]).itemsize)
+        self.check(self._L.ddj_fl_fetch(self._h, dtype, bits, lanes, stream.dev or None, stream.nbytes, n,
+                                        rows.dev or None, count, out.dev or None), "fl_fetch")
+        return out
+
+    def fl_pack_lanes(self, buf: DeviceBuffer, dtype: int, bits: int, n: int) -> DeviceBuffer:
+        """The 32-lane interleaved FL stream (run_afl_compress_gpu<T,32>)."""
+        out = self.alloc(self._L.ddj_fl_lanes_bytes(dtype, bits, n))
+        self.check(self._L.ddj_fl_pack_lanes(self._h, dtype, bits, buf.dev or None, n, out.dev or None),
+                   "fl_pack_lanes")
+        return out
+
+    def fl_unpack_lanes(self, buf: DeviceBuffer, dtype: int, bits: int, n: int) -> DeviceBuffer:
+        out = self.alloc(n * np.dtype(NP_DTYPE[dtype]).itemsize)
+        self.check(self._L.ddj_fl_unpack_lanes(self._h, dtype, bits, buf.dev or None, buf.nbytes, n,
+                                               out.dev or None), "fl_unpack_lanes")
         return out
 
 
@@ -390,6 +433,15 @@ class CompressionTree:
         dt = ctypes.c_int()
         self.ctx.check(self.ctx._L.ddj_tree_decompress(self.ctx._h, blob.dev or None, blob.nbytes, ctypes.byref(out),
                                                        ctypes.byref(dt)), "tree decompress")
+        self.dtype = dt.value
+        return self.ctx._wrap(out)
+
+    def DecompressRows(self, blob: DeviceBuffer, begin: int, count: int) -> DeviceBuffer:
+        """Rows [begin, begin + count) of the blob's column (partial decode)."""
+        out = _Buf()
+        dt = ctypes.c_int()
+        self.ctx.check(self.ctx._L.ddj_tree_decompress_rows(self.ctx._h, blob.dev or None, blob.nbytes, begin, count,
+                                                            ctypes.byref(out), ctypes.byref(dt)), "tree rows")
         self.dtype = dt.value
         return self.ctx._wrap(out)
 
@@ -522,6 +574,108 @@ class ParallelTSCompressor:
         finally:
             self.ctx._L.ddj_ts_reader_close(h)
         return [np.concatenate(p) if p else np.empty(0, NP_DTYPE[d]) for p, d in zip(parts, self.dtypes)]
+
+
+FORMAT_BIN, FORMAT_CSV = 0, 1
+TYPE_NAMES = {D_INT: "int", D_TIME: "time", D_FLOAT: "float", D_DOUBLE: "double", D_SHORT: "short", D_CHAR: "char"}
+
+
+def _io_check(st: int, what: str):
+    if st != 0:
+        raise DdjError(st, f"{what}: {(load_library().ddj_io_last_error() or b'').decode()}")
+
+
+class FileDefinition:
+    """The columns of a time-series file: name, DataType, decimals (the
+    header files of the reference's sample_data/, read by
+    TimeSeriesReader::ReadFileDefinition, main.cpp:92-94)."""
+
+    def __init__(self, names: Sequence[str], dtypes: Sequence[int], decimals: Optional[Sequence[int]] = None):
+        self.names = list(names)
+        self.dtypes = list(dtypes)
+        self.decimals = list(decimals) if decimals is not None else [0] * len(self.dtypes)
+
+    @staticmethod
+    def read(header_path: str) -> "FileDefinition":
+        L = load_library()
+        h = ctypes.c_void_p()
+        _io_check(L.ddj_file_def_read(os.fsencode(header_path), ctypes.byref(h)), "read header")
+        try:
+            names, dts, decs = [], [], []
+            for i in range(L.ddj_file_def_columns(h)):
+                nm, dt, dc = ctypes.c_char_p(), ctypes.c_int(), ctypes.c_int()
+                _io_check(L.ddj_file_def_column(h, i, ctypes.byref(nm), ctypes.byref(dt), ctypes.byref(dc)), "column")
+                names.append(nm.value.decode())
+                dts.append(dt.value)
+                decs.append(dc.value)
+        finally:
+            L.ddj_file_def_free(h)
+        return FileDefinition(names, dts, decs)
+
+    def _handle(self):
+        L = load_library()
+        n = len(self.dtypes)
+        names = (ctypes.c_char_p * n)(*[x.encode() for x in self.names])
+        h = ctypes.c_void_p()
+        _io_check(L.ddj_file_def_create(n, names, (ctypes.c_int * n)(*self.dtypes), (ctypes.c_int * n)(*self.decimals),
+                                        ctypes.byref(h)), "file definition")
+        return h
+
+    def write(self, header_path: str):
+        L = load_library()
+        h = self._handle()
+        try:
+            _io_check(L.ddj_file_def_write(h, os.fsencode(header_path)), "write header")
+        finally:
+            L.ddj_file_def_free(h)
+
+
+def read_series(definition: FileDefinition, path: str, fmt: int = FORMAT_BIN, padding: int = 0,
+                batch: int = 1 << 20) -> List[np.ndarray]:
+    """Every record of a binary or CSV time-series file, one array per column."""
+    L = load_library()
+    d = definition._handle()
+    f = ctypes.c_void_p()
+    try:
+        _io_check(L.ddj_series_open(d, os.fsencode(path), fmt, padding, 0, ctypes.byref(f)), "open")
+    finally:
+        L.ddj_file_def_free(d)
+    parts = [[] for _ in definition.dtypes]
+    try:
+        bufs = [np.empty(batch, NP_DTYPE[t]) for t in definition.dtypes]
+        ptrs = (ctypes.c_void_p * len(bufs))(*[b.ctypes.data for b in bufs])
+        rows = ctypes.c_size_t()
+        while True:
+            _io_check(L.ddj_series_read(f, batch, ptrs, ctypes.byref(rows)), "read")
+            for k, b in enumerate(bufs):
+                parts[k].append(b[: rows.value].copy())
+            if rows.value < batch:
+                break
+    finally:
+        L.ddj_series_close(f)
+    return [np.concatenate(p) for p in parts]
+
+
+def write_series(definition: FileDefinition, path: str, columns: Sequence[np.ndarray], fmt: int = FORMAT_BIN,
+                 padding: int = 0):
+    L = load_library()
+    cols = [np.ascontiguousarray(c, dtype=NP_DTYPE[t]) for c, t in zip(columns, definition.dtypes)]
+    if len(cols) != len(definition.dtypes) or len({c.size for c in cols}) != 1:
+        raise ValueError("one array per column, all of the same length")
+    d = definition._handle()
+    f = ctypes.c_void_p()
+    try:
+        _io_check(L.ddj_series_open(d, os.fsencode(path), fmt, padding, 1, ctypes.byref(f)), "open")
+    finally:
+        L.ddj_file_def_free(d)
+    try:
+        ptrs = (ctypes.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
+        _io_check(L.ddj_series_write(f, ptrs, cols[0].size), "write")
+    finally:
+        L.ddj_series_close(f)
+
+
+CLI_PATH = os.path.join(HERE, "ddj")
 
 
 MLP_FEATURES, MLP_HIDDEN, MLP_OUTPUTS, MLP_OUTPUTS_PADDED = 32, 128, 48, 64

@@ -1,7 +1,9 @@
-"""Build libddj_hip.so in-tree with hipcc for gfx950.
+"""Build libddj_hip.so in-tree with hipcc for gfx950, and the `ddj` CLI.
 
 Every .hip / .cpp under csrc/ is compiled to an object in _build/ (in
-parallel) and linked into gputimeseriescompressionoptimizer_amd/libddj_hip.so.
+parallel) and linked into gputimeseriescompressionoptimizer_amd/libddj_hip.so;
+cli/main.cpp (host code over the C ABI only) is linked against it into
+gputimeseriescompressionoptimizer_amd/ddj.
 """
 from __future__ import annotations
 
@@ -14,6 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libddj_hip.so")
+CLI = os.path.join(HERE, "ddj")
+CXX = os.environ.get("CXX", "g++")
 ARCH = os.environ.get("DDJ_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -56,9 +60,23 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cli()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CLI}")
     return LIB
+
+
+def build_cli() -> str:
+    """The command-line compressor: host C++ linked against libddj_hip.so (rpath $ORIGIN)."""
+    tmp = CLI + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(HERE, '..', 'include')}",
+           os.path.join(HERE, "cli", "main.cpp"), "-o", tmp, f"-L{HERE}", "-l:libddj_hip.so",
+           "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CLI build failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -93,7 +93,8 @@ struct ddj_optimizer {
 
 namespace {
 
-// The device error flag (2 = encoded input failed a device-side check) is
+// The device error flag (2 = encoded input failed a device-side check, 3 = a
+// requested row lies past the end of its column) is
 // zero between calls.  A call whose kernels may raise it arms it; the flag
 // is then read and cleared when the call ends, also when it fails.
 void check_device_error(Ctx* c, bool report = true) {
@@ -104,6 +105,7 @@ void check_device_error(Ctx* c, bool report = true) {
     if (!err) return;
     const uint32_t zero = 0;
     h2d(c, &c->control->error_flag, &zero, 4);
+    if (report && err == 3) fail(DDJ_EINVAL, "row index beyond the end of the column");
     if (report) fail(DDJ_ECORRUPT, "encoded input failed a device-side check");
 }
 
@@ -324,6 +326,51 @@ ddj_status ddj_fl_unpack(ddj_ctx* ctx, int dtype, int bits, const void* d_in, si
     });
 }
 
+static void check_int_type(int dtype) {
+    if (!dtype_ok(dtype) || dtype == DDJ_D_FLOAT || dtype == DDJ_D_DOUBLE) fail(DDJ_EINVAL, "integer types only");
+}
+
+ddj_status ddj_fl_fetch(ddj_ctx* ctx, int dtype, int bits, int lanes, const void* d_stream, size_t stream_bytes,
+                        size_t n, const uint64_t* d_rows, size_t count, void* d_out) {
+    return guarded(ctx, [&] {
+        check_int_type(dtype);
+        if (count && (!d_stream || !d_rows || !d_out)) fail(DDJ_EINVAL, "null argument");
+        launch_fl_fetch(&ctx->c, dtype, bits, lanes, static_cast<const uint8_t*>(d_stream), stream_bytes, n, d_rows,
+                        count, d_out);
+    });
+}
+
+size_t ddj_fl_lanes_bytes(int dtype, int bits, size_t n) {
+    if (!dtype_ok(dtype) || dtype == DDJ_D_FLOAT || dtype == DDJ_D_DOUBLE) return 0;
+    if (bits <= 0 || bits > 8 * (int)dtype_size(dtype)) return 0;
+    return fl_lanes_bytes(dtype, bits, n);
+}
+
+ddj_status ddj_fl_pack_lanes(ddj_ctx* ctx, int dtype, int bits, const void* d_in, size_t n, void* d_out) {
+    return guarded(ctx, [&] {
+        check_int_type(dtype);
+        const size_t es = dtype_size(dtype);
+        if (n && (!d_in || !d_out)) fail(DDJ_EINVAL, "null argument");
+        if (reinterpret_cast<uintptr_t>(d_in) % es || reinterpret_cast<uintptr_t>(d_out) % es)
+            fail(DDJ_EINVAL, "column and stream must be aligned to the row type");
+        launch_fl_pack_lanes(&ctx->c, dtype, bits, d_in, n, d_out);
+    });
+}
+
+ddj_status ddj_fl_unpack_lanes(ddj_ctx* ctx, int dtype, int bits, const void* d_in, size_t in_bytes, size_t n,
+                               void* d_out) {
+    return guarded(ctx, [&] {
+        check_int_type(dtype);
+        const size_t es = dtype_size(dtype);
+        if (n && (!d_in || !d_out)) fail(DDJ_EINVAL, "null argument");
+        if (reinterpret_cast<uintptr_t>(d_in) % es || reinterpret_cast<uintptr_t>(d_out) % es)
+            fail(DDJ_EINVAL, "column and stream must be aligned to the row type");
+        if (bits <= 0 || bits > 8 * (int)es) fail(DDJ_EINVAL, "bit width out of range");
+        if (in_bytes < fl_lanes_bytes(dtype, bits, n)) fail(DDJ_EINVAL, "stream shorter than n values of b bits");
+        launch_fl_unpack_lanes(&ctx->c, dtype, bits, d_in, n, d_out);
+    });
+}
+
 ddj_status ddj_tree_compress(ddj_ctx* ctx, const int* preorder_path, size_t path_len, int dtype, const void* d_in,
                              size_t in_bytes, ddj_buf* blob, double* node_ratios) {
     return guarded(ctx, [&] {
@@ -349,6 +396,18 @@ ddj_status ddj_tree_decompress(ddj_ctx* ctx, const void* d_blob, size_t blob_byt
         if (!out) fail(DDJ_EINVAL, "null output");
         int dt = 0;
         Part p = decompress_tree(&ctx->c, DView{static_cast<const uint8_t*>(d_blob), blob_bytes}, &dt);
+        *out = to_buf(&ctx->c, p);
+        if (dtype) *dtype = dt;
+    });
+}
+
+ddj_status ddj_tree_decompress_rows(ddj_ctx* ctx, const void* d_blob, size_t blob_bytes, size_t row_begin,
+                                   size_t row_count, ddj_buf* out, int* dtype) {
+    return guarded(ctx, [&] {
+        if (!out) fail(DDJ_EINVAL, "null output");
+        int dt = 0;
+        Part p = decompress_tree_rows(&ctx->c, DView{static_cast<const uint8_t*>(d_blob), blob_bytes}, row_begin,
+                                      row_count, &dt);
         *out = to_buf(&ctx->c, p);
         if (dtype) *dtype = dt;
     });

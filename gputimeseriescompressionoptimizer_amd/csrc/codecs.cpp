@@ -722,6 +722,84 @@ Part delta_afl_decode(Ctx* c, int dtype, DView dmeta, DView ameta, DView data) {
     return own(std::move(out));
 }
 
+namespace {
+// ----------------------------------------------------------------- unique
+// UniqueEncoding::Encode / Decode (compression/unique/unique_encoding.cu:133-181,
+// :261-300): [size_t U][size_t n][U values] and ceil(n / (32 / b)) code words.
+std::vector<Part> unique_encode(Ctx* c, int dtype, DView in) {
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    if (n == 0) return {empty(), empty()};
+    Part x = aligned_view(c, in, es);
+    DevBuf meta, data;
+    unique_encode_dev(c, dtype, x.p, n, meta, data);
+    return {own(std::move(meta)), own(std::move(data))};
+}
+Part unique_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
+    if (in[1].n == 0) return empty();
+    if (in[0].n < 16) fail(DDJ_ECORRUPT, "unique metadata shorter than its two sizes");
+    uint64_t head[2];
+    read_small(c, head, in[0].p, sizeof head);
+    const size_t es = dtype_size(dtype);
+    const uint64_t U = head[0], n = head[1];
+    if (U == 0 || U > (in[0].n - 16) / es) fail(DDJ_ECORRUPT, "unique metadata shorter than its values");
+    if (n > ((uint64_t)1 << 40)) fail(DDJ_ECORRUPT, "unique row count out of range");
+    Part words = aligned_view(c, in[1], 4);
+    DevBuf out(c, n * es);
+    unique_decode_dev(c, dtype, in[0].p + 16, U, n, words.p, in[1].n / 4, out.get());
+    return own(std::move(out));
+}
+}  // namespace
+
+namespace {
+// Width and value count of an integer AFL stream from its metadata
+// (DecodeAfl, afl_encoding.cu:74-87: n = (8 * bytes - rest) / b).
+size_t afl_stream_rows(Ctx* c, int dtype, DView meta, size_t data_bytes, int* b_out) {
+    if (data_bytes == 0) {
+        *b_out = 0;
+        return 0;
+    }
+    if (meta.n < 2) fail(DDJ_ECORRUPT, "AFL metadata shorter than 2 bytes");
+    int8_t mb[2];
+    read_small(c, mb, meta.p, 2);
+    const int b = mb[0], rest = mb[1];
+    if (b <= 0 || b > 8 * (int)dtype_size(dtype)) fail(DDJ_ECORRUPT, "AFL bit width out of range");
+    const long long bits = (long long)data_bytes * 8 - rest;
+    if (bits < 0) fail(DDJ_ECORRUPT, "AFL rest larger than the data");
+    *b_out = b;
+    return (size_t)(bits / b);
+}
+}  // namespace
+
+Part afl_decode_rows(Ctx* c, int dtype, DView meta, DView data, size_t begin, size_t count) {
+    int b = 0;
+    const size_t n = afl_stream_rows(c, dtype, meta, data.n, &b);
+    if (begin > n || count > n - begin) fail(DDJ_EINVAL, "rows beyond the end of the column");
+    const size_t es = dtype_size(dtype);
+    DevBuf out(c, count * es);
+    if (count) {
+        launch_fl_unpack(c, dtype, data.p, data.n, count, b, out.get(), begin);
+        kernel_bytes(c, "k_fl_unpack", (double)((count * b + 7) / 8 + count * es));
+    }
+    return own(std::move(out));
+}
+
+Part delta_afl_decode_rows(Ctx* c, int dtype, DView dmeta, DView ameta, DView data, size_t begin, size_t count) {
+    const size_t es = dtype_size(dtype);
+    if (dmeta.n < es) fail(DDJ_ECORRUPT, "delta metadata shorter than one value");
+    int b = 0;
+    const size_t m = afl_stream_rows(c, dtype, ameta, data.n, &b);
+    if (begin > m + 1 || count > m + 1 - begin) fail(DDJ_EINVAL, "rows beyond the end of the column");
+    DevBuf out(c, count * es);
+    if (count == 0) return own(std::move(out));
+    if (m == 0) {  // the first row alone
+        d2d(c, out.get(), dmeta.p, es);
+        return own(std::move(out));
+    }
+    launch_fl_unpack_delta_rows(c, (int)es, dmeta.p, data.p, data.n, b, begin, count, out.get());
+    kernel_bytes(c, "k_fl_unpack_delta_scan", (double)(((count - 1) * b + 7) / 8 + count * es));
+    return own(std::move(out));
+}
+
 std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in, bool defer) {
     const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
     if (n < 2) fail(DDJ_EINVAL, "fused delta/FL needs two rows");
@@ -824,7 +902,7 @@ std::vector<Part> encode(Ctx* c, int enc, int dtype, DView in, const MinMaxResul
     case DDJ_ENC_OUTSIDE_PATCH: return patch_encode(c, dtype, in, true);
     case DDJ_ENC_FLOAT_TO_INT: return float_encode(c, dtype, in);
     case DDJ_ENC_GFC: return gfc_encode_any(c, dtype, in, defer_fl);
-    case DDJ_ENC_UNIQUE: fail(DDJ_EUNSUPPORTED, "standalone unique encoding is not on the planner's path");
+    case DDJ_ENC_UNIQUE: return unique_encode(c, dtype, in);
     default: fail(DDJ_EINVAL, "unknown encoding " + std::to_string(enc));
     }
 }
@@ -852,7 +930,7 @@ Part decode(Ctx* c, int enc, int dtype, const std::vector<DView>& in) {
     case DDJ_ENC_OUTSIDE_PATCH: return patch_decode(c, dtype, in);
     case DDJ_ENC_FLOAT_TO_INT: return float_decode(c, dtype, in);
     case DDJ_ENC_GFC: return gfc_decode_any(c, dtype, in);
-    case DDJ_ENC_UNIQUE: fail(DDJ_EUNSUPPORTED, "standalone unique encoding is not on the planner's path");
+    case DDJ_ENC_UNIQUE: return unique_decode(c, dtype, in);
     default: fail(DDJ_EINVAL, "unknown encoding " + std::to_string(enc));
     }
 }
@@ -903,7 +981,19 @@ void predict_sizes(Ctx* c, int enc, int dtype, DView in, size_t* meta, size_t* d
         *data = o[1].n;
         return;
     }
-    case DDJ_ENC_UNIQUE: fail(DDJ_EUNSUPPORTED, "standalone unique encoding is not on the planner's path");
+    case DDJ_ENC_UNIQUE: {  // unique_encoding.cu:307-328: FindUnique(data, type) exists for int and float
+        if (!n) { *meta = 0; *data = 0; return; }
+        if (dtype != DDJ_D_INT && dtype != DDJ_D_FLOAT) fail(DDJ_EUNSUPPORTED, "unique size predictor: int/float only");
+        Part x = aligned_view(c, in, es);
+        const uint64_t U = unique_count(c, dtype, x.p, n);
+        uint64_t v = U - 1;
+        int b = 1;
+        while (v >>= 1) b++;
+        const size_t P = 32 / (size_t)b;
+        *meta = 16 + (size_t)U * es;
+        *data = (n + P - 1) / P * 4;
+        return;
+    }
     default: fail(DDJ_EINVAL, "unknown encoding");
     }
 }

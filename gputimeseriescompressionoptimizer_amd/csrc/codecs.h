@@ -93,4 +93,11 @@ std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in, bool defer_fl = 
 // The inverse: AFL data (non-empty) decoded and prefix-summed in one pass.
 Part delta_afl_decode(Ctx* c, int dtype, DView delta_meta, DView afl_meta, DView afl_data);
 
+// Rows [begin, begin + count) of what an integer AFL stream (metadata,
+// data) decodes to, unpacking only those rows.
+Part afl_decode_rows(Ctx* c, int dtype, DView meta, DView data, size_t begin, size_t count);
+// The same for a column stored as delta -> AFL: the deltas before the range
+// are summed (per tile, never written), the range unpacked and scanned.
+Part delta_afl_decode_rows(Ctx* c, int dtype, DView dmeta, DView ameta, DView data, size_t begin, size_t count);
+
 }  // namespace ddj

@@ -56,15 +56,17 @@ __device__ __forceinline__ uint32_t byte_mask(int lo, int hi) {
 }
 
 // Stages the b-bit values [v0, v0 + nv) of the stream `in` (any byte
-// address) in LDS as 32-bit words, from the 16-byte chunks that hold them
-// (an aligned 16-byte chunk holding a valid byte never crosses a page), four
-// chunks in flight per thread.  Returns the byte offset of the first value's
-// byte inside w[0]; bytes outside the stream are zero, and one zero chunk
-// follows the last staged one.
+// address; v0 any row, so the first value may start inside a byte) in LDS
+// as 32-bit words, from the 16-byte chunks that hold them (an aligned
+// 16-byte chunk holding a valid byte never crosses a page), four chunks in
+// flight per thread.  Returns the bit offset of the first value inside w[0];
+// bytes outside the stream are zero, and one zero chunk follows the last
+// staged one.
 __device__ __forceinline__ int stage_fl_words(const uint8_t* __restrict__ in, size_t in_bytes, size_t v0, int nv, int b,
                                               uint32_t* w) {
-    const size_t B0 = v0 * (size_t)b / 8;  // exact: v0 is a multiple of 8
-    const size_t need = ((size_t)nv * b + 7) / 8;
+    const size_t B0 = v0 * (size_t)b / 8;
+    const int s0 = (int)((v0 * (size_t)b) & 7);  // 0 whenever v0 is a multiple of 8
+    const size_t need = ((size_t)s0 + (size_t)nv * b + 7) / 8;
     const size_t avail_total = in_bytes > B0 ? in_bytes - B0 : 0;
     const int avail = (int)(need < avail_total ? need : avail_total);
     const uint8_t* base = in + B0;
@@ -97,12 +99,13 @@ __device__ __forceinline__ int stage_fl_words(const uint8_t* __restrict__ in, si
             }
         }
     }
-    return a;
+    return 8 * a + s0;
 }
 
-// Value k (0-based in the tile) from staged words.
-__device__ __forceinline__ uint64_t fl_value(const uint32_t* w, int k, int b, int a, uint64_t mask) {
-    const uint64_t p = (uint64_t)k * b + 8 * a;
+// Value k (0-based in the tile) from staged words whose first value starts
+// at bit `abit` of w[0].
+__device__ __forceinline__ uint64_t fl_value(const uint32_t* w, int k, int b, int abit, uint64_t mask) {
+    const uint64_t p = (uint64_t)k * b + abit;
     const uint32_t q = (uint32_t)(p >> 5), s = (uint32_t)(p & 31);
     uint64_t val = ((uint64_t)w[q] | ((uint64_t)w[q + 1] << 32)) >> s;
     if (s + b > 64) val |= (uint64_t)w[q + 2] << (64 - s);
@@ -111,11 +114,12 @@ __device__ __forceinline__ uint64_t fl_value(const uint32_t* w, int k, int b, in
 
 template <typename T, int MAXB>
 __global__ __launch_bounds__(kBlock) void k_fl_unpack(const uint8_t* __restrict__ in, size_t in_bytes, size_t n,
-                                                      int b, T* __restrict__ out) {
+                                                      int b, T* __restrict__ out, size_t vbase) {
+    // rows [vbase, vbase + n) of the stream into out[0, n)
     __shared__ __attribute__((aligned(16))) uint32_t w[fl_stage_words<MAXB>()];
     const size_t v0 = (size_t)blockIdx.x * kFlTile;
     const int nv = (int)((n - v0) < (size_t)kFlTile ? (n - v0) : (size_t)kFlTile);
-    const int a = stage_fl_words(in, in_bytes, v0, nv, b, w);
+    const int a = stage_fl_words(in, in_bytes, vbase + v0, nv, b, w);
     __syncthreads();
     const uint64_t mask = lowmask64(b);
     // consecutive lanes take consecutive values: their LDS reads spread over the banks
@@ -131,12 +135,13 @@ __global__ __launch_bounds__(kBlock) void k_fl_unpack(const uint8_t* __restrict_
 // costs far less than writing and re-reading the deltas.  out has m + 1 rows.
 template <int MAXB>
 __global__ __launch_bounds__(kBlock) void k_fl_delta_sums(const uint8_t* __restrict__ in, size_t in_bytes, size_t m,
-                                                          int b, uint64_t* __restrict__ vals) {
+                                                          int b, uint64_t* __restrict__ vals, size_t vbase) {
+    // sums of the deltas [vbase, vbase + m), one per tile of 4096
     __shared__ __attribute__((aligned(16))) uint32_t w[fl_stage_words<MAXB>()];
     __shared__ uint64_t red[kBlock / kWave];
     const size_t v0 = (size_t)blockIdx.x * kFlTile;
     const int nv = (int)((m - v0) < (size_t)kFlTile ? (m - v0) : (size_t)kFlTile);
-    const int a = stage_fl_words(in, in_bytes, v0, nv, b, w);
+    const int a = stage_fl_words(in, in_bytes, vbase + v0, nv, b, w);
     __syncthreads();
     const uint64_t mask = lowmask64(b);
     uint64_t sum = 0;
@@ -149,7 +154,8 @@ __global__ __launch_bounds__(kBlock) void k_fl_delta_sums(const uint8_t* __restr
 template <typename T, int MAXB>
 __global__ __launch_bounds__(kBlock) void k_fl_unpack_delta_scan(const uint8_t* first_bytes, const uint8_t* __restrict__ in,
                                                                  size_t in_bytes, size_t m, int b, T* __restrict__ out,
-                                                                 const uint64_t* __restrict__ prefix) {
+                                                                 const uint64_t* __restrict__ prefix, size_t vbase) {
+    // out[0] = first + prefix[0]; out[i + 1] = out[i] + d[vbase + i] for i < m
     // the staged words, then the tile's results
     __shared__ __attribute__((aligned(16))) uint32_t w[cmax(fl_stage_words<MAXB>(), kFlTile * (int)sizeof(T) / 4)];
     __shared__ uint64_t red[kBlock / kWave];
@@ -159,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void k_fl_unpack_delta_scan(const uint8_t* 
     const uint32_t tile = blockIdx.x;
     const size_t v0 = (size_t)tile * kFlTile;
     const int nv = (int)((m - v0) < (size_t)kFlTile ? (m - v0) : (size_t)kFlTile);
-    const int a = stage_fl_words(in, in_bytes, v0, nv, b, w);
+    const int a = stage_fl_words(in, in_bytes, vbase + v0, nv, b, w);
     __syncthreads();
     const uint64_t mask = lowmask64(b);
     uint64_t d[kPer], sum = 0;
@@ -180,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_fl_unpack_delta_scan(const uint8_t* 
     }
     __syncthreads();
     for (int i = threadIdx.x; i < nv; i += kBlock) out[v0 + 1 + i] = res[i];
-    if (tile == 0 && threadIdx.x == 0) out[0] = first;
+    if (tile == 0 && threadIdx.x == 0) out[0] = (T)(ubits(first) + prefix[0]);
 }
 
 }  // namespace
@@ -255,17 +261,18 @@ void launch_fl_unpack_delta(Ctx* c, int elem_size, const uint8_t* first, const u
         with_maxb(bits, [&](auto mb) {
             constexpr int MAXB = decltype(mb)::value;
             { KernelTimer kt_(c, "k_fl_delta_sums"); hipLaunchKernelGGL(k_fl_delta_sums<MAXB>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                               in, in_bytes, m, bits, vals); }
+                               in, in_bytes, m, bits, vals, (size_t)0); }
             HIPC(hipGetLastError());
             launch_scan_tiles(c, vals, tiles, prefix);
             { KernelTimer kt_(c, "k_fl_unpack_delta_scan"); hipLaunchKernelGGL((k_fl_unpack_delta_scan<T, MAXB>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
-                               first, in, in_bytes, m, bits, static_cast<T*>(out), (const uint64_t*)prefix); }
+                               first, in, in_bytes, m, bits, static_cast<T*>(out), (const uint64_t*)prefix, (size_t)0); }
         });
         HIPC(hipGetLastError());
     });
 }
 
-void launch_fl_unpack(Ctx* c, int dtype, const uint8_t* in, size_t in_bytes, size_t n, int bits, void* out) {
+void launch_fl_unpack(Ctx* c, int dtype, const uint8_t* in, size_t in_bytes, size_t n, int bits, void* out,
+                      size_t first_row) {
     if (n == 0) return;
     const int esize = (int)dtype_size(dtype);
     dispatch_int(esize, [&](auto tag) {
@@ -276,7 +283,43 @@ void launch_fl_unpack(Ctx* c, int dtype, const uint8_t* in, size_t in_bytes, siz
             constexpr int MAXB = decltype(mb)::value;
             KernelTimer kt_(c, "k_fl_unpack");
             hipLaunchKernelGGL((k_fl_unpack<T, MAXB>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, in, in_bytes, n,
-                               bits, static_cast<T*>(out));
+                               bits, static_cast<T*>(out), first_row);
+        });
+        HIPC(hipGetLastError());
+    });
+}
+
+void launch_fl_unpack_delta_rows(Ctx* c, int elem_size, const uint8_t* first, const uint8_t* in, size_t in_bytes,
+                                 int bits, size_t begin, size_t count, void* out) {
+    if (count == 0) return;
+    dispatch_int(elem_size, [&](auto tag) {
+        using T = decltype(tag);
+        if (bits <= 0 || bits > 8 * (int)sizeof(T)) fail(DDJ_ECORRUPT, "FL bit width out of range");
+        // tile sums of the deltas before the range (part A) and inside it
+        // (part B) in one array: one scan gives part B's prefixes on top of
+        // the sum of everything before the range
+        const size_t mb = count - 1;
+        const size_t ta = (begin + kFlTile - 1) / kFlTile;
+        const size_t tb = mb ? (mb + kFlTile - 1) / kFlTile : 1;
+        TileScan ts(c, ta + tb);
+        kernel_bytes(c, "k_fl_delta_sums", (double)(((begin + mb) * (uint64_t)bits + 7) / 8));
+        with_maxb(bits, [&](auto mbits) {
+            constexpr int MAXB = decltype(mbits)::value;
+            if (ta) {
+                KernelTimer kt_(c, "k_fl_delta_sums");
+                hipLaunchKernelGGL(k_fl_delta_sums<MAXB>, dim3((unsigned)ta), dim3(kBlock), 0, c->stream, in, in_bytes,
+                                   begin, bits, ts.vals, (size_t)0);
+            }
+            {
+                KernelTimer kt_(c, "k_fl_delta_sums");
+                hipLaunchKernelGGL(k_fl_delta_sums<MAXB>, dim3((unsigned)tb), dim3(kBlock), 0, c->stream, in, in_bytes,
+                                   mb, bits, ts.vals + ta, begin);
+            }
+            HIPC(hipGetLastError());
+            ts.scan(c);
+            KernelTimer kt_(c, "k_fl_unpack_delta_scan");
+            hipLaunchKernelGGL((k_fl_unpack_delta_scan<T, MAXB>), dim3((unsigned)tb), dim3(kBlock), 0, c->stream, first, in,
+                               in_bytes, mb, bits, static_cast<T*>(out), (const uint64_t*)(ts.prefix + ta), begin);
         });
         HIPC(hipGetLastError());
     });

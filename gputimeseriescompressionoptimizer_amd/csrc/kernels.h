@@ -25,6 +25,22 @@ struct TileScan {
 };
 // k_scan_tiles over caller memory: prefix[t] for t <= tiles.
 void launch_scan_tiles(Ctx* c, const uint64_t* vals, size_t tiles, uint64_t* prefix);
+// The same for arrays of any length: one workgroup up to 16K values, else
+// block sums, their scan and a per-block rescan (three launches).
+void launch_scan_large(Ctx* c, const uint64_t* vals, size_t count, uint64_t* prefix);
+
+// Standalone unique encoding (k_unique.hip): the number of listed values U;
+// the metadata [size_t U][size_t n][U values] and the code words of n rows
+// (aligned x); and the decode of n rows from the listed values (any
+// address) and 4-byte aligned code words.
+uint64_t unique_count(Ctx* c, int dtype, const void* x, size_t n);
+void unique_encode_dev(Ctx* c, int dtype, const void* x, size_t n, DevBuf& meta, DevBuf& data);
+void unique_decode_dev(Ctx* c, int dtype, const uint8_t* uvals, uint64_t U, uint64_t n, const uint8_t* words,
+                       size_t nwords, void* out);
+// Stable LSD radix sort (8-bit digits) of n unsigned keys whose low
+// key_bits bits vary; keys and tmp (n keys each) are used in turn; returns
+// the one holding the sorted keys.
+template <typename K> K* radix_sort(Ctx* c, K* keys, K* tmp, size_t n, int key_bits);
 
 // Kernels that can raise the device error flag (encoded input failing a
 // device-side check) arm it first; the call then reads and clears it.
@@ -88,8 +104,27 @@ void launch_fl_pack(Ctx* c, int dtype, const void* x, size_t n, const MinMaxResu
 size_t fl_unpack_delta_scratch_words(size_t m);
 void launch_fl_unpack_delta(Ctx* c, int elem_size, const uint8_t* first, const uint8_t* in, size_t in_bytes, size_t m,
                            int bits, void* out, uint64_t* tile_scratch = nullptr);
-// FL unpack of n values of b bits from an arbitrary byte address.
-void launch_fl_unpack(Ctx* c, int dtype, const uint8_t* in, size_t in_bytes, size_t n, int bits, void* out);
+// FL unpack of the n values [first_row, first_row + n) of a b-bit stream at
+// an arbitrary byte address.
+void launch_fl_unpack(Ctx* c, int dtype, const uint8_t* in, size_t in_bytes, size_t n, int bits, void* out,
+                      size_t first_row = 0);
+// Rows [begin, begin + count) of the column that delta + FL encoded as
+// `first` and a stream of b-bit deltas (out[0] = first + d[0] + ... +
+// d[begin - 1]); reads the stream up to the range's end, twice.
+void launch_fl_unpack_delta_rows(Ctx* c, int elem_size, const uint8_t* first, const uint8_t* in, size_t in_bytes,
+                                 int bits, size_t begin, size_t count, void* out);
+// Random access (afl_decompress_value_gpu, compression/afl/afl_gpu.cu:113-121,
+// 197-223): out[i] = value rows[i] of an n-value b-bit stream in `lanes`
+// layout (1: contiguous; 32: interleaved blocks of 32 lanes x W values).  A
+// row >= n reads as 0 and raises the device error flag with code 3.
+void launch_fl_fetch(Ctx* c, int dtype, int bits, int lanes, const uint8_t* in, size_t in_bytes, size_t n,
+                     const uint64_t* rows, size_t count, void* out);
+// The interleaved layout (run_afl_compress_gpu<T,32> /
+// run_afl_decompress_gpu<T,32>, afl_gpu.cu:67-111): stream bytes for n
+// values, and pack / unpack.
+size_t fl_lanes_bytes(int dtype, int bits, size_t n);
+void launch_fl_pack_lanes(Ctx* c, int dtype, int bits, const void* x, size_t n, void* out);
+void launch_fl_unpack_lanes(Ctx* c, int dtype, int bits, const void* in, size_t n, void* out);
 
 // Elementwise kernels.
 void launch_delta_encode(Ctx* c, int elem_size, const void* x, size_t n, void* out /* n-1 */);

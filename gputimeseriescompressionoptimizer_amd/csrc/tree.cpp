@@ -151,7 +151,7 @@ NodeP tree_from_path(const int* path, size_t len, int dtype) {
     std::function<NodeP(int, int)> make = [&](int dt, int depth) -> NodeP {
         if (depth > kMaxTreeDepth) fail(DDJ_EINVAL, "tree path too deep");
         const int et = path[pos];
-        if (!enc_ok(et) || et == DDJ_ENC_UNIQUE) fail(DDJ_EINVAL, "bad encoding in tree path");
+        if (!enc_ok(et)) fail(DDJ_EINVAL, "bad encoding in tree path");
         NodeP node = make_node(et, dt);
         if (et == DDJ_ENC_NONE) return node;
         int rt[2];
@@ -276,82 +276,156 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
     return own(std::move(blob));
 }
 
-Part decompress_tree(Ctx* c, DView blob, int* dtype_out, NodeP* tree_out) {
-    if (blob.n < 8) fail(DDJ_ECORRUPT, "blob shorter than one node header");
-    // status words and node table in one buffer, read back with one copy
-    // (trees have a few nodes; 64 table entries cover every planner tree)
-    constexpr int kHead = 64;  // bytes before the table
-    DevBuf parsed(c, kHead + sizeof(ParsedNode) * kMaxTreeNodes);
-    ParsedNode* table = reinterpret_cast<ParsedNode*>(parsed.bytes_ptr() + kHead);
-    launch_parse_tree(c, blob.p, blob.n, table, kMaxTreeNodes, parsed.get<int>());
-    constexpr int kFirst = 64;
-    std::vector<uint8_t> host(kHead + sizeof(ParsedNode) * kFirst);
-    d2h(c, host.data(), parsed.get(), host.size());
-    int st[3];
-    memcpy(st, host.data(), sizeof st);
-    if (st[0] != 0) fail(DDJ_ECORRUPT, "malformed tree blob (parse status " + std::to_string(st[0]) + ")");
-    const int count = st[1];
-    if (count <= 0 || count > kMaxTreeNodes) fail(DDJ_ECORRUPT, "malformed tree blob (node count)");
-    std::vector<ParsedNode> nodes((size_t)count);
-    if (count <= kFirst)
-        memcpy(nodes.data(), host.data() + kHead, sizeof(ParsedNode) * count);
-    else
-        d2h(c, nodes.data(), table, sizeof(ParsedNode) * count);
-    std::vector<std::vector<int>> kids((size_t)count);
-    for (int i = 1; i < count; i++) kids[(size_t)nodes[i].parent].push_back(i);
-    // The table carries each node's leading metadata (and leaf data) bytes:
-    // decoders read their small metadata from these host copies.
-    struct ShadowScope {
-        Ctx* c;
-        size_t keep;
-        ~ShadowScope() { c->shadows.resize(keep); }
-    } scope{c, c->shadows.size()};
-    for (int i = 0; i < count; i++) {
-        const ParsedNode& nd = nodes[(size_t)i];
-        c->shadows.push_back({blob.p + nd.meta_off, (size_t)std::min<uint64_t>(nd.meta_len, kMetaHead), nd.meta_head});
-        if (nd.enc == DDJ_ENC_NONE)
+namespace {
+
+// A parsed blob: the node table read back from k_parse_tree, each node's
+// children, and while alive the host copies of every node's leading metadata
+// (and leaf data) bytes registered as shadows, from which decoders read
+// their small metadata without a device round trip.
+struct ParsedBlob {
+    Ctx* c;
+    DView blob;
+    std::vector<ParsedNode> nodes;
+    std::vector<std::vector<int>> kids;
+    size_t shadows_before = 0;
+
+    ParsedBlob(Ctx* cc, DView b) : c(cc), blob(b) {
+        if (blob.n < 8) fail(DDJ_ECORRUPT, "blob shorter than one node header");
+        // status words and node table in one buffer, read back with one copy
+        // (trees have a few nodes; 64 table entries cover every planner tree)
+        constexpr int kHead = 64;  // bytes before the table
+        DevBuf parsed(c, kHead + sizeof(ParsedNode) * kMaxTreeNodes);
+        ParsedNode* table = reinterpret_cast<ParsedNode*>(parsed.bytes_ptr() + kHead);
+        launch_parse_tree(c, blob.p, blob.n, table, kMaxTreeNodes, parsed.get<int>());
+        constexpr int kFirst = 64;
+        std::vector<uint8_t> host(kHead + sizeof(ParsedNode) * kFirst);
+        d2h(c, host.data(), parsed.get(), host.size());
+        int st[3];
+        memcpy(st, host.data(), sizeof st);
+        if (st[0] != 0) fail(DDJ_ECORRUPT, "malformed tree blob (parse status " + std::to_string(st[0]) + ")");
+        const int count = st[1];
+        if (count <= 0 || count > kMaxTreeNodes) fail(DDJ_ECORRUPT, "malformed tree blob (node count)");
+        nodes.resize((size_t)count);
+        if (count <= kFirst)
+            memcpy(nodes.data(), host.data() + kHead, sizeof(ParsedNode) * count);
+        else
+            d2h(c, nodes.data(), table, sizeof(ParsedNode) * count);
+        kids.resize((size_t)count);
+        for (int i = 1; i < count; i++) kids[(size_t)nodes[i].parent].push_back(i);
+        shadows_before = c->shadows.size();
+        for (int i = 0; i < count; i++) {
+            const ParsedNode& nd = nodes[(size_t)i];
             c->shadows.push_back(
-                {blob.p + nd.data_off, (size_t)std::min<uint64_t>(nd.data_len, kMetaHead), nd.data_head});
+                {blob.p + nd.meta_off, (size_t)std::min<uint64_t>(nd.meta_len, kMetaHead), nd.meta_head});
+            if (nd.enc == DDJ_ENC_NONE)
+                c->shadows.push_back(
+                    {blob.p + nd.data_off, (size_t)std::min<uint64_t>(nd.data_len, kMetaHead), nd.data_head});
+        }
     }
-    if (tree_out) {
-        std::vector<NodeP> ns((size_t)count);
-        for (int i = 0; i < count; i++) ns[i] = make_node(nodes[i].enc, nodes[i].dtype);
-        for (int i = 1; i < count; i++) ns[(size_t)nodes[i].parent]->ch.push_back(ns[i]);
-        *tree_out = ns[0];
+    ~ParsedBlob() { c->shadows.resize(shadows_before); }
+    ParsedBlob(const ParsedBlob&) = delete;
+    ParsedBlob& operator=(const ParsedBlob&) = delete;
+
+    DView meta(int i) const { return DView{blob.p + nodes[(size_t)i].meta_off, (size_t)nodes[(size_t)i].meta_len}; }
+    DView leaf_data(int i) const {
+        return DView{blob.p + nodes[(size_t)i].data_off, (size_t)nodes[(size_t)i].data_len};
     }
-    // CompressionNode::Decompress (compression_node.cpp:140-161): children first.
-    std::function<Part(int)> dec = [&](int i) -> Part {
+    bool is_leaf(int i) const { return nodes[(size_t)i].enc == DDJ_ENC_NONE && kids[(size_t)i].empty(); }
+    // The integer node i is DELTA over AFL over a none leaf: the AFL node.
+    int delta_afl_child(int i) const {
         const ParsedNode& nd = nodes[(size_t)i];
-        const DView meta{blob.p + nd.meta_off, (size_t)nd.meta_len};
-        if (nd.enc == DDJ_ENC_NONE) return borrow(DView{blob.p + nd.data_off, (size_t)nd.data_len});
+        if (nd.enc != DDJ_ENC_DELTA || !is_int_type(nd.dtype) || nd.nchild != 1 || kids[(size_t)i].size() != 1)
+            return -1;
+        const int a = kids[(size_t)i][0];
+        const ParsedNode& an = nodes[(size_t)a];
+        if (an.enc != DDJ_ENC_AFL || an.dtype != nd.dtype || an.nchild != 1 || kids[(size_t)a].size() != 1) return -1;
+        return is_leaf(kids[(size_t)a][0]) ? a : -1;
+    }
+
+    // CompressionNode::Decompress (compression_node.cpp:140-161): children first.
+    Part decode_node(int i) const {
+        const ParsedNode& nd = nodes[(size_t)i];
+        const DView m = meta(i);
+        if (nd.enc == DDJ_ENC_NONE) return borrow(leaf_data(i));
         if ((int)kids[(size_t)i].size() != nd.nchild) fail(DDJ_ECORRUPT, "node child count mismatch");
         // DELTA over AFL over none (integer): unpack and prefix-sum in one pass
-        if (nd.enc == DDJ_ENC_DELTA && is_int_type(nd.dtype) && nd.nchild == 1) {
-            const ParsedNode& an = nodes[(size_t)kids[(size_t)i][0]];
-            const auto& ak = kids[(size_t)kids[(size_t)i][0]];
-            if (an.enc == DDJ_ENC_AFL && an.dtype == nd.dtype && an.nchild == 1 && ak.size() == 1) {
-                const ParsedNode& ln = nodes[(size_t)ak[0]];
-                if (ln.enc == DDJ_ENC_NONE && ln.data_len > 0 && meta.n >= dtype_size(nd.dtype))
-                    return delta_afl_decode(c, nd.dtype, meta, DView{blob.p + an.meta_off, (size_t)an.meta_len},
-                                            DView{blob.p + ln.data_off, (size_t)ln.data_len});
-            }
+        const int a = delta_afl_child(i);
+        if (a >= 0) {
+            const int leaf = kids[(size_t)a][0];
+            if (nodes[(size_t)leaf].data_len > 0 && m.n >= dtype_size(nd.dtype))
+                return delta_afl_decode(c, nd.dtype, m, meta(a), leaf_data(leaf));
         }
         std::vector<Part> parts;
-        std::vector<DView> in{meta};
+        std::vector<DView> in{m};
         for (int k : kids[(size_t)i]) {
-            parts.push_back(dec(k));
+            parts.push_back(decode_node(k));
             in.push_back(parts.back().view());
         }
         return decode(c, nd.enc, nd.dtype, in);
-    };
-    if (dtype_out) *dtype_out = nodes[0].dtype;
-    Part r = dec(0);
-    if (nodes[0].enc == DDJ_ENC_NONE) {  // a bare leaf: hand out a copy
-        DevBuf b(c, r.n);
-        d2d(c, b.get(), r.p, r.n);
-        r = own(std::move(b));
     }
+
+    // Rows [begin, begin + count) of node i's decoded output.  Leaves, FL
+    // streams and delta -> FL columns decode only those rows; scale maps
+    // them from its child's; any other node decodes whole and is sliced.
+    Part decode_rows(int i, size_t begin, size_t count) const {
+        const ParsedNode& nd = nodes[(size_t)i];
+        const size_t es = dtype_size(nd.dtype);
+        auto check = [&](size_t total_rows) {
+            if (begin > total_rows || count > total_rows - begin) fail(DDJ_EINVAL, "rows beyond the end of the column");
+        };
+        if (nd.enc == DDJ_ENC_NONE) {
+            const DView d = leaf_data(i);
+            check(d.n / es);
+            return borrow(DView{d.p + begin * es, count * es});
+        }
+        if ((int)kids[(size_t)i].size() != nd.nchild) fail(DDJ_ECORRUPT, "node child count mismatch");
+        if (nd.enc == DDJ_ENC_AFL && is_int_type(nd.dtype) && nd.nchild == 1 && is_leaf(kids[(size_t)i][0]))
+            return afl_decode_rows(c, nd.dtype, meta(i), leaf_data(kids[(size_t)i][0]), begin, count);
+        const int a = delta_afl_child(i);
+        if (a >= 0 && meta(i).n >= es)
+            return delta_afl_decode_rows(c, nd.dtype, meta(i), meta(a), leaf_data(kids[(size_t)a][0]), begin, count);
+        if (nd.enc == DDJ_ENC_SCALE && is_int_type(nd.dtype) && nd.nchild == 1 && meta(i).n > 0) {
+            Part ch = decode_rows(kids[(size_t)i][0], begin, count);
+            if (ch.n == 0) return ch;
+            return decode(c, DDJ_ENC_SCALE, nd.dtype, {meta(i), ch.view()});
+        }
+        Part whole = decode_node(i);
+        check(whole.n / es);
+        return borrow(DView{whole.p + begin * es, count * es}, whole.owner);
+    }
+};
+
+// A part the caller may own: borrowed bytes (a view into the blob, or into
+// a larger decoded buffer) are copied into a buffer of their own size.
+Part owned_copy(Ctx* c, Part r, bool exact) {
+    if (r.owner && !exact) return r;
+    if (r.owner && r.owner->get() == r.p && r.owner->size() == r.n) return r;
+    DevBuf b(c, r.n);
+    if (r.n) d2d(c, b.get(), r.p, r.n);
+    return own(std::move(b));
+}
+
+}  // namespace
+
+Part decompress_tree(Ctx* c, DView blob, int* dtype_out, NodeP* tree_out) {
+    ParsedBlob pb(c, blob);
+    const int count = (int)pb.nodes.size();
+    if (tree_out) {
+        std::vector<NodeP> ns((size_t)count);
+        for (int i = 0; i < count; i++) ns[i] = make_node(pb.nodes[i].enc, pb.nodes[i].dtype);
+        for (int i = 1; i < count; i++) ns[(size_t)pb.nodes[i].parent]->ch.push_back(ns[i]);
+        *tree_out = ns[0];
+    }
+    if (dtype_out) *dtype_out = pb.nodes[0].dtype;
+    Part r = pb.decode_node(0);
+    if (pb.nodes[0].enc == DDJ_ENC_NONE) r = owned_copy(c, r, true);  // a bare leaf: hand out a copy
     return r;
+}
+
+Part decompress_tree_rows(Ctx* c, DView blob, size_t begin, size_t count, int* dtype_out) {
+    ParsedBlob pb(c, blob);
+    if (dtype_out) *dtype_out = pb.nodes[0].dtype;
+    return owned_copy(c, pb.decode_rows(0, begin, count), true);
 }
 
 size_t predict_tree(Ctx* c, const NodeP& n, DView data) {

@@ -39,6 +39,9 @@ std::string tree_string(const NodeP& n);
 Part compress_tree(Ctx* c, const NodeP& root, DView data);
 // CompressionTree::Decompress (compression_tree.cpp:148-155)
 Part decompress_tree(Ctx* c, DView blob, int* dtype_out, NodeP* tree_out = nullptr);
+// Partial-column decode: rows [begin, begin + count) of the blob's column
+// (random access, afl_gpu.cu:113-121, 197-223, extended to trees).
+Part decompress_tree_rows(Ctx* c, DView blob, size_t begin, size_t count, int* dtype_out);
 // CompressionNode::PredictCompressionSize (compression_node.cpp:163-177)
 size_t predict_tree(Ctx* c, const NodeP& n, DView data);
 

@@ -133,6 +133,32 @@ ddj_status ddj_stats(ddj_ctx* ctx, int dtype, const void* d_in, size_t in_bytes,
 ddj_status ddj_fl_pack(ddj_ctx* ctx, int dtype, int bits, const void* d_in, size_t n, void* d_out);
 ddj_status ddj_fl_unpack(ddj_ctx* ctx, int dtype, int bits, const void* d_in, size_t in_bytes, size_t n, void* d_out);
 
+/* Random access into an FL stream: afl_decompress_value_gpu /
+ * afl_decompress_base_value_gpu (compression/afl/afl_gpu.cu:113-121, :197-223),
+ * generalised from "rows 0..length" to any set of rows.  d_out[i] = value
+ * d_rows[i] (uint64, device) of the n-value b-bit stream at d_stream (any
+ * byte address), in `lanes` layout: 1 = the contiguous stream AflEncoding
+ * writes (and ddj_fl_pack), 32 = the interleaved stream of ddj_fl_pack_lanes.
+ * A row >= n returns DDJ_EINVAL (its output slot reads 0). */
+ddj_status ddj_fl_fetch(ddj_ctx* ctx, int dtype, int bits, int lanes, const void* d_stream, size_t stream_bytes,
+                        size_t n, const uint64_t* d_rows, size_t count, void* d_out);
+
+/* The interleaved FL layout: run_afl_compress_gpu<T,32> /
+ * run_afl_decompress_gpu<T,32> (compression/afl/afl_gpu.cu:67-111, the
+ * CWARP_SIZE = 32 instantiation of afl_encoding_benchmark.cpp:44).  Blocks
+ * of 32 lanes x W values (W = bits of T); lane l of block k packs values
+ * k*32*W + l + 32*i LSB first into b words stored at k*32*b + l + 32*j.
+ * The last block is stored whole with its missing values packed as zero:
+ * ddj_fl_lanes_bytes(dtype, bits, n) = ceil(n / (32*W)) * 32 * b words of T
+ * (0 for a bad type or width).  For n a multiple of 32*W the stream equals
+ * the reference's byte for byte; for other n the reference keeps only its
+ * first ceil(b*n/W) words and so loses rows of the last block.  Column and
+ * stream are aligned to the row type. */
+size_t ddj_fl_lanes_bytes(int dtype, int bits, size_t n);
+ddj_status ddj_fl_pack_lanes(ddj_ctx* ctx, int dtype, int bits, const void* d_in, size_t n, void* d_out);
+ddj_status ddj_fl_unpack_lanes(ddj_ctx* ctx, int dtype, int bits, const void* d_in, size_t in_bytes, size_t n,
+                               void* d_out);
+
 /* ------------------------------------------------------------------- tree */
 /* CompressionTree built from a pre-order path of encodings
  * (PathGenerator::GenerateTree, optimizer/path_generator.cpp:70-91); leaves
@@ -141,6 +167,15 @@ ddj_status ddj_tree_compress(ddj_ctx* ctx, const int* preorder, size_t path_len,
                              size_t in_bytes, ddj_buf* blob, double* node_ratios /* [path_len] or NULL */);
 /* CompressionTree::Decompress (tree/compression_tree.cpp:148-155) */
 ddj_status ddj_tree_decompress(ddj_ctx* ctx, const void* d_blob, size_t blob_bytes, ddj_buf* out, int* dtype);
+/* Partial-column decode: rows [row_begin, row_begin + row_count) of the
+ * blob's column (the thesis' random-access use of FL streams,
+ * afl_gpu.cu:113-121, :197-223, carried through the tree).  none leaves,
+ * afl -> none and delta -> afl -> none (integer) decode only what the range
+ * needs (delta sums the packed deltas before the range without writing
+ * them); scale maps its child's rows; other nodes decode whole and are
+ * sliced.  Rows past the column's end return DDJ_EINVAL. */
+ddj_status ddj_tree_decompress_rows(ddj_ctx* ctx, const void* d_blob, size_t blob_bytes, size_t row_begin,
+                                   size_t row_count, ddj_buf* out, int* dtype);
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, serving loops, harnesses that forbid
  * hipMalloc).  Decompresses a blob whose tree is none, afl -> none or
@@ -262,6 +297,40 @@ ddj_status ddj_ts_reader_open(ddj_ctx* ctx, const char* in_path, uint32_t n_colu
                               ddj_ts_reader** out);
 ddj_status ddj_ts_reader_next(ddj_ts_reader* reader, void* const* host_columns, size_t cap_rows, size_t* rows);
 void ddj_ts_reader_close(ddj_ts_reader* reader);
+
+/* ------------------------------------------------ time-series record files */
+/* The record files the reference's column API and CLI read and write: its
+ * File / FileDefinition / TimeSeriesReaderBinary / TimeSeriesReaderCSV come
+ * from the data-manager submodule, which is absent from the reference tree
+ * (used by main.cpp:59-76, parallel_ts_compressor.cpp:39-102).  Host code
+ * only, no context.  Formats:
+ *   header  one column per line, "name,type,decimals" (sample_data/*.header),
+ *           type int | time | float | double | short | char;
+ *   binary  records of every column's native little-endian value in order,
+ *           then `padding` bytes (main.cpp's -p option);
+ *   CSV     one record per line, comma-separated; floats written with the
+ *           column's decimals.
+ * Errors carry a message in ddj_io_last_error() (per thread). */
+enum { DDJ_FORMAT_BIN = 0, DDJ_FORMAT_CSV = 1 };
+typedef struct ddj_file_def ddj_file_def;
+const char* ddj_io_last_error(void);
+/* TimeSeriesReader::ReadFileDefinition (main.cpp:92-94) */
+ddj_status ddj_file_def_read(const char* header_path, ddj_file_def** out);
+ddj_status ddj_file_def_create(uint32_t n_columns, const char* const* names, const int* dtypes, const int* decimals,
+                               ddj_file_def** out);
+ddj_status ddj_file_def_write(const ddj_file_def* def, const char* header_path);
+uint32_t ddj_file_def_columns(const ddj_file_def* def);
+ddj_status ddj_file_def_column(const ddj_file_def* def, uint32_t i, const char** name, int* dtype, int* decimals);
+void ddj_file_def_free(ddj_file_def* def);
+/* TimeSeriesReader::Read(file, batch) / Write(file, ts) (parallel_ts_compressor.cpp:46, :93):
+ * up to max_rows records per call into one host array per column; *rows = 0
+ * at end of file. */
+typedef struct ddj_series_file ddj_series_file;
+ddj_status ddj_series_open(const ddj_file_def* def, const char* path, int format, int padding, int write,
+                           ddj_series_file** out);
+ddj_status ddj_series_read(ddj_series_file* file, size_t max_rows, void* const* host_columns, size_t* rows);
+ddj_status ddj_series_write(ddj_series_file* file, const void* const* host_columns, size_t rows);
+void ddj_series_close(ddj_series_file* file);
 
 #ifdef __cplusplus
 }

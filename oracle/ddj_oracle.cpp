@@ -557,13 +557,19 @@ template <typename T> int decompress_unique(const Bytes& b, std::vector<T>& out)
     size_t off = 16 + U * sizeof(T);
     return decode_codes(b.data() + off, (b.size() - off) / 4, uniq.data(), U, m, out);
 }
-/* UniqueEncoding::Encode / Decode (:133-181, :261-300) */
+/* UniqueEncoding::Encode / Decode (:133-181, :261-300).  FindUnique
+ * (:35-46): thrust::sort (device radix order, as for the histogram above:
+ * -0.0 just before +0.0, sign-set NaNs first, other NaNs last) then
+ * thrust::unique with equal_to, which keeps the first of each run of equal
+ * neighbours: -0.0 and +0.0 merge into whichever sorts first (-0.0), and
+ * no NaN equals anything, so every NaN row stays in the list.  A NaN row
+ * matches no listed value and is coded 0. */
 template <typename T> int unique_encode(const T* x, size_t n, Outs& outs) {
     outs.assign(2, Bytes());
     if (n == 0) return OK;
     std::vector<T> uniq(x, x + n);
-    std::sort(uniq.begin(), uniq.end());
-    uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+    std::stable_sort(uniq.begin(), uniq.end(), [](T a, T b) { return radix_key(a) < radix_key(b); });
+    uniq.erase(std::unique(uniq.begin(), uniq.end(), [](T a, T b) { return a == b; }), uniq.end());
     std::vector<T> xv(x, x + n);
     std::vector<uint32_t> words;
     pack_codes(xv, uniq, words);
@@ -1249,6 +1255,42 @@ int orc_fl_unpack(int dtype, int bits, const void* in, size_t n, void* out) {
         const uint64_t W = 8 * sizeof(T);
         size_t words = (size_t)((bits * (uint64_t)n + W - 1) / W);
         fl_decompress<T>(bits, (const T*)in, (T*)out, n, words);
+        return OK;
+    });
+}
+
+/* run_afl_compress_cpu<T,32> / run_afl_decompress_cpu<T,32> over whole
+ * 32-lane blocks; out_words = ceil(n / (32*W)) * 32 * b. */
+int orc_fl_pack_lanes(int dtype, int bits, const void* in, size_t n, void* out, size_t out_words) {
+    return by_int_view(dtype, [&](auto* tag) {
+        typedef typename std::remove_pointer<decltype(tag)>::type T;
+        if (bits <= 0 || bits > (int)(8 * sizeof(T))) return EINVAL_;
+        memset(out, 0, out_words * sizeof(T));
+        fl_compress_lanes<T>(bits, (const T*)in, (T*)out, n, out_words);
+        return OK;
+    });
+}
+
+int orc_fl_unpack_lanes(int dtype, int bits, const void* in, size_t in_words, size_t n, void* out) {
+    return by_int_view(dtype, [&](auto* tag) {
+        typedef typename std::remove_pointer<decltype(tag)>::type T;
+        if (bits <= 0 || bits > (int)(8 * sizeof(T))) return EINVAL_;
+        fl_decompress_lanes<T>(bits, (const T*)in, (T*)out, n, in_words);
+        return OK;
+    });
+}
+
+/* afl_decompress_base_value_gpu for every row in rows[0..count): lanes 1 or 32. */
+int orc_fl_fetch(int dtype, int bits, int lanes, const void* in, size_t n, const uint64_t* rows, size_t count,
+                 void* out) {
+    return by_int_view(dtype, [&](auto* tag) {
+        typedef typename std::remove_pointer<decltype(tag)>::type T;
+        if (bits <= 0 || bits > (int)(8 * sizeof(T)) || (lanes != 1 && lanes != 32)) return EINVAL_;
+        for (size_t i = 0; i < count; i++) {
+            if (rows[i] >= n) return EINVAL_;
+            ((T*)out)[i] = lanes == 1 ? afl_value<T, 1>(bits, (const T*)in, rows[i])
+                                      : afl_value<T, 32>(bits, (const T*)in, rows[i]);
+        }
         return OK;
     });
 }

@@ -76,6 +76,15 @@ int orc_stencil_unpack(const uint8_t* p, size_t nbytes, orc_buf* out);
 int orc_fl_pack(int dtype, int bits, const void* in, size_t n, void* out, size_t out_words);
 int orc_fl_unpack(int dtype, int bits, const void* in, size_t n, void* out);
 
+/* The CWARP_SIZE = 32 instantiation of the same loops (afl_gpu.cu:67-111,
+ * afl_encoding_benchmark.cpp:44) over whole blocks of 32 lanes x W values:
+ * out_words = ceil(n / (32*W)) * 32 * b, words no lane writes stay zero. */
+int orc_fl_pack_lanes(int dtype, int bits, const void* in, size_t n, void* out, size_t out_words);
+int orc_fl_unpack_lanes(int dtype, int bits, const void* in, size_t in_words, size_t n, void* out);
+/* afl_decompress_base_value_gpu (afl_gpu.cu:197-223) at each of rows[0..count). */
+int orc_fl_fetch(int dtype, int bits, int lanes, const void* in, size_t n, const uint64_t* rows, size_t count,
+                 void* out);
+
 void orc_free(orc_buf* b);
 
 #ifdef __cplusplus

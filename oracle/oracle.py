@@ -86,6 +86,12 @@ def lib():
         L.orc_fl_pack.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p,
                                   ctypes.c_size_t]
         L.orc_fl_unpack.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+        L.orc_fl_pack_lanes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                        ctypes.c_void_p, ctypes.c_size_t]
+        L.orc_fl_unpack_lanes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                          ctypes.c_size_t, ctypes.c_void_p]
+        L.orc_fl_fetch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t,
+                                   ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
         L.orc_encoding_info.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_int)]
         L.orc_bitlen.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
@@ -194,6 +200,39 @@ def fl_unpack(dtype: int, bits: int, words: np.ndarray, n: int) -> np.ndarray:
     words = np.ascontiguousarray(words)
     out = np.zeros(n, dtype=words.dtype)
     _check(lib().orc_fl_unpack(dtype, bits, words.ctypes.data, n, out.ctypes.data), "fl_unpack")
+    return out
+
+
+def fl_lanes_words(itemsize: int, bits: int, n: int) -> int:
+    """Words of T in the 32-lane stream of n values: whole blocks of 32 lanes x W values."""
+    W = 8 * itemsize
+    return (n + 32 * W - 1) // (32 * W) * 32 * bits
+
+
+def fl_pack_lanes(dtype: int, bits: int, x: np.ndarray) -> np.ndarray:
+    """run_afl_compress_cpu<T,32> restated over whole 32-lane blocks."""
+    x = np.ascontiguousarray(x)
+    words = fl_lanes_words(x.itemsize, bits, x.size)
+    out = np.zeros(words, dtype=x.dtype)
+    _check(lib().orc_fl_pack_lanes(dtype, bits, x.ctypes.data, x.size, out.ctypes.data, words), "fl_pack_lanes")
+    return out
+
+
+def fl_unpack_lanes(dtype: int, bits: int, words: np.ndarray, n: int) -> np.ndarray:
+    words = np.ascontiguousarray(words)
+    out = np.zeros(n, dtype=words.dtype)
+    _check(lib().orc_fl_unpack_lanes(dtype, bits, words.ctypes.data, words.size, n, out.ctypes.data),
+           "fl_unpack_lanes")
+    return out
+
+
+def fl_fetch(dtype: int, bits: int, lanes: int, words: np.ndarray, n: int, rows: np.ndarray) -> np.ndarray:
+    """afl_decompress_base_value_gpu at each row (compression/afl/afl_gpu.cu:197-223)."""
+    words = np.ascontiguousarray(words)
+    rows = np.ascontiguousarray(rows, dtype=np.uint64)
+    out = np.zeros(rows.size, dtype=words.dtype)
+    _check(lib().orc_fl_fetch(dtype, bits, lanes, words.ctypes.data, n, rows.ctypes.data, rows.size,
+                              out.ctypes.data), "fl_fetch")
     return out
 
 

@@ -77,7 +77,8 @@ def check(ctx, enc, dtype, x):
 
 
 CASES = []
-for enc in [g.DELTA, g.SCALE, g.RLE, g.DICT, g.CONST, g.PATCH, g.OUTSIDE_PATCH, g.FLOAT_TO_INT, g.GFC, g.NONE]:
+for enc in [g.DELTA, g.SCALE, g.RLE, g.DICT, g.CONST, g.PATCH, g.OUTSIDE_PATCH, g.FLOAT_TO_INT, g.GFC, g.NONE,
+            g.UNIQUE]:
     for dt in ALL_TYPES:
         CASES.append((enc, dt))
 for dt in INT_TYPES + [g.D_FLOAT]:
@@ -98,7 +99,8 @@ def test_codec_parity(ctx, enc, dtype):
                 assert dec == x.tobytes(), (n, fam)
 
 
-@pytest.mark.parametrize("enc", [g.DELTA, g.SCALE, g.RLE, g.DICT, g.CONST, g.PATCH, g.FLOAT_TO_INT, g.GFC, g.AFL])
+@pytest.mark.parametrize("enc", [g.DELTA, g.SCALE, g.RLE, g.DICT, g.CONST, g.PATCH, g.FLOAT_TO_INT, g.GFC, g.AFL,
+                                 g.UNIQUE])
 def test_empty_input(ctx, enc):
     dtype = g.D_INT
     outs, got = gpu_encode(ctx, enc, dtype, np.zeros(0, np.int32))
@@ -278,3 +280,44 @@ def test_delta_afl_tree_many_tiles(ctx):
     ref.fix()
     assert blob.to_bytes() == O.compress(ref, x)
     assert np.array_equal(tree.Decompress(blob).to_numpy(np.int32), x)
+
+
+@pytest.mark.parametrize("dtype", [g.D_INT, g.D_FLOAT])
+def test_unique_sizes_and_tree(ctx, dtype):
+    """GetMetadataSize / GetCompressedSize of unique (unique_encoding.cu:302-328)
+    equal the encoder's outputs, and a unique -> none tree matches the
+    oracle's blob."""
+    for n, k in ((1, 1), (100, 3), (70001, 50), (70001, 3000)):
+        x = (np.random.default_rng(n + k).integers(0, k, n) * 7 - 20).astype(g.NP_DTYPE[dtype])
+        outs, got = gpu_encode(ctx, g.UNIQUE, dtype, x)
+        enc = g.Encoding(ctx, g.UNIQUE)
+        buf = ctx.upload(x)
+        assert (enc.GetMetadataSize(buf, dtype), enc.GetCompressedSize(buf, dtype)) == O.predict_sizes(
+            g.UNIQUE, dtype, x)
+        assert (len(got[0]), len(got[1])) == O.predict_sizes(g.UNIQUE, dtype, x)
+        tree = g.CompressionTree(ctx, [g.UNIQUE, g.NONE], dtype)
+        blob = tree.Compress(buf)
+        assert blob.to_bytes() == O.compress(O.tree_from_path([O.UNIQUE, O.NONE], dtype), x)
+        assert tree.Decompress(blob).to_bytes() == x.tobytes()
+
+
+def test_unique_many_distinct_values(ctx):
+    """More listed values than the coder keeps in LDS (binary search in
+    global memory) and a radix sort over many tiles; int64 keys sort in eight
+    passes."""
+    r = np.random.default_rng(11)
+    for dtype, x in ((g.D_INT, r.integers(-2**31, 2**31 - 1, 300_000).astype(np.int32)),
+                     (g.D_TIME, r.integers(-2**62, 2**62, 200_000).astype(np.int64)),
+                     (g.D_DOUBLE, r.normal(0, 1, 200_000))):
+        print(f"[unique] dtype {dtype} rows {x.size}", flush=True)
+        check(ctx, g.UNIQUE, dtype, x)
+
+
+def test_unique_corrupt_code_is_an_error(ctx):
+    x = np.repeat(np.arange(5, dtype=np.int32), 9)
+    outs = g.Encoding(ctx, g.UNIQUE).Encode(ctx.upload(x), g.D_INT)
+    words = outs[1].to_numpy(np.uint32).copy()
+    words[2] |= 7  # code 7 of a 5-value list (3-bit codes)
+    with pytest.raises(g.DdjError):
+        g.Encoding(ctx, g.UNIQUE).Decode([outs[0], ctx.upload(words)], g.D_INT)
+    assert g.Encoding(ctx, g.UNIQUE).Decode(outs, g.D_INT).to_bytes() == x.tobytes()
