@@ -264,21 +264,26 @@ def fl_plumbing_section(ctx):
 
 
 def learned_planner_section(ctx, blocks: int, rows: int, regime: int, exhaustive_every: int):
-    """Learned vs exhaustive planning on a stream of int32 blocks whose
-    generator family changes every `regime` blocks (weights trained by
-    gputimeseriescompressionoptimizer_amd.train_planner)."""
+    """Learned vs exhaustive planning on streams of blocks whose generator
+    family changes every `regime` blocks: int32 blocks with the integer
+    network, float32/float64 blocks with the float network (weights trained
+    by gputimeseriescompressionoptimizer_amd.train_planner)."""
     import gputimeseriescompressionoptimizer_amd as g
     from gputimeseriescompressionoptimizer_amd import planner_data as PD
-    from gputimeseriescompressionoptimizer_amd.train_planner import WEIGHTS, evaluate
+    from gputimeseriescompressionoptimizer_amd.train_planner import WEIGHTS, WEIGHTS_FLOAT, evaluate
 
-    if not os.path.exists(WEIGHTS):
-        return None
-    net = g.LearnedPlanner(ctx)
-    net.Load(WEIGHTS)
-    ev = evaluate(ctx, net, PD.regime_stream(blocks, rows, regime, seed=4242), exhaustive_every)
-    ev["stream"] = f"{blocks} blocks x {rows} int32 rows, family changes every {regime} blocks"
-    ev["ratio_gap_vs_exhaustive"] = (ev["learned_vs_exhaustive"] - 1.0) if ev["learned_vs_exhaustive"] else None
-    return ev
+    out = {}
+    for name, weights, cls, stream in (("int32", WEIGHTS, g.PLANNER_INT, PD.regime_stream),
+                                       ("float", WEIGHTS_FLOAT, g.PLANNER_FLOAT, PD.float_regime_stream)):
+        if not os.path.exists(weights):
+            continue
+        net = g.LearnedPlanner(ctx, cls=cls)
+        net.Load(weights)
+        ev = evaluate(ctx, net, stream(blocks, rows, regime, seed=4242), exhaustive_every)
+        ev["stream"] = f"{blocks} blocks x {rows} rows, family changes every {regime} blocks"
+        ev["ratio_gap_vs_exhaustive"] = (ev["learned_vs_exhaustive"] - 1.0) if ev["learned_vs_exhaustive"] else None
+        out[name] = ev
+    return out or None
 
 
 def copy_calibration():
@@ -305,9 +310,14 @@ def main():
     ap.add_argument("--rows", type=int, default=1 << 26)
     ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--planner-blocks", type=int, default=200)
+    ap.add_argument("--planner-blocks", type=int, default=1000)
+    ap.add_argument("--planner-regime", type=int, default=500, help="blocks per regime of the planner's stream")
     ap.add_argument("--planner-rows", type=int, default=65536)
     ap.add_argument("--no-planner", action="store_true")
+    ap.add_argument("--no-experiment", action="store_true",
+                    help="skip the optimizer-vs-single-scheme experiment on regime-shifting columns")
+    ap.add_argument("--experiment-rows", type=int, default=10_000,
+                    help="rows per part in that experiment (the reference's unittest size)")
     ap.add_argument("--float-rows", type=int, default=1 << 26)
     ap.add_argument("--batch-columns", type=int, default=256)
     ap.add_argument("--batch-max-rows", type=int, default=16 << 20)
@@ -414,6 +424,31 @@ def main():
     ratios = {name: round(x.nbytes / b.nbytes, 3) for (name, _, _, x), b in zip(dev, blobs)}
     out_bytes = sum(b.nbytes for b in blobs)
 
+    # whole-tree rooflines: (input + blob bytes) / time of each column's tree
+    # alone, and of the whole step, against the HBM peak (intermediate
+    # buffers do not count, so only fusion moves these figures)
+    per_tree = {}
+    for t, (name, buf, _, x), b in zip(trees, dev, blobs):
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            t.Compress(buf)
+        ctx.sync()
+        tc = (time.perf_counter() - t0) / args.steps
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            t.Decompress(b)
+        ctx.sync()
+        td = (time.perf_counter() - t0) / args.steps
+        algo = x.nbytes + b.nbytes
+        per_tree[name] = {"compress_ms": round(tc * 1e3, 4), "decompress_ms": round(td * 1e3, 4),
+                          "compress_frac": round(algo / tc / 1e9 / HBM_PEAK_GBPS, 4),
+                          "decompress_frac": round(algo / td / 1e9 / HBM_PEAK_GBPS, 4)}
+    tree_roofline = {"algorithmic_bytes_per_step": in_bytes + out_bytes,
+                     "compress_frac": round((in_bytes + out_bytes) / t_comp / 1e9 / HBM_PEAK_GBPS, 4),
+                     "decompress_frac": round((in_bytes + out_bytes) / t_dec / 1e9 / HBM_PEAK_GBPS, 4),
+                     "per_tree": per_tree}
+
     # dominant kernel of the compress step (largest total device time)
     dom = max(ktab.items(), key=lambda kv: kv[1][0])
     dname, (dms, dlaunch, dbytes) = dom
@@ -452,6 +487,7 @@ def main():
         "compression_ratio": round(in_bytes / out_bytes, 3),
         "ratios": ratios,
         "roofline": roofline,
+        "tree_roofline": tree_roofline,
         "profiled_ms_per_step": {"compress": round(t_comp_prof * 1e3, 3), "decompress": round(t_dec_prof * 1e3, 3)},
         "concurrent_columns": concurrent,
         "kernels": kernels,
@@ -472,8 +508,13 @@ def main():
     if not args.no_planner:
         step("learned planner")
         res["learned_planner"] = learned_planner_section(ctx, args.planner_blocks, args.planner_rows,
-                                                         regime=max(1, args.planner_blocks // 4),
+                                                         regime=max(1, min(args.planner_regime, args.planner_blocks)),
                                                          exhaustive_every=10)
+    if not args.no_experiment:
+        step("optimizer vs single scheme")
+        from gputimeseriescompressionoptimizer_amd.experiments import optimizer_vs_single_scheme
+
+        res["optimizer_vs_single_scheme"] = optimizer_vs_single_scheme(ctx, size=args.experiment_rows)
     ctx.close()
     if not args.no_cpu:
         step("cpu baseline")

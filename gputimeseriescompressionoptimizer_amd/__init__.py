@@ -135,6 +135,8 @@ def load_library(path: str = LIB_PATH):
         "ddj_batch_compress_config": (i32, [vp, vp, u32, vp, P(_Buf)]),
         "ddj_batch_autotune": (i32, [vp, vp, u32, vp, P(ctypes.c_double)]),
         "ddj_planner_candidates": (i32, [P(i32), sz, P(sz)]),
+        "ddj_planner_candidates_class": (i32, [i32, P(i32), sz, P(sz)]),
+        "ddj_planner_create_class": (i32, [vp, i32, u64, P(vp)]),
         "ddj_planner_features": (i32, [vp, i32, vp, sz, P(ctypes.c_float)]),
         "ddj_planner_labels": (i32, [vp, i32, vp, sz, P(ctypes.c_float)]),
         "ddj_planner_create": (i32, [vp, u64, P(vp)]),
@@ -682,13 +684,21 @@ MLP_FEATURES, MLP_HIDDEN, MLP_OUTPUTS, MLP_OUTPUTS_PADDED = 32, 128, 48, 64
 MLP_PARAM_COUNT = 32 * 128 + 128 + 128 * 128 + 128 + 128 * 64 + 64
 
 
-def planner_candidates() -> List[List[int]]:
-    """The 48 candidate trees of the learned planner (pre-order paths)."""
+PLANNER_INT, PLANNER_FLOAT = 0, 1
+
+
+def planner_class(dtype: int) -> int:
+    """The learned planner's column class of a DataType."""
+    return PLANNER_FLOAT if dtype in (D_FLOAT, D_DOUBLE) else PLANNER_INT
+
+
+def planner_candidates(cls: int = PLANNER_INT) -> List[List[int]]:
+    """The 48 candidate trees of a learned-planner class (pre-order paths)."""
     L = load_library()
     offs = (ctypes.c_size_t * (MLP_OUTPUTS + 1))()
-    L.ddj_planner_candidates(None, 0, offs)
+    L.ddj_planner_candidates_class(cls, None, 0, offs)
     paths = (ctypes.c_int * offs[MLP_OUTPUTS])()
-    st = L.ddj_planner_candidates(paths, offs[MLP_OUTPUTS], offs)
+    st = L.ddj_planner_candidates_class(cls, paths, offs[MLP_OUTPUTS], offs)
     if st:
         raise DdjError(st, "planner candidates")
     return [list(paths[offs[i]:offs[i + 1]]) for i in range(MLP_OUTPUTS)]
@@ -709,10 +719,11 @@ class LearnedPlanner:
     block -> predicted log2(compressed / input bytes) of 48 candidate trees,
     trained with the library's own matrix-core forward/backward kernels."""
 
-    def __init__(self, ctx: Context, seed: int = 0):
+    def __init__(self, ctx: Context, seed: int = 0, cls: int = PLANNER_INT):
         self.ctx = ctx
+        self.cls = cls
         h = ctypes.c_void_p()
-        ctx.check(ctx._L.ddj_planner_create(ctx._h, seed, ctypes.byref(h)), "planner create")
+        ctx.check(ctx._L.ddj_planner_create_class(ctx._h, cls, seed, ctypes.byref(h)), "planner create")
         self._h = h
         ctx._track(self)
 

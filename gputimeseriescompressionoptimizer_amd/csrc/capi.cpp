@@ -498,6 +498,7 @@ ddj_status ddj_optimizer_tree(ddj_optimizer* opt, int* path_out, size_t path_cap
 struct ddj_planner {
     ddj_ctx* ctx;
     std::unique_ptr<PlannerNet> net;
+    int cls = kClassInt;  // the column class its candidates and weights are for
 };
 
 struct ddj_ts_compressor {
@@ -646,7 +647,12 @@ ddj_status ddj_ts_reader_next(ddj_ts_reader* r, void* const* host_columns, size_
 extern "C" {
 
 ddj_status ddj_planner_candidates(int* paths, size_t cap, size_t* offsets) {
-    const auto& c = candidate_paths();
+    return ddj_planner_candidates_class(kClassInt, paths, cap, offsets);
+}
+
+ddj_status ddj_planner_candidates_class(int cls, int* paths, size_t cap, size_t* offsets) {
+    if (cls != kClassInt && cls != kClassFloat) return DDJ_EINVAL;
+    const auto& c = candidate_paths(cls);
     size_t total = 0;
     for (auto& p : c) total += p.size();
     if (!offsets || (paths && cap < total)) return DDJ_EINVAL;
@@ -677,10 +683,16 @@ ddj_status ddj_planner_labels(ddj_ctx* ctx, int dtype, const void* d_in, size_t 
 }
 
 ddj_status ddj_planner_create(ddj_ctx* ctx, uint64_t seed, ddj_planner** out) {
+    return ddj_planner_create_class(ctx, kClassInt, seed, out);
+}
+
+ddj_status ddj_planner_create_class(ddj_ctx* ctx, int cls, uint64_t seed, ddj_planner** out) {
     return guarded(ctx, [&] {
         if (!out) fail(DDJ_EINVAL, "null output");
+        if (cls != kClassInt && cls != kClassFloat) fail(DDJ_EINVAL, "unknown column class");
         auto* p = new ddj_planner;
         p->ctx = ctx;
+        p->cls = cls;
         p->net.reset(new PlannerNet(&ctx->c, seed));
         *out = p;
     });
@@ -729,16 +741,23 @@ ddj_status ddj_planner_plan(ddj_planner* p, int dtype, const void* d_in, size_t 
     if (!p) return DDJ_EINVAL;
     return guarded(p->ctx, [&] {
         if (!dtype_ok(dtype) || !path_len) fail(DDJ_EINVAL, "bad argument");
+        if (planner_class(dtype) != p->cls) fail(DDJ_EINVAL, "the planner's weights are for another column class");
         Ctx* c = &p->ctx->c;
         const DView data{static_cast<const uint8_t*>(d_in), in_bytes};
         auto f = block_features(c, dtype, data);
         float y[kMlpOutputs];
         p->net->predict(f.data(), 1, y);
-        int best = 0, second = -1;
-        for (int i = 1; i < kMlpOutputs; i++)
-            if (y[i] < y[best]) best = i;
-        for (int i = 0; i < kMlpOutputs; i++)
-            if (i != best && (second < 0 || y[i] < y[second])) second = i;
+        int best = -1, second = -1;  // among the candidates this type can take
+        for (int i = 0; i < kMlpOutputs; i++) {
+            if (!candidate_valid(i, dtype)) continue;
+            if (best < 0 || y[i] < y[best]) {
+                second = best;
+                best = i;
+            } else if (second < 0 || y[i] < y[second]) {
+                second = i;
+            }
+        }
+        if (best < 0 || second < 0) fail(DDJ_EHIP, "fewer than two candidate trees for this type");
         const float m = y[second] - y[best];
         if (margin) *margin = m;
         NodeP tree;
@@ -746,7 +765,7 @@ ddj_status ddj_planner_plan(ddj_planner* p, int dtype, const void* d_in, size_t 
             tree = plan_exhaustive(c, data, dtype, nullptr, nullptr);
             if (used_fallback) *used_fallback = 1;
         } else {
-            const auto& cp = candidate_paths()[best];
+            const auto& cp = candidate_paths(p->cls)[best];
             tree = tree_from_path(cp.data(), cp.size(), dtype);
             fix_tree(tree);
             if (used_fallback) *used_fallback = 0;

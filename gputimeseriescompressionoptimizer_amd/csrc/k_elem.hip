@@ -462,6 +462,10 @@ TileScan::TileScan(Ctx* c, size_t n_tiles) : mem(c, 8 * (2 * n_tiles + 1)), tile
 
 void TileScan::scan(Ctx* c, const uint64_t* keys, MinMaxResult* mm_a, MinMaxResult* mm_b) {
     if (keys && (!mm_a || !mm_b)) fail(DDJ_EINVAL, "tile key fold without outputs");
+    if (!keys && tiles > (size_t)kScanRows * kScanThreads) {  // past one step of the one-workgroup scan
+        launch_scan_large(c, vals, tiles, prefix);
+        return;
+    }
     kernel_bytes(c, "k_scan_tiles", 8.0 * (double)(2 * tiles + 1) + (keys ? 32.0 * (double)tiles : 0.0));
     { KernelTimer kt_(c, "k_scan_tiles"); hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, c->stream, vals, tiles, prefix, keys, mm_a, mm_b); }
     HIPC(hipGetLastError());

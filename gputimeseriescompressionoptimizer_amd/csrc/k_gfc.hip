@@ -411,6 +411,156 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     if (threadIdx.x == 0 && s_bad) __hip_atomic_store(error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// One wave per chunk (chunks of at most MAXPER rows): the chunk is staged
+// in LDS, the wave walks the subchunk headers, then each lane decodes
+// MAXPER / 64 consecutive rows (16 for float, 32 for double) and stores them
+// as whole vectors.  A workgroup of one wave needs a quarter of the LDS of a
+// chunk per 256-thread workgroup's idle waves, so four times as many chunks
+// are in flight on a CU while their headers are walked.
+template <typename U, int MAXPER>
+__global__ __launch_bounds__(kWave) void k_gfc_decode_wave(const uint8_t* __restrict__ payload, size_t payload_bytes,
+                                                           const uint8_t* __restrict__ offsets, long long n,
+                                                           long long N, long long per, int S, U* __restrict__ out,
+                                                           const uint64_t* __restrict__ prefix, uint32_t* error) {
+    constexpr bool dbl = sizeof(U) == 8;
+    constexpr int RPL = MAXPER / kWave;  // rows per lane
+    constexpr int LPS = 32 / RPL;        // lanes per subchunk
+    static_assert(LPS == 1 || LPS == 2, "one or two lanes per 32-row subchunk");
+    constexpr int kMaxBytes = MAXPER / 32 * (16 + 32 * (int)sizeof(U));
+    constexpr int kVec = (kMaxBytes + 15 + 15) / 16;
+    constexpr int kVR = (kVec + kWave - 1) / kWave;
+    __shared__ __attribute__((aligned(16))) uint8_t buf[16 * kVec + 16];
+    __shared__ uint32_t sub_off[MAXPER / 32 + 1];
+    const int lane = threadIdx.x;
+    const uint32_t chunk = blockIdx.x;
+    const long long c0 = (long long)chunk * per;
+    const int rows = gfc_chunk(chunk, N, per).rows;
+    const int nsub = rows / 32;
+    uint64_t chunk_bytes = gfc_chunk_bytes<U>(offsets, chunk, N, per, S);
+    bool bad = false;
+    if (chunk_bytes > (uint64_t)kMaxBytes) {  // more bytes than MAXPER rows can hold: corrupt
+        bad = true;
+        chunk_bytes = 0;
+    }
+    const uint64_t P = prefix[chunk];
+    const uintptr_t a0 = reinterpret_cast<uintptr_t>(payload + P);
+    const uint32_t base = (uint32_t)(a0 & 15);
+    {
+        const u32x4* gv = reinterpret_cast<const u32x4*>(a0 - base);
+        const uintptr_t pend = reinterpret_cast<uintptr_t>(payload) + payload_bytes;
+        const uint32_t nvec = (base + (uint32_t)chunk_bytes + 15) / 16;
+        u32x4 t[kVR];
+#pragma unroll
+        for (int r = 0; r < kVR; r++) {
+            const uint32_t k = lane + r * kWave;
+            t[r] = (k < nvec && reinterpret_cast<uintptr_t>(gv + k) < pend) ? gv[k] : u32x4{0u, 0u, 0u, 0u};
+        }
+        u32x4* b4 = reinterpret_cast<u32x4*>(buf);
+#pragma unroll
+        for (int r = 0; r < kVR; r++) {
+            const uint32_t k = lane + r * kWave;
+            if (k < nvec) b4[k] = t[r];
+        }
+        if (P + chunk_bytes > payload_bytes) bad = true;
+    }
+    __syncthreads();
+    // header walk, lanes 0..31 reading one nibble each
+    {
+        uint32_t off = 0;
+        for (int s = 0; s < nsub; s++) {
+            int nb = 0;
+            if (lane < 32 && off + 16 <= chunk_bytes) {
+                const uint8_t byte = buf[base + off + (lane >> 1)];
+                const int nib = (lane & 1) ? (byte >> 4) : (byte & 15);
+                const int code = nib & 7;
+                nb = dbl ? (code >= 2 ? code + 1 : code) : code;
+            }
+            const int tot = __builtin_amdgcn_readlane(wave_inclusive_scan(nb), 31);
+            if (lane == 0) sub_off[s] = off;
+            off += 16 + tot;
+        }
+        if (off != chunk_bytes) bad = true;
+    }
+    __syncthreads();
+    const uint32_t* bw = reinterpret_cast<const uint32_t*>(buf);
+    constexpr uint32_t kBufWords = sizeof(buf) / 4;
+    auto lds_word = [&](uint32_t a) -> uint32_t {  // chunk bytes [a, a + 4); past the buffer: zero
+        a += base;
+        const uint32_t q = a >> 2, sh = a & 3;
+        const uint32_t lo = q < kBufWords ? bw[q] : 0u;
+        if (!sh) return lo;
+        const uint32_t hi = q + 1 < kBufWords ? bw[q + 1] : 0u;
+        return __builtin_amdgcn_alignbyte(hi, lo, sh);
+    };
+    auto nb_of = [](uint32_t nib) -> uint32_t {
+        const uint32_t code = nib & 7;
+        return dbl ? (code >= 2 ? code + 1 : code) : code;
+    };
+    const int sub = lane / LPS, part = lane % LPS;
+    const int r0 = sub * 32 + part * RPL;  // first row of this lane in the chunk
+    const bool mine = r0 < rows;
+    const uint32_t so = mine ? sub_off[sub] : 0u;
+    uint32_t hw[RPL / 8];
+    uint32_t own = 0;
+#pragma unroll
+    for (int w = 0; w < RPL / 8; w++) {
+        hw[w] = mine ? lds_word(so + (uint32_t)(part * RPL / 2 + 4 * w)) : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; k++) own += nb_of(hw[w] >> (4 * k));
+    }
+    uint32_t pre = 0;
+    if (LPS == 2) {
+        const uint32_t up = __shfl_up(own, 1, 2);
+        pre = part ? up : 0u;
+    }
+    U dec[RPL];
+    uint32_t pos = so + 16 + pre;
+#pragma unroll
+    for (int k = 0; k < RPL; k++) {
+        const uint32_t nib = (hw[k / 8] >> (4 * (k % 8))) & 15;
+        const uint32_t nb = mine ? nb_of(nib) : 0u;
+        uint64_t dv = mine ? lds_word(pos) : 0u;
+        if (dbl && nb > 4) dv |= (uint64_t)lds_word(pos + 4) << 32;
+        dv &= nb >= 8 ? ~0ull : (1ull << (8 * nb)) - 1ull;
+        pos += nb;
+        U diff = (U)dv;
+        if (nib & 8) diff = (U)(U(0) - diff);
+        dec[k] = diff;
+    }
+    // row 31 of the chunk (the last row of lane LPS - 1) predicts every later subchunk
+    U last;
+    if constexpr (dbl) {
+        const uint64_t v = (uint64_t)dec[RPL - 1];
+        const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, LPS - 1);
+        const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), LPS - 1);
+        last = (U)(((uint64_t)hi << 32) | lo);
+    } else {
+        last = (U)__builtin_amdgcn_readlane((uint32_t)dec[RPL - 1], LPS - 1);
+    }
+    if (sub == 0) last = U(0);
+    if (mine) {
+        const long long g0 = c0 + r0;
+        U v[RPL];
+#pragma unroll
+        for (int k = 0; k < RPL; k++) v[k] = (U)(last + dec[k]);
+        if (r0 + RPL <= rows && g0 + RPL <= n && (reinterpret_cast<uintptr_t>(out + g0) & 15) == 0) {
+            constexpr int V = 16 / (int)sizeof(U);
+            u32x4* d4 = reinterpret_cast<u32x4*>(out + g0);
+#pragma unroll
+            for (int q = 0; q < RPL / V; q++) {
+                u32x4 t;
+                __builtin_memcpy(&t, &v[q * V], 16);
+                d4[q] = t;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < RPL; k++)
+                if (g0 + k < n && r0 + k < rows) out[g0 + k] = v[k];
+        }
+    }
+    if (lane == 0 && bad) __hip_atomic_store(error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 }  // namespace
 
 GfcPlan gfc_plan(Ctx* c, bool dbl, const void* x, size_t n, DevBuf& meta, DevBuf& offsets) {
@@ -494,9 +644,16 @@ void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const 
         ts.scan(c);
         kernel_bytes(c, "k_gfc_chunk_bytes", 12.0 * (double)tiles);
         kernel_bytes(c, "k_gfc_decode", (double)payload_bytes + 4.0 * (double)tiles + (double)n * sizeof(U));
-        { KernelTimer kt_(c, "k_gfc_decode"); hipLaunchKernelGGL(k_gfc_decode<U>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
-                           payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<U>(), (const uint64_t*)ts.prefix,
-                           err_flag); }
+        constexpr int kWaveMaxPer = sizeof(U) == 8 ? 2048 : 1024;
+        KernelTimer kt_(c, "k_gfc_decode");
+        if (g.per <= kWaveMaxPer)
+            hipLaunchKernelGGL((k_gfc_decode_wave<U, kWaveMaxPer>), dim3((unsigned)tiles), dim3(kWave), 0, c->stream,
+                               payload, payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<U>(),
+                               (const uint64_t*)ts.prefix, err_flag);
+        else
+            hipLaunchKernelGGL(k_gfc_decode<U>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
+                               payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<U>(),
+                               (const uint64_t*)ts.prefix, err_flag);
         HIPC(hipGetLastError());
     };
     if (dbl)

@@ -5,7 +5,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 
@@ -13,13 +15,77 @@
 
 namespace ddj {
 
+namespace {
+
+// The float network's 48 outputs (float and double columns).
+const std::vector<std::vector<int>>& float_candidate_paths() {
+    static const std::vector<std::vector<int>> paths = [] {
+        const int D = DDJ_ENC_DELTA, A = DDJ_ENC_AFL, S = DDJ_ENC_SCALE, R = DDJ_ENC_RLE, K = DDJ_ENC_DICT,
+                  P = DDJ_ENC_PATCH, N = DDJ_ENC_NONE, C = DDJ_ENC_CONST, F = DDJ_ENC_FLOAT_TO_INT, G = DDJ_ENC_GFC;
+        return std::vector<std::vector<int>>{
+            {G, N, N},
+            {N},
+            {F, D, A, N},
+            {F, S, D, A, N},
+            {F, A, N},
+            {F, S, A, N},
+            {F, K, A, N, A, N},
+            {F, R, A, N, A, N},
+            {F, P, A, N, A, N},
+            {F, C, A, N},
+            {D, G, N, N},
+            {K, N, G, N, N},
+            {R, A, N, G, N, N},
+            {C, G, N, N},
+            {P, G, N, N, G, N, N},
+            {R, A, N, N},
+            {K, N, N},
+            {F, D, K, A, N, A, N},
+            {F, D, R, A, N, A, N},
+            {F, D, P, A, N, A, N},
+            {F, D, C, A, N},
+            {F, S, K, A, N, A, N},
+            {F, S, R, A, N, A, N},
+            {F, S, P, A, N, A, N},
+            {F, S, C, A, N},
+            {F, R, A, N, D, A, N},
+            {F, R, A, N, S, A, N},
+            {F, K, N, S, A, N},
+            {F, K, N, D, A, N},
+            {F, C, D, A, N},
+            {F, P, D, A, N, D, A, N},
+            {F, P, S, A, N, S, A, N},
+            {D, K, N, G, N, N},
+            {D, R, A, N, G, N, N},
+            {D, C, G, N, N},
+            {D, P, G, N, N, G, N, N},
+            {S, G, N, N},
+            {S, K, N, G, N, N},
+            {R, A, N, K, N, G, N, N},
+            {R, A, N, D, G, N, N},
+            {K, N, D, G, N, N},
+            {K, N, R, A, N, G, N, N},
+            {C, D, G, N, N},
+            {C, R, A, N, G, N, N},
+            {P, D, G, N, N, G, N, N},
+            {P, K, N, N, G, N, N},
+            {F, R, K, N, A, N, A, N},
+            {F, K, N, R, A, N, A, N},
+        };
+    }();
+    return paths;
+}
+
+}  // namespace
+
 // The network's 48 outputs.  Chosen by select_candidates.py (--per-family 64,
 // 65536-row blocks, seed 11): the trees the exhaustive planner picks on
 // blocks of every synthetic family, plus the previous list, measured
 // exactly on every block and added greedily by how much they lower the sum
 // of log2(best size).  The first 26 already reach the pool's best size on
 // every one of those blocks; the rest (in pool order) add nothing there.
-const std::vector<std::vector<int>>& candidate_paths() {
+const std::vector<std::vector<int>>& candidate_paths(int cls) {
+    if (cls == kClassFloat) return float_candidate_paths();
     static const std::vector<std::vector<int>> paths = [] {
         const int D = DDJ_ENC_DELTA, A = DDJ_ENC_AFL, S = DDJ_ENC_SCALE, R = DDJ_ENC_RLE, K = DDJ_ENC_DICT,
                   P = DDJ_ENC_PATCH, N = DDJ_ENC_NONE, C = DDJ_ENC_CONST, F = DDJ_ENC_FLOAT_TO_INT;
@@ -90,7 +156,8 @@ std::array<float, kMlpFeatures> block_features(Ctx* c, int dtype, DView data) {
         while (m >>= 1) bl++;
     }
     f[0] = (float)(std::log2(n + 1) / 32);
-    f[1] = (float)(bl / W);
+    // bit width of the values (floats: always W, so their row width instead)
+    f[1] = (float)(dtype == DDJ_D_FLOAT || dtype == DDJ_D_DOUBLE ? W / 64.0 : bl / W);
     f[2] = (float)(s.delta_bits_max / (W + 1));
     f[3] = (float)(s.runs / n);
     f[4] = (float)(lg((double)s.runs) / 32);
@@ -151,19 +218,49 @@ uint64_t splitmix(uint64_t& s) {
 
 }  // namespace
 
+int planner_class(int dtype) { return dtype == DDJ_D_FLOAT || dtype == DDJ_D_DOUBLE ? kClassFloat : kClassInt; }
+
 std::array<float, kCandidates> candidate_labels(Ctx* c, int dtype, DView data) {
-    if (dtype == DDJ_D_FLOAT || dtype == DDJ_D_DOUBLE) fail(DDJ_EUNSUPPORTED, "the learned planner covers integer columns");
     std::array<float, kCandidates> out{};
     SizeEval ev(c);
     const Part root = borrow(data);
-    const auto& paths = candidate_paths();
+    const auto& paths = candidate_paths(planner_class(dtype));
     for (int i = 0; i < kCandidates; i++) {
         NodeP t = tree_from_path(paths[i].data(), paths[i].size(), dtype);
         fix_tree(t);
-        const size_t sz = ev.size(t, root);
-        out[i] = (float)std::log2((double)sz / (double)std::max<size_t>(data.n, 1));
+        try {
+            const size_t sz = ev.size(t, root);
+            out[i] = (float)std::log2((double)sz / (double)std::max<size_t>(data.n, 1));
+        } catch (const Error& e) {
+            if (e.status != DDJ_EUNSUPPORTED) throw;
+            out[i] = kInvalidLabel;  // a tree this type cannot take (e.g. AFL on double)
+        }
     }
     return out;
+}
+
+bool candidate_valid(int idx, int dtype) {
+    static std::mutex mu;
+    static std::map<std::pair<int, int>, bool> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_pair(idx, dtype);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    // AFL on double is the one encoding a type refuses outright
+    // (afl_encoding.cu:268-271): a tree is valid unless it puts AFL on a
+    // double output anywhere
+    const auto& p = candidate_paths(planner_class(dtype))[idx];
+    NodeP t = tree_from_path(p.data(), p.size(), dtype);
+    fix_tree(t);
+    std::function<bool(const NodeP&)> ok = [&](const NodeP& n) -> bool {
+        if (n->enc == DDJ_ENC_AFL && n->dtype == DDJ_D_DOUBLE) return false;
+        for (auto& ch : n->ch)
+            if (!ok(ch)) return false;
+        return true;
+    };
+    const bool v = ok(t);
+    cache[key] = v;
+    return v;
 }
 
 PlannerNet::PlannerNet(Ctx* c, uint64_t seed)

@@ -16,10 +16,16 @@ namespace ddj {
 
 constexpr int kCandidates = kMlpOutputs;
 
-// The 48 candidate trees for integer columns (pre-order paths): the
-// reference's 30 generated paths (PathGenerator::GeneratePaths) first, then
-// 18 shallower or alternative cascades.
-const std::vector<std::vector<int>>& candidate_paths();
+// Two networks of the same shape: one for integer columns, one for float
+// and double columns, each with its own 48 candidate trees.
+constexpr int kClassInt = 0, kClassFloat = 1;
+int planner_class(int dtype);
+// The candidate trees of a class (pre-order paths).
+const std::vector<std::vector<int>>& candidate_paths(int cls = kClassInt);
+// A candidate the type cannot take (AFL on a double output) is never
+// chosen; its training label is kInvalidLabel (2^8 times the input).
+constexpr float kInvalidLabel = 8.0f;
+bool candidate_valid(int idx, int dtype);
 
 // 32 features of one block from its statistics pass.
 std::array<float, kMlpFeatures> block_features(Ctx* c, int dtype, DView data);

@@ -46,34 +46,42 @@ def main():
     ap.add_argument("--k", type=int, default=g.MLP_OUTPUTS)
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--out", default="candidates.json")
+    ap.add_argument("--float", action="store_true", help="float32 / float64 blocks of the float families")
     args = ap.parse_args()
+    cls = g.PLANNER_FLOAT if args.float else g.PLANNER_INT
+    families = P.FLOAT_FAMILIES if args.float else P.FAMILIES
+    make = (lambda f, r: P.float_block(f, P._float_params(f, r), args.rows, r)) if args.float else \
+        (lambda f, r: P.block(f, P._params(f, r), args.rows, r))
     r = np.random.default_rng(args.seed)
     t0 = time.perf_counter()
     with g.Context(0) as ctx:
-        pool = [tuple(p) for p in g.planner_candidates()]
+        pool = [tuple(p) for p in g.planner_candidates(cls)]
         blocks = []
-        for fam in P.FAMILIES:
+        for fam in families:
             for _ in range(args.per_family):
-                x = P.block(fam, P._params(fam, r), args.rows, r)
+                x = make(fam, r)
                 buf = ctx.upload(x)
-                tree, _, _ = g.CompressionOptimizer(ctx).PlanExhaustive(buf, g.D_INT)
+                tree, _, _ = g.CompressionOptimizer(ctx).PlanExhaustive(buf, g.DTYPE_OF_NP[x.dtype])
                 if tuple(tree.path) not in pool:
                     pool.append(tuple(tree.path))
                 blocks.append((fam, x))
             print(f"[exhaustive] {fam}: pool {len(pool)} trees, {time.perf_counter() - t0:.0f} s", flush=True)
         sizes = np.zeros((len(blocks), len(pool)))
-        trees = [g.CompressionTree(ctx, list(p), g.D_INT) for p in pool]
         for b, (_, x) in enumerate(blocks):
             buf = ctx.upload(x)
-            for t, tree in enumerate(trees):
-                sizes[b, t] = tree.Compress(buf).nbytes
+            dt = g.DTYPE_OF_NP[x.dtype]
+            for t, p in enumerate(pool):
+                try:
+                    sizes[b, t] = g.CompressionTree(ctx, list(p), dt).Compress(buf).nbytes
+                except g.DdjError:  # a tree this type cannot take (AFL on double)
+                    sizes[b, t] = 256.0 * x.nbytes
             if (b + 1) % 64 == 0:
                 print(f"[sizes] {b + 1}/{len(blocks)} blocks, {time.perf_counter() - t0:.0f} s", flush=True)
     pick = greedy(sizes, args.k)
     chosen = [list(pool[i]) for i in pick]
     best_all = sizes.min(1).sum()
     best_k = sizes[:, pick].min(1).sum()
-    old = sizes[:, : len(g.planner_candidates())].min(1).sum()
+    old = sizes[:, : len(g.planner_candidates(cls))].min(1).sum()
     res = {"candidates": chosen, "pool": len(pool), "blocks": len(blocks),
            "bytes_best_of_pool": best_all, "bytes_best_of_chosen": best_k, "bytes_best_of_previous": old,
            "chosen_vs_pool": best_k / best_all, "previous_vs_pool": old / best_all}

@@ -1,8 +1,10 @@
 """Trains the learned planner on the GPU and evaluates it.
 
 1. Labels: `--blocks` random int32 blocks of `--rows` rows (planner_data
-   families with random parameters); for each, the 32 features (one
-   statistics pass) and the exact blob size of all 48 candidate trees.
+   families with random parameters; with `--float`, float32 and float64
+   blocks of the float families for the float network); for each, the 32
+   features (one statistics pass) and the exact blob size of all 48
+   candidate trees.
 2. Training: the network's own HIP forward/backward/Adam kernels, MSE on
    log2(blob bytes / input bytes).
 3. Evaluation on a stream whose family changes every `--regime` blocks:
@@ -31,16 +33,21 @@ import gputimeseriescompressionoptimizer_amd as g  # noqa: E402
 from gputimeseriescompressionoptimizer_amd import planner_data as PD  # noqa: E402
 
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "planner_weights.npy")
+WEIGHTS_FLOAT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "planner_weights_float.npy")
 
 
-def label_blocks(ctx, blocks, log_every=500):
-    net = g.LearnedPlanner(ctx)
+def dtype_of(x: np.ndarray) -> int:
+    return g.DTYPE_OF_NP[x.dtype]
+
+
+def label_blocks(ctx, blocks, log_every=500, cls=g.PLANNER_INT):
+    net = g.LearnedPlanner(ctx, cls=cls)
     X, T, fams = [], [], []
     t0 = time.perf_counter()
     for i, (fam, x) in enumerate(blocks):
         buf = ctx.upload(x)
-        X.append(net.Features(buf, g.D_INT))
-        T.append(net.Labels(buf, g.D_INT))
+        X.append(net.Features(buf, dtype_of(x)))
+        T.append(net.Labels(buf, dtype_of(x)))
         fams.append(fam)
         if log_every and (i + 1) % log_every == 0:
             print(f"[label] {i + 1} blocks, {time.perf_counter() - t0:.1f} s", flush=True)
@@ -50,23 +57,24 @@ def label_blocks(ctx, blocks, log_every=500):
 def evaluate(ctx, net, stream, exhaustive_every: int, min_margin: float = 0.0, log_every: int = 0):
     """Per block: learned choice (timed), best-of-48 size, and every k-th
     block the exhaustive planner's choice (timed) and its size."""
-    paths = g.planner_candidates()
+    paths = g.planner_candidates(net.cls)
     learned_bytes = best_bytes = raw = 0.0
     ex_learned = ex_bytes = 0.0
     t_learned = t_ex = 0.0
     n_ex = n = fallbacks = 0
-    lab = g.LearnedPlanner(ctx)
+    lab = g.LearnedPlanner(ctx, cls=net.cls)
     for i, (_, fam, x) in enumerate(stream):
+        dt = dtype_of(x)
         buf = ctx.upload(x)
         ctx.sync()
         t0 = time.perf_counter()
-        path, fb, _ = net.Plan(buf, g.D_INT, min_margin=min_margin)
+        path, fb, _ = net.Plan(buf, dt, min_margin=min_margin)
         t_learned += time.perf_counter() - t0
         fallbacks += int(fb)
-        labels = lab.Labels(buf, g.D_INT)
+        labels = lab.Labels(buf, dt)
         sizes = np.exp2(labels.astype(np.float64)) * x.nbytes
         idx = paths.index(path) if path in paths else -1
-        chosen = sizes[idx] if idx >= 0 else g.CompressionTree(ctx, path, g.D_INT).Compress(buf).nbytes
+        chosen = sizes[idx] if idx >= 0 else g.CompressionTree(ctx, path, dt).Compress(buf).nbytes
         learned_bytes += chosen
         best_bytes += sizes.min()
         raw += x.nbytes
@@ -75,7 +83,7 @@ def evaluate(ctx, net, stream, exhaustive_every: int, min_margin: float = 0.0, l
             opt = g.CompressionOptimizer(ctx)
             ctx.sync()
             t0 = time.perf_counter()
-            tree, _, _ = opt.PlanExhaustive(buf, g.D_INT)
+            tree, _, _ = opt.PlanExhaustive(buf, dt)
             t_ex += time.perf_counter() - t0
             ex_bytes += tree.Compress(buf).nbytes
             ex_learned += chosen
@@ -111,13 +119,17 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--eval-only", metavar="WEIGHTS", help="skip labelling and training; evaluate these weights")
     ap.add_argument("--no-eval", action="store_true", help="label and train only")
+    ap.add_argument("--float", action="store_true", help="the float network: float32 / float64 blocks")
+    ap.add_argument("--dataset", help="label set (npz with X, T) to train on instead of labelling")
     args = ap.parse_args()
     os.makedirs(args.out_dir, exist_ok=True)
     ctx = g.Context(0)
+    cls = g.PLANNER_FLOAT if args.float else g.PLANNER_INT
+    stream = PD.float_regime_stream if args.float else PD.regime_stream
     if args.eval_only:
-        net = g.LearnedPlanner(ctx)
+        net = g.LearnedPlanner(ctx, cls=cls)
         net.Load(args.eval_only)
-        ev = evaluate(ctx, net, PD.regime_stream(args.eval_blocks, args.rows, args.regime, args.seed + 1000),
+        ev = evaluate(ctx, net, stream(args.eval_blocks, args.rows, args.regime, args.seed + 1000),
                       args.exhaustive_every, log_every=500)
         res = {"weights": os.path.basename(args.eval_only), "rows_per_block": args.rows, "regime": args.regime,
                "eval": ev}
@@ -125,11 +137,16 @@ def main():
             json.dump(res, f, indent=1)
         print(json.dumps(res))
         return
-    X, T, fams = label_blocks(ctx, PD.random_blocks(args.blocks, args.rows, args.seed))
-    np.savez_compressed(os.path.join(args.out_dir, "planner_dataset.npz"), X=X, T=T, fams=np.array(fams))
+    if args.dataset:
+        d = np.load(args.dataset)
+        X, T = d["X"], d["T"]
+    else:
+        blocks = (PD.random_float_blocks if args.float else PD.random_blocks)(args.blocks, args.rows, args.seed)
+        X, T, fams = label_blocks(ctx, blocks, cls=cls)
+        np.savez_compressed(os.path.join(args.out_dir, "planner_dataset.npz"), X=X, T=T, fams=np.array(fams))
     n_val = max(1, len(X) // 10)
     Xtr, Ttr, Xva, Tva = X[n_val:], T[n_val:], X[:n_val], T[:n_val]
-    net = g.LearnedPlanner(ctx, seed=args.seed)
+    net = g.LearnedPlanner(ctx, seed=args.seed, cls=cls)
     t0 = time.perf_counter()
     history = []
     for e in range(0, args.epochs, 10):
@@ -151,7 +168,7 @@ def main():
             json.dump({"train_blocks": len(Xtr), "rows_per_block": args.rows, "epochs": args.epochs,
                        "train_s": train_s, "history": history}, f, indent=1)
         return
-    ev = evaluate(ctx, net, PD.regime_stream(args.eval_blocks, args.rows, args.regime, args.seed + 1000),
+    ev = evaluate(ctx, net, stream(args.eval_blocks, args.rows, args.regime, args.seed + 1000),
                   args.exhaustive_every, log_every=50)
     res = {"train_blocks": len(Xtr), "rows_per_block": args.rows, "epochs": args.epochs, "train_s": train_s,
            "history": history, "eval": ev}

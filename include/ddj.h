@@ -249,17 +249,24 @@ ddj_status ddj_batch_autotune(ddj_ctx* ctx, const ddj_column* cols, uint32_t n_c
 /* No counterpart in the reference.  A 32-128-128-48 network (ReLU, BF16
  * matrix-core products, FP32 accumulation, kernels in k_mlp.hip) maps one
  * statistics pass over a block to the predicted log2(compressed bytes /
- * input bytes) of 48 candidate trees for integer columns: the reference's
- * 30 generated paths (optimizer/path_generator.cpp:20-68) and 18 more.  Its
- * labels are exact serialized sizes from the library's encoders. */
+ * input bytes) of 48 candidate trees.  Two column classes, each with its own
+ * candidates and weights: integer columns (class 0) and float / double
+ * columns (class 1).  Labels are exact serialized sizes from the library's
+ * encoders; a candidate a type cannot take (AFL on a double output) is
+ * labelled 8 (2^8 times the input) and never chosen. */
+enum { DDJ_PLANNER_INT = 0, DDJ_PLANNER_FLOAT = 1 };
 typedef struct ddj_planner ddj_planner;
-/* The 48 candidate trees as pre-order paths, concatenated; offsets[49]. */
+/* The 48 candidate trees as pre-order paths, concatenated; offsets[49]
+ * (ddj_planner_candidates: the integer class). */
 ddj_status ddj_planner_candidates(int* paths, size_t cap, size_t* offsets);
+ddj_status ddj_planner_candidates_class(int cls, int* paths, size_t cap, size_t* offsets);
 /* 32 features of one block (one statistics pass). */
 ddj_status ddj_planner_features(ddj_ctx* ctx, int dtype, const void* d_in, size_t in_bytes, float* out /* [32] */);
 /* log2(exact blob bytes / input bytes) of every candidate tree on a block. */
 ddj_status ddj_planner_labels(ddj_ctx* ctx, int dtype, const void* d_in, size_t in_bytes, float* out /* [48] */);
+/* ddj_planner_create: the integer class. */
 ddj_status ddj_planner_create(ddj_ctx* ctx, uint64_t seed, ddj_planner** out);
+ddj_status ddj_planner_create_class(ddj_ctx* ctx, int cls, uint64_t seed, ddj_planner** out);
 void ddj_planner_destroy(ddj_planner* planner);
 /* Parameters (FP32, W1[32][128] b1[128] W2[128][128] b2[128] W3[128][64] b3[64]):
  * copied out to get_out and/or in from set_in (either may be NULL). */
@@ -304,7 +311,7 @@ void ddj_ts_reader_close(ddj_ts_reader* reader);
  * from the data-manager submodule, which is absent from the reference tree
  * (used by main.cpp:59-76, parallel_ts_compressor.cpp:39-102).  Host code
  * only, no context.  Formats:
- *   header  one column per line, "name,type,decimals" (sample_data/*.header),
+ *   header  one column per line, "name,type,decimals" (the .header files of sample_data/),
  *           type int | time | float | double | short | char;
  *   binary  records of every column's native little-endian value in order,
  *           then `padding` bytes (main.cpp's -p option);

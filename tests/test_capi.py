@@ -75,3 +75,20 @@ def test_profiled_kernel_names_are_listed():
             timed |= set(re.findall(r'KernelTimer kt_\(c, "([a-z_0-9]+)"\)', open(os.path.join(csrc, f)).read()))
     assert timed, "no timed kernels found"
     assert timed <= set(g.Context.KERNELS), sorted(timed - set(g.Context.KERNELS))
+
+
+def test_headers_compile_warning_free(tmp_path):
+    """include/ddj.h (C and C++) and include/ddj.hpp compile with -Wall -Werror."""
+    import subprocess
+
+    inc = os.path.join(ROOT, "include")
+    src = tmp_path / "t.cpp"
+    src.write_text('#include "ddj.h"\n#include "ddj.hpp"\nint main() { return 0; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", f"-I{inc}", str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    csrc = tmp_path / "t.c"
+    csrc.write_text('#include "ddj.h"\nint main(void) { return 0; }\n')
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", f"-I{inc}", str(csrc)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

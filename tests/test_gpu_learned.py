@@ -157,3 +157,40 @@ def test_trained_weights_stay_near_exhaustive_and_beat_untrained(ctx):
     print(tot, flush=True)
     assert tot["learned"] <= 1.02 * tot["exhaustive"]
     assert tot["untrained"] > 1.05 * tot["learned"]
+
+
+@pytest.mark.parametrize("dtype", [g.D_FLOAT, g.D_DOUBLE])
+def test_float_labels_are_exact_tree_sizes(ctx, dtype):
+    """The float class: every label is the oracle's blob size of its
+    candidate on a float32 / float64 block, and a candidate the type cannot
+    take (AFL on a double output) is labelled 8 and refused by the oracle."""
+    x = D.price_walk_f64(20_000, seed=13) if dtype == g.D_DOUBLE else D.price_walk_f32(20_000, seed=13)
+    net = g.LearnedPlanner(ctx, seed=0, cls=g.PLANNER_FLOAT)
+    labels = net.Labels(ctx.upload(x), dtype)
+    invalid = 0
+    for i, path in enumerate(g.planner_candidates(g.PLANNER_FLOAT)):
+        t = O.tree_from_path(path, dtype)
+        t.fix()
+        try:
+            size = len(O.compress(t, x))
+        except Exception:
+            assert labels[i] == np.float32(8.0), (i, path)
+            invalid += 1
+            continue
+        assert labels[i] == np.float32(np.log2(size / x.nbytes)), (i, path)
+    if dtype == g.D_FLOAT:
+        assert invalid == 0
+
+
+def test_float_plan_takes_only_trees_the_type_can_take(ctx):
+    """On double blocks the float network never proposes a tree with AFL on a
+    double output, and a network of the other class is refused."""
+    net = g.LearnedPlanner(ctx, seed=8, cls=g.PLANNER_FLOAT)
+    for seed in range(6):
+        x = D.price_walk_f64(5000, seed=seed)
+        buf = ctx.upload(x)
+        path, _, _ = net.Plan(buf, g.D_DOUBLE)
+        blob = g.CompressionTree(ctx, path, g.D_DOUBLE).Compress(buf).to_bytes()  # AFL on double would raise
+        assert blob == O.compress(O.tree_from_path(path, O.D_DOUBLE), x)
+    with pytest.raises(g.DdjError):
+        g.LearnedPlanner(ctx, cls=g.PLANNER_INT).Plan(ctx.upload(D.price_walk_f64(100, seed=1)), g.D_DOUBLE)

@@ -48,7 +48,7 @@ def test_noalloc_matches_tree_decompress(ctx, name):
 
 
 def test_noalloc_small_and_empty_columns(ctx):
-    out = ctx.alloc(64)
+    out = ctx.alloc(4 * 33)
     for n in (1, 2, 33):
         x = D.timestamps(n, seed=n)
         tree = g.CompressionTree(ctx, [g.DELTA, g.AFL, g.NONE], g.D_INT)
