@@ -286,6 +286,75 @@ def learned_planner_section(ctx, blocks: int, rows: int, regime: int, exhaustive
     return out or None
 
 
+def access_section(ctx, cols, rows: int, steps: int):
+    """The secondary paths on the benchmark's columns: unique -> none against
+    dict on the Zipf column (its codes), the 32-lane FL layout against the
+    contiguous one (the reference's BM_Pure_Afl_Aligned shape: int32 at 10
+    bits), random-access fetches and a partial decode of the timestamp
+    column's delta -> afl blob."""
+    import numpy as np
+    import gputimeseriescompressionoptimizer_amd as g
+    from gputimeseriescompressionoptimizer_amd import datasets as D
+
+    def timeit(fn, reps):
+        fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = fn()
+            del r
+        ctx.sync()
+        return (time.perf_counter() - t0) / reps
+
+    out = {}
+    # unique -> none vs dict -> none, none on the Zipf codes
+    zipf = next(x for name, x, _ in cols if name.startswith("zipf"))
+    buf = ctx.upload(zipf)
+    for name, path in (("unique", [g.UNIQUE, g.NONE]), ("dict", [g.DICT, g.NONE, g.NONE])):
+        tree = g.CompressionTree(ctx, path, g.D_INT)
+        blob = tree.Compress(buf)
+        exact = tree.Decompress(blob).to_bytes() == zipf.tobytes()
+        tc = timeit(lambda: tree.Compress(buf), steps)
+        td = timeit(lambda: tree.Decompress(blob), steps)
+        out[f"zipf_{name}"] = {"compress_gbps": round(zipf.nbytes / tc / 1e9, 2),
+                               "decompress_gbps": round(zipf.nbytes / td / 1e9, 2),
+                               "ratio": round(zipf.nbytes / blob.nbytes, 3), "bit_exact": exact}
+        del blob
+    del buf
+    # 32-lane vs contiguous FL: int32 at 10 bits
+    x = D.uniform_int(rows, 1024, seed=0)
+    xb = ctx.upload(x)
+    lanes = ctx.fl_pack_lanes(xb, g.D_INT, 10, rows)
+    flat = ctx.fl_pack(xb, g.D_INT, 10, rows)
+    exact = np.array_equal(ctx.fl_unpack_lanes(lanes, g.D_INT, 10, rows).to_numpy(np.int32), x)
+    res = {"rows": rows, "bits": 10, "round_trip_exact": exact}
+    for name, pack, unpack in (("lanes32", lambda: ctx.fl_pack_lanes(xb, g.D_INT, 10, rows),
+                                lambda: ctx.fl_unpack_lanes(lanes, g.D_INT, 10, rows)),
+                               ("contiguous", lambda: ctx.fl_pack(xb, g.D_INT, 10, rows),
+                                lambda: ctx.fl_unpack(flat, g.D_INT, 10, rows))):
+        res[f"{name}_pack_gbps"] = round(x.nbytes / timeit(pack, steps) / 1e9, 2)
+        res[f"{name}_unpack_gbps"] = round(x.nbytes / timeit(unpack, steps) / 1e9, 2)
+    out["fl_layouts"] = res
+    # random access into the contiguous stream, and a row range of a delta blob
+    r = np.random.default_rng(5)
+    k = 1 << 20
+    idx = ctx.upload(r.integers(0, rows, k).astype(np.uint64))
+    got = ctx.fl_fetch(flat, g.D_INT, 10, rows, idx).to_numpy(np.int32)
+    ok = np.array_equal(got, x[idx.to_numpy(np.uint64).astype(np.int64)])
+    tf = timeit(lambda: ctx.fl_fetch(flat, g.D_INT, 10, rows, idx), steps)
+    ts = next(x for name, x, _ in cols if name.startswith("timestamps"))
+    tree = g.CompressionTree(ctx, [g.DELTA, g.AFL, g.NONE], g.D_INT)
+    blob = tree.Compress(ctx.upload(ts))
+    b0, cnt = len(ts) // 2, 1 << 20
+    part_ok = tree.DecompressRows(blob, b0, cnt).to_bytes() == ts[b0:b0 + cnt].tobytes()
+    tr = timeit(lambda: tree.DecompressRows(blob, b0, cnt), steps)
+    tfull = timeit(lambda: tree.Decompress(blob), steps)
+    out["random_access"] = {"fetch_rows": k, "fetch_mrows_per_s": round(k / tf / 1e6, 1), "fetch_exact": ok,
+                            "range_rows": cnt, "range_begin": b0, "range_ms": round(tr * 1e3, 4),
+                            "full_decode_ms": round(tfull * 1e3, 4), "range_exact": part_ok}
+    return out
+
+
 def copy_calibration():
     import numpy as np
     import gputimeseriescompressionoptimizer_amd as g
@@ -324,6 +393,8 @@ def main():
     ap.add_argument("--batch-workers", type=int, default=0, help="0: autotune the batch configuration")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-float", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip unique / 32-lane FL / random access on the benchmark's columns")
     ap.add_argument("--compress-only", action="store_true",
                     help="warmup and timed compress steps only (for hardware-counter runs)")
     ap.add_argument("--no-concurrent", action="store_true",
@@ -498,6 +569,9 @@ def main():
 
     step("fl plumbing")
     res["fl_plumbing"] = fl_plumbing_section(ctx)
+    if not args.no_secondary:
+        step("secondary paths")
+        res["secondary_paths"] = access_section(ctx, cols, args.rows, max(1, args.steps // 2))
     if not args.no_float:
         step("float columns")
         res["float_columns"] = float_section(ctx, args.float_rows, max(1, args.steps // 2))

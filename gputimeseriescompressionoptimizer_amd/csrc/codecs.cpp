@@ -390,11 +390,8 @@ size_t stencil_rows(Ctx* c, DView st) {
 // PatchEncodingFactory::Get(data) + LowerPatchEncoding / OutsidePatchEncoding
 // (compression/patch/patch_encoding_factory.hpp:40-57, lower_patch_encoding.cu:22-66,
 // outside_patch_encoding.cu:22-60)
-std::vector<Part> patch_encode(Ctx* c, int dtype, DView in, bool outside) {
-    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
-    if (n == 0) return {empty(), empty(), empty()};
-    Part x = aligned_view(c, in, es);
-    MinMax mm = minmax(c, dtype, x.p, n);
+PredArgs patch_pred(Ctx* c, int dtype, const uint8_t* x, size_t n, bool outside) {
+    MinMax mm = minmax(c, dtype, x, n);
     PredArgs p{};
     dispatch_native(dtype, [&](auto tag) {
         using T = decltype(tag);
@@ -412,6 +409,15 @@ std::vector<Part> patch_encode(Ctx* c, int dtype, DView in, bool outside) {
             p.b_bits = tbits((T)(mx - prod));
         }
     });
+    kernel_bytes(c, "k_minmax", (double)(n * dtype_size(dtype)));
+    return p;
+}
+
+std::vector<Part> patch_encode(Ctx* c, int dtype, DView in, bool outside) {
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    if (n == 0) return {empty(), empty(), empty()};
+    Part x = aligned_view(c, in, es);
+    const PredArgs p = patch_pred(c, dtype, x.p, n, outside);
     DevBuf st(c, 1 + (n + 7) / 8), yes(c, n * es), no(c, n * es), cnt(c, 8);
     // integer outputs carry their min/max (from the split's counting pass)
     const bool integral = dtype != DDJ_D_FLOAT && dtype != DDJ_D_DOUBLE;
@@ -433,7 +439,6 @@ std::vector<Part> patch_encode(Ctx* c, int dtype, DView in, bool outside) {
     }
     yes.set_size(m * es);
     no.set_size((n - m) * es);
-    kernel_bytes(c, "k_minmax", (double)(n * es));
     kernel_bytes(c, "k_split", (double)(2 * n * es + 1 + (n + 7) / 8));
     Part py = own(std::move(yes)), pn = own(std::move(no));
     py.minmax = mm_yes;
@@ -454,6 +459,28 @@ Part patch_decode(Ctx* c, int dtype, const std::vector<DView>& in) {
     launch_merge(c, dtype, in[0].p + 1, n, y.p, y.n / es, z.p, z.n / es, false, 0, out.get());
     return own(std::move(out));
 }
+
+// Patch whose two outputs go to FL leaves (patch -> (afl -> none), (afl ->
+// none), integer columns): the counting pass brings back the yes count and
+// both sides' min/max, so the FL widths and every size are known on the
+// host; the writing pass runs when the blob is assembled and writes the
+// stencil and both FL streams straight into it (launch_split_fl).  The
+// blob is byte-identical to patch_encode followed by the two AFL encoders.
+struct PatchFlJob {
+    SplitPlan plan;
+    Part x;
+    int by = 1, bn = 1;
+    uint64_t sbytes = 0, ybytes = 0, nbytes = 0;
+    uint8_t* dst[3] = {nullptr, nullptr, nullptr};  // stencil, yes stream, no stream
+    int expected = 0, seen = 0;
+    std::shared_ptr<DevBuf> count;
+    void place(Ctx* c, int which, uint8_t* p) {
+        dst[which] = p;
+        if (++seen < expected) return;
+        launch_split_fl(c, plan, dst[0], FlSide{dst[1], ybytes, by, nullptr}, FlSide{dst[2], nbytes, bn, nullptr},
+                        count->get<uint64_t>());
+    }
+};
 
 // ------------------------------------------------------------------ const
 // ConstEncoding (compression/const/const_encoding.cu:43-131)
@@ -703,6 +730,58 @@ Part gfc_decode_any(Ctx* c, int dtype, const std::vector<DView>& in) {
 }
 
 }  // namespace
+
+PatchAflParts patch_afl_encode(Ctx* c, int dtype, DView in, bool outside) {
+    const size_t es = dtype_size(dtype), n = rows(dtype, in.n);
+    if (n == 0 || dtype == DDJ_D_FLOAT || dtype == DDJ_D_DOUBLE) fail(DDJ_EINVAL, "fused patch: integer rows only");
+    Part x = aligned_view(c, in, es);
+    const PredArgs p = patch_pred(c, dtype, x.p, n, outside);
+    DevBuf mmb(c, 2 * sizeof(MinMaxResult));
+    MinMaxResult* mmy = mmb.get<MinMaxResult>();
+    auto job = std::make_shared<PatchFlJob>();
+    job->plan = launch_split_count(c, dtype, x.p, n, p, mmy, mmy + 1);
+    job->x = x;
+    job->count = std::make_shared<DevBuf>(c, 8);
+    uint64_t m = 0;
+    MinMaxResult h[2];
+    const ReadBack rb[2] = {{&m, job->plan.ts->prefix + job->plan.ts->tiles, 8}, {h, mmb.get(), sizeof h}};
+    d2h_many(c, rb, 2);
+    const uint64_t sides[2] = {m, n - m};
+    PatchAflParts out;
+    uint64_t bytes[2];
+    int bits[2];
+    for (int k = 0; k < 2; k++) {
+        bits[k] = sides[k] ? bits_of(dtype, h[k]) : 1;
+        bytes[k] = sides[k] ? fl_bytes(es, bits[k], sides[k]) : 0;
+        out.side_in[k].n = sides[k] * es;  // never materialised: only its size is read
+        if (sides[k]) {
+            const uint8_t meta[4] = {(uint8_t)bits[k], (uint8_t)((bytes[k] * 8 - sides[k] * (uint64_t)bits[k]) & 0xff),
+                                     0, 0};
+            out.side_meta[k] = host_part(meta, 4);
+        }
+    }
+    job->by = bits[0];
+    job->bn = bits[1];
+    job->sbytes = 1 + (n + 7) / 8;
+    job->ybytes = bytes[0];
+    job->nbytes = bytes[1];
+    job->expected = 1 + (bytes[0] ? 1 : 0) + (bytes[1] ? 1 : 0);
+    kernel_bytes(c, "k_split_count", (double)(n * es));
+    auto deferred = [&](int which, uint64_t nb) {
+        Part part;
+        part.n = (size_t)nb;
+        if (!nb) return part;
+        auto fj = std::make_shared<FlJob>();
+        fj->dtype = dtype;
+        fj->write = [job, which](Ctx* cc, uint8_t* dst, size_t) { job->place(cc, which, dst); };
+        part.fl = std::move(fj);
+        return part;
+    };
+    out.stencil = deferred(0, job->sbytes);
+    out.side_data[0] = deferred(1, bytes[0]);
+    out.side_data[1] = deferred(2, bytes[1]);
+    return out;
+}
 
 Part delta_afl_decode(Ctx* c, int dtype, DView dmeta, DView ameta, DView data) {
     const size_t es = dtype_size(dtype);

@@ -93,6 +93,20 @@ std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in, bool defer_fl = 
 // The inverse: AFL data (non-empty) decoded and prefix-summed in one pass.
 Part delta_afl_decode(Ctx* c, int dtype, DView delta_meta, DView afl_meta, DView afl_data);
 
+// Patch (lower, or outside with `outside`) whose two outputs each go to an
+// AFL node stored unchanged, on an integer column of >= 1 row: the patch
+// metadata (the stencil) and both FL data streams are FlJob parts written
+// together by one fused split when the blob is assembled; each side's AFL
+// metadata is host-resident and its input size (rows * sizeof(T)) is the
+// size of side_in (no bytes).
+struct PatchAflParts {
+    Part stencil;
+    Part side_in[2];
+    Part side_meta[2];
+    Part side_data[2];
+};
+PatchAflParts patch_afl_encode(Ctx* c, int dtype, DView in, bool outside);
+
 // Rows [begin, begin + count) of what an integer AFL stream (metadata,
 // data) decodes to, unpacking only those rows.
 Part afl_decode_rows(Ctx* c, int dtype, DView meta, DView data, size_t begin, size_t count);

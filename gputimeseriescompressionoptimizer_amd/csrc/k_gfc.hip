@@ -317,7 +317,22 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
         if (threadIdx.x == 0 && P + chunk_bytes > payload_bytes) s_bad = 1;
     }
     __syncthreads();
-    // header walk: one wave, lanes 0..31 read one nibble each
+    const uint32_t* bw = reinterpret_cast<const uint32_t*>(buf);
+    constexpr uint32_t kBufWords = sizeof(buf) / 4;
+    // chunk bytes [a, a + 4); words past the buffer (corrupt input only)
+    // read as zero
+    auto lds_word = [&](uint32_t a) -> uint32_t {
+        a += base;
+        const uint32_t q = a >> 2, s = a & 3;
+        const uint32_t lo = q < kBufWords ? bw[q] : 0u;
+        if (!s) return lo;
+        const uint32_t hi = q + 1 < kBufWords ? bw[q + 1] : 0u;
+        return __builtin_amdgcn_alignbyte(hi, lo, s);
+    };
+    // header walk (the format's one sequential step): one wave, lanes 0..31
+    // reading one nibble each, their byte counts summed by a wave scan.  (One
+    // lane summing the header's four words nibble-parallel instead takes
+    // twice as long: its dependent LDS reads are the chain.)
     if (threadIdx.x < kWave) {
         const int lane = threadIdx.x;
         uint32_t off = 0;
@@ -340,18 +355,6 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     }
     __syncthreads();
     const int sub = threadIdx.x / 4;
-    const uint32_t* bw = reinterpret_cast<const uint32_t*>(buf);
-    constexpr uint32_t kBufWords = sizeof(buf) / 4;
-    // chunk bytes [a, a + 4); words past the buffer (corrupt input only)
-    // read as zero
-    auto lds_word = [&](uint32_t a) -> uint32_t {
-        a += base;
-        const uint32_t q = a >> 2, s = a & 3;
-        const uint32_t lo = q < kBufWords ? bw[q] : 0u;
-        if (!s) return lo;
-        const uint32_t hi = q + 1 < kBufWords ? bw[q + 1] : 0u;
-        return __builtin_amdgcn_alignbyte(hi, lo, s);
-    };
     auto nb_of = [](uint32_t nib) -> uint32_t {
         const uint32_t code = nib & 7;
         return dbl ? (code >= 2 ? code + 1 : code) : code;
@@ -464,6 +467,16 @@ __global__ __launch_bounds__(kWave) void k_gfc_decode_wave(const uint8_t* __rest
         if (P + chunk_bytes > payload_bytes) bad = true;
     }
     __syncthreads();
+    const uint32_t* bw = reinterpret_cast<const uint32_t*>(buf);
+    constexpr uint32_t kBufWords = sizeof(buf) / 4;
+    auto lds_word = [&](uint32_t a) -> uint32_t {  // chunk bytes [a, a + 4); past the buffer: zero
+        a += base;
+        const uint32_t q = a >> 2, sh = a & 3;
+        const uint32_t lo = q < kBufWords ? bw[q] : 0u;
+        if (!sh) return lo;
+        const uint32_t hi = q + 1 < kBufWords ? bw[q + 1] : 0u;
+        return __builtin_amdgcn_alignbyte(hi, lo, sh);
+    };
     // header walk, lanes 0..31 reading one nibble each
     {
         uint32_t off = 0;
@@ -482,16 +495,6 @@ __global__ __launch_bounds__(kWave) void k_gfc_decode_wave(const uint8_t* __rest
         if (off != chunk_bytes) bad = true;
     }
     __syncthreads();
-    const uint32_t* bw = reinterpret_cast<const uint32_t*>(buf);
-    constexpr uint32_t kBufWords = sizeof(buf) / 4;
-    auto lds_word = [&](uint32_t a) -> uint32_t {  // chunk bytes [a, a + 4); past the buffer: zero
-        a += base;
-        const uint32_t q = a >> 2, sh = a & 3;
-        const uint32_t lo = q < kBufWords ? bw[q] : 0u;
-        if (!sh) return lo;
-        const uint32_t hi = q + 1 < kBufWords ? bw[q + 1] : 0u;
-        return __builtin_amdgcn_alignbyte(hi, lo, sh);
-    };
     auto nb_of = [](uint32_t nib) -> uint32_t {
         const uint32_t code = nib & 7;
         return dbl ? (code >= 2 ? code + 1 : code) : code;
@@ -644,16 +647,23 @@ void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const 
         ts.scan(c);
         kernel_bytes(c, "k_gfc_chunk_bytes", 12.0 * (double)tiles);
         kernel_bytes(c, "k_gfc_decode", (double)payload_bytes + 4.0 * (double)tiles + (double)n * sizeof(U));
-        constexpr int kWaveMaxPer = sizeof(U) == 8 ? 2048 : 1024;
+        // float chunks (at most 1024 rows, 4.6 KB staged) decode one wave
+        // each: 2.1x faster than four waves a chunk (float32 walk, 2^26 rows,
+        // 273 -> 132 us).  Double chunks stage 17 KB, so one wave each would
+        // leave nine waves a CU: 367 -> 456 us; they keep four.
         KernelTimer kt_(c, "k_gfc_decode");
-        if (g.per <= kWaveMaxPer)
-            hipLaunchKernelGGL((k_gfc_decode_wave<U, kWaveMaxPer>), dim3((unsigned)tiles), dim3(kWave), 0, c->stream,
-                               payload, payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<U>(),
-                               (const uint64_t*)ts.prefix, err_flag);
-        else
-            hipLaunchKernelGGL(k_gfc_decode<U>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
-                               payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<U>(),
-                               (const uint64_t*)ts.prefix, err_flag);
+        if constexpr (sizeof(U) == 4) {
+            if (g.per <= 1024) {
+                hipLaunchKernelGGL((k_gfc_decode_wave<U, 1024>), dim3((unsigned)tiles), dim3(kWave), 0, c->stream,
+                                   payload, payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<U>(),
+                                   (const uint64_t*)ts.prefix, err_flag);
+                HIPC(hipGetLastError());
+                return;
+            }
+        }
+        hipLaunchKernelGGL(k_gfc_decode<U>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
+                           payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<U>(),
+                           (const uint64_t*)ts.prefix, err_flag);
         HIPC(hipGetLastError());
     };
     if (dbl)

@@ -271,6 +271,201 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     if (threadIdx.x == 0 && base + cnt == n) d_count[0] = Y0 + total;
 }
 
+// The split of a patch node whose outputs go to FL leaves (patch -> (afl ->
+// none), (afl -> none)), with the FL encoding fused in: the stencil and the
+// compaction of k_split, then each side's rows are packed at their bit
+// offset in their FL stream (a side's row r at bits [r b, r b + b)), so that
+// neither side is ever written unpacked.  Thread g < 128 packs yes rows
+// [32g, 32g + 32) of the tile, thread 128 + g the no rows, into b words in
+// LDS as if the side's tile stream started at bit 0; the words are then
+// funnel-shifted to the stream's true bit offset.  Words the tile fills
+// alone are stored at any byte address (16-byte stores realigned to the
+// destination, single bytes at the ends); the first and last word of a side
+// that the tile shares with its neighbours go to edges[4 t + 2 side + 0/1],
+// assembled by k_fl_edges.
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, size_t n, Pred<T> p,
+                                                     uint8_t* __restrict__ stencil, FlSide yside, FlSide nside,
+                                                     const uint64_t* __restrict__ prefix, uint64_t* d_count) {
+    using U = typename UnsignedOf<T>::type;
+    static_assert(sizeof(T) * kTilePad >= 4 * (kTile / 32 + 2) * 8 * sizeof(T), "packed words fit the tile buffer");
+    __shared__ __attribute__((aligned(16))) T buf[kTilePad];
+    __shared__ uint64_t red[kBlock / kWave];
+    const uint32_t tile = blockIdx.x;
+    const size_t base = (size_t)tile * kTile;
+    const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
+    const T padv = x[n - 1];
+    const uint64_t Y0 = prefix[tile];
+    if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
+        constexpr int V = 16 / sizeof(T);
+        const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
+        for (int i = threadIdx.x; i < kTile / V; i += kBlock) {
+            const u32x4 w = __builtin_nontemporal_load(s4 + i);
+            T vals[V];
+            __builtin_memcpy(vals, &w, 16);
+#pragma unroll
+            for (int k = 0; k < V; k++) buf[P(V * i + k)] = vals[k];
+        }
+    } else {
+        for (int i = threadIdx.x; i < kTile; i += kBlock) buf[P(i)] = i < cnt ? x[base + i] : padv;
+    }
+    __syncthreads();
+    T v[kItems];
+    uint32_t mask = 0;
+    int c = 0;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        v[k] = buf[P(threadIdx.x * kItems + k)];
+        uint32_t code;
+        const bool f = p.template eval<K>(v[k], code);
+        mask |= (uint32_t)f << k;
+        if (threadIdx.x * kItems + k < cnt && f) c++;
+    }
+    const size_t g0 = base + (size_t)threadIdx.x * kItems;
+    if (g0 < n) stencil[1 + g0 / 8] = (uint8_t)(mask & 0xff);
+    if (g0 + 8 < n) stencil[2 + g0 / 8] = (uint8_t)(mask >> 8);
+    if (tile == 0 && threadIdx.x == 0) stencil[0] = (uint8_t)(n % 8);
+    uint64_t total;
+    const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)c, &total, red);
+    int yr = (int)excl, nr = threadIdx.x * kItems - (int)excl;
+#pragma unroll
+    for (int k = 0; k < kItems; k++) {
+        if (threadIdx.x * kItems + k >= cnt) break;
+        if ((mask >> k) & 1)
+            buf[P(yr++)] = v[k];
+        else
+            buf[P((int)total + nr++)] = v[k];
+    }
+    __syncthreads();
+    const int cy = (int)total, cn = cnt - cy;
+    // pack: thread g < 128 the yes group g, thread 128 + g the no group g
+    const bool is_yes = threadIdx.x < kBlock / 2;
+    const int grp = threadIdx.x & (kBlock / 2 - 1);
+    const int cs = is_yes ? cy : cn;
+    const int b = is_yes ? yside.bits : nside.bits;
+    U w32[32];
+    const bool packs = 32 * grp < cs;
+    if (packs) {
+        const int sb = is_yes ? 0 : cy;
+#pragma unroll
+        for (int k = 0; k < 32; k++) {
+            const int i = 32 * grp + k;
+            w32[k] = i < cs ? (U)buf[P(sb + i)] : U(0);
+        }
+    }
+    __syncthreads();  // the packed words overwrite buf
+    uint32_t* words = reinterpret_cast<uint32_t*>(buf);
+    const int wy = (cy + 31) / 32 * yside.bits;  // yes words first, then the no words
+    if (packs) {
+        uint32_t* w = words + (is_yes ? 0 : wy) + grp * b;
+        const uint64_t m = lowmask64((unsigned)b);
+        uint64_t acc = 0;
+        int nb = 0, k = 0;
+        auto push = [&](uint64_t bits, int len) {  // len <= 32
+            acc |= bits << nb;
+            nb += len;
+            if (nb >= 32) {
+                w[k++] = (uint32_t)acc;
+                acc >>= 32;
+                nb -= 32;
+            }
+        };
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+            const uint64_t val = (uint64_t)w32[i] & m;
+            if (sizeof(T) <= 4 || b <= 32) {
+                push(val, b);
+            } else {
+                push(val & 0xffffffffull, 32);
+                push(val >> 32, b - 32);
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        const FlSide fs = side == 0 ? yside : nside;
+        const int ns = side == 0 ? cy : cn;
+        if (ns == 0) continue;  // uniform
+        const uint64_t R0 = side == 0 ? Y0 : (uint64_t)(base - Y0);
+        const int bb = fs.bits;
+        const uint64_t bit0 = R0 * (uint64_t)bb;
+        const uint32_t o = (uint32_t)(bit0 & 31);
+        const uint64_t wf = bit0 >> 5;
+        const uint64_t L = (uint64_t)ns * bb;
+        const int nw = (int)((o + L + 31) >> 5);
+        const int nloc = (int)((L + 31) >> 5);
+        const uint32_t* lw = words + (side == 0 ? 0 : wy);
+        auto local = [&](int q) -> uint32_t { return (q >= 0 && q < nloc) ? lw[q] : 0u; };
+        auto outw = [&](int q) -> uint32_t {  // stream word wf + q as this tile fills it
+            return o ? ((local(q) << o) | (local(q - 1) >> (32 - o))) : local(q);
+        };
+        const int q_lo = o ? 1 : 0;
+        const int q_hi = (int)((o + L) >> 5);  // words this tile fills to their end
+        if (threadIdx.x == 0 && (o || q_hi == 0)) fs.edges[4 * (size_t)tile + 2 * side] = outw(0);
+        if (threadIdx.x == 1 && nw > 1 && q_hi < nw) fs.edges[4 * (size_t)tile + 2 * side + 1] = outw(nw - 1);
+        if (q_hi <= q_lo) continue;
+        // stream bytes [4 (wf + q_lo), 4 (wf + q_hi)), clipped to the stream
+        uint64_t B0 = 4 * (wf + (uint64_t)q_lo), B1 = 4 * (wf + (uint64_t)q_hi);
+        if (B1 > fs.bytes) B1 = fs.bytes;
+        if (B1 <= B0) continue;
+        uint8_t* d = fs.dst + B0;
+        const int Lb = (int)(B1 - B0);
+        auto byte_at = [&](int j) -> uint8_t { return (uint8_t)(outw(q_lo + (j >> 2)) >> (8 * (j & 3))); };
+        int head = (int)((16 - (reinterpret_cast<uintptr_t>(d) & 15)) & 15);
+        head = head < Lb ? head : Lb;
+        if ((int)threadIdx.x < head) d[threadIdx.x] = byte_at(threadIdx.x);
+        const uint32_t r = (uint32_t)(head & 3);
+        const int nq = (Lb - head) / 16;
+        u32x4* d4 = reinterpret_cast<u32x4*>(d + head);
+        for (int i = threadIdx.x; i < nq; i += kBlock) {
+            const int q = q_lo + ((head + 16 * i) >> 2);
+            const uint32_t a0 = outw(q), a1 = outw(q + 1), a2 = outw(q + 2), a3 = outw(q + 3), a4 = outw(q + 4);
+            if (r)
+                __builtin_nontemporal_store(u32x4{__builtin_amdgcn_alignbyte(a1, a0, r), __builtin_amdgcn_alignbyte(a2, a1, r),
+                                                  __builtin_amdgcn_alignbyte(a3, a2, r), __builtin_amdgcn_alignbyte(a4, a3, r)},
+                                            d4 + i);
+            else
+                __builtin_nontemporal_store(u32x4{a0, a1, a2, a3}, d4 + i);
+        }
+        for (int j = head + 16 * nq + threadIdx.x; j < Lb; j += kBlock) d[j] = byte_at(j);
+    }
+    if (threadIdx.x == 0 && base + cnt == n) d_count[0] = Y0 + total;
+}
+
+// Stream words of a fused patch split that several tiles fill: the tile
+// holding a word's first bit assembles it from its own last-word edge and
+// the first-word edges of the following tiles whose rows reach into it, and
+// writes its bytes (those inside the stream).  Side 0 = yes rows (tile t
+// starts at row prefix[t]), side 1 = no rows (t * 4096 - prefix[t]).
+__global__ __launch_bounds__(kBlock) void k_fl_edges(const uint64_t* __restrict__ prefix, size_t tiles, size_t n,
+                                                     FlSide yside, FlSide nside) {
+    const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= 2 * tiles) return;
+    const size_t t = i >> 1;
+    const int side = (int)(i & 1);
+    const FlSide fs = side ? nside : yside;
+    auto rows0 = [&](size_t u) -> uint64_t { return side ? (uint64_t)(u * kTile) - prefix[u] : prefix[u]; };
+    auto rows_of = [&](size_t u) -> uint64_t {
+        const uint64_t c = (n - u * kTile) < (uint64_t)kTile ? (n - u * kTile) : (uint64_t)kTile;
+        const uint64_t y = prefix[u + 1] - prefix[u];
+        return side ? c - y : y;
+    };
+    const uint64_t ns = rows_of(t);
+    if (ns == 0) return;
+    const uint64_t b = (uint64_t)fs.bits;
+    const uint64_t first = rows0(t) * b, end = (rows0(t) + ns) * b;
+    const uint64_t wf = first >> 5, wl = (end - 1) >> 5;
+    if (32 * wl < first || end >= 32 * wl + 32) return;  // the word's bit 0 lies earlier, or the tile fills it
+    uint32_t w = fs.edges[4 * t + 2 * side + (wl == wf ? 0 : 1)];
+    for (size_t u = t + 1; u < tiles; u++) {
+        if (rows0(u) * b >= 32 * wl + 32) break;
+        if (rows_of(u)) w |= fs.edges[4 * u + 2 * side];  // its first word is this one
+    }
+    for (int j = 0; j < 4; j++)
+        if (4 * wl + j < fs.bytes) fs.dst[4 * wl + j] = (uint8_t)(w >> (8 * j));
+}
+
 // Code words that several tiles fill: tile t's codes are [prefix[t],
 // prefix[t+1]); a word whose field 0 lies in tile t and which t leaves
 // partly empty is assembled by t's thread from t's contribution and the
@@ -935,6 +1130,40 @@ void launch_split(Ctx* c, int dtype, const void* x, size_t n, const PredArgs& pa
     if (n == 0) return;
     const SplitPlan plan = launch_split_count(c, dtype, x, n, pa, mm_yes, mm_no);
     launch_split_write(c, plan, stencil, yes, static_cast<uint8_t*>(no), codes, d_counts);
+}
+
+void launch_split_fl(Ctx* c, const SplitPlan& plan, uint8_t* stencil, FlSide yes, FlSide no, uint64_t* d_counts) {
+    if (plan.n == 0) return;
+    if (plan.p.kind != PRED_LOWER && plan.p.kind != PRED_INSIDE) fail(DDJ_EINVAL, "FL split: patch predicates only");
+    const size_t tiles = plan.ts->tiles;
+    DevBuf edges(c, 16 * tiles);
+    yes.edges = edges.get<uint32_t>();
+    no.edges = yes.edges;
+    dispatch_int((int)dtype_size(plan.dtype), [&](auto tag) {
+        using T = decltype(tag);
+        if (plan.dtype == DDJ_D_FLOAT || plan.dtype == DDJ_D_DOUBLE) fail(DDJ_EINVAL, "FL split: integer columns only");
+        if (yes.bits <= 0 || yes.bits > 8 * (int)sizeof(T) || no.bits <= 0 || no.bits > 8 * (int)sizeof(T))
+            fail(DDJ_EINVAL, "FL split: bit width out of range");
+        const Pred<T> p = make_pred<T>(plan.p);
+        const uint64_t* pre = plan.ts->prefix;
+        kernel_bytes(c, "k_split", (double)plan.n * sizeof(T) + 1 + (double)((plan.n + 7) / 8) + (double)yes.bytes +
+                                       (double)no.bytes);
+        {
+            KernelTimer kt_(c, "k_split");
+            with_kind(p.kind, [&](auto kind) {
+                constexpr int K = decltype(kind)::value;
+                if constexpr (K == PRED_LOWER || K == PRED_INSIDE)
+                    hipLaunchKernelGGL((k_split_fl<T, K>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream,
+                                       static_cast<const T*>(plan.x), plan.n, p, stencil, yes, no, pre, d_counts);
+            });
+        }
+        HIPC(hipGetLastError());
+        no.edges = yes.edges + 0;  // both sides share the array: 4 words a tile
+        KernelTimer kt_(c, "k_fl_edges");
+        hipLaunchKernelGGL(k_fl_edges, dim3((unsigned)((2 * tiles + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
+                           pre, tiles, plan.n, yes, no);
+        HIPC(hipGetLastError());
+    });
 }
 
 namespace {

@@ -170,6 +170,18 @@ SplitPlan launch_split_count(Ctx* c, int dtype, const void* x, size_t n, const P
                              MinMaxResult* mm_yes = nullptr, MinMaxResult* mm_no = nullptr);
 void launch_split_write(Ctx* c, const SplitPlan& plan, uint8_t* stencil, void* yes, uint8_t* no, uint8_t* codes,
                         uint64_t* d_counts);
+// One output of a patch split FL-encoded straight into its stream: `bytes`
+// stream bytes at dst (any byte address), values of `bits` bits.
+struct FlSide {
+    uint8_t* dst;
+    uint64_t bytes;
+    int bits;
+    uint32_t* edges;  // set by the launcher
+};
+// The writing pass of a patch split whose two outputs are FL leaves: the
+// stencil, and both sides packed into their streams (k_split_fl, then
+// k_fl_edges for the words neighbouring tiles share).
+void launch_split_fl(Ctx* c, const SplitPlan& plan, uint8_t* stencil, FlSide yes, FlSide no, uint64_t* d_counts);
 // Merge: out[i] = bit i ? yes[k++] : no[j++] (const_no: `no_const` for every
 // clear bit).  yes / no may sit at any byte address (outputs stored
 // unchanged in a blob); n_yes / n_no rows are available; a stencil asking

@@ -61,12 +61,23 @@ bool fusable_delta_afl(const TNode& node, const Part& data) {
     return data.n / dtype_size(node.dtype) >= 2;
 }
 
+// PATCH -> (AFL -> none), (AFL -> none) on an integer column: the split
+// FL-packs both sides straight into the blob (patch_afl_encode).
+bool fusable_patch_afl(const TNode& node, const Part& data) {
+    if ((node.enc != DDJ_ENC_PATCH && node.enc != DDJ_ENC_OUTSIDE_PATCH) || !is_int_type(node.dtype) ||
+        node.ch.size() != 2)
+        return false;
+    for (const NodeP& a : node.ch)
+        if (a->enc != DDJ_ENC_AFL || a->dtype != node.dtype || !output_is_leaf(*a)) return false;
+    return data.n / dtype_size(node.dtype) >= 1;
+}
+
 // CompressionNode::Compress (compression_node.cpp:103-138) and PrepareMetadata (:84-101).
 // `pre` holds this node's encoder outputs when a parent already produced them.
 size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& out,
                      const std::vector<Part>* pre = nullptr) {
     std::vector<Part> res;
-    std::vector<Part> child_pre;
+    std::vector<std::vector<Part>> child_pre;  // per child: its outputs when this node produced them
     if (data.owner) {
         Piece k;
         k.keep_alive = true;
@@ -85,7 +96,11 @@ size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& 
         Part deltas;  // never materialised: only its size is read
         deltas.n = (data.n / es - 1) * es;
         res = {f[0], deltas};
-        child_pre = {f[1], f[2]};
+        child_pre = {{f[1], f[2]}};
+    } else if (fusable_patch_afl(node, data)) {
+        auto f = patch_afl_encode(c, node.dtype, data.view(), node.enc == DDJ_ENC_OUTSIDE_PATCH);
+        res = {f.stencil, f.side_in[0], f.side_in[1]};
+        child_pre = {{f.side_meta[0], f.side_data[0]}, {f.side_meta[1], f.side_data[1]}};
     } else {
         // an integer AFL whose data goes to the blob unchanged packs it there
         // when the blob is assembled; a dictionary split whose two outputs
@@ -120,7 +135,7 @@ size_t compress_node(Ctx* c, TNode& node, const Part& data, std::vector<Piece>& 
     } else {
         for (size_t i = 0; i < node.ch.size(); i++) {
             if (i + 1 >= res.size()) fail(DDJ_EINVAL, "tree node has more children than its encoding has outputs");
-            total += compress_node(c, *node.ch[i], res[i + 1], out, child_pre.empty() ? nullptr : &child_pre);
+            total += compress_node(c, *node.ch[i], res[i + 1], out, i < child_pre.size() ? &child_pre[i] : nullptr);
         }
     }
     node.ratio = ratio_of(data.n, total);

@@ -194,3 +194,26 @@ def test_float_plan_takes_only_trees_the_type_can_take(ctx):
         assert blob == O.compress(O.tree_from_path(path, O.D_DOUBLE), x)
     with pytest.raises(g.DdjError):
         g.LearnedPlanner(ctx, cls=g.PLANNER_INT).Plan(ctx.upload(D.price_walk_f64(100, seed=1)), g.D_DOUBLE)
+
+
+def test_float_trained_weights_stay_near_exhaustive_and_beat_untrained(ctx):
+    """The committed float weights on a float32 / float64 stream whose family
+    changes every 10 blocks: within 2% of the exhaustive planner's total
+    compressed size, and clearly better than an untrained network."""
+    from gputimeseriescompressionoptimizer_amd import planner_data as PD
+    from gputimeseriescompressionoptimizer_amd.train_planner import WEIGHTS_FLOAT
+
+    net = g.LearnedPlanner(ctx, cls=g.PLANNER_FLOAT)
+    net.Load(WEIGHTS_FLOAT)
+    untrained = g.LearnedPlanner(ctx, seed=321, cls=g.PLANNER_FLOAT)
+    tot = {"learned": 0, "untrained": 0, "exhaustive": 0}
+    for _, fam, x in PD.float_regime_stream(90, 65536, 10, seed=77):
+        dt = g.DTYPE_OF_NP[x.dtype]
+        buf = ctx.upload(x)
+        tot["learned"] += g.CompressionTree(ctx, net.Plan(buf, dt)[0], dt).Compress(buf).nbytes
+        tot["untrained"] += g.CompressionTree(ctx, untrained.Plan(buf, dt)[0], dt).Compress(buf).nbytes
+        tree, _, _ = g.CompressionOptimizer(ctx).PlanExhaustive(buf, dt)
+        tot["exhaustive"] += tree.Compress(buf).nbytes
+    print(tot, flush=True)
+    assert tot["learned"] <= 1.02 * tot["exhaustive"]
+    assert tot["untrained"] > 1.05 * tot["learned"]
