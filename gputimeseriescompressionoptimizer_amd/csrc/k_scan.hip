@@ -430,7 +430,14 @@ __global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, si
         }
         for (int j = head + 16 * nq + threadIdx.x; j < Lb; j += kBlock) d[j] = byte_at(j);
     }
-    if (threadIdx.x == 0 && base + cnt == n) d_count[0] = Y0 + total;
+    if (base + cnt == n && threadIdx.x < 2) {
+        // A 64-bit stream ends on an 8-byte boundary: the word after the
+        // side's last bit (if any) holds no row, so no tile writes it.
+        const FlSide fs = threadIdx.x == 0 ? yside : nside;
+        const uint64_t rows = threadIdx.x == 0 ? Y0 + total : n - (Y0 + total);
+        for (uint64_t j = 4 * ((rows * (uint64_t)fs.bits + 31) >> 5); j < fs.bytes; j++) fs.dst[j] = 0;
+        if (threadIdx.x == 0) d_count[0] = Y0 + total;
+    }
 }
 
 // Stream words of a fused patch split that several tiles fill: the tile
@@ -1138,7 +1145,7 @@ void launch_split_fl(Ctx* c, const SplitPlan& plan, uint8_t* stencil, FlSide yes
     const size_t tiles = plan.ts->tiles;
     DevBuf edges(c, 16 * tiles);
     yes.edges = edges.get<uint32_t>();
-    no.edges = yes.edges;
+    no.edges = yes.edges;  // one array for both sides: 4 words a tile
     dispatch_int((int)dtype_size(plan.dtype), [&](auto tag) {
         using T = decltype(tag);
         if (plan.dtype == DDJ_D_FLOAT || plan.dtype == DDJ_D_DOUBLE) fail(DDJ_EINVAL, "FL split: integer columns only");
@@ -1146,10 +1153,10 @@ void launch_split_fl(Ctx* c, const SplitPlan& plan, uint8_t* stencil, FlSide yes
             fail(DDJ_EINVAL, "FL split: bit width out of range");
         const Pred<T> p = make_pred<T>(plan.p);
         const uint64_t* pre = plan.ts->prefix;
-        kernel_bytes(c, "k_split", (double)plan.n * sizeof(T) + 1 + (double)((plan.n + 7) / 8) + (double)yes.bytes +
+        kernel_bytes(c, "k_split_fl", (double)plan.n * sizeof(T) + 1 + (double)((plan.n + 7) / 8) + (double)yes.bytes +
                                        (double)no.bytes);
         {
-            KernelTimer kt_(c, "k_split");
+            KernelTimer kt_(c, "k_split_fl");
             with_kind(p.kind, [&](auto kind) {
                 constexpr int K = decltype(kind)::value;
                 if constexpr (K == PRED_LOWER || K == PRED_INSIDE)
@@ -1158,7 +1165,6 @@ void launch_split_fl(Ctx* c, const SplitPlan& plan, uint8_t* stencil, FlSide yes
             });
         }
         HIPC(hipGetLastError());
-        no.edges = yes.edges + 0;  // both sides share the array: 4 words a tile
         KernelTimer kt_(c, "k_fl_edges");
         hipLaunchKernelGGL(k_fl_edges, dim3((unsigned)((2 * tiles + kBlock - 1) / kBlock)), dim3(kBlock), 0, c->stream,
                            pre, tiles, plan.n, yes, no);
