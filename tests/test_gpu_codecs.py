@@ -245,6 +245,26 @@ def test_dict_codes_shared_between_tiles(ctx, every):
     check(ctx, g.DICT, g.D_INT, x)
 
 
+@pytest.mark.parametrize("case", ["zipf_tail", "load_over_70pct", "all_distinct_f64", "small_all_distinct"])
+def test_dict_histogram_table_growth(ctx, case):
+    """Most-frequent values of columns whose distinct values outgrow the
+    first hash table (a full table that drops rows, or one past 70% load),
+    and of a small column whose first table is sized by its length: the
+    dictionary and codes are the oracle's."""
+    print(f"[dict table growth] {case}", flush=True)
+    r = np.random.default_rng(11)
+    dt = g.D_INT
+    if case == "zipf_tail":  # ~10^5 distinct values, most rows on a few
+        x = ((r.zipf(1.5, 2_000_000) % 100_000) * 100).astype(np.int32)
+    elif case == "load_over_70pct":  # 50,000 values four times each
+        x = r.permutation(np.repeat(np.arange(50_000, dtype=np.int32), 4))
+    elif case == "all_distinct_f64":  # a random walk: every row distinct
+        x, dt = np.cumsum(r.normal(0, 1, 1_000_000)), g.D_DOUBLE
+    else:
+        x = r.permutation(3000).astype(np.int32)
+    check(ctx, g.DICT, dt, x)
+
+
 def test_rle_long_runs_with_a_stretch_of_short_ones(ctx):
     """Long runs around a stretch of single-row runs: expansion tiles see
     from one run to as many runs as rows."""
