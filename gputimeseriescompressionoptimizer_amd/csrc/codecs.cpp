@@ -849,6 +849,20 @@ size_t afl_stream_rows(Ctx* c, int dtype, DView meta, size_t data_bytes, int* b_
 }
 }  // namespace
 
+// PatchEncoding::Decode over two AflEncoding::Decode (lower_patch_encoding.cu,
+// afl_encoding.cu:74-113): the merge reads both FL streams directly.
+Part patch_afl_decode(Ctx* c, int dtype, DView stencil, DView ymeta, DView ydata, DView nmeta, DView ndata) {
+    int by = 0, bn = 0;
+    const size_t ny = afl_stream_rows(c, dtype, ymeta, ydata.n, &by);
+    const size_t nn = afl_stream_rows(c, dtype, nmeta, ndata.n, &bn);
+    if (ny == 0 && nn == 0) return empty();
+    const size_t n = stencil_rows(c, stencil);
+    if (ny + nn < n) fail(DDJ_ECORRUPT, "patch outputs shorter than the stencil");
+    DevBuf out(c, n * dtype_size(dtype));
+    launch_merge_fl(c, dtype, stencil.p + 1, n, ydata.p, ydata.n, ny, by, ndata.p, ndata.n, nn, bn, out.get());
+    return own(std::move(out));
+}
+
 Part afl_decode_rows(Ctx* c, int dtype, DView meta, DView data, size_t begin, size_t count) {
     int b = 0;
     const size_t n = afl_stream_rows(c, dtype, meta, data.n, &b);

@@ -109,6 +109,9 @@ PatchAflParts patch_afl_encode(Ctx* c, int dtype, DView in, bool outside);
 
 // Rows [begin, begin + count) of what an integer AFL stream (metadata,
 // data) decodes to, unpacking only those rows.
+// patch -> (afl -> none), (afl -> none) decoded in one merge over both FL
+// streams (the patch's stencil, each AFL node's metadata and leaf data).
+Part patch_afl_decode(Ctx* c, int dtype, DView stencil, DView ymeta, DView ydata, DView nmeta, DView ndata);
 Part afl_decode_rows(Ctx* c, int dtype, DView meta, DView data, size_t begin, size_t count);
 // The same for a column stored as delta -> AFL: the deltas before the range
 // are summed (per tile, never written), the range unpacked and scanned.

@@ -1,4 +1,6 @@
-// The FL tile packer shared by the single-column and batched kernels.
+// The FL tile packer shared by the single-column and batched kernels, and
+// the staging of an FL stream's row range in LDS shared by the unpack and
+// merge kernels.
 #pragma once
 
 #include "k_common.h"
@@ -120,6 +122,80 @@ __device__ __forceinline__ void fl_pack_tile(const T* __restrict__ x, size_t n, 
         __builtin_nontemporal_store(u32x4{src_word(s), src_word(s + 4), src_word(s + 8), src_word(s + 12)}, d4 + i);
     }
     for (int j = head + 16 * nq + threadIdx.x; j < L; j += kBlock) d[j] = wb[j];
+}
+
+// Staged words of one tile at b <= MAXB bits: at most kFlTile * MAXB bits,
+// plus the bytes before the stream start in its first 16-byte chunk and one
+// zero chunk after it.  The unpack kernels come in MAXB = 16, 32, 64, so that
+// narrow streams stage in less LDS and more workgroups fit on a CU.
+template <int MAXB> constexpr int fl_stage_words() { return kFlTile * MAXB / 32 + 16; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
+// Bytes [lo, hi) of a 32-bit word kept, the others cleared.
+__device__ __forceinline__ uint32_t byte_mask(int lo, int hi) {
+    lo = lo < 0 ? 0 : (lo > 4 ? 4 : lo);
+    hi = hi < 0 ? 0 : (hi > 4 ? 4 : hi);
+    if (hi <= lo) return 0u;
+    const uint32_t upto = hi == 4 ? ~0u : ((1u << (8 * hi)) - 1u);
+    const uint32_t from = lo == 4 ? 0u : ~((1u << (8 * lo)) - 1u);
+    return upto & from;
+}
+
+// Stages the b-bit values [v0, v0 + nv) of the stream `in` (any byte
+// address; v0 any row, so the first value may start inside a byte) in LDS
+// as 32-bit words, from the 16-byte chunks that hold them (an aligned
+// 16-byte chunk holding a valid byte never crosses a page), four chunks in
+// flight per thread.  Returns the bit offset of the first value inside w[0];
+// bytes outside the stream are zero, and one zero chunk follows the last
+// staged one.
+__device__ __forceinline__ int stage_fl_words(const uint8_t* __restrict__ in, size_t in_bytes, size_t v0, int nv, int b,
+                                              uint32_t* w) {
+    const size_t B0 = v0 * (size_t)b / 8;
+    const int s0 = (int)((v0 * (size_t)b) & 7);  // 0 whenever v0 is a multiple of 8
+    const size_t need = ((size_t)s0 + (size_t)nv * b + 7) / 8;
+    const size_t avail_total = in_bytes > B0 ? in_bytes - B0 : 0;
+    const int avail = (int)(need < avail_total ? need : avail_total);
+    const uint8_t* base = in + B0;
+    const int a = (int)(reinterpret_cast<uintptr_t>(base) & 15);
+    const u32x4* wp4 = reinterpret_cast<const u32x4*>(base - a);
+    const int n4_valid = (a + avail + 15) / 16;
+    const int n4_fill = n4_valid + 1;  // <= fl_stage_words<MAXB>() / 4 for b <= MAXB
+    u32x4* w4 = reinterpret_cast<u32x4*>(w);
+    constexpr int U = 4;
+    for (int i0 = threadIdx.x; i0 < n4_fill; i0 += U * kBlock) {
+        u32x4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + u * kBlock;
+            v[u] = i < n4_valid ? wp4[i] : u32x4{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = i0 + u * kBlock;
+            if (i < n4_fill) {
+                const int b0 = 16 * i;  // first byte of the chunk, relative to wp4
+                u32x4 m = v[u];
+                if (b0 < a || b0 + 16 > a + avail) {  // only the chunks at either end hold bytes outside the stream
+                    m.x &= byte_mask(a - b0, a + avail - b0);
+                    m.y &= byte_mask(a - b0 - 4, a + avail - b0 - 4);
+                    m.z &= byte_mask(a - b0 - 8, a + avail - b0 - 8);
+                    m.w &= byte_mask(a - b0 - 12, a + avail - b0 - 12);
+                }
+                w4[i] = m;
+            }
+        }
+    }
+    return 8 * a + s0;
+}
+
+// Value k (0-based in the tile) from staged words whose first value starts
+// at bit `abit` of w[0].
+__device__ __forceinline__ uint64_t fl_value(const uint32_t* w, int k, int b, int abit, uint64_t mask) {
+    const uint64_t p = (uint64_t)k * b + abit;
+    const uint32_t q = (uint32_t)(p >> 5), s = (uint32_t)(p & 31);
+    uint64_t val = ((uint64_t)w[q] | ((uint64_t)w[q + 1] << 32)) >> s;
+    if (s + b > 64) val |= (uint64_t)w[q + 2] << (64 - s);
+    return val & mask;
 }
 
 }  // namespace ddj

@@ -16,6 +16,7 @@
 #include "dispatch.h"
 #include "k_common.h"
 #include "kernels.h"
+#include "k_fl_tile.h"
 
 namespace ddj {
 
@@ -640,16 +641,21 @@ constexpr int kCodeLdsWords = kTile / 4 + 2 + 8;
 // known at the start: they are staged in LDS (yes first, then no) with
 // 16-byte loads, the stencil scan gives each row its source slot, and the
 // tile is written back striped.  Codes of at most 8 bits are staged the
-// same way (as the bytes of their words) and looked up per row.
-template <typename T, bool CONST_NO, bool CODES>
+// same way (as the bytes of their words) and looked up per row.  FL: both
+// sides are FL streams (patch -> (afl -> none), (afl -> none)); each side's
+// packed words for the tile's rows are staged in LDS and unpacked into the
+// same slots, so that the sides are never written unpacked.
+template <typename T, bool CONST_NO, bool CODES, bool FL>
 __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bits, size_t n, const uint8_t* __restrict__ yes,
                                                   size_t n_yes, DictCodes dc, const uint8_t* __restrict__ no, size_t n_no,
                                                   T no_const, T* __restrict__ out, const uint64_t* __restrict__ prefix,
-                                                  uint32_t* error) {
+                                                  uint32_t* error, FlSrc fy, FlSrc fn) {
+    using U = typename UnsignedOf<T>::type;
     __shared__ T buf[kTilePad];
     __shared__ uint64_t red[kBlock / kWave];
     __shared__ T sdict[CODES ? kLdsDict : 1];
     __shared__ __attribute__((aligned(16))) uint32_t cw[CODES ? kCodeLdsWords : 4];
+    __shared__ __attribute__((aligned(16))) uint32_t wfl[FL ? fl_stage_words<8 * (int)sizeof(T)>() : 4];
     const uint32_t tile = blockIdx.x;
     const size_t base = (size_t)tile * kTile;
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
@@ -678,10 +684,25 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
             u32x4* l4 = reinterpret_cast<u32x4*>(cw);
             for (int k = threadIdx.x; k < nchunks; k += kBlock) l4[k] = __builtin_nontemporal_load(c4 + k);
         }
+    } else if constexpr (FL) {
+        // one side at a time through wfl; rows past a stream read as zero
+        if (Y0 + (uint64_t)total > n_yes || N0 + (uint64_t)(cnt - total) > n_no) bad = true;
+#pragma unroll
+        for (int side = 0; side < 2; side++) {
+            const FlSrc& f = side == 0 ? fy : fn;
+            const int rows = side == 0 ? total : cnt - total;
+            if (rows == 0) continue;  // uniform
+            const int a = stage_fl_words(side == 0 ? yes : no, f.bytes, side == 0 ? Y0 : N0, rows, f.bits, wfl);
+            __syncthreads();
+            const uint64_t mask = lowmask64((unsigned)f.bits);
+            const int dst0 = side == 0 ? 0 : total;
+            for (int k = threadIdx.x; k < rows; k += kBlock) buf[P(dst0 + k)] = (T)(U)fl_value(wfl, k, f.bits, a, mask);
+            __syncthreads();  // wfl is restaged for the other side
+        }
     } else {
         bad |= stage_rows<T>(yes, Y0, total, n_yes, buf, 0);
     }
-    if (!CONST_NO) bad |= stage_rows<T>(no, N0, cnt - total, n_no, buf, total);
+    if (!CONST_NO && !FL) bad |= stage_rows<T>(no, N0, cnt - total, n_no, buf, total);
     const size_t g0 = base + (size_t)threadIdx.x * kItems;
     uint32_t mask = 0;
     if (g0 < n) mask = bits[g0 / 8];
@@ -1193,21 +1214,54 @@ void merge_impl(Ctx* c, const uint8_t* bits, size_t n, const uint8_t* yes, size_
     if (dc.words) {
         kernel_bytes(c, "k_merge", (double)((n + 7) / 8) + (double)n_yes * dc.bits / 8 + (double)n_no * sizeof(T) +
                                        (double)n * sizeof(T));
-        hipLaunchKernelGGL((k_merge<T, false, true>), grid, dim3(kBlock), 0, c->stream, bits, n, yes, n_yes, dc, no, n_no,
-                           T(0), out, pre, err);
+        hipLaunchKernelGGL((k_merge<T, false, true, false>), grid, dim3(kBlock), 0, c->stream, bits, n, yes, n_yes, dc, no, n_no,
+                           T(0), out, pre, err, FlSrc{}, FlSrc{});
     } else if (const_no) {
         kernel_bytes(c, "k_merge", (double)((n + 7) / 8) + (double)n_yes * sizeof(T) + (double)n * sizeof(T));
-        hipLaunchKernelGGL((k_merge<T, true, false>), grid, dim3(kBlock), 0, c->stream, bits, n, yes, n_yes, dc, no, n_no,
-                           no_const, out, pre, err);
+        hipLaunchKernelGGL((k_merge<T, true, false, false>), grid, dim3(kBlock), 0, c->stream, bits, n, yes, n_yes, dc, no, n_no,
+                           no_const, out, pre, err, FlSrc{}, FlSrc{});
     } else {
         kernel_bytes(c, "k_merge", (double)((n + 7) / 8) + 2.0 * (double)n * sizeof(T));
-        hipLaunchKernelGGL((k_merge<T, false, false>), grid, dim3(kBlock), 0, c->stream, bits, n, yes, n_yes, dc, no, n_no,
-                           T(0), out, pre, err);
+        hipLaunchKernelGGL((k_merge<T, false, false, false>), grid, dim3(kBlock), 0, c->stream, bits, n, yes, n_yes, dc, no, n_no,
+                           T(0), out, pre, err, FlSrc{}, FlSrc{});
     }
     HIPC(hipGetLastError());
 }
 
+// The merge of a patch whose sides are FL streams.
+template <typename T>
+void merge_fl_impl(Ctx* c, const uint8_t* bits, size_t n, const uint8_t* yes, size_t n_yes, FlSrc fy,
+                   const uint8_t* no, size_t n_no, FlSrc fn, T* out) {
+    const size_t tiles = (n + kTile - 1) / kTile;
+    TileScan ts(c, tiles);
+    arm_error_flag(c);
+    const unsigned cg = (unsigned)((tiles + kBlock / kWave - 1) / (kBlock / kWave));
+    kernel_bytes(c, "k_stencil_count", (double)((n + 7) / 8));
+    { KernelTimer kt_(c, "k_stencil_count"); hipLaunchKernelGGL(k_stencil_count, dim3(cg), dim3(kBlock), 0, c->stream, bits, n, tiles, ts.vals); }
+    HIPC(hipGetLastError());
+    ts.scan(c);
+    KernelTimer kt_(c, "k_merge_fl");
+    kernel_bytes(c, "k_merge_fl", (double)((n + 7) / 8) + (double)fy.bytes + (double)fn.bytes + (double)n * sizeof(T));
+    hipLaunchKernelGGL((k_merge<T, false, false, true>), dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, bits, n, yes,
+                       n_yes, DictCodes{}, no, n_no, T(0), out, ts.prefix, &c->control->error_flag, fy, fn);
+    HIPC(hipGetLastError());
+}
+
 }  // namespace
+
+void launch_merge_fl(Ctx* c, int dtype, const uint8_t* bits, size_t n, const uint8_t* yes, size_t yes_bytes,
+                     size_t n_yes, int by, const uint8_t* no, size_t no_bytes, size_t n_no, int bn, void* out) {
+    if (n == 0) return;
+    dispatch_int((int)dtype_size(dtype), [&](auto tag) {
+        using T = decltype(tag);
+        if (dtype == DDJ_D_FLOAT || dtype == DDJ_D_DOUBLE) fail(DDJ_EINVAL, "FL merge: integer columns only");
+        const int W = 8 * (int)sizeof(T);
+        if ((n_yes && (by <= 0 || by > W)) || (n_no && (bn <= 0 || bn > W))) fail(DDJ_ECORRUPT, "FL merge: bit width out of range");
+        // a side with no rows is never staged: give it a width that keeps the arithmetic in range
+        merge_fl_impl<T>(c, bits, n, yes, n_yes, FlSrc{yes_bytes, n_yes ? by : 1}, no, n_no,
+                         FlSrc{no_bytes, n_no ? bn : 1}, static_cast<T*>(out));
+    });
+}
 
 void launch_merge(Ctx* c, int dtype, const uint8_t* bits, size_t n, const uint8_t* yes, size_t n_yes,
                   const uint8_t* no, size_t n_no, bool const_no, uint64_t no_const, void* out) {

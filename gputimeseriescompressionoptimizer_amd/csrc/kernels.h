@@ -172,6 +172,11 @@ void launch_split_write(Ctx* c, const SplitPlan& plan, uint8_t* stencil, void* y
                         uint64_t* d_counts);
 // One output of a patch split FL-encoded straight into its stream: `bytes`
 // stream bytes at dst (any byte address), values of `bits` bits.
+// An FL stream read by a merge: its bytes and bit width.
+struct FlSrc {
+    size_t bytes;
+    int bits;
+};
 struct FlSide {
     uint8_t* dst;
     uint64_t bytes;
@@ -189,6 +194,11 @@ void launch_split_fl(Ctx* c, const SplitPlan& plan, uint8_t* stencil, FlSide yes
 void launch_merge(Ctx* c, int dtype, const uint8_t* stencil_bits /* packed bits, no header */, size_t n,
                   const uint8_t* yes, size_t n_yes, const uint8_t* no, size_t n_no, bool const_no, uint64_t no_const,
                   void* out);
+// Merge whose two sides are FL streams (n_yes rows of `by` bits in
+// yes_bytes at yes, n_no rows of `bn` bits at no; any byte addresses):
+// each tile's rows of either stream are unpacked in LDS.  Integer columns.
+void launch_merge_fl(Ctx* c, int dtype, const uint8_t* stencil_bits, size_t n, const uint8_t* yes, size_t yes_bytes,
+                     size_t n_yes, int by, const uint8_t* no, size_t no_bytes, size_t n_no, int bn, void* out);
 // Merge whose set-bit rows are dict[code j] for the m `code_bits`-wide codes
 // in `words` (the dictionary decoder without materialising those rows).
 void launch_merge_codes(Ctx* c, int dtype, const uint8_t* stencil_bits, size_t n, const uint8_t* words, size_t m,

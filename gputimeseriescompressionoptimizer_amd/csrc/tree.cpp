@@ -357,12 +357,34 @@ struct ParsedBlob {
         return is_leaf(kids[(size_t)a][0]) ? a : -1;
     }
 
+    // The integer node i is a patch whose two children are AFL over a none
+    // leaf: whether it is.
+    bool patch_afl_kids(int i) const {
+        const ParsedNode& nd = nodes[(size_t)i];
+        if ((nd.enc != DDJ_ENC_PATCH && nd.enc != DDJ_ENC_OUTSIDE_PATCH) || !is_int_type(nd.dtype) || nd.nchild != 2 ||
+            kids[(size_t)i].size() != 2)
+            return false;
+        for (int a : kids[(size_t)i]) {
+            const ParsedNode& an = nodes[(size_t)a];
+            if (an.enc != DDJ_ENC_AFL || an.dtype != nd.dtype || an.nchild != 1 || kids[(size_t)a].size() != 1 ||
+                !is_leaf(kids[(size_t)a][0]))
+                return false;
+        }
+        return true;
+    }
+
     // CompressionNode::Decompress (compression_node.cpp:140-161): children first.
     Part decode_node(int i) const {
         const ParsedNode& nd = nodes[(size_t)i];
         const DView m = meta(i);
         if (nd.enc == DDJ_ENC_NONE) return borrow(leaf_data(i));
         if ((int)kids[(size_t)i].size() != nd.nchild) fail(DDJ_ECORRUPT, "node child count mismatch");
+        // PATCH over two AFL -> none (integer): merge straight from both FL streams
+        if (patch_afl_kids(i)) {
+            const int y = kids[(size_t)i][0], z = kids[(size_t)i][1];
+            return patch_afl_decode(c, nd.dtype, m, meta(y), leaf_data(kids[(size_t)y][0]), meta(z),
+                                    leaf_data(kids[(size_t)z][0]));
+        }
         // DELTA over AFL over none (integer): unpack and prefix-sum in one pass
         const int a = delta_afl_child(i);
         if (a >= 0) {

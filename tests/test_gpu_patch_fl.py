@@ -1,10 +1,12 @@
 """patch -> (afl -> none), (afl -> none): the split FL-packs both sides
 straight into the blob (one fused pass writing the stencil and both streams,
-with the words that neighbouring tiles share assembled afterwards).  Blobs
-must be byte-identical to the oracle's patch followed by two AFL encoders,
-whatever the side sizes: sparse outliers (most tiles give the no side
-nothing), every row on one side, widths from 1 bit to the full word, every
-integer type, and the outside-patch predicate."""
+with the words that neighbouring tiles share assembled afterwards), and the
+decoder's merge unpacks both streams in LDS.  Blobs must be byte-identical
+to the oracle's patch followed by two AFL encoders, whatever the side sizes:
+sparse outliers (most tiles give the no side nothing), every row on one
+side, widths from 1 bit to the full word, every integer type, and the
+outside-patch predicate; decoding must give the column back from any blob
+alignment and report a stencil that asks for more rows than a stream holds."""
 import numpy as np
 import pytest
 
@@ -66,3 +68,33 @@ def test_fused_patch_many_tiles(ctx):
     offsets in both streams."""
     x = D.normal_with_outliers(1 << 24, seed=77)
     check(ctx, x, g.D_INT, P_AFL)
+
+
+@pytest.mark.parametrize("dtype", INT_TYPES)
+def test_fused_patch_decode_from_any_blob_offset(ctx, dtype):
+    for kind in ("sparse", "negative"):
+        x = column(dtype, 70_001, seed=3, kind=kind)
+        tree = g.CompressionTree(ctx, P_AFL, dtype)
+        blob = tree.Compress(ctx.upload(x)).to_bytes()
+        for off in (1, 2, 3, 5, 6):
+            big = ctx.upload(b"\xab" * off + blob + b"\xcd" * 7)
+            assert tree.Decompress(big.view(off, len(blob))).to_bytes() == x.tobytes(), (kind, off)
+
+
+def test_fused_patch_decode_rejects_a_stencil_past_the_streams(ctx):
+    """One clear stencil bit set: the stencil then asks the yes stream for
+    one row more than it holds."""
+    x = column(g.D_INT, 20_000, seed=9, kind="sparse")
+    tree = g.CompressionTree(ctx, P_AFL, g.D_INT)
+    blob = bytearray(tree.Compress(ctx.upload(x)).to_bytes())
+    stencil = O.encode(g.PATCH, g.D_INT, x)[0]
+    at = bytes(blob).find(stencil)
+    assert at > 0
+    flags = np.unpackbits(np.frombuffer(stencil[1:], np.uint8), bitorder="little")[: len(x)]
+    row = int(np.flatnonzero(flags == 0)[0])
+    blob[at + 1 + row // 8] |= 1 << (row % 8)
+    with pytest.raises(g.DdjError):
+        tree.Decompress(ctx.upload(bytes(blob)))
+    for cut in (9, len(blob) // 2, len(blob) - 1):
+        with pytest.raises(g.DdjError):
+            g.CompressionTree(ctx).Decompress(ctx.upload(bytes(blob[:cut])))
