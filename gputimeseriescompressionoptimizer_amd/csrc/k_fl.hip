@@ -48,8 +48,23 @@ __global__ __launch_bounds__(kBlock) void k_fl_unpack(const uint8_t* __restrict_
     const int a = stage_fl_words(in, in_bytes, vbase + v0, nv, b, w);
     __syncthreads();
     const uint64_t mask = lowmask64(b);
-    // consecutive lanes take consecutive values: their LDS reads spread over the banks
-    for (int k = threadIdx.x; k < nv; k += kBlock) out[v0 + k] = (T)fl_value(w, k, b, a, mask);
+    constexpr int V = 16 / (int)sizeof(T);
+    if (nv == kFlTile && (reinterpret_cast<uintptr_t>(out + v0) & 15) == 0) {
+        // whole tile: each lane unpacks V consecutive values into one 16-byte
+        // store (a cached store: RLE's values and lengths are read next)
+        u32x4* o4 = reinterpret_cast<u32x4*>(out + v0);
+        for (int i = threadIdx.x; i < kFlTile / V; i += kBlock) {
+            T vals[V];
+#pragma unroll
+            for (int k = 0; k < V; k++) vals[k] = (T)fl_value(w, V * i + k, b, a, mask);
+            u32x4 q;
+            __builtin_memcpy(&q, vals, 16);
+            o4[i] = q;
+        }
+    } else {
+        // consecutive lanes take consecutive values: their LDS reads spread over the banks
+        for (int k = threadIdx.x; k < nv; k += kBlock) out[v0 + k] = (T)fl_value(w, k, b, a, mask);
+    }
 }
 
 // DeltaEncoding::Decode over AflEncoding::Decode without writing the deltas
@@ -111,7 +126,28 @@ __global__ __launch_bounds__(kBlock) void k_fl_unpack_delta_scan(const uint8_t* 
         res[threadIdx.x * kPer + k] = (T)run;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nv; i += kBlock) out[v0 + 1 + i] = res[i];
+    // out[v0 + 1 ..]: single rows up to a 16-byte boundary, then 16-byte
+    // stores from consecutive lanes, then single rows
+    T* dst = out + v0 + 1;
+    constexpr int V = 16 / (int)sizeof(T);
+    int head = (int)(((16 - (reinterpret_cast<uintptr_t>(dst) & 15)) & 15) / sizeof(T));
+    head = head < nv ? head : nv;
+    int nq = (nv - head) / V;
+    if ((reinterpret_cast<uintptr_t>(dst) % sizeof(T)) != 0) {  // not on a row boundary: rows only
+        head = 0;
+        nq = 0;
+    }
+    if ((int)threadIdx.x < head) dst[threadIdx.x] = res[threadIdx.x];
+    u32x4* d4 = reinterpret_cast<u32x4*>(dst + head);
+    for (int i = threadIdx.x; i < nq; i += kBlock) {
+        T vals[V];
+#pragma unroll
+        for (int k = 0; k < V; k++) vals[k] = res[head + V * i + k];
+        u32x4 q;
+        __builtin_memcpy(&q, vals, 16);
+        __builtin_nontemporal_store(q, d4 + i);
+    }
+    for (int i = head + V * nq + threadIdx.x; i < nv; i += kBlock) dst[i] = res[i];
     if (tile == 0 && threadIdx.x == 0) out[0] = (T)(ubits(first) + prefix[0]);
 }
 

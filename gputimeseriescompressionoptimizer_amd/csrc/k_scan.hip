@@ -751,7 +751,21 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
     for (int k = 0; k < kItems; k++)
         if (threadIdx.x * kItems + k < cnt) buf[P(threadIdx.x * kItems + k)] = v[k];
     __syncthreads();
-    for (int i = threadIdx.x; i < cnt; i += kBlock) out[base + i] = buf[P(i)];
+    if (cnt == kTile && (reinterpret_cast<uintptr_t>(out + base) & 15) == 0) {
+        // whole tile: consecutive lanes store consecutive 16-byte vectors
+        constexpr int V = 16 / (int)sizeof(T);
+        u32x4* o4 = reinterpret_cast<u32x4*>(out + base);
+        for (int i = threadIdx.x; i < kTile / V; i += kBlock) {
+            T vals[V];
+#pragma unroll
+            for (int k = 0; k < V; k++) vals[k] = buf[P(V * i + k)];
+            u32x4 w;
+            __builtin_memcpy(&w, vals, 16);
+            __builtin_nontemporal_store(w, o4 + i);
+        }
+    } else {
+        for (int i = threadIdx.x; i < cnt; i += kBlock) out[base + i] = buf[P(i)];
+    }
 }
 
 
