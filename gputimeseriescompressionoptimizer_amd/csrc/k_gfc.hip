@@ -14,9 +14,11 @@
 // k_scan_tiles places the chunks, and the encode pass recomputes the byte
 // counts, scans them in the workgroup, assembles the chunk payload in LDS and
 // writes it at its place.  Decode takes the chunk sizes from the offsets
-// table (scanned the same way), stages the chunk payload in LDS, walks the
-// subchunk headers with one wave (the only sequential step of the format)
-// and then decodes all rows in parallel.
+// table (scanned the same way) and stages each chunk's payload in LDS; the
+// walk over the subchunk headers (the format's only sequential step) runs
+// for double chunks as a kernel of its own, one lane per chunk, and inside
+// the one-wave decoder for float chunks; all rows then decode in parallel
+// and the chunk goes out through LDS as consecutive 16-byte vectors.
 #include "dispatch.h"
 #include "k_common.h"
 #include "kernels.h"
@@ -46,6 +48,17 @@ GfcGeom gfc_geom(bool dbl, long long n) {
 }
 
 constexpr int kGfcRpt = kGfcMaxPer / kBlock;  // rows per thread (8)
+
+// Residual bytes announced by one 32-bit word of a subchunk header (the
+// sign|code nibbles of 8 rows): the sum of the codes, plus one for every
+// double code >= 2 (double codes 2..7 stand for 3..8 bytes).
+template <bool DBL> __device__ __forceinline__ uint32_t gfc_word_bytes(uint32_t w) {
+    const uint32_t m = w & 0x77777777u;
+    uint32_t s = (m & 0x0f0f0f0fu) + ((m >> 4) & 0x0f0f0f0fu);
+    s = (s * 0x01010101u) >> 24;
+    if (DBL) s += (uint32_t)__builtin_popcount(((m >> 1) | (m >> 2)) & 0x11111111u);
+    return s;
+}
 
 // Thread t's rows [8t, 8t+8) of the chunk starting at row c0: |diff| against
 // the predictor, byte counts and the sign|code nibbles.
@@ -268,11 +281,56 @@ __global__ __launch_bounds__(kBlock) void k_gfc_chunk_bytes(const uint8_t* __res
     if (i < chunks) vals[i] = gfc_chunk_bytes<U>(offsets, (uint32_t)i, N, per, S);
 }
 
+// The header walk of every chunk, one lane per chunk, reading the headers
+// straight from the payload (each step's four header words are one round
+// trip; the lanes of a wave walk 64 chunks side by side):
+// sub_off[chunk * stride + s] = subchunk s's offset in the chunk, s <= nsub.
+// A walk that does not end at the chunk's size raises the error flag.
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_gfc_walk(const uint8_t* __restrict__ payload, size_t payload_bytes,
+                                                     const uint8_t* __restrict__ offsets, size_t chunks, long long N,
+                                                     long long per, int S, const uint64_t* __restrict__ prefix,
+                                                     uint32_t* __restrict__ sub_off, int stride, uint32_t* error) {
+    constexpr bool dbl = sizeof(U) == 8;
+    const size_t chunk = (size_t)blockIdx.x * kBlock + threadIdx.x;
+    if (chunk >= chunks) return;
+    const int nsub = gfc_chunk((uint32_t)chunk, N, per).rows / 32;
+    const uint64_t cb = gfc_chunk_bytes<U>(offsets, (uint32_t)chunk, N, per, S);
+    const uint64_t P = prefix[chunk];
+    const uint64_t lim = P >= payload_bytes ? 0 : (payload_bytes - P < cb ? payload_bytes - P : cb);  // readable
+    const uint8_t* base = payload + (P < payload_bytes ? P : 0);
+    uint32_t* so = sub_off + chunk * (size_t)stride;
+    uint32_t off = 0;
+    for (int s = 0; s < nsub; s++) {
+        so[s] = off;
+        uint32_t tot = 0;
+        if ((uint64_t)off + 16 <= lim) {
+            // the aligned words holding header bytes [off, off + 16): each
+            // holds at least one of them, so none leaves the payload's pages
+            const uintptr_t a = reinterpret_cast<uintptr_t>(base + off);
+            const uint32_t* w = reinterpret_cast<const uint32_t*>(a & ~(uintptr_t)3);
+            const uint32_t r = (uint32_t)(a & 3);
+            uint32_t h[5];
+#pragma unroll
+            for (int k = 0; k < 4; k++) h[k] = w[k];
+            h[4] = r ? w[4] : 0u;
+#pragma unroll
+            for (int k = 0; k < 4; k++) tot += gfc_word_bytes<dbl>(r ? __builtin_amdgcn_alignbyte(h[k + 1], h[k], r) : h[k]);
+        }
+        off += 16 + tot;
+    }
+    so[nsub] = off;
+    if (off != cb || P + cb > payload_bytes) __hip_atomic_store(error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Decode of chunks of up to 2048 rows, one 256-thread workgroup each, the
+// subchunk offsets taken from k_gfc_walk (walked, stride per chunk).
 template <typename U>
 __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict__ payload, size_t payload_bytes,
                                                        const uint8_t* __restrict__ offsets, long long n, long long N,
                                                        long long per, int S, U* __restrict__ out,
-                                                       const uint64_t* __restrict__ prefix, uint32_t* error) {
+                                                       const uint64_t* __restrict__ prefix, uint32_t* error,
+                                                       const uint32_t* __restrict__ walked, int stride) {
     constexpr int RPT = kGfcMaxPer / kBlock;
     constexpr bool dbl = sizeof(U) == 8;
     constexpr int kMaxBytes = kGfcMaxPer / 32 * (16 + 32 * (int)sizeof(U));
@@ -283,6 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     __shared__ __attribute__((aligned(16))) uint8_t buf[16 * kVec + 16];
     __shared__ uint32_t sub_off[kGfcMaxPer / 32 + 1];
     __shared__ U s_last;
+    static_assert(sizeof(buf) >= kGfcMaxPer * sizeof(U), "the decoded chunk fits the staging buffer");
     __shared__ int s_bad;
     const uint32_t chunk = blockIdx.x;
     const long long c0 = (long long)chunk * per;
@@ -329,29 +388,10 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
         const uint32_t hi = q + 1 < kBufWords ? bw[q + 1] : 0u;
         return __builtin_amdgcn_alignbyte(hi, lo, s);
     };
-    // header walk (the format's one sequential step): one wave, lanes 0..31
-    // reading one nibble each, their byte counts summed by a wave scan.  (One
-    // lane summing the header's four words nibble-parallel instead takes
-    // twice as long: its dependent LDS reads are the chain.)
-    if (threadIdx.x < kWave) {
-        const int lane = threadIdx.x;
-        uint32_t off = 0;
-        for (int s = 0; s < nsub; s++) {
-            int nb = 0;
-            if (lane < 32 && off + 16 <= chunk_bytes) {
-                const uint8_t byte = buf[base + off + (lane >> 1)];
-                const int nib = (lane & 1) ? (byte >> 4) : (byte & 15);
-                const int code = nib & 7;
-                nb = dbl ? (code >= 2 ? code + 1 : code) : code;
-            }
-            const int tot = __builtin_amdgcn_readlane(wave_inclusive_scan(nb), 31);  // lanes 0..31
-            if (lane == 0) sub_off[s] = off;
-            off += 16 + tot;
-        }
-        if (lane == 0) {
-            sub_off[nsub] = off;
-            if (off != chunk_bytes) s_bad = 1;
-        }
+    // subchunk offsets from the header walk (k_gfc_walk)
+    {
+        const uint32_t* so = walked + (size_t)chunk * stride;
+        for (int s = threadIdx.x; s <= nsub; s += kBlock) sub_off[s] = so[s];
     }
     __syncthreads();
     const int sub = threadIdx.x / 4;
@@ -389,8 +429,23 @@ __global__ __launch_bounds__(kBlock) void k_gfc_decode(const uint8_t* __restrict
     }
     // row 31 of the chunk is the predictor of every later subchunk
     if (threadIdx.x == 3 && rows > 0) s_last = dec[RPT - 1];
-    __syncthreads();
-    if (mine) {
+    __syncthreads();  // also: every read of the staged payload is done
+    if (c0 + rows <= n && (reinterpret_cast<uintptr_t>(out + c0) & 15) == 0) {  // uniform
+        // the decoded rows go through LDS so that consecutive lanes store
+        // consecutive 16-byte vectors of the chunk (a lane's own 8 rows,
+        // stored straight, put 64 scattered pieces in each store)
+        U* ob = reinterpret_cast<U*>(buf);
+        if (mine) {
+            const U last = sub == 0 ? U(0) : s_last;
+#pragma unroll
+            for (int k = 0; k < RPT; k++) ob[threadIdx.x * RPT + k] = (U)(last + dec[k]);
+        }
+        __syncthreads();
+        const u32x4* b4 = reinterpret_cast<const u32x4*>(buf);
+        u32x4* d4 = reinterpret_cast<u32x4*>(out + c0);
+        const int nvec = rows * (int)sizeof(U) / 16;  // rows is a multiple of 32
+        for (int i = threadIdx.x; i < nvec; i += kBlock) d4[i] = b4[i];
+    } else if (mine) {
         const U last = sub == 0 ? U(0) : s_last;
         U v8[RPT];
 #pragma unroll
@@ -434,6 +489,7 @@ __global__ __launch_bounds__(kWave) void k_gfc_decode_wave(const uint8_t* __rest
     constexpr int kVR = (kVec + kWave - 1) / kWave;
     __shared__ __attribute__((aligned(16))) uint8_t buf[16 * kVec + 16];
     __shared__ uint32_t sub_off[MAXPER / 32 + 1];
+    static_assert(sizeof(buf) >= MAXPER * sizeof(U), "the decoded chunk fits the staging buffer");
     const int lane = threadIdx.x;
     const uint32_t chunk = blockIdx.x;
     const long long c0 = (long long)chunk * per;
@@ -541,7 +597,21 @@ __global__ __launch_bounds__(kWave) void k_gfc_decode_wave(const uint8_t* __rest
         last = (U)__builtin_amdgcn_readlane((uint32_t)dec[RPL - 1], LPS - 1);
     }
     if (sub == 0) last = U(0);
-    if (mine) {
+    __syncthreads();  // every read of the staged payload is done
+    if (c0 + rows <= n && (reinterpret_cast<uintptr_t>(out + c0) & 15) == 0) {  // uniform
+        // through LDS, so that consecutive lanes store consecutive 16-byte
+        // vectors of the chunk
+        U* ob = reinterpret_cast<U*>(buf);
+        if (mine) {
+#pragma unroll
+            for (int k = 0; k < RPL; k++) ob[r0 + k] = (U)(last + dec[k]);
+        }
+        __syncthreads();
+        const u32x4* b4 = reinterpret_cast<const u32x4*>(buf);
+        u32x4* d4 = reinterpret_cast<u32x4*>(out + c0);
+        const int nvec = rows * (int)sizeof(U) / 16;  // rows is a multiple of 32
+        for (int i = lane; i < nvec; i += kWave) d4[i] = b4[i];
+    } else if (mine) {
         const long long g0 = c0 + r0;
         U v[RPL];
 #pragma unroll
@@ -648,12 +718,11 @@ void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const 
         kernel_bytes(c, "k_gfc_chunk_bytes", 12.0 * (double)tiles);
         kernel_bytes(c, "k_gfc_decode", (double)payload_bytes + 4.0 * (double)tiles + (double)n * sizeof(U));
         // float chunks (at most 1024 rows, 4.6 KB staged) decode one wave
-        // each: 2.1x faster than four waves a chunk (float32 walk, 2^26 rows,
-        // 273 -> 132 us).  Double chunks stage 17 KB, so one wave each would
-        // leave nine waves a CU: 367 -> 456 us; they keep four.
-        KernelTimer kt_(c, "k_gfc_decode");
+        // each, walk included: 2.1x faster than four waves a chunk (float32
+        // walk, 2^26 rows, 273 -> 132 us).
         if constexpr (sizeof(U) == 4) {
             if (g.per <= 1024) {
+                KernelTimer kt_(c, "k_gfc_decode");
                 hipLaunchKernelGGL((k_gfc_decode_wave<U, 1024>), dim3((unsigned)tiles), dim3(kWave), 0, c->stream,
                                    payload, payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<U>(),
                                    (const uint64_t*)ts.prefix, err_flag);
@@ -661,9 +730,24 @@ void gfc_decode(Ctx* c, bool dbl, const uint8_t* meta, size_t meta_bytes, const 
                 return;
             }
         }
+        // Double chunks stage 17 KB (nine a CU): the header walks run first,
+        // one lane per chunk (k_gfc_walk), so that a decoding workgroup holds
+        // its LDS only while it loads, decodes and stores (float64 walk, 2^26
+        // rows, decode end to end 348 -> 326 us).
+        const int stride = (int)(g.per / 32 + 1);
+        DevBuf subs(c, 4 * tiles * (size_t)stride);
+        kernel_bytes(c, "k_gfc_walk", 20.0 * (double)(g.N / 32) + 4.0 * (double)tiles);
+        {
+            KernelTimer kw_(c, "k_gfc_walk");
+            hipLaunchKernelGGL(k_gfc_walk<U>, dim3((unsigned)((tiles + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                               c->stream, payload, payload_bytes, offsets, tiles, g.N, g.per, (int)g.S,
+                               (const uint64_t*)ts.prefix, subs.get<uint32_t>(), stride, err_flag);
+            HIPC(hipGetLastError());
+        }
+        KernelTimer kt_(c, "k_gfc_decode");
         hipLaunchKernelGGL(k_gfc_decode<U>, dim3((unsigned)tiles), dim3(kBlock), 0, c->stream, payload,
                            payload_bytes, offsets, g.n, g.N, g.per, (int)g.S, out.get<U>(),
-                           (const uint64_t*)ts.prefix, err_flag);
+                           (const uint64_t*)ts.prefix, err_flag, subs.get<uint32_t>(), stride);
         HIPC(hipGetLastError());
     };
     if (dbl)
