@@ -263,11 +263,14 @@ def fl_plumbing_section(ctx):
     return res
 
 
-def learned_planner_section(ctx, blocks: int, rows: int, regime: int, exhaustive_every: int):
+def learned_planner_section(ctx, blocks: int, rows: int, regime: int, exhaustive_every: int, big_blocks: int = 0,
+                            big_rows: int = 1 << 20):
     """Learned vs exhaustive planning on streams of blocks whose generator
     family changes every `regime` blocks: int32 blocks with the integer
     network, float32/float64 blocks with the float network (weights trained
-    by gputimeseriescompressionoptimizer_amd.train_planner)."""
+    by gputimeseriescompressionoptimizer_amd.train_planner on 65,536-row
+    blocks), and `big_blocks` int32 blocks of `big_rows` rows (the 1M-row
+    blocks of the learned-planner workload) with the integer network."""
     import gputimeseriescompressionoptimizer_amd as g
     from gputimeseriescompressionoptimizer_amd import planner_data as PD
     from gputimeseriescompressionoptimizer_amd.train_planner import WEIGHTS, WEIGHTS_FLOAT, evaluate
@@ -283,6 +286,11 @@ def learned_planner_section(ctx, blocks: int, rows: int, regime: int, exhaustive
         ev["stream"] = f"{blocks} blocks x {rows} rows, family changes every {regime} blocks"
         ev["ratio_gap_vs_exhaustive"] = (ev["learned_vs_exhaustive"] - 1.0) if ev["learned_vs_exhaustive"] else None
         out[name] = ev
+        if name == "int32" and big_blocks > 0:
+            ev = evaluate(ctx, net, PD.regime_stream(big_blocks, big_rows, regime, seed=4243), exhaustive_every)
+            ev["stream"] = f"{big_blocks} blocks x {big_rows} rows, family changes every {regime} blocks"
+            ev["ratio_gap_vs_exhaustive"] = (ev["learned_vs_exhaustive"] - 1.0) if ev["learned_vs_exhaustive"] else None
+            out["int32_1m_rows"] = ev
     return out or None
 
 
@@ -382,6 +390,9 @@ def main():
     ap.add_argument("--planner-blocks", type=int, default=1000)
     ap.add_argument("--planner-regime", type=int, default=500, help="blocks per regime of the planner's stream")
     ap.add_argument("--planner-rows", type=int, default=65536)
+    ap.add_argument("--planner-big-blocks", type=int, default=1000,
+                    help="int32 blocks of --planner-big-rows rows also planned by the integer network (0: none)")
+    ap.add_argument("--planner-big-rows", type=int, default=1 << 20)
     ap.add_argument("--no-planner", action="store_true")
     ap.add_argument("--no-experiment", action="store_true",
                     help="skip the optimizer-vs-single-scheme experiment on regime-shifting columns")
@@ -583,7 +594,8 @@ def main():
         step("learned planner")
         res["learned_planner"] = learned_planner_section(ctx, args.planner_blocks, args.planner_rows,
                                                          regime=max(1, min(args.planner_regime, args.planner_blocks)),
-                                                         exhaustive_every=10)
+                                                         exhaustive_every=10, big_blocks=args.planner_big_blocks,
+                                                         big_rows=args.planner_big_rows)
     if not args.no_experiment:
         step("optimizer vs single scheme")
         from gputimeseriescompressionoptimizer_amd.experiments import optimizer_vs_single_scheme

@@ -147,6 +147,7 @@ def load_library(path: str = LIB_PATH):
         "ddj_planner_predict": (i32, [vp, vp, sz, vp]),
         "ddj_planner_plan": (i32, [vp, i32, vp, sz, ctypes.c_float, P(i32), sz, P(sz), P(i32),
                                    P(ctypes.c_float)]),
+        "ddj_planner_plan_checked": (i32, [vp, i32, vp, sz, ctypes.c_float, i32, P(i32), sz, P(sz), P(i32)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -795,6 +796,19 @@ class LearnedPlanner:
                                                     4096, ctypes.byref(n), ctypes.byref(fb), ctypes.byref(m)),
                        "learned plan")
         return list(path[: n.value]), bool(fb.value), m.value
+
+    def PlanChecked(self, data: DeviceBuffer, dtype: int, within: float = 0.3,
+                    max_checked: int = 4) -> Tuple[List[int], int]:
+        """The smallest prediction, or among the candidates predicted within
+        `within` (log2) of it (at most max_checked) the smallest exact size:
+        (pre-order path, candidates compressed)."""
+        path = (ctypes.c_int * 4096)()
+        n = ctypes.c_size_t()
+        k = ctypes.c_int()
+        self.ctx.check(self.ctx._L.ddj_planner_plan_checked(self._h, dtype, data.dev or None, data.nbytes, within,
+                                                            max_checked, path, 4096, ctypes.byref(n),
+                                                            ctypes.byref(k)), "learned plan (checked)")
+        return list(path[: n.value]), k.value
 
     def Save(self, path: str):
         np.save(path, self.Params(), allow_pickle=False)

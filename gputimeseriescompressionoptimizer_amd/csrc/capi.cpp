@@ -1,4 +1,5 @@
 // The C ABI (include/ddj.h): status codes at the boundary, C++ inside.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -770,6 +771,43 @@ ddj_status ddj_planner_plan(ddj_planner* p, int dtype, const void* d_in, size_t 
             fix_tree(tree);
             if (used_fallback) *used_fallback = 0;
         }
+        write_path(tree, path_out, path_cap, path_len);
+    });
+}
+
+ddj_status ddj_planner_plan_checked(ddj_planner* p, int dtype, const void* d_in, size_t in_bytes, float within,
+                                    int max_checked, int* path_out, size_t path_cap, size_t* path_len, int* checked) {
+    if (!p) return DDJ_EINVAL;
+    return guarded(p->ctx, [&] {
+        if (!dtype_ok(dtype) || !path_len || !(within >= 0.0f)) fail(DDJ_EINVAL, "bad argument");
+        if (planner_class(dtype) != p->cls) fail(DDJ_EINVAL, "the planner's weights are for another column class");
+        Ctx* c = &p->ctx->c;
+        const DView data{static_cast<const uint8_t*>(d_in), in_bytes};
+        auto f = block_features(c, dtype, data);
+        float y[kMlpOutputs];
+        p->net->predict(f.data(), 1, y);
+        std::vector<int> order;
+        for (int i = 0; i < kMlpOutputs; i++)
+            if (candidate_valid(i, dtype)) order.push_back(i);
+        if (order.empty()) fail(DDJ_EHIP, "no candidate tree for this type");
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return y[a] < y[b]; });
+        // the candidates predicted within `within` of the best, best first
+        std::vector<int> near;
+        for (int i : order) {
+            if ((int)near.size() >= std::max(1, max_checked) || y[i] - y[order[0]] > within) break;
+            near.push_back(i);
+        }
+        size_t bk = 0;  // the smallest exact size; ties go to the better prediction
+        if (near.size() > 1) {
+            const std::vector<size_t> sz = candidate_sizes(c, dtype, data, near);
+            for (size_t k = 1; k < near.size(); k++)
+                if (sz[k] < sz[bk]) bk = k;
+        }
+        const int pick = near[bk];
+        if (checked) *checked = near.size() > 1 ? (int)near.size() : 0;
+        const auto& cp = candidate_paths(p->cls)[(size_t)pick];
+        NodeP tree = tree_from_path(cp.data(), cp.size(), dtype);
+        fix_tree(tree);
         write_path(tree, path_out, path_cap, path_len);
     });
 }

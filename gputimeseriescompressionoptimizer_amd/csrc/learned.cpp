@@ -83,12 +83,15 @@ const std::vector<std::vector<int>>& float_candidate_paths() {
 
 }  // namespace
 
-// The network's 48 outputs.  Chosen by select_candidates.py (--per-family 64,
-// 65536-row blocks, seed 11): the trees the exhaustive planner picks on
-// blocks of every synthetic family, plus the previous list, measured
-// exactly on every block and added greedily by how much they lower the sum
-// of log2(best size).  The first 26 already reach the pool's best size on
-// every one of those blocks; the rest (in pool order) add nothing there.
+// The network's 48 outputs.  Chosen by select_candidates.py (--per-family 24
+// at 65,536 rows plus --big-per-family 12 at 1,048,576 rows, seed 11): the
+// trees the exhaustive planner picks on blocks of every synthetic family,
+// plus the previous list, measured exactly on all 324 blocks and added
+// greedily by how much they lower the sum of log2(best size); the 48 reach
+// the 64-tree pool's best on every block.  The previous list, chosen on
+// 65,536-row blocks only, was 0.17% larger there and lacked const -> scale
+// -> afl, which the exhaustive planner picks on 1M-row Zipf blocks of a few
+// values (15% smaller than const -> afl on such a block).
 const std::vector<std::vector<int>>& candidate_paths(int cls) {
     if (cls == kClassFloat) return float_candidate_paths();
     static const std::vector<std::vector<int>> paths = [] {
@@ -100,26 +103,32 @@ const std::vector<std::vector<int>>& candidate_paths(int cls) {
             {D, C, K, N, A, N},
             {R, A, N, A, N},
             {S, A, N},
-            {D, S, A, N},
-            {K, N, S, K, N, A, N},
-            {R, K, N, A, N, K, C, N, A, N},
-            {D, K, N, S, A, N},
-            {D, C, S, A, N},
-            {C, A, N},
-            {R, K, N, S, A, N, A, N},
+            {R, K, N, S, A, N, K, C, N, A, N},
             {D, P, S, A, N, S, A, N},
+            {K, N, S, K, N, A, N},
+            {C, S, A, N},
+            {D, S, A, N},
+            {D, R, K, N, A, N, S, A, N},
+            {R, K, N, S, A, N, A, N},
             {D, P, R, N, N, S, A, N},
             {D, C, R, A, N, A, N},
-            {D, R, K, N, A, N, A, N},
-            {C, P, R, N, N, S, A, N},
-            {D, K, N, R, A, N, N},
+            {R, K, N, A, N, D, S, A, N},
+            {D, K, N, S, A, N},
+            {R, K, N, S, A, N, P, A, N, C, A, N},
+            {R, K, N, A, N, K, C, N, A, N},
             {R, K, N, A, N, A, N},
             {C, K, N, A, N},
+            {D, C, S, A, N},
             {A, N},
+            {C, A, N},
             {D, K, N, A, N},
             {D, P, A, N, S, A, N},
             {D, A, N},
             {D, C, A, N},
+            {D, R, K, N, A, N, A, N},
+            {N},
+            {C, P, R, N, N, S, A, N},
+            {D, K, N, R, A, N, N},
             {K, N, S, A, N},
             {K, A, N, A, N},
             {P, A, N, A, N},
@@ -137,12 +146,6 @@ const std::vector<std::vector<int>>& candidate_paths(int cls) {
             {S, P, A, N, A, N},
             {D, F, K, A, N, A, N},
             {D, F, C, A, N},
-            {D, F, R, A, N, A, N},
-            {D, F, A, N},
-            {D, F, P, A, N, A, N},
-            {S, F, K, A, N, A, N},
-            {S, F, C, A, N},
-            {S, F, R, A, N, A, N},
         };
     }();
     return paths;
@@ -239,6 +242,25 @@ std::array<float, kCandidates> candidate_labels(Ctx* c, int dtype, DView data) {
         } catch (const Error& e) {
             if (e.status != DDJ_EUNSUPPORTED) throw;
             out[i] = kInvalidLabel;  // a tree this type cannot take (e.g. AFL on double)
+        }
+    }
+    return out;
+}
+
+std::vector<size_t> candidate_sizes(Ctx* c, int dtype, DView data, const std::vector<int>& idx) {
+    std::vector<size_t> out;
+    SizeEval ev(c);
+    const Part root = borrow(data);
+    const auto& paths = candidate_paths(planner_class(dtype));
+    for (int i : idx) {
+        if (i < 0 || i >= (int)paths.size()) fail(DDJ_EINVAL, "candidate index out of range");
+        NodeP t = tree_from_path(paths[(size_t)i].data(), paths[(size_t)i].size(), dtype);
+        fix_tree(t);
+        try {
+            out.push_back(ev.size(t, root));
+        } catch (const Error& e) {
+            if (e.status != DDJ_EUNSUPPORTED) throw;
+            out.push_back(SIZE_MAX);
         }
     }
     return out;

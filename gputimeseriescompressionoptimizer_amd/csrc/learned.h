@@ -33,6 +33,9 @@ std::array<float, kMlpFeatures> block_features(Ctx* c, int dtype, DView data);
 // Exact blob size of every candidate tree on `data` (shared prefixes are
 // encoded once), as log2(size / input bytes).
 std::array<float, kCandidates> candidate_labels(Ctx* c, int dtype, DView data);
+// Exact blob sizes of the listed candidates on `data` (SIZE_MAX for a tree
+// the type cannot take).
+std::vector<size_t> candidate_sizes(Ctx* c, int dtype, DView data, const std::vector<int>& idx);
 
 class PlannerNet {
 public:

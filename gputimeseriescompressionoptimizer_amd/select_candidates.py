@@ -10,6 +10,7 @@ far), until 48 are chosen.  The result is printed as JSON pre-order paths
 (the list compiled into learned.cpp) and written to --out.
 
     python -m gputimeseriescompressionoptimizer_amd.select_candidates --per-family 64 --out cands.json
+    (--big-rows 1048576 --big-per-family 16: also blocks of 1M rows)
 """
 from __future__ import annotations
 
@@ -43,6 +44,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--per-family", type=int, default=64)
     ap.add_argument("--rows", type=int, default=65536)
+    ap.add_argument("--big-rows", type=int, default=0, help="also blocks of this many rows (0: none)")
+    ap.add_argument("--big-per-family", type=int, default=0)
     ap.add_argument("--k", type=int, default=g.MLP_OUTPUTS)
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--out", default="candidates.json")
@@ -50,21 +53,23 @@ def main():
     args = ap.parse_args()
     cls = g.PLANNER_FLOAT if args.float else g.PLANNER_INT
     families = P.FLOAT_FAMILIES if args.float else P.FAMILIES
-    make = (lambda f, r: P.float_block(f, P._float_params(f, r), args.rows, r)) if args.float else \
-        (lambda f, r: P.block(f, P._params(f, r), args.rows, r))
+    make = (lambda f, r, n: P.float_block(f, P._float_params(f, r), n, r)) if args.float else \
+        (lambda f, r, n: P.block(f, P._params(f, r), n, r))
+    sizes_of = [(args.rows, args.per_family)] + ([(args.big_rows, args.big_per_family)] if args.big_rows else [])
     r = np.random.default_rng(args.seed)
     t0 = time.perf_counter()
     with g.Context(0) as ctx:
         pool = [tuple(p) for p in g.planner_candidates(cls)]
         blocks = []
         for fam in families:
-            for _ in range(args.per_family):
-                x = make(fam, r)
-                buf = ctx.upload(x)
-                tree, _, _ = g.CompressionOptimizer(ctx).PlanExhaustive(buf, g.DTYPE_OF_NP[x.dtype])
-                if tuple(tree.path) not in pool:
-                    pool.append(tuple(tree.path))
-                blocks.append((fam, x))
+            for rows, count in sizes_of:
+                for _ in range(count):
+                    x = make(fam, r, rows)
+                    buf = ctx.upload(x)
+                    tree, _, _ = g.CompressionOptimizer(ctx).PlanExhaustive(buf, g.DTYPE_OF_NP[x.dtype])
+                    if tuple(tree.path) not in pool:
+                        pool.append(tuple(tree.path))
+                    blocks.append((fam, x))
             print(f"[exhaustive] {fam}: pool {len(pool)} trees, {time.perf_counter() - t0:.0f} s", flush=True)
         sizes = np.zeros((len(blocks), len(pool)))
         for b, (_, x) in enumerate(blocks):

@@ -54,14 +54,17 @@ def label_blocks(ctx, blocks, log_every=500, cls=g.PLANNER_INT):
     return np.array(X, np.float32), np.array(T, np.float32), fams
 
 
-def evaluate(ctx, net, stream, exhaustive_every: int, min_margin: float = 0.0, log_every: int = 0):
-    """Per block: learned choice (timed), best-of-48 size, and every k-th
-    block the exhaustive planner's choice (timed) and its size."""
+def evaluate(ctx, net, stream, exhaustive_every: int, min_margin: float = 0.0, log_every: int = 0,
+             check_within: float = 0.3, max_checked: int = 4):
+    """Per block: learned choice (timed), the checked learned choice (exact
+    sizes for the candidates predicted within `check_within` of the best;
+    timed), best-of-48 size, and every k-th block the exhaustive planner's
+    choice (timed) and its size."""
     paths = g.planner_candidates(net.cls)
-    learned_bytes = best_bytes = raw = 0.0
-    ex_learned = ex_bytes = 0.0
-    t_learned = t_ex = 0.0
-    n_ex = n = fallbacks = 0
+    learned_bytes = best_bytes = raw = checked_bytes = 0.0
+    ex_learned = ex_checked = ex_bytes = 0.0
+    t_learned = t_checked = t_ex = 0.0
+    n_ex = n = fallbacks = n_checked = 0
     lab = g.LearnedPlanner(ctx, cls=net.cls)
     for i, (_, fam, x) in enumerate(stream):
         dt = dtype_of(x)
@@ -71,11 +74,18 @@ def evaluate(ctx, net, stream, exhaustive_every: int, min_margin: float = 0.0, l
         path, fb, _ = net.Plan(buf, dt, min_margin=min_margin)
         t_learned += time.perf_counter() - t0
         fallbacks += int(fb)
+        ctx.sync()
+        t0 = time.perf_counter()
+        cpath, k = net.PlanChecked(buf, dt, within=check_within, max_checked=max_checked)
+        t_checked += time.perf_counter() - t0
+        n_checked += int(k > 0)
         labels = lab.Labels(buf, dt)
         sizes = np.exp2(labels.astype(np.float64)) * x.nbytes
         idx = paths.index(path) if path in paths else -1
         chosen = sizes[idx] if idx >= 0 else g.CompressionTree(ctx, path, dt).Compress(buf).nbytes
+        cchosen = sizes[paths.index(cpath)]
         learned_bytes += chosen
+        checked_bytes += cchosen
         best_bytes += sizes.min()
         raw += x.nbytes
         n += 1
@@ -87,6 +97,7 @@ def evaluate(ctx, net, stream, exhaustive_every: int, min_margin: float = 0.0, l
             t_ex += time.perf_counter() - t0
             ex_bytes += tree.Compress(buf).nbytes
             ex_learned += chosen
+            ex_checked += cchosen
             n_ex += 1
         if log_every and (i + 1) % log_every == 0:
             print(f"[eval] {i + 1} blocks", flush=True)
@@ -102,6 +113,11 @@ def evaluate(ctx, net, stream, exhaustive_every: int, min_margin: float = 0.0, l
         "exhaustive_blocks_per_s": n_ex / t_ex if t_ex else None,
         "planning_speedup": (n / t_learned) / (n_ex / t_ex) if t_ex and t_learned else None,
         "fallbacks": fallbacks,
+        "checked": {"within_log2": check_within, "max_checked": max_checked, "blocks_checked": n_checked,
+                    "vs_best_of_48": checked_bytes / best_bytes,
+                    "vs_exhaustive": ex_checked / ex_bytes if ex_bytes else None,
+                    "blocks_per_s": n / t_checked if t_checked else None,
+                    "planning_speedup": (n / t_checked) / (n_ex / t_ex) if t_ex and t_checked else None},
     }
 
 
@@ -121,6 +137,9 @@ def main():
     ap.add_argument("--no-eval", action="store_true", help="label and train only")
     ap.add_argument("--float", action="store_true", help="the float network: float32 / float64 blocks")
     ap.add_argument("--dataset", help="label set (npz with X, T) to train on instead of labelling")
+    ap.add_argument("--big-blocks", type=int, default=0, help="also label this many blocks of --big-rows rows")
+    ap.add_argument("--big-rows", type=int, default=1 << 20)
+    ap.add_argument("--big-eval-blocks", type=int, default=0, help="also evaluate on a stream of --big-rows blocks")
     args = ap.parse_args()
     os.makedirs(args.out_dir, exist_ok=True)
     ctx = g.Context(0)
@@ -141,8 +160,13 @@ def main():
         d = np.load(args.dataset)
         X, T = d["X"], d["T"]
     else:
-        blocks = (PD.random_float_blocks if args.float else PD.random_blocks)(args.blocks, args.rows, args.seed)
-        X, T, fams = label_blocks(ctx, blocks, cls=cls)
+        gen = PD.random_float_blocks if args.float else PD.random_blocks
+        X, T, fams = label_blocks(ctx, gen(args.blocks, args.rows, args.seed), cls=cls)
+        if args.big_blocks:  # blocks of both sizes, shuffled together
+            Xb, Tb, fb = label_blocks(ctx, gen(args.big_blocks, args.big_rows, args.seed + 1), cls=cls)
+            order = np.random.default_rng(args.seed).permutation(len(X) + len(Xb))
+            X, T = np.concatenate([X, Xb])[order], np.concatenate([T, Tb])[order]
+            fams = [(fams + fb)[i] for i in order]
         np.savez_compressed(os.path.join(args.out_dir, "planner_dataset.npz"), X=X, T=T, fams=np.array(fams))
     n_val = max(1, len(X) // 10)
     Xtr, Ttr, Xva, Tva = X[n_val:], T[n_val:], X[:n_val], T[:n_val]
@@ -172,6 +196,12 @@ def main():
                   args.exhaustive_every, log_every=50)
     res = {"train_blocks": len(Xtr), "rows_per_block": args.rows, "epochs": args.epochs, "train_s": train_s,
            "history": history, "eval": ev}
+    if args.big_blocks:
+        res["big_blocks"] = args.big_blocks
+        res["big_rows"] = args.big_rows
+    if args.big_eval_blocks:
+        res["eval_big_rows"] = evaluate(ctx, net, stream(args.big_eval_blocks, args.big_rows, args.regime,
+                                                         args.seed + 2000), args.exhaustive_every, log_every=50)
     with open(os.path.join(args.out_dir, "planner_eval.json"), "w") as f:
         json.dump(res, f, indent=1)
     print(json.dumps(res))

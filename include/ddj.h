@@ -282,6 +282,15 @@ ddj_status ddj_planner_predict(ddj_planner* planner, const float* X, size_t n, f
  * planner's choice when the runner-up is within min_margin (log2 units). */
 ddj_status ddj_planner_plan(ddj_planner* planner, int dtype, const void* d_in, size_t in_bytes, float min_margin,
                             int* path_out, size_t path_cap, size_t* path_len, int* used_fallback, float* margin);
+/* No counterpart in the reference (its planner is the exhaustive search):
+ * for callers that want the learned planner's rate with exact sizes where
+ * the network is unsure.  The candidates predicted within `within` (log2
+ * units) of the best, at most max_checked of them, are compressed exactly
+ * (shared prefixes once) and the smallest is chosen; *checked = how many
+ * were compressed (0 when the best prediction stood alone). */
+ddj_status ddj_planner_plan_checked(ddj_planner* planner, int dtype, const void* d_in, size_t in_bytes,
+                                    float within, int max_checked, int* path_out, size_t path_cap,
+                                    size_t* path_len, int* checked);
 
 /* ------------------------------------------------------------- column API */
 /* ParallelTSCompressor (parallel_ts_compressor.cpp:17-102): rows of several

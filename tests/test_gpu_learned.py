@@ -132,6 +132,33 @@ def test_plan_follows_the_smallest_prediction(ctx):
     assert fallback2 and path2 == O.plan_exhaustive(x, O.D_INT)[0].preorder()
 
 
+def test_checked_plan_takes_the_smallest_of_the_near_predictions(ctx):
+    """ddj_planner_plan_checked: with no margin it is the plain prediction;
+    with a wide one it compresses the best-predicted candidates and keeps
+    the smallest exact size, never larger than the plain choice's."""
+    net = g.LearnedPlanner(ctx, seed=4)
+    paths = g.planner_candidates()
+    for seed, x in enumerate([D.step_function(30_000, seed=2), D.zipf_codes(40_000, seed=3),
+                              D.timestamps(25_000, seed=4)]):
+        buf = ctx.upload(x)
+        y = net.Predict(net.Features(buf, g.D_INT)[None, :])[0]
+        plain, _, _ = net.Plan(buf, g.D_INT)
+        p0, k0 = net.PlanChecked(buf, g.D_INT, within=0.0, max_checked=4)
+        assert p0 == plain and k0 == 0
+        p1, k1 = net.PlanChecked(buf, g.D_INT, within=1e9, max_checked=5)
+        assert k1 == 5
+        def size(i):
+            return g.CompressionTree(ctx, paths[i], g.D_INT).Compress(buf).nbytes
+
+        top = [int(i) for i in np.argsort(y, kind="stable")[:5]]
+        exact = [size(i) for i in top]
+        want = top[int(np.argmin(exact))]  # the first of equal sizes: the better prediction
+        t = O.tree_from_path(paths[want], O.D_INT)
+        t.fix()
+        assert p1 == t.preorder(), seed
+        assert min(exact) <= size(int(np.argmin(y)))
+
+
 def test_trained_weights_stay_near_exhaustive_and_beat_untrained(ctx):
     """The committed weights, on a stream whose generator family changes every
     10 blocks, choose trees whose total compressed size is within 2% of the
