@@ -766,7 +766,6 @@ PatchAflParts patch_afl_encode(Ctx* c, int dtype, DView in, bool outside) {
     job->ybytes = bytes[0];
     job->nbytes = bytes[1];
     job->expected = 1 + (bytes[0] ? 1 : 0) + (bytes[1] ? 1 : 0);
-    kernel_bytes(c, "k_split_count", (double)(n * es));
     auto deferred = [&](int which, uint64_t nb) {
         Part part;
         part.n = (size_t)nb;
