@@ -56,7 +56,7 @@ __device__ __forceinline__ void range_minmax(const T* __restrict__ x, bool delta
     for (uint64_t b = v0; b < v1; b += kBlock) {  // uniform trip count: every lane takes part in the shuffles
         const uint64_t i = b + threadIdx.x;
         const bool valid = i < v1;
-        u32x4 w = valid ? __builtin_nontemporal_load(x4 + i) : u32x4{0u, 0u, 0u, 0u};
+        u32x4 w = valid ? x4[i] : u32x4{0u, 0u, 0u, 0u};
         T nx = T(0);
         if (delta && valid && (lane == kWave - 1 || i + 1 >= v1)) nx = x[(i + 1) * V];  // row (i+1)V <= end exists
         const uint32_t nb0 = (uint32_t)__shfl_down((int)w.x, 1, kWave);

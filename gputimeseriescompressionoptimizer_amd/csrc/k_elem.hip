@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kBlock) void k_delta_encode(const T* __restrict__ x
     const size_t nv = vec ? m / V : 0;
     for (size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x; g < nv; g += stride) {
         const size_t i0 = g * V;
-        u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(x) + g);
+        u32x4 a = reinterpret_cast<const u32x4*>(x)[g];
         T v[V + 1];
         __builtin_memcpy(v, &a, 16);
         v[V] = x[i0 + V];  // i0 + V <= m < n
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kBlock) void k_add_scalar(const T* __restrict__ x, 
     const bool vec = ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
     const size_t nv = vec ? n / V : 0;
     for (size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x; g < nv; g += stride) {
-        u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(x) + g);
+        u32x4 a = reinterpret_cast<const u32x4*>(x)[g];
         T v[V];
         __builtin_memcpy(v, &a, 16);
 #pragma unroll
@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kBlock) void k_float_to_int(const T* __restrict__ x
     for (size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x; g < n4; g += stride) {
         u32x4 a[L];
 #pragma unroll
-        for (int l = 0; l < L; l++) a[l] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(x + 4 * g) + l);
+        for (int l = 0; l < L; l++) a[l] = reinterpret_cast<const u32x4*>(x + 4 * g)[l];
         T v[4] = {};
         if constexpr (sizeof(T) >= 4) __builtin_memcpy(v, a, 16 * L);
         int32_t r[4];
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void k_int_to_float(const int32_t* __restri
     const bool vec = sizeof(T) == 4 && ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(out)) & 15) == 0;
     const size_t n4 = vec ? n / 4 : 0;
     for (size_t g = (size_t)blockIdx.x * kBlock + threadIdx.x; g < n4; g += stride) {
-        const u32x4 a = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(x + 4 * g));
+        const u32x4 a = *reinterpret_cast<const u32x4*>(x + 4 * g);
         int32_t v[4];
         __builtin_memcpy(v, &a, 16);
         T r[4];
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_delta_sums(const uint8_t* __restrict
         const u32x4* s4 = reinterpret_cast<const u32x4*>(tp);
 #pragma unroll
         for (int q = 0; q < kScanTile / V / kBlock; q++) {
-            const u32x4 w = __builtin_nontemporal_load(s4 + threadIdx.x + kBlock * q);
+            const u32x4 w = s4[threadIdx.x + kBlock * q];
             T v[V];
             __builtin_memcpy(v, &w, 16);
 #pragma unroll
@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_delta_decode(const uint8_t* first_by
 #pragma unroll
         for (int q = 0; q < kScanTile / V / kBlock; q++) {
             const int i = threadIdx.x + kBlock * q;
-            const u32x4 w = __builtin_nontemporal_load(s4 + i);
+            const u32x4 w = s4[i];
             T v[V];
             __builtin_memcpy(v, &w, 16);
 #pragma unroll

@@ -44,7 +44,7 @@ __device__ __forceinline__ void fl_pack_tile(const T* __restrict__ x, size_t n, 
         const u32x4* s4 = reinterpret_cast<const u32x4*>(src);
         u32x4 q[N4 / kBlock];
 #pragma unroll
-        for (int r = 0; r < N4 / kBlock; r++) q[r] = __builtin_nontemporal_load(s4 + threadIdx.x + r * kBlock);
+        for (int r = 0; r < N4 / kBlock; r++) q[r] = s4[threadIdx.x + r * kBlock];
 #pragma unroll
         for (int r = 0; r < N4 / kBlock; r++) {
             T t[V];

@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kHistThreads) void k_hist_count(const T* __restrict
             const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
             u32x4 w[Q];
 #pragma unroll
-            for (int q = 0; q < Q; q++) w[q] = __builtin_nontemporal_load(s4 + threadIdx.x + kHistThreads * q);
+            for (int q = 0; q < Q; q++) w[q] = s4[threadIdx.x + kHistThreads * q];
 #pragma unroll
             for (int q = 0; q < Q; q++) {
                 T v[V];

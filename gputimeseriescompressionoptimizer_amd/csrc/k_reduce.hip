@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_minmax(const T* __restrict__ x, size
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const size_t i = i0 + u * stride;
-                if (i < nv) v[u] = __builtin_nontemporal_load(xv + i);
+                if (i < nv) v[u] = xv[i];
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -188,7 +188,7 @@ __global__ __launch_bounds__(kBlock) void k_minmax_delta(const T* __restrict__ x
             for (int u = 0; u < Uf; u++) {
                 const size_t i = i0 + u * stride;
                 if (i < nv) {
-                    v[u] = __builtin_nontemporal_load(xv + i);
+                    v[u] = xv[i];
                     if (lane == kWave - 1 || i + 1 >= nv) nx[u] = x[(i + 1) * V];  // (i + 1) * V <= m < n
                 }
             }
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_minmax_prec(const T* __restrict__ x,
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const size_t i = i0 + u * stride;
-                if (i < nv) v[u] = __builtin_nontemporal_load(xv + i);
+                if (i < nv) v[u] = xv[i];
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -438,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void k_stats(const T* __restrict__ x, size_
         const u32x4* xv = reinterpret_cast<const u32x4*>(x);
         const size_t nv = n / V;
         for (size_t i = tid; i < nv; i += stride) {
-            u32x4 v = __builtin_nontemporal_load(xv + i);
+            u32x4 v = xv[i];
             T vals[V + 1];
             __builtin_memcpy(vals, &v, 16);
             const size_t base = i * V;

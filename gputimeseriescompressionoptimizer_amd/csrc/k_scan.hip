@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_split_count(const T* __restrict__ x,
         const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
 #pragma unroll
         for (int q = 0; q < kTile / V / kBlock; q++) {
-            const u32x4 w = __builtin_nontemporal_load(s4 + threadIdx.x + kBlock * q);
+            const u32x4 w = s4[threadIdx.x + kBlock * q];
             T v[V];
             __builtin_memcpy(v, &w, 16);
 #pragma unroll
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
         constexpr int V = 16 / sizeof(T);
         const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
         for (int i = threadIdx.x; i < kTile / V; i += kBlock) {
-            const u32x4 w = __builtin_nontemporal_load(s4 + i);
+            const u32x4 w = s4[i];
             T vals[V];
             __builtin_memcpy(vals, &w, 16);
 #pragma unroll
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, si
         constexpr int V = 16 / sizeof(T);
         const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
         for (int i = threadIdx.x; i < kTile / V; i += kBlock) {
-            const u32x4 w = __builtin_nontemporal_load(s4 + i);
+            const u32x4 w = s4[i];
             T vals[V];
             __builtin_memcpy(vals, &w, 16);
 #pragma unroll
@@ -580,7 +580,7 @@ __device__ __forceinline__ bool stage_rows(const uint8_t* __restrict__ src, uint
         const int nv = (sh + ok * S + 15) / 16;
         if (sh % S == 0) {  // rows on the chunk grid (uniform)
             for (int k = threadIdx.x; k < nv; k += kBlock) {
-                const u32x4 w = __builtin_nontemporal_load(v4 + k);
+                const u32x4 w = v4[k];
                 T ev[V];
                 __builtin_memcpy(ev, &w, 16);
                 const int e0 = V * k - sh / S;
@@ -593,7 +593,7 @@ __device__ __forceinline__ bool stage_rows(const uint8_t* __restrict__ src, uint
         } else if constexpr (S > 1) {
             const int r = sh % S;
             for (int k = threadIdx.x; k < nv; k += kBlock) {
-                const u32x4 c0 = __builtin_nontemporal_load(v4 + k);
+                const u32x4 c0 = v4[k];
                 const u32x4 c1 = k + 1 < nv ? v4[k + 1] : u32x4{0u, 0u, 0u, 0u};
                 const uint32_t w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
                 const int e0 = (16 * k + r - sh) / S;  // the first row starting in chunk k
@@ -682,7 +682,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
             const u32x4* c4 = reinterpret_cast<const u32x4*>(a & ~(uintptr_t)15);
             const int nchunks = (off0 + 4 * (int)(w_hi - w_lo + 1) + 15) / 16;  // <= kCodeLdsWords / 4
             u32x4* l4 = reinterpret_cast<u32x4*>(cw);
-            for (int k = threadIdx.x; k < nchunks; k += kBlock) l4[k] = __builtin_nontemporal_load(c4 + k);
+            for (int k = threadIdx.x; k < nchunks; k += kBlock) l4[k] = c4[k];
         }
     } else if constexpr (FL) {
         // one side at a time through wfl; rows past a stream read as zero
@@ -717,24 +717,51 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
     if (bad) __hip_atomic_store(error, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     T v[kItems];
     int yr = (int)excl, nr = total + threadIdx.x * kItems - (int)excl;
+    // CODES, staged: the thread's yes rows are the consecutive codes j0 =
+    // Y0 + excl, j0 + 1, ...; its first code's word and field are found
+    // once, then codes are shifted out of the current word, the next word
+    // read from LDS every 32 / bits codes
+    auto code_word = [&](int b) -> uint32_t {  // staged code bytes [b, b + 4)
+        const int q = b >> 2, r = b & 3;
+        return r ? __builtin_amdgcn_alignbyte(cw[q + 1], cw[q], (uint32_t)r) : cw[q];
+    };
+    uint32_t cur = 0, cper = 1;
+    int cq = 0, fleft = 1, ylim = 0;
+    if constexpr (CODES) {
+        if (staged) {
+            cper = 32u / dc.bits;
+            const uint64_t j0 = Y0 + excl;
+            const uint64_t w0 = dc.pshift >= 0 ? (j0 >> dc.pshift) : j0 / cper;
+            const uint32_t f0 = (uint32_t)(j0 - w0 * cper);
+            cq = off0 + 4 * (int)(w0 - w_lo);
+            cur = code_word(cq) >> (dc.bits * f0);
+            fleft = (int)(cper - f0);
+            ylim = j0 < n_yes ? (int)((n_yes - j0) < (uint64_t)kItems ? (n_yes - j0) : (uint64_t)kItems) : 0;
+        }
+    }
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
         if ((mask >> k) & 1) {
             if constexpr (CODES) {
-                const uint64_t j = Y0 + (uint64_t)yr;
                 T val = T(0);
-                if (j < n_yes) {
-                    if (staged) {
-                        const uint64_t per = 32 / dc.bits;
-                        const uint64_t w = dc.pshift >= 0 ? (j >> dc.pshift) : j / per;
-                        const uint32_t f = (uint32_t)(j - w * per);
-                        const int boff = off0 + 4 * (int)(w - w_lo);
-                        const int q = boff >> 2, r = boff & 3;
-                        const uint32_t word = r ? __builtin_amdgcn_alignbyte(cw[q + 1], cw[q], (uint32_t)r) : cw[q];
-                        val = dict_key<T>(dc, sdict, word, f);
-                    } else {
-                        val = dict_value<T>(dc, sdict, j);
+                if (staged) {
+                    uint32_t code = cur & ((1u << dc.bits) - 1u);
+                    cur >>= dc.bits;
+                    if (--fleft == 0) {
+                        cq += 4;
+                        cur = code_word(cq);
+                        fleft = (int)cper;
                     }
+                    if (yr - (int)excl < ylim) {  // codes past the stream read as zero (flagged above)
+                        code = code < dc.K ? code : dc.K - 1;
+                        if (dc.K <= kLdsDict)
+                            val = sdict[code];
+                        else
+                            __builtin_memcpy(&val, dc.dict + sizeof(T) * code, sizeof(T));
+                    }
+                } else {
+                    const uint64_t j = Y0 + (uint64_t)yr;
+                    if (j < n_yes) val = dict_value<T>(dc, sdict, j);
                 }
                 v[k] = val;
             } else {
@@ -804,7 +831,7 @@ __global__ __launch_bounds__(kBlock) void k_rle_count(const T* __restrict__ x, s
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             const size_t g = base + (size_t)V * (threadIdx.x + kBlock * q);
-            w[q] = __builtin_nontemporal_load(s4 + threadIdx.x + kBlock * q);
+            w[q] = s4[threadIdx.x + kBlock * q];
             if (lane0 && g > 0) before[q] = x[g - 1];
         }
 #pragma unroll
@@ -845,7 +872,7 @@ __global__ __launch_bounds__(kBlock) void k_rle_encode(const T* __restrict__ x, 
         constexpr int V = 16 / sizeof(T);
         const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
         for (int i = threadIdx.x; i < kTile / V; i += kBlock) {
-            const u32x4 w = __builtin_nontemporal_load(s4 + i);
+            const u32x4 w = s4[i];
             T vals[V];
             __builtin_memcpy(vals, &w, 16);
 #pragma unroll
