@@ -84,6 +84,7 @@ struct MinMax {
     MinMaxResult host;
 };
 MinMax minmax(Ctx* c, int dtype, const void* x, size_t n) {
+    if (dtype != DDJ_D_FLOAT && dtype != DDJ_D_DOUBLE) return MinMax{DevBuf(), minmax_to_host(c, dtype, x, n, false)};
     MinMax m{DevBuf(c, sizeof(MinMaxResult)), {}};
     launch_minmax(c, dtype, x, n, m.buf.get<MinMaxResult>());
     d2h(c, &m.host, m.buf.get(), sizeof(MinMaxResult));
@@ -215,17 +216,20 @@ std::vector<Part> afl_encode_int(Ctx* c, int dtype, DView in, const MinMaxResult
     }
     DevBuf mmb;
     const MinMaxResult* mm = known;
-    if (!mm) {
+    MinMaxResult h;
+    if (!mm && defer) {  // the width only, folded on the host
+        h = minmax_to_host(c, dtype, x.p, n, false);
+        kernel_bytes(c, "k_minmax", (double)(n * es));
+    } else if (!mm) {
         mmb = DevBuf(c, sizeof(MinMaxResult));
         launch_minmax(c, dtype, x.p, n, mmb.get<MinMaxResult>());
         kernel_bytes(c, "k_minmax", (double)(n * es));
         mm = mmb.get<MinMaxResult>();
     }
     const uint64_t W = 8 * es;
-    MinMaxResult h;
     Part data;
     if (defer) {  // the width first; the pack runs when the blob is assembled
-        d2h(c, &h, mm, sizeof h);
+        if (mm) d2h(c, &h, mm, sizeof h);
         auto job = std::make_shared<FlJob>();
         job->dtype = dtype;
         job->bits = bits_of(dtype, h);
@@ -897,14 +901,18 @@ std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in, bool defer) {
     if (n < 2) fail(DDJ_EINVAL, "fused delta/FL needs two rows");
     Part x = aligned_view(c, in, es);
     const size_t m = n - 1;
-    DevBuf mmb(c, sizeof(MinMaxResult));
-    launch_minmax_delta(c, (int)es, x.p, n, mmb.get<MinMaxResult>());
+    DevBuf mmb;
+    MinMaxResult h;
+    if (defer) {  // the width only, folded on the host
+        h = minmax_to_host(c, dtype, x.p, n, true);
+    } else {
+        mmb = DevBuf(c, sizeof(MinMaxResult));
+        launch_minmax_delta(c, (int)es, x.p, n, mmb.get<MinMaxResult>());
+    }
     kernel_bytes(c, "k_minmax", (double)(n * es));
     const uint64_t W = 8 * es;
-    MinMaxResult h;
     Part data;
     if (defer) {  // the width first; the pack runs when the blob is assembled
-        d2h(c, &h, mmb.get(), sizeof h);
         auto job = std::make_shared<FlJob>();
         job->dtype = dtype;
         job->delta = true;

@@ -376,9 +376,15 @@ __global__ __launch_bounds__(kBlock) void k_hist_topk(const unsigned long long* 
     __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ void k_fill_u64(unsigned long long* p, size_t n, unsigned long long v) {
+// The histogram's global table emptied (keys ~0, counts 0) and its control
+// block cleared, in one launch.
+__global__ void k_hist_init(unsigned long long* keys, unsigned long long* counts, size_t cap, HistCtl* ctl) {
     const size_t stride = (size_t)gridDim.x * kBlock;
-    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) p[i] = v;
+    for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < cap; i += stride) {
+        keys[i] = kEmpty;
+        counts[i] = 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ctl = HistCtl{};
 }
 
 uint64_t pow2_at_least(uint64_t v) {
@@ -400,10 +406,11 @@ void topk_most_frequent(Ctx* c, int dtype, const void* x, size_t n, int K, uint6
         for (int attempt = 0; attempt < 3; attempt++) {
             DevBuf keys(c, 8 * cap), counts(c, 8 * cap), ctl(c, sizeof(HistCtl));
             const unsigned fg = (unsigned)std::min<uint64_t>((cap + kBlock - 1) / kBlock, (uint64_t)c->cu_count * 16);
-            { KernelTimer kt_(c, "k_fill_u64"); hipLaunchKernelGGL(k_fill_u64, dim3(fg), dim3(kBlock), 0, c->stream, keys.get<unsigned long long>(), cap,
-                               (unsigned long long)kEmpty); }
-            launch_fill_zero(c, counts.get(), 8 * cap);
-            launch_fill_zero(c, ctl.get(), sizeof(HistCtl));
+            {
+                KernelTimer kt_(c, "k_hist_init");
+                hipLaunchKernelGGL(k_hist_init, dim3(fg), dim3(kBlock), 0, c->stream, keys.get<unsigned long long>(),
+                                   counts.get<unsigned long long>(), cap, ctl.get<HistCtl>());
+            }
             size_t g = (n + 16 * kHistThreads - 1) / (16 * kHistThreads);  // at most one workgroup per tile and per CU
             g = std::max<size_t>(1, std::min<size_t>(g, (size_t)c->cu_count));
             DevBuf nan_keys(c, 8 * kMaxK * g), nan_n(c, 4 * g);

@@ -10,6 +10,7 @@
 // [16t, 16t+16) of the tile, i.e. exactly two stencil bytes.  Inputs are
 // staged through LDS with striped (coalesced) loads, outputs are compacted in
 // LDS and written back striped.
+#include <vector>
 #include <limits>
 #include <type_traits>
 
@@ -1356,16 +1357,21 @@ uint64_t launch_rle_encode(Ctx* c, int dtype, const void* x, size_t n, int32_t* 
         { KernelTimer kt_(c, "k_rle_lengths"); hipLaunchKernelGGL(k_rle_lengths, dim3(grid_for(c, n)), dim3(kBlock), 0, c->stream,
                            starts.get<uint32_t>(), (const uint64_t*)d_runs, n, lengths); }
         HIPC(hipGetLastError());
-        DevBuf mm(c, 2 * sizeof(MinMaxResult));
-        MinMaxResult* dmm = mm.get<MinMaxResult>();
-        if (h_values && std::is_integral<T>::value) launch_minmax(c, dtype, values, n, dmm, d_runs);
-        if (h_lengths) launch_minmax(c, DDJ_D_INT, lengths, n, dmm + 1, d_runs);
+        // the two min/max reductions' partials come back with the run count
+        // and are folded on the host
+        const bool mv = h_values && std::is_integral<T>::value;
+        DevBuf pv, pl;
+        const int gv = mv ? launch_minmax_partials(c, dtype, values, n, d_runs, pv) : 0;
+        const int gl = h_lengths ? launch_minmax_partials(c, DDJ_D_INT, lengths, n, d_runs, pl) : 0;
+        std::vector<uint64_t> hv(3 * (size_t)gv), hl(3 * (size_t)gl);
         uint64_t runs = 0;
-        ReadBack rb[3] = {{&runs, d_runs, 8}, {nullptr, dmm, 0}, {nullptr, dmm + 1, 0}};
+        ReadBack rb[3] = {{&runs, d_runs, 8}, {nullptr, nullptr, 0}, {nullptr, nullptr, 0}};
         size_t k = 1;
-        if (h_values && std::is_integral<T>::value) rb[k++] = ReadBack{h_values, dmm, sizeof(MinMaxResult)};
-        if (h_lengths) rb[k++] = ReadBack{h_lengths, dmm + 1, sizeof(MinMaxResult)};
+        if (mv) rb[k++] = ReadBack{hv.data(), pv.get(), 8 * hv.size()};
+        if (h_lengths) rb[k++] = ReadBack{hl.data(), pl.get(), 8 * hl.size()};
         d2h_many(c, rb, k);
+        if (mv) *h_values = fold_minmax_partials(hv.data(), gv, 3);
+        if (h_lengths) *h_lengths = fold_minmax_partials(hl.data(), gl, 3);
         return runs;
     });
 }

@@ -65,6 +65,15 @@ void launch_minmax(Ctx* c, int dtype, const void* x, size_t n, MinMaxResult* d_r
                    const uint64_t* d_rows = nullptr /* device row count (<= n) read by the kernel, or null */);
 // min/max of the wrapping differences x[i+1] - x[i] of n integer rows.
 void launch_minmax_delta(Ctx* c, int elem_size, const void* x, size_t n, MinMaxResult* d_result);
+// Integer columns whose min/max the host needs: the reduction's partials
+// are read back and folded on the host (no fold launch).  delta: of the
+// differences x[i+1] - x[i].
+MinMaxResult minmax_to_host(Ctx* c, int dtype, const void* x, size_t n, bool delta);
+// The reduction alone (integer columns, rows from *d_rows when given):
+// `partials` receives 3 order keys per workgroup, of which the returned
+// count; fold_minmax_partials folds them (stride 3; 2 for delta) on the host.
+int launch_minmax_partials(Ctx* c, int dtype, const void* x, size_t n, const uint64_t* d_rows, DevBuf& partials);
+MinMaxResult fold_minmax_partials(const uint64_t* h, int grid, int stride);
 // FL pack of the m = n - 1 differences of n integer rows (bits from mm).
 // With mm null the width is `bits`; no byte at or past out + out_bytes is
 // written (0: room for the widest b, rounded up to whole words).
