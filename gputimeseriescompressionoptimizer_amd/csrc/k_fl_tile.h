@@ -84,18 +84,38 @@ __device__ __forceinline__ void fl_pack_tile(const T* __restrict__ x, size_t n, 
             }
         };
         const int lim = nv - 32 * g;  // values of the group inside the stream
+        if constexpr (sizeof(T) <= 4) {
+            // b <= 32: a 32-bit accumulator; nb depends on b and i only, so
+            // it is uniform and the shifts take scalar amounts
+            const uint32_t m32 = (uint32_t)mask;
+            uint32_t a32 = 0;
+            int nb32 = 0;
 #pragma unroll
-        for (int i = 0; i < 32; i++) {
-            // past the stream's end the deltas, like the values, are zero
-            const U d = DELTA ? (i < lim ? (U)(v[i + 1] - v[i]) : U(0)) : v[i];
-            const uint64_t val = (uint64_t)d & mask;
-            if (sizeof(T) <= 4) {
-                push(val, b);
-            } else if (b <= 32) {
-                push(val, b);
-            } else {
-                push(val & 0xffffffffull, 32);
-                push(val >> 32, b - 32);
+            for (int i = 0; i < 32; i++) {
+                const U d = DELTA ? (i < lim ? (U)(v[i + 1] - v[i]) : U(0)) : v[i];
+                const uint32_t val = (uint32_t)d & m32;
+                a32 |= val << nb32;
+                const int nn = nb32 + b;
+                if (nn >= 32) {
+                    w[k++] = a32;
+                    a32 = nb32 ? (val >> (32 - nb32)) : 0u;  // the bits that did not fit
+                    nb32 = nn - 32;
+                } else {
+                    nb32 = nn;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 32; i++) {
+                // past the stream's end the deltas, like the values, are zero
+                const U d = DELTA ? (i < lim ? (U)(v[i + 1] - v[i]) : U(0)) : v[i];
+                const uint64_t val = (uint64_t)d & mask;
+                if (b <= 32) {
+                    push(val, b);
+                } else {
+                    push(val & 0xffffffffull, 32);
+                    push(val >> 32, b - 32);
+                }
             }
         }
     }

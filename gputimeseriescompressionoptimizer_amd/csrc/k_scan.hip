@@ -372,14 +372,34 @@ __global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, si
                 nb -= 32;
             }
         };
+        if constexpr (sizeof(T) <= 4) {
+            // b <= 32: a 32-bit accumulator whose bit count is uniform
+            const uint32_t m32 = (uint32_t)m;
+            uint32_t a32 = 0;
+            int nb32 = 0;
 #pragma unroll
-        for (int i = 0; i < 32; i++) {
-            const uint64_t val = (uint64_t)w32[i] & m;
-            if (sizeof(T) <= 4 || b <= 32) {
-                push(val, b);
-            } else {
-                push(val & 0xffffffffull, 32);
-                push(val >> 32, b - 32);
+            for (int i = 0; i < 32; i++) {
+                const uint32_t val = (uint32_t)w32[i] & m32;
+                a32 |= val << nb32;
+                const int nn = nb32 + b;
+                if (nn >= 32) {
+                    w[k++] = a32;
+                    a32 = nb32 ? (val >> (32 - nb32)) : 0u;  // the bits that did not fit
+                    nb32 = nn - 32;
+                } else {
+                    nb32 = nn;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 32; i++) {
+                const uint64_t val = (uint64_t)w32[i] & m;
+                if (b <= 32) {
+                    push(val, b);
+                } else {
+                    push(val & 0xffffffffull, 32);
+                    push(val >> 32, b - 32);
+                }
             }
         }
     }
