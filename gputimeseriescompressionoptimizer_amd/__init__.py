@@ -265,11 +265,12 @@ class Context:
                "k_fl_pack_lanes", "k_fl_unpack", "k_fl_unpack_delta_scan", "k_fl_unpack_lanes", "k_float_compose",
                "k_float_split", "k_float_to_int", "k_gather_copy", "k_gfc_chunk_bytes", "k_gfc_decode", "k_gfc_encode",
                "k_gfc_sizes", "k_gfc_walk", "k_hist_count", "k_hist_init", "k_hist_topk", "k_int_to_float",
-               "k_length_sums", "k_merge", "k_merge_fl", "k_minmax", "k_minmax_fold", "k_minmax_prec",
-               "k_mlp_backward", "k_mlp_forward", "k_order_keys", "k_parse_tree", "k_radix_count", "k_radix_scatter",
-               "k_rle_count", "k_rle_encode", "k_rle_expand", "k_rle_lengths", "k_rle_tile_runs", "k_scan_block_apply",
-               "k_scan_block_sums", "k_scan_lengths", "k_scan_tiles", "k_split", "k_split_count", "k_split_fl",
-               "k_stats", "k_stencil_count", "k_unique_count", "k_unique_decode", "k_unique_encode", "k_unique_write"]
+               "k_length_sums", "k_merge", "k_merge_fl", "k_minmax", "k_minmax_delta", "k_minmax_fold",
+               "k_minmax_prec", "k_mlp_backward", "k_mlp_forward", "k_order_keys", "k_parse_tree", "k_radix_count",
+               "k_radix_scatter", "k_rle_count", "k_rle_encode", "k_rle_expand", "k_rle_lengths", "k_rle_tile_runs",
+               "k_scan_block_apply", "k_scan_block_sums", "k_scan_lengths", "k_scan_tiles", "k_split", "k_split_count",
+               "k_split_fl", "k_stats", "k_stencil_count", "k_unique_count", "k_unique_decode", "k_unique_encode",
+               "k_unique_write"]
 
     def kernel_table(self):
         """{kernel: (ms, launches, algorithmic bytes)} for every kernel that ran."""

@@ -909,7 +909,7 @@ std::vector<Part> delta_afl_encode(Ctx* c, int dtype, DView in, bool defer) {
         mmb = DevBuf(c, sizeof(MinMaxResult));
         launch_minmax_delta(c, (int)es, x.p, n, mmb.get<MinMaxResult>());
     }
-    kernel_bytes(c, "k_minmax", (double)(n * es));
+    kernel_bytes(c, "k_minmax_delta", (double)(n * es));
     const uint64_t W = 8 * es;
     Part data;
     if (defer) {  // the width first; the pack runs when the blob is assembled

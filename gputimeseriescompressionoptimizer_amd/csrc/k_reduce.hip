@@ -633,7 +633,7 @@ MinMaxResult minmax_to_host(Ctx* c, int dtype, const void* x, size_t n, bool del
             const int stride = delta ? 2 : 3;
             DevBuf partials(c, 8 * (size_t)stride * (size_t)grid);
             {
-                KernelTimer kt_(c, "k_minmax");
+                KernelTimer kt_(c, delta ? "k_minmax_delta" : "k_minmax");
                 if (delta)
                     hipLaunchKernelGGL(k_minmax_delta<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x),
                                        n, partials.get<uint64_t>());
@@ -656,7 +656,7 @@ void launch_minmax_delta(Ctx* c, int elem_size, const void* x, size_t n, MinMaxR
         using T = decltype(tag);
         const int grid = reduce_grid(c, n, sizeof(T));
         DevBuf partials(c, 16 * (size_t)grid);
-        { KernelTimer kt_(c, "k_minmax"); hipLaunchKernelGGL(k_minmax_delta<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
+        { KernelTimer kt_(c, "k_minmax_delta"); hipLaunchKernelGGL(k_minmax_delta<T>, dim3(grid), dim3(kBlock), 0, c->stream, static_cast<const T*>(x), n,
                            partials.get<uint64_t>()); }
         HIPC(hipGetLastError());
         { KernelTimer kt_(c, "k_minmax_fold"); hipLaunchKernelGGL((k_minmax_fold<T, 2>), dim3(1), dim3(kBlock), 0, c->stream,

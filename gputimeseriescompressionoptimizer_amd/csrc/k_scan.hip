@@ -356,13 +356,19 @@ __global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, si
         }
     }
     __syncthreads();  // the packed words overwrite buf
+    // Each side's words are laid out on the stream's word grid: local word q
+    // is stream word wf + q, the side's first row starting at bit o of word
+    // 0.  Group g then starts at bit o of its first word, whose low o bits
+    // are the previous group's last bits (carry[side][g - 1]).
+    __shared__ uint32_t carry[2][kTile / 32];
     uint32_t* words = reinterpret_cast<uint32_t*>(buf);
-    const int wy = (cy + 31) / 32 * yside.bits;  // yes words first, then the no words
+    const int wy = (cy + 31) / 32 * yside.bits + 1;  // yes words (and the partial word after them), then the no words
+    const int o_me = (int)(((is_yes ? Y0 : (uint64_t)(base - Y0)) * (uint64_t)b) & 31);
     if (packs) {
         uint32_t* w = words + (is_yes ? 0 : wy) + grp * b;
         const uint64_t m = lowmask64((unsigned)b);
         uint64_t acc = 0;
-        int nb = 0, k = 0;
+        int nb = o_me, k = 0;
         auto push = [&](uint64_t bits, int len) {  // len <= 32
             acc |= bits << nb;
             nb += len;
@@ -376,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, si
             // b <= 32: a 32-bit accumulator whose bit count is uniform
             const uint32_t m32 = (uint32_t)m;
             uint32_t a32 = 0;
-            int nb32 = 0;
+            int nb32 = o_me;
 #pragma unroll
             for (int i = 0; i < 32; i++) {
                 const uint32_t val = (uint32_t)w32[i] & m32;
@@ -390,6 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, si
                     nb32 = nn;
                 }
             }
+            carry[is_yes ? 0 : 1][grp] = a32;  // the group's last o bits
         } else {
 #pragma unroll
             for (int i = 0; i < 32; i++) {
@@ -401,7 +408,14 @@ __global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, si
                     push(val >> 32, b - 32);
                 }
             }
+            carry[is_yes ? 0 : 1][grp] = (uint32_t)acc;  // the group's last o bits
         }
+    }
+    __syncthreads();
+    if (packs) {  // each group's first word takes the previous group's last bits
+        uint32_t* w = words + (is_yes ? 0 : wy) + grp * b;
+        if (grp > 0) w[0] |= carry[is_yes ? 0 : 1][grp - 1];
+        if (32 * (grp + 1) >= cs) w[b] = carry[is_yes ? 0 : 1][grp];  // the last group: the word after it
     }
     __syncthreads();
 #pragma unroll
@@ -416,12 +430,8 @@ __global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, si
         const uint64_t wf = bit0 >> 5;
         const uint64_t L = (uint64_t)ns * bb;
         const int nw = (int)((o + L + 31) >> 5);
-        const int nloc = (int)((L + 31) >> 5);
         const uint32_t* lw = words + (side == 0 ? 0 : wy);
-        auto local = [&](int q) -> uint32_t { return (q >= 0 && q < nloc) ? lw[q] : 0u; };
-        auto outw = [&](int q) -> uint32_t {  // stream word wf + q as this tile fills it
-            return o ? ((local(q) << o) | (local(q - 1) >> (32 - o))) : local(q);
-        };
+        auto outw = [&](int q) -> uint32_t { return lw[q]; };  // stream word wf + q as this tile fills it
         const int q_lo = o ? 1 : 0;
         const int q_hi = (int)((o + L) >> 5);  // words this tile fills to their end
         if (threadIdx.x == 0 && (o || q_hi == 0)) fs.edges[4 * (size_t)tile + 2 * side] = outw(0);
