@@ -269,8 +269,8 @@ class Context:
                "k_minmax_prec", "k_mlp_backward", "k_mlp_forward", "k_order_keys", "k_parse_tree", "k_radix_count",
                "k_radix_scatter", "k_rle_count", "k_rle_encode", "k_rle_expand", "k_rle_lengths", "k_rle_tile_runs",
                "k_scan_block_apply", "k_scan_block_sums", "k_scan_lengths", "k_scan_tiles", "k_split", "k_split_count",
-               "k_split_fl", "k_stats", "k_stencil_count", "k_unique_count", "k_unique_decode", "k_unique_encode",
-               "k_unique_write"]
+               "k_split_fl", "k_stats", "k_stencil_count", "k_to_host", "k_unique_count", "k_unique_decode",
+               "k_unique_encode", "k_unique_write"]
 
     def kernel_table(self):
         """{kernel: (ms, launches, algorithmic bytes)} for every kernel that ran."""

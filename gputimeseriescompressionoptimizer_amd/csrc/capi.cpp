@@ -32,8 +32,17 @@ static hipError_t init_ctx(Ctx& c, int device) {
     if (e == hipSuccess) {
         c.staging_size = 4u << 20;
         e = hipHostMalloc(reinterpret_cast<void**>(&c.staging), c.staging_size, hipHostMallocDefault);
+        if (e == hipSuccess && hipHostGetDevicePointer(reinterpret_cast<void**>(&c.staging_dev), c.staging, 0) != hipSuccess) {
+            (void)hipGetLastError();  // tables are then uploaded
+            c.staging_dev = nullptr;
+        }
     }
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c.readback), Ctx::kReadbackSize, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c.readback), Ctx::kReadbackSize + 64, hipHostMallocDefault);
+    if (e == hipSuccess) memset(c.readback, 0, Ctx::kReadbackSize + 64);
+    if (e == hipSuccess && hipHostGetDevicePointer(reinterpret_cast<void**>(&c.readback_dev), c.readback, 0) != hipSuccess) {
+        (void)hipGetLastError();  // read-backs then go through the runtime's copies
+        c.readback_dev = nullptr;
+    }
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         fini_ctx(c);
@@ -56,7 +65,9 @@ static void fini_ctx(Ctx& c) {
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.control = nullptr;
     c.staging = nullptr;
+    c.staging_dev = nullptr;
     c.readback = nullptr;
+    c.readback_dev = nullptr;
     c.stream = nullptr;
 }
 
@@ -263,7 +274,7 @@ ddj_status ddj_memcpy_h2d(ddj_ctx* ctx, void* dst, const void* src, size_t bytes
 }
 
 ddj_status ddj_memcpy_d2h(ddj_ctx* ctx, void* dst, const void* src, size_t bytes) {
-    return guarded(ctx, [&] { d2h(&ctx->c, dst, src, bytes); });
+    return guarded(ctx, [&] { d2h_any(&ctx->c, dst, src, bytes); });
 }
 
 ddj_status ddj_memcpy_d2d(ddj_ctx* ctx, void* dst, const void* src, size_t bytes) {

@@ -90,11 +90,16 @@ struct Ctx {
     // Pinned staging ring for small uploads: h2d copies the source here and
     // returns without waiting; the ring restarts after the stream drains.
     uint8_t* staging = nullptr;
+    uint8_t* staging_dev = nullptr;  // its device address, when the runtime gives one
     size_t staging_size = 0, staging_off = 0;
     // Pinned landing area for small read-backs (a copy into pageable memory
     // is staged by the runtime itself, which costs a further round trip).
+    // The area holds kReadbackSize bytes and, after them, the completion
+    // flag that the read-back launch sets to readback_seq.
     static constexpr size_t kReadbackSize = 64 << 10;
     uint8_t* readback = nullptr;
+    uint8_t* readback_dev = nullptr;  // its device address, when the runtime gives one
+    uint32_t readback_seq = 0;
     // set when a launch may raise the device error flag (device-side input
     // checks); the C ABI reads the flag back only then
     bool error_flag_armed = false;
@@ -196,8 +201,18 @@ struct ReadBack {
     const void* src;
     size_t bytes;
 };
+// Read-backs of library-owned device memory (small ones take one launch
+// that writes the landing area directly).  d2h_any is for pointers a
+// caller passed in, which the runtime's copy checks.
 void d2h_many(Ctx* c, const ReadBack* rb, size_t count);
+void d2h_any(Ctx* c, void* dst, const void* src, size_t bytes);
 void h2d(Ctx* c, void* dst, const void* src, size_t bytes);  // stream-ordered
+// A slot of `bytes` in the staging ring for a small table that kernels read
+// directly through the returned device address (no upload launch); the
+// caller fills staging + (address - staging_dev) before its launch.  Null
+// when the ring has no device address or the table is too big.  Like a
+// staged upload, the slot stays valid until the stream next drains.
+const uint8_t* staging_slot(Ctx* c, size_t bytes);
 void d2d(Ctx* c, void* dst, const void* src, size_t bytes);  // stream-ordered
 DevBuf make_buf(Ctx* c, const void* host, size_t bytes);     // stream-ordered upload
 // A few bytes of device memory to the host: from a registered shadow when

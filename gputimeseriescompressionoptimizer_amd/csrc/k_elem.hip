@@ -214,6 +214,35 @@ template <typename T> __device__ __forceinline__ T load_row(const uint8_t* p, si
     return t;
 }
 
+// Small read-backs: the ranges go straight into the context's pinned
+// landing area through its device address, all of them in one launch of
+// one workgroup (the runtime's copy takes a launch per range, or for larger
+// ranges a DMA transfer).  Every thread releases its stores at system scope
+// (a release alone: the full fence would also invalidate the L2), then,
+// after the barrier, thread 0 stores `seq` to the flag the host polls.
+__global__ __launch_bounds__(kBlock) void k_to_host(ToHostPieces t, uint8_t* __restrict__ dst, uint32_t* flag,
+                                                    uint32_t seq) {
+    const int tid = threadIdx.x;
+    for (int p = 0; p < t.count; p++) {  // uniform
+        const uint8_t* s = t.src[p];
+        uint8_t* d = dst + t.off[p];
+        const uint32_t nb = t.bytes[p];
+        const uintptr_t al = reinterpret_cast<uintptr_t>(s) | t.off[p] | nb;
+        if ((al & 15) == 0) {
+            for (uint32_t i = tid; i < nb / 16; i += kBlock)
+                reinterpret_cast<u32x4*>(d)[i] = reinterpret_cast<const u32x4*>(s)[i];
+        } else if ((al & 7) == 0) {
+            for (uint32_t i = tid; i < nb / 8; i += kBlock)
+                reinterpret_cast<uint64_t*>(d)[i] = reinterpret_cast<const uint64_t*>(s)[i];
+        } else {
+            for (uint32_t i = tid; i < nb; i += kBlock) d[i] = s[i];
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    __syncthreads();
+    if (tid == 0) __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_delta_sums(const uint8_t* __restrict__ din, size_t m,
                                                        uint64_t* __restrict__ vals) {
@@ -290,41 +319,39 @@ __global__ __launch_bounds__(kBlock) void k_delta_decode(const uint8_t* first_by
     if (tile == 0 && threadIdx.x == 0) out[0] = first;
 }
 
-// Exclusive prefixes of per-tile values; one workgroup of kScanThreads.
+// Exclusive prefixes of per-tile values; workgroup 0 of kScanThreads.
 // A step covers kScanRows rows of kScanThreads consecutive values: thread t
-// holds value t of every row (all loads and stores coalesced and in flight
-// together), each row is scanned across the workgroup (wave scans, then the
-// waves' totals in LDS), and the rows are chained by their totals.
-// With `keys`, the same workgroup first folds per-tile {min, max, min, max}
-// order keys into *mm_a and *mm_b.
+// holds value t of every row (all loads and stores in flight together),
+// each row is scanned across the workgroup (wave scans, then the waves'
+// totals in LDS), and the rows are chained by their totals.
+// With `keys`, workgroups 1..F fold per-tile {min, max, min, max} order keys
+// into *mm_a and *mm_b while workgroup 0 scans: each folds a slice into
+// key_part, and the last to finish (a ticket on `ticket`, F arrivals, reset
+// by that workgroup) folds the F partials.
 __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const uint64_t* __restrict__ vals, size_t tiles,
                                                              uint64_t* __restrict__ prefix,
                                                              const uint64_t* __restrict__ keys, MinMaxResult* mm_a,
-                                                             MinMaxResult* mm_b) {
+                                                             MinMaxResult* mm_b, uint64_t* key_part, uint32_t* ticket) {
     constexpr int R = kScanRows, NW = kScanThreads / kWave;
     __shared__ uint64_t wsum[R][NW];  // per row: the waves' totals, then their exclusive offsets
     __shared__ uint64_t rowtot[R];
     __shared__ uint64_t rowoff[R + 1];
     __shared__ unsigned long long mm[4];
+    __shared__ bool last;
     const int t = threadIdx.x, lane = t & (kWave - 1), wid = t / kWave;
-    if (keys) {  // uniform
+    if (blockIdx.x > 0) {  // key folding (launched only with keys)
+        const uint32_t F = gridDim.x - 1, b = blockIdx.x - 1;
+        const size_t per = (tiles + F - 1) / F, t0 = (size_t)b * per, t1 = t0 + per < tiles ? t0 + per : tiles;
         if (t < 4) mm[t] = (t & 1) ? 0ull : ~0ull;
         __syncthreads();
         uint64_t r[4] = {~0ull, 0, ~0ull, 0};
-        constexpr int KR = 8;  // tiles per thread and step, loads all in flight
-        for (size_t t0 = 0; t0 < tiles; t0 += (size_t)KR * kScanThreads) {
-            uint64_t v[KR][4];
+        for (size_t i = t0 + t; i < t1; i += kScanThreads) {
+            const u32x4* k4 = reinterpret_cast<const u32x4*>(keys + 4 * i);
+            const u32x4 lo = k4[0], hi = k4[1];
+            const uint64_t v[4] = {lo[0] | (uint64_t)lo[1] << 32, lo[2] | (uint64_t)lo[3] << 32,
+                                   hi[0] | (uint64_t)hi[1] << 32, hi[2] | (uint64_t)hi[3] << 32};
 #pragma unroll
-            for (int k = 0; k < KR; k++) {
-                const size_t i = t0 + (size_t)k * kScanThreads + t;
-#pragma unroll
-                for (int j = 0; j < 4; j++) v[k][j] = i < tiles ? keys[4 * i + j] : ((j & 1) ? 0ull : ~0ull);
-            }
-#pragma unroll
-            for (int k = 0; k < KR; k++)
-#pragma unroll
-                for (int j = 0; j < 4; j++)
-                    r[j] = (j & 1) ? (v[k][j] > r[j] ? v[k][j] : r[j]) : (v[k][j] < r[j] ? v[k][j] : r[j]);
+            for (int j = 0; j < 4; j++) r[j] = (j & 1) ? (v[j] > r[j] ? v[j] : r[j]) : (v[j] < r[j] ? v[j] : r[j]);
         }
         r[0] = wave_min(r[0]);
         r[1] = wave_max(r[1]);
@@ -337,12 +364,27 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const uint64_t* __r
             atomicMax(&mm[3], (unsigned long long)r[3]);
         }
         __syncthreads();
-        if (t == 0) {
-            mm_a->min_key = mm[0];
-            mm_a->max_key = mm[1];
-            mm_b->min_key = mm[2];
-            mm_b->max_key = mm[3];
+        if (t < 4) st_agent(&key_part[4 * (size_t)b + t], (uint64_t)mm[t]);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (t == 0) last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == F - 1;
+        __syncthreads();
+        if (!last) return;
+        if (t < 4) {
+            uint64_t f = (t & 1) ? 0ull : ~0ull;
+            for (uint32_t q = 0; q < F; q++) {
+                const uint64_t v = __hip_atomic_load(&key_part[4 * (size_t)q + t], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                f = (t & 1) ? (v > f ? v : f) : (v < f ? v : f);
+            }
+            MinMaxResult* o = t < 2 ? mm_a : mm_b;
+            if (t & 1)
+                o->max_key = f;
+            else
+                o->min_key = f;
         }
+        if (t == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
     }
     uint64_t carry = 0;
     for (size_t c0 = 0; c0 < tiles; c0 += (size_t)R * kScanThreads) {  // uniform
@@ -467,7 +509,12 @@ void TileScan::scan(Ctx* c, const uint64_t* keys, MinMaxResult* mm_a, MinMaxResu
         return;
     }
     kernel_bytes(c, "k_scan_tiles", 8.0 * (double)(2 * tiles + 1) + (keys ? 32.0 * (double)tiles : 0.0));
-    { KernelTimer kt_(c, "k_scan_tiles"); hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, c->stream, vals, tiles, prefix, keys, mm_a, mm_b); }
+    // with keys: one folding workgroup per 1024 tiles (at most 64) beside the scan
+    const unsigned F = keys ? (unsigned)std::min<size_t>(64, std::max<size_t>(1, (tiles + kScanThreads - 1) / kScanThreads)) : 0;
+    DevBuf part;
+    if (F) part = DevBuf(c, 32 * (size_t)F);
+    { KernelTimer kt_(c, "k_scan_tiles"); hipLaunchKernelGGL(k_scan_tiles, dim3(1 + F), dim3(kScanThreads), 0, c->stream, vals, tiles, prefix, keys, mm_a, mm_b,
+                                                           F ? part.get<uint64_t>() : nullptr, &c->control->counters[3]); }
     HIPC(hipGetLastError());
 }
 
@@ -475,7 +522,7 @@ void launch_scan_tiles(Ctx* c, const uint64_t* vals, size_t tiles, uint64_t* pre
     kernel_bytes(c, "k_scan_tiles", 8.0 * (double)(2 * tiles + 1));
     { KernelTimer kt_(c, "k_scan_tiles"); hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, c->stream, vals, tiles, prefix,
                                                            (const uint64_t*)nullptr, (MinMaxResult*)nullptr,
-                                                           (MinMaxResult*)nullptr); }
+                                                           (MinMaxResult*)nullptr, (uint64_t*)nullptr, (uint32_t*)nullptr); }
     HIPC(hipGetLastError());
 }
 
@@ -494,6 +541,12 @@ void launch_gather_copy(Ctx* c, const CopyRange* d_ranges, int n_ranges, uint8_t
     unsigned gx = grid_for(c, max_bytes, 64);  // 4 x 16 bytes per thread
     gx = gx > 2048 ? 2048 : gx;
     { KernelTimer kt_(c, "k_gather_copy"); hipLaunchKernelGGL(k_gather_copy, dim3(gx, n_ranges), dim3(kBlock), 0, c->stream, d_ranges, dst); }
+    HIPC(hipGetLastError());
+}
+
+void launch_to_host(Ctx* c, const ToHostPieces& t, uint8_t* host_dev, uint32_t* flag_dev, uint32_t seq) {
+    if (t.count < 0 || t.count > kToHostPieces) fail(DDJ_EINVAL, "too many read-back ranges");
+    { KernelTimer kt_(c, "k_to_host"); hipLaunchKernelGGL(k_to_host, dim3(1), dim3(kBlock), 0, c->stream, t, host_dev, flag_dev, seq); }
     HIPC(hipGetLastError());
 }
 

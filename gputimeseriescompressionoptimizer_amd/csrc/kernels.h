@@ -311,5 +311,17 @@ struct CopyPair {
     uint64_t bytes;
 };
 void launch_copy_pairs(Ctx* c, const CopyPair* d_pairs, int n_pairs, uint64_t max_bytes);
+// Up to kToHostPieces device ranges copied by one launch into pinned host
+// memory that the device addresses directly (the context's read-back area).
+constexpr int kToHostPieces = 8;
+struct ToHostPieces {
+    const uint8_t* src[kToHostPieces];
+    uint32_t off[kToHostPieces];
+    uint32_t bytes[kToHostPieces];
+    int count;
+};
+// After the data, the launch stores `seq` at flag_dev (host memory too), so
+// the host can poll for completion instead of waiting on the stream.
+void launch_to_host(Ctx* c, const ToHostPieces& t, uint8_t* host_dev, uint32_t* flag_dev, uint32_t seq);
 
 }  // namespace ddj

@@ -89,7 +89,7 @@ void tree_decompress_noalloc(Ctx* c, const uint8_t* blob, size_t blob_bytes, voi
     // one read-back into the context's pinned landing area
     uint8_t head[64] = {};
     const size_t hb = blob_bytes < sizeof head ? blob_bytes : sizeof head;
-    d2h(c, head, blob, hb);
+    d2h_any(c, head, blob, hb);
     size_t at = 0;
     auto header = [&]() -> Hdr {
         Hdr h;

@@ -1,9 +1,12 @@
 // Context, stream, caching allocator and copies.
 #include "ddj_internal.h"
+#include "kernels.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 
 namespace ddj {
 
@@ -113,7 +116,7 @@ Allocator::~Allocator() {
     for (auto& kv : live_) (void)hipFree(kv.first);
 }
 
-void d2h(Ctx* c, void* dst, const void* src, size_t bytes) {
+void d2h_any(Ctx* c, void* dst, const void* src, size_t bytes) {
     if (!bytes) return;
     const bool small = c->readback && bytes <= Ctx::kReadbackSize;
     HIPC(hipMemcpyAsync(small ? c->readback : dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -122,20 +125,65 @@ void d2h(Ctx* c, void* dst, const void* src, size_t bytes) {
     c->staging_off = 0;
 }
 
+void d2h(Ctx* c, void* dst, const void* src, size_t bytes) {
+    if (!bytes) return;
+    const ReadBack rb{dst, src, bytes};
+    d2h_many(c, &rb, 1);
+}
+
 void d2h_many(Ctx* c, const ReadBack* rb, size_t count) {
     size_t total = 0;
     for (size_t i = 0; i < count; i++) total += (rb[i].bytes + 63) & ~(size_t)63;
     if (!c->readback || total > Ctx::kReadbackSize) {
-        for (size_t i = 0; i < count; i++) d2h(c, rb[i].dst, rb[i].src, rb[i].bytes);
+        for (size_t i = 0; i < count; i++) d2h_any(c, rb[i].dst, rb[i].src, rb[i].bytes);
         return;
     }
     size_t off = 0;
-    for (size_t i = 0; i < count; i++) {
-        if (rb[i].bytes)
-            HIPC(hipMemcpyAsync(c->readback + off, rb[i].src, rb[i].bytes, hipMemcpyDeviceToHost, c->stream));
-        off += (rb[i].bytes + 63) & ~(size_t)63;
+    if (c->readback_dev && count <= (size_t)kToHostPieces) {
+        ToHostPieces t{};
+        for (size_t i = 0; i < count; i++) {
+            if (rb[i].bytes) {
+                t.src[t.count] = static_cast<const uint8_t*>(rb[i].src);
+                t.off[t.count] = (uint32_t)off;
+                t.bytes[t.count] = (uint32_t)rb[i].bytes;
+                t.count++;
+            }
+            off += (rb[i].bytes + 63) & ~(size_t)63;
+        }
+        const uint32_t seq = ++c->readback_seq;
+        uint32_t* flag = reinterpret_cast<uint32_t*>(c->readback + Ctx::kReadbackSize);
+        launch_to_host(c, t, c->readback_dev, reinterpret_cast<uint32_t*>(c->readback_dev + Ctx::kReadbackSize), seq);
+        // Poll the flag (a blocking stream wait sleeps once the work outlasts
+        // its short spin, and wakes late).  After ~200 us the loop yields
+        // between polls, leaving the cores to threads with host work; after
+        // ~1 ms it also asks the runtime whether the stream has gone idle,
+        // so that a failed launch is reported instead of waited on.
+        const auto t0 = std::chrono::steady_clock::now();
+        int phase = 0;  // 0 spin, 1 yield, 2 yield and query the stream
+        for (uint32_t spins = 1;; spins++) {
+            if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == seq) break;
+            if (phase) std::this_thread::yield();
+            if ((spins & 255) != 0) continue;
+            if (phase < 2) {
+                const auto us = std::chrono::duration_cast<std::chrono::microseconds>(
+                                    std::chrono::steady_clock::now() - t0).count();
+                phase = us > 1000 ? 2 : us > 200 ? 1 : 0;
+                continue;
+            }
+            const hipError_t q = hipStreamQuery(c->stream);
+            if (q == hipErrorNotReady) continue;
+            HIPC(q);  // idle: the flag is set by now
+            if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) != seq) fail(DDJ_EHIP, "read-back flag not set");
+            break;
+        }
+    } else {
+        for (size_t i = 0; i < count; i++) {
+            if (rb[i].bytes)
+                HIPC(hipMemcpyAsync(c->readback + off, rb[i].src, rb[i].bytes, hipMemcpyDeviceToHost, c->stream));
+            off += (rb[i].bytes + 63) & ~(size_t)63;
+        }
+        HIPC(hipStreamSynchronize(c->stream));
     }
-    HIPC(hipStreamSynchronize(c->stream));
     off = 0;
     for (size_t i = 0; i < count; i++) {
         if (rb[i].bytes) memcpy(rb[i].dst, c->readback + off, rb[i].bytes);
@@ -152,7 +200,7 @@ void read_small(Ctx* c, void* dst, const void* src, size_t bytes) {
             return;
         }
     }
-    d2h(c, dst, src, bytes);
+    d2h_any(c, dst, src, bytes);
 }
 
 void h2d(Ctx* c, void* dst, const void* src, size_t bytes) {
@@ -172,6 +220,18 @@ void h2d(Ctx* c, void* dst, const void* src, size_t bytes) {
     HIPC(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
     HIPC(hipStreamSynchronize(c->stream));  // the source may be a temporary
     c->staging_off = 0;
+}
+
+const uint8_t* staging_slot(Ctx* c, size_t bytes) {
+    if (!c->staging_dev || !bytes || bytes > c->staging_size / 4) return nullptr;
+    const size_t need = (bytes + 63) & ~(size_t)63;
+    if (c->staging_off + need > c->staging_size) {
+        HIPC(hipStreamSynchronize(c->stream));  // every staged copy and table has been read
+        c->staging_off = 0;
+    }
+    const uint8_t* d = c->staging_dev + c->staging_off;
+    c->staging_off += need;
+    return d;
 }
 
 void d2d(Ctx* c, void* dst, const void* src, size_t bytes) {

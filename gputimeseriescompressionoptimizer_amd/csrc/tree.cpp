@@ -224,8 +224,9 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
     // Concatenate (util/copy/cuda_array_copy.cu:14-33): deferred FL packs
     // write straight into their blob ranges, everything else moves in one
     // gather kernel.  The copy table, the node headers and the host-resident
-    // metadata travel in one upload: their ranges point into the table
-    // buffer itself.
+    // metadata form one table, read by the kernel from the pinned staging
+    // ring (or, without a device address for it, uploaded in one copy):
+    // their ranges point into the table itself.
     DevBuf blob(c, total);
     size_t n_ranges = 0, inline_bytes = 0;
     for (auto& p : pieces) {
@@ -236,9 +237,9 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
             inline_bytes += p.part.n;
         if (p.is_header || (p.part.n && !p.part.fl && !p.part.split)) n_ranges++;
     }
-    DevBuf rdev(c, sizeof(CopyRange) * n_ranges + inline_bytes);
-    const uint8_t* inl_dev = rdev.bytes_ptr() + sizeof(CopyRange) * n_ranges;
-    std::vector<uint8_t> table(sizeof(CopyRange) * n_ranges + inline_bytes);
+    const size_t table_bytes = sizeof(CopyRange) * n_ranges + inline_bytes;
+    std::vector<uint8_t> table(table_bytes);
+    std::vector<size_t> inline_ranges;  // their src holds the offset in the inline bytes until placed
     CopyRange* ranges = reinterpret_cast<CopyRange*>(table.data());
     uint8_t* inl = table.data() + sizeof(CopyRange) * n_ranges;
     size_t off = 0, io = 0, ri = 0;
@@ -258,7 +259,8 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
         }
         if (p.is_header) {
             memcpy(inl + io, p.hdr, 8);
-            ranges[ri++] = CopyRange{inl_dev + io, off, 8};
+            inline_ranges.push_back(ri);
+            ranges[ri++] = CopyRange{reinterpret_cast<const uint8_t*>(io), off, 8};
             io += 8;
             off += 8;
             copied += 8;
@@ -268,7 +270,8 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
                 jobs.push_back(JobAt{p.part.fl.get(), off, len});
             } else if (p.part.on_host()) {
                 memcpy(inl + io, p.part.host->data(), len);
-                ranges[ri++] = CopyRange{inl_dev + io, off, len};
+                inline_ranges.push_back(ri);
+                ranges[ri++] = CopyRange{reinterpret_cast<const uint8_t*>(io), off, len};
                 io += len;
             } else {
                 ranges[ri++] = CopyRange{p.part.p, off, len};
@@ -283,9 +286,23 @@ Part compress_tree(Ctx* c, const NodeP& root, DView data) {
     if (off != total || ri != n_ranges) fail(DDJ_EHIP, "blob size mismatch");
     // deferred splits first: they also fill outputs that the gather copies
     for (const auto& sj : splits) run_split_job(c, *sj.first, blob.bytes_ptr() + sj.second);
-    h2d(c, rdev.get(), table.data(), table.size());
+    // the table's device address: a slot of the staging ring, taken now (no
+    // staged copy may follow before the launch), else a device buffer
+    DevBuf rdev;
+    const uint8_t* tdev = staging_slot(c, table_bytes);
+    if (!tdev) {
+        rdev = DevBuf(c, table_bytes);
+        tdev = rdev.bytes_ptr();
+    }
+    for (size_t k : inline_ranges)
+        ranges[k].src = tdev + sizeof(CopyRange) * n_ranges + reinterpret_cast<uintptr_t>(ranges[k].src);
+    if (rdev.get())
+        h2d(c, rdev.get(), table.data(), table.size());
+    else
+        memcpy(c->staging + (tdev - c->staging_dev), table.data(), table.size());
     kernel_bytes(c, "k_gather_copy", 2.0 * (double)copied);
-    launch_gather_copy(c, rdev.get<CopyRange>(), (int)n_ranges, blob.bytes_ptr(), std::max<uint64_t>(max_bytes, 8));
+    launch_gather_copy(c, reinterpret_cast<const CopyRange*>(tdev), (int)n_ranges, blob.bytes_ptr(),
+                       std::max<uint64_t>(max_bytes, 8));
     // the FL streams go straight to their blob ranges (any byte offset)
     for (const JobAt& j : jobs) run_fl_job(c, *j.job, blob.bytes_ptr() + j.off, j.bytes);
     return own(std::move(blob));
