@@ -278,11 +278,12 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
 // compaction of k_split, then each side's rows are packed at their bit
 // offset in their FL stream (a side's row r at bits [r b, r b + b)), so that
 // neither side is ever written unpacked.  Thread g < 128 packs yes rows
-// [32g, 32g + 32) of the tile, thread 128 + g the no rows, into b words in
-// LDS as if the side's tile stream started at bit 0; the words are then
-// funnel-shifted to the stream's true bit offset.  Words the tile fills
-// alone are stored at any byte address (16-byte stores realigned to the
-// destination, single bytes at the ends); the first and last word of a side
+// [32g, 32g + 32) of the tile, thread 128 + g the no rows, into LDS words
+// already at the stream's bit offset (a group starts at bit (first row * b)
+// mod 32, so the LDS words are the stream's words); the word a group shares
+// with the next goes through a carry slot, ORed in after a barrier.  Words
+// the tile fills alone are stored at any byte address (16-byte stores
+// realigned to the destination, single bytes at the ends); the first and last word of a side
 // that the tile shares with its neighbours go to edges[4 t + 2 side + 0/1],
 // assembled by k_fl_edges.
 template <typename T, int K>
