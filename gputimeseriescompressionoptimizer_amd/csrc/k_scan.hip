@@ -197,17 +197,32 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
         atomicOr(&cwl[bit >> 5], cchunk << sh);
         if (sh && sh + 2 * c > 32) atomicOr(&cwl[(bit >> 5) + 1], cchunk >> (32 - sh));
     }
-    int yr = (int)excl, nr = threadIdx.x * kItems - (int)excl;
+    // compaction without branches (with codes, only the no rows are kept);
+    // the guard for rows past the column's end is needed on its last tile only
     const int nbase = CODES ? 0 : (int)total;
-#pragma unroll
-    for (int k = 0; k < kItems; k++) {
-        if (threadIdx.x * kItems + k >= cnt) break;
-        if ((mask >> k) & 1) {
-            if (!CODES) buf[P(yr)] = v[k];
-            yr++;
+    int yr = (int)excl, nr = nbase + threadIdx.x * kItems - (int)excl;
+    auto place = [&](int k, int f) {
+        if (CODES) {
+            if (!f) buf[P(nr)] = v[k];
         } else {
-            buf[P(nbase + nr)] = v[k];
-            nr++;
+            buf[P(f ? yr : nr)] = v[k];
+        }
+    };
+    if (cnt == kTile) {  // uniform
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const int f = (mask >> k) & 1;
+            place(k, f);
+            yr += f;
+            nr += 1 - f;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const int f = (mask >> k) & 1;
+            if (threadIdx.x * kItems + k < cnt) place(k, f);
+            yr += f;
+            nr += 1 - f;
         }
     }
     __syncthreads();
@@ -330,22 +345,36 @@ __global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, si
     if (tile == 0 && threadIdx.x == 0) stencil[0] = (uint8_t)(n % 8);
     uint64_t total;
     const uint64_t excl = block_exclusive_scan<uint64_t>((uint64_t)c, &total, red);
-    int yr = (int)excl, nr = threadIdx.x * kItems - (int)excl;
+    // compaction without branches: each row's slot is a select, and the
+    // guard for rows past the column's end is needed on its last tile only
+    int yr = (int)excl, nr = (int)total + threadIdx.x * kItems - (int)excl;
+    if (cnt == kTile) {  // uniform
 #pragma unroll
-    for (int k = 0; k < kItems; k++) {
-        if (threadIdx.x * kItems + k >= cnt) break;
-        if ((mask >> k) & 1)
-            buf[P(yr++)] = v[k];
-        else
-            buf[P((int)total + nr++)] = v[k];
+        for (int k = 0; k < kItems; k++) {
+            const int f = (mask >> k) & 1;
+            buf[P(f ? yr : nr)] = v[k];
+            yr += f;
+            nr += 1 - f;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kItems; k++) {
+            const int f = (mask >> k) & 1;
+            if (threadIdx.x * kItems + k < cnt) buf[P(f ? yr : nr)] = v[k];
+            yr += f;
+            nr += 1 - f;
+        }
     }
     __syncthreads();
     const int cy = (int)total, cn = cnt - cy;
     // pack: thread g < 128 the yes group g, thread 128 + g the no group g
-    const bool is_yes = threadIdx.x < kBlock / 2;
+    static_assert((kBlock / 2) % kWave == 0, "each wave packs one side");
+    const bool is_yes = threadIdx.x < kBlock / 2;  // uniform per wave
     const int grp = threadIdx.x & (kBlock / 2 - 1);
     const int cs = is_yes ? cy : cn;
-    const int b = is_yes ? yside.bits : nside.bits;
+    // width and start bit are the same for every group of a side, so
+    // scalar: the pack's word boundaries are then uniform branches
+    const int b = __builtin_amdgcn_readfirstlane(is_yes ? yside.bits : nside.bits);
     U w32[32];
     const bool packs = 32 * grp < cs;
     if (packs) {
@@ -364,7 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, si
     __shared__ uint32_t carry[2][kTile / 32];
     uint32_t* words = reinterpret_cast<uint32_t*>(buf);
     const int wy = (cy + 31) / 32 * yside.bits + 1;  // yes words (and the partial word after them), then the no words
-    const int o_me = (int)(((is_yes ? Y0 : (uint64_t)(base - Y0)) * (uint64_t)b) & 31);
+    const int o_me = __builtin_amdgcn_readfirstlane((int)(((is_yes ? Y0 : (uint64_t)(base - Y0)) * (uint64_t)b) & 31));
     if (packs) {
         uint32_t* w = words + (is_yes ? 0 : wy) + grp * b;
         const uint64_t m = lowmask64((unsigned)b);
