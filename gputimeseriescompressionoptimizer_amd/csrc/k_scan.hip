@@ -800,8 +800,43 @@ __global__ __launch_bounds__(kBlock) void k_merge(const uint8_t* __restrict__ bi
             ylim = j0 < n_yes ? (int)((n_yes - j0) < (uint64_t)kItems ? (n_yes - j0) : (uint64_t)kItems) : 0;
         }
     }
+    bool merged = false;  // uniform
+    if constexpr (CODES) {
+        if (staged && dc.bits <= 2) {
+            // 1- and 2-bit codes: the thread's (at most 16) codes lie in a
+            // 64-bit window of the staged words, and the (at most 4) values
+            // sit in registers; each row is then a select, no branch
+            const uint64_t win = (uint64_t)cur | ((uint64_t)code_word(cq + 4) << (32 - (int)dc.bits * ((int)cper - fleft)));
+            const uint32_t cm = (1u << dc.bits) - 1u, kmax = dc.K - 1;
+            const T d0 = sdict[0], d1 = sdict[kmax < 1 ? kmax : 1], d2 = sdict[kmax < 2 ? kmax : 2],
+                    d3 = sdict[kmax < 3 ? kmax : 3];
+            int i = 0;
 #pragma unroll
-    for (int k = 0; k < kItems; k++) {
+            for (int k = 0; k < kItems; k++) {
+                const int f = (mask >> k) & 1;
+                uint32_t code = (uint32_t)(win >> (dc.bits * i)) & cm;
+                code = code < kmax ? code : kmax;
+                T dv = code == 0 ? d0 : code == 1 ? d1 : code == 2 ? d2 : d3;
+                dv = i < ylim ? dv : T(0);  // codes past the stream read as zero (flagged above)
+                const T nv = CONST_NO ? no_const : buf[P(nr < kTile ? nr : kTile - 1)];
+                v[k] = f ? dv : nv;
+                i += f;
+                nr += 1 - f;
+            }
+            merged = true;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kItems && !merged; k++) {
+        if constexpr (!CODES) {  // rows from LDS: a select, no branch
+            const int f = (mask >> k) & 1;
+            const int idx = f ? yr : nr;
+            const T t = buf[P(idx < kTile ? idx : kTile - 1)];
+            v[k] = (CONST_NO && !f) ? no_const : t;
+            yr += f;
+            nr += 1 - f;
+            continue;
+        }
         if ((mask >> k) & 1) {
             if constexpr (CODES) {
                 T val = T(0);
