@@ -153,27 +153,27 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
     const uint64_t Y0 = prefix[tile];  // loaded with the tile
     if (CODES)
         for (int q = threadIdx.x; q < kCodeWords; q += kBlock) cwl[q] = 0;
-    if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
+    T v[kItems];
+    if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {  // uniform
+        // a whole tile: each thread's 16 rows straight into registers
         constexpr int V = 16 / sizeof(T);
-        const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
-        for (int i = threadIdx.x; i < kTile / V; i += kBlock) {
-            const u32x4 w = s4[i];
-            T vals[V];
-            __builtin_memcpy(vals, &w, 16);
+        const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base + (size_t)threadIdx.x * kItems);
 #pragma unroll
-            for (int k = 0; k < V; k++) buf[P(V * i + k)] = vals[k];
+        for (int q = 0; q < kItems / V; q++) {
+            const u32x4 w = s4[q];
+            __builtin_memcpy(&v[q * V], &w, 16);
         }
     } else {
         for (int i = threadIdx.x; i < kTile; i += kBlock) buf[P(i)] = i < cnt ? x[base + i] : padv;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kItems; k++) v[k] = buf[P(threadIdx.x * kItems + k)];
     }
-    __syncthreads();
-    T v[kItems];
     uint32_t mask = 0;
     uint32_t cchunk = 0;  // CODES: the thread's codes, 2 bits each, in row order
     int c = 0;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
-        v[k] = buf[P(threadIdx.x * kItems + k)];
         uint32_t code;
         bool f = p.template eval<K>(v[k], code);
         mask |= (uint32_t)f << k;
@@ -314,26 +314,26 @@ __global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, si
     const int cnt = (int)((n - base) < (size_t)kTile ? (n - base) : (size_t)kTile);
     const T padv = x[n - 1];
     const uint64_t Y0 = prefix[tile];
-    if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {
+    T v[kItems];
+    if (cnt == kTile && (reinterpret_cast<uintptr_t>(x + base) & 15) == 0) {  // uniform
+        // a whole tile: each thread's 16 rows straight into registers
         constexpr int V = 16 / sizeof(T);
-        const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base);
-        for (int i = threadIdx.x; i < kTile / V; i += kBlock) {
-            const u32x4 w = s4[i];
-            T vals[V];
-            __builtin_memcpy(vals, &w, 16);
+        const u32x4* s4 = reinterpret_cast<const u32x4*>(x + base + (size_t)threadIdx.x * kItems);
 #pragma unroll
-            for (int k = 0; k < V; k++) buf[P(V * i + k)] = vals[k];
+        for (int q = 0; q < kItems / V; q++) {
+            const u32x4 w = s4[q];
+            __builtin_memcpy(&v[q * V], &w, 16);
         }
     } else {
         for (int i = threadIdx.x; i < kTile; i += kBlock) buf[P(i)] = i < cnt ? x[base + i] : padv;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kItems; k++) v[k] = buf[P(threadIdx.x * kItems + k)];
     }
-    __syncthreads();
-    T v[kItems];
     uint32_t mask = 0;
     int c = 0;
 #pragma unroll
     for (int k = 0; k < kItems; k++) {
-        v[k] = buf[P(threadIdx.x * kItems + k)];
         uint32_t code;
         const bool f = p.template eval<K>(v[k], code);
         mask |= (uint32_t)f << k;
