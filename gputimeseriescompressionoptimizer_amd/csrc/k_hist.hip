@@ -167,7 +167,6 @@ __global__ __launch_bounds__(kHistThreads) void k_hist_count(const T* __restrict
     __shared__ uint32_t scnt[kLdsSlots];
     __shared__ uint64_t s_nk[kHistThreads / kWave * kMaxK];
     __shared__ int s_nc[kHistThreads / kWave];
-    __shared__ uint64_t swin[kHistThreads / kWave + 1];
     Small4 nan4;
     unsigned long long nans = 0;
     for (int i = threadIdx.x; i < kLdsSlots; i += kHistThreads) {
@@ -192,12 +191,9 @@ __global__ __launch_bounds__(kHistThreads) void k_hist_count(const T* __restrict
         }
         global_insert(keys, counts, cap_mask, key, (unsigned long long)add, ctl);
     };
-    // Two keys per thread counted in registers: in the workgroup's first
-    // tile the first two distinct keys the thread meets, afterwards the
-    // workgroup's two most frequent keys of that tile in every lane.  On
-    // skewed columns the rows that hit them skip the LDS atomics, whose
-    // same-address conflicts are the cost (a lane's first keys hold ~23% of
-    // the rows of the 50-code Zipf column, its two most frequent 45%).
+    // Two keys per thread counted in registers (the first two distinct keys
+    // the thread meets): on skewed columns most rows hit them and skip the
+    // LDS atomics, whose same-address conflicts are the cost.
     uint64_t rk0 = kEmpty, rk1 = kEmpty;
     uint32_t rc0 = 0, rc1 = 0;
     auto count = [&](T v) {
@@ -232,8 +228,6 @@ __global__ __launch_bounds__(kHistThreads) void k_hist_count(const T* __restrict
     constexpr int Q = kRows / V / kHistThreads;  // 16-byte loads per thread and tile
     const size_t tiles = (n + kRows - 1) / kRows;
     const bool aligned = (reinterpret_cast<uintptr_t>(x) & 15) == 0;
-    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    bool first = true;  // uniform
     for (size_t t = blockIdx.x; t < tiles; t += gridDim.x) {  // uniform per workgroup
         const size_t base = t * kRows;
         if (aligned && base + kRows <= n) {
@@ -251,42 +245,6 @@ __global__ __launch_bounds__(kHistThreads) void k_hist_count(const T* __restrict
         } else {
             const int cnt = (int)((n - base) < (size_t)kRows ? (n - base) : (size_t)kRows);
             for (int i = threadIdx.x; i < cnt; i += kHistThreads) count(x[base + i]);
-        }
-        if (first && t + gridDim.x < tiles) {  // uniform: more tiles follow
-            // every lane's register keys become the two keys with the most
-            // rows in the table so far (the lanes' own keys added first)
-            first = false;
-            if (rk0 != kEmpty) lds_add(rk0, rc0);
-            if (rk1 != kEmpty) lds_add(rk1, rc1);
-            rk0 = rk1 = kEmpty;
-            rc0 = rc1 = 0;
-            __syncthreads();
-            uint32_t won = ~0u;
-            for (int r = 0; r < 2; r++) {
-                uint64_t best = 0;  // (count << 32) | slot; 0: none
-                for (int q = threadIdx.x; q < kLdsSlots; q += kHistThreads)
-                    if (skey[q] != kEmpty && (uint32_t)q != won) {
-                        const uint64_t c = ((uint64_t)scnt[q] << 32) | (uint32_t)q;
-                        best = c > best ? c : best;
-                    }
-                best = wave_max(best);
-                if (lane == 0) swin[wid] = best;
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    uint64_t b = 0;
-                    for (int w = 0; w < kHistThreads / kWave; w++) b = swin[w] > b ? swin[w] : b;
-                    swin[kHistThreads / kWave] = b;
-                }
-                __syncthreads();
-                const uint64_t b = swin[kHistThreads / kWave];
-                __syncthreads();  // swin is rewritten by the next round
-                if ((b >> 32) == 0) break;  // uniform: no more keys
-                won = (uint32_t)b;
-                if (r == 0)
-                    rk0 = skey[won];
-                else
-                    rk1 = skey[won];
-            }
         }
     }
     if (rk0 != kEmpty) lds_add(rk0, rc0);
