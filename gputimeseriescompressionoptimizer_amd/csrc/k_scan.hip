@@ -37,6 +37,7 @@ template <typename T> struct Pred {
     T a, b;
     T set[4];
     int set_n;
+    uint32_t active;  // bit j: set[j] is in use (j < set_n)
     template <int K> __device__ __forceinline__ bool eval(T v, uint32_t& code) const {
         code = 0;
         switch (K) {
@@ -44,14 +45,16 @@ template <typename T> struct Pred {
         case PRED_INSIDE: return !(v > b || v < a);   // OutsideOperator: true inside [low, high]
         case PRED_NOT_EQUAL: return v != a;           // _constEncodeStencilKernel (const_encoding.cu:19-25)
         default: {                                    // getMostFrequentStencilKernel (dict_encoding.cu:15-34)
-            bool r = false;
-#pragma unroll
-            for (int j = 0; j < 4; j++)               // the dictionary holds at most 4 keys
-                if (j < set_n && v == set[j]) {
-                    r = true;
-                    code |= (uint32_t)j;              // _compressUniqueKernel ORs every matching index
-                }
-            return r;
+            // the dictionary holds at most 4 keys: bit j of m is a match of
+            // key j, and the code ORs every matching index, as
+            // _compressUniqueKernel does (indices 1, 2, 3 contribute 1, 2, 3).
+            // Integers, not per-key masks, so that 16 rows x 4 keys of
+            // comparisons never sit in scalar registers at once.
+            const uint32_t m = ((v == set[0]) ? 1u : 0u) | ((v == set[1]) ? 2u : 0u) |
+                               ((v == set[2]) ? 4u : 0u) | ((v == set[3]) ? 8u : 0u);
+            const uint32_t mm = m & active;
+            code = ((mm >> 1) & 3u) | ((mm & 8u) ? 3u : 0u);
+            return mm != 0;
         }
         }
     }
@@ -1204,6 +1207,7 @@ template <typename T> Pred<T> make_pred(const PredArgs& pa) {
     p.b = bits_to<T>(pa.b_bits);
     for (int j = 0; j < 4; j++) p.set[j] = bits_to<T>(pa.set_bits[j]);
     p.set_n = pa.set_n;
+    p.active = pa.set_n >= 4 ? 0xFu : pa.set_n <= 0 ? 0u : (1u << pa.set_n) - 1u;
     return p;
 }
 }  // namespace
