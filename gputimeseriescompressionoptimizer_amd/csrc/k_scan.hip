@@ -301,9 +301,9 @@ __global__ __launch_bounds__(kBlock) void k_split(const T* __restrict__ x, size_
 // mod 32, so the LDS words are the stream's words); the word a group shares
 // with the next goes through a carry slot, ORed in after a barrier.  Words
 // the tile fills alone are stored at any byte address (16-byte stores
-// realigned to the destination, single bytes at the ends); the first and last word of a side
-// that the tile shares with its neighbours go to edges[4 t + 2 side + 0/1],
-// assembled by k_fl_edges.
+// realigned to the destination, single bytes at the ends); the first and
+// last word of a side that the tile shares with its neighbours go to
+// edges[4 t + 2 side + 0/1], assembled by k_fl_edges.
 template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_split_fl(const T* __restrict__ x, size_t n, Pred<T> p,
                                                      uint8_t* __restrict__ stencil, FlSide yside, FlSide nside,
